@@ -1,0 +1,183 @@
+/*
+ * mpsort_gpu.h -- C ABI of the MI355X histogram-sort library (libmpsort_gpu.so).
+ *
+ * Plain C types only: pointers, sizes, int64 counts.  One process drives one
+ * GPU; the P ranks of the reference's MPI sort are P "virtual ranks": P
+ * consecutive segments of one input array and P consecutive segments of one
+ * output array.  The output is the reference's output: the records of all
+ * virtual ranks, stably sorted by radix, cut into the requested per-rank
+ * output counts.
+ *
+ * Reference interfaces replaced (paths relative to the MP-sort tree):
+ *   mpsort_gpu_newarray     mpsort_mpi_newarray_impl  mpsort.h:36-47,
+ *                           mpsort-mpi.c:161-331 (with the radix callback
+ *                           replaced by the key descriptor that
+ *                           radix_data_init builds, binding.pyx:50-79)
+ *   mpsort_gpu_newarray_radix  the same call with an opaque radix callback
+ *   mpsort_gpu_last_stats / mpsort_gpu_report_last_run
+ *                           mpsort_mpi_report_last_run mpsort.h:49,
+ *                           mpsort-mpi.c:110-119 (phase timers)
+ *   mpsort_gpu_last_counts  no exported counterpart: the splitters, C, CLT/CLE
+ *                           and send counts the reference keeps on the stack
+ *                           of mpsort_mpi_histogram_sort (mpsort-mpi.c:333-364)
+ *   mpsort_gpu_set_allocator  MPIU_SetMalloc mp-mpiu.h:4-16 (device side)
+ *
+ * All entry points return MPSORT_OK (0) or a negative MPSORT_ERR_* code; the
+ * reference aborts instead (mpsort-mpi.c:74,185,231,329,500,509,707-716).
+ * mpsort_gpu_last_error() gives the message of the last failure.
+ * The library is not reentrant (neither is the reference: mpsort-mpi.c:17,105).
+ */
+#ifndef MPSORT_GPU_H
+#define MPSORT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPSORT_OK 0
+#define MPSORT_ERR_ARG (-1)          /* invalid argument */
+#define MPSORT_ERR_SIZE_MISMATCH (-2)/* sum(nin) != sum(nout): mpsort-mpi.c:225-232 */
+#define MPSORT_ERR_LAYOUT (-3)       /* layout solver inconsistency: mpsort-mpi.c:496-510,707-716 */
+#define MPSORT_ERR_CHECKSUM (-4)     /* data changed: mpsort-mpi.c:324-330 */
+#define MPSORT_ERR_HIP (-5)          /* HIP runtime failure */
+#define MPSORT_ERR_NOMEM (-6)        /* device allocation failed */
+#define MPSORT_ERR_UNSUPPORTED (-7)  /* key or record shape not supported */
+#define MPSORT_ERR_NODEVICE (-8)     /* no GPU */
+#define MPSORT_ERR_TIMEOUT (-9)      /* a device wait exceeded its spin limit */
+
+/* Key of a record: nwords words of width 4 or 8 bytes starting at byte
+ * `offset`; the last word is the most significant (binding.pyx:50-121,
+ * lexsort convention).  Signed words order as signed integers. */
+typedef struct {
+    int64_t offset;
+    int32_t width;     /* 4 or 8 */
+    int32_t is_signed; /* 0 or 1 */
+    int32_t nwords;    /* >= 1; width*nwords <= 28 unless the record itself
+                          is 4/8/16/32 bytes and holds only the key at a
+                          4-byte aligned offset */
+    int32_t reserved;
+} mpsort_key_desc;
+
+/* flags */
+#define MPSORT_GPU_DEVICE_PTRS (1u << 0) /* base and out are device pointers */
+#define MPSORT_GPU_NO_CHECKSUM (1u << 1) /* skip the byte-sum invariant check */
+#define MPSORT_GPU_TIMING (1u << 2)      /* record per-pass kernel times */
+#define MPSORT_GPU_LEARNED (1u << 3)     /* learned-CDF splitters + bucket sort */
+#define MPSORT_GPU_FORCE_GENERAL (1u << 4) /* always sort packed {key, index} records */
+
+/*
+ * Sort the P virtual ranks of `base` (rank t holds nin[t] records, in rank
+ * order) into `out` (rank t receives nout[t] records).  elsize is the record
+ * size in bytes.  out may equal base (in-place, as mpsort_mpi_impl).  With
+ * MPSORT_GPU_DEVICE_PTRS the pointers are device memory; otherwise host
+ * memory that is staged through the device.
+ */
+int mpsort_gpu_newarray(const void *base, const int64_t *nin, int P,
+                        void *out, const int64_t *nout, size_t elsize,
+                        const mpsort_key_desc *key, unsigned flags);
+
+/* The same with the reference's opaque radix callback (mpsort.h:1-4): the
+ * radix of every record is materialised once on the host (rsize <= 28 bytes,
+ * compared as a little-endian unsigned integer, radixsort.c:146-195), the
+ * {radix, index} pairs are sorted on the GPU and the records gathered.  Host
+ * pointers only. */
+int mpsort_gpu_newarray_radix(const void *base, const int64_t *nin, int P,
+                              void *out, const int64_t *nout, size_t elsize,
+                              void (*radix)(const void *ptr, void *radix, void *arg),
+                              size_t rsize, void *arg, unsigned flags);
+
+typedef struct {
+    double t_first_sort_ms;  /* FirstSort (mpsort-mpi.c:369) */
+    double t_findp_ms;       /* PmaxPmin + bisection (mpsort-mpi.c:375-441) */
+    double t_laysolve_ms;    /* LayDistr + LaySolve (mpsort-mpi.c:443-510) */
+    double t_exchange_ms;    /* Exchange (mpsort-mpi.c:571-592) */
+    double t_second_sort_ms; /* SecondSort (mpsort-mpi.c:597) */
+    double t_total_ms;       /* device time of the whole call */
+    double t_h2d_ms;         /* host staging (host-pointer calls only) */
+    double t_d2h_ms;
+    int32_t nvranks;
+    int32_t record_bytes;    /* sort record size actually moved */
+    int32_t first_passes;    /* digit passes run by FirstSort */
+    int32_t second_passes;
+    int32_t bisect_rounds;   /* max over splitters */
+    int32_t general_path;    /* 1 when packed {key, index} records were sorted */
+    int64_t nrecords;
+    int64_t checksum_in;
+    int64_t checksum_out;
+    int32_t npass_ms;        /* entries used in pass_ms */
+    int32_t learned;         /* 1 when the learned path ran */
+    double pass_ms[64];      /* per digit pass kernel time (FirstSort then SecondSort) */
+    /* learned path */
+    int32_t splitters_exact;   /* proposals accepted without repair */
+    int32_t splitters_repaired;
+    int32_t buckets;
+    int32_t buckets_overflow;  /* buckets that fell back to the plain sort */
+    int32_t repair_rounds;
+    int32_t reserved2;
+    double t_model_ms;
+    double t_bucket_ms;
+} mpsort_gpu_stats;
+
+int mpsort_gpu_last_stats(mpsort_gpu_stats *stats);
+
+/* Prints the phase times of the last call in the reference's format
+ * (mpsort-mpi.c:110-119). */
+void mpsort_gpu_report_last_run(void);
+
+/*
+ * Per-virtual-rank counts of the last mpsort_gpu_newarray call, in the
+ * reference's layout (P = nvranks):
+ *   splitters  (P-1) radix values of rsize bytes each (rsize = 8*nwords for
+ *              the descriptor form, the callback's rsize otherwise)
+ *   pmin/pmax  rsize bytes each
+ *   C          P+1 cumulative output counts
+ *   myCLT/myCLE  P x (P+1): rank j's counts of radix < / <= splitter i at
+ *              [j*(P+1) + i+1], with columns 0 and P fixed at 0 and n_j
+ *   myC        P x (P+1): cumulative send offsets of rank j
+ *   send       P x P: records rank j sends to rank r at [j*P + r]
+ * Any pointer may be NULL.
+ */
+int mpsort_gpu_last_counts(unsigned char *splitters, unsigned char *pmin, unsigned char *pmax,
+                           int64_t *C, int64_t *myCLT, int64_t *myCLE, int64_t *myC,
+                           int64_t *send, int64_t *rsize_out);
+
+/* Device allocator hook (MPIU_SetMalloc mp-mpiu.h:4-16).  NULL restores
+ * hipMalloc/hipFree.  Frees every cached workspace. */
+typedef void *(*mpsort_gpu_malloc_fn)(size_t bytes, void *userdata);
+typedef void (*mpsort_gpu_free_fn)(void *ptr, void *userdata);
+int mpsort_gpu_set_allocator(mpsort_gpu_malloc_fn m, mpsort_gpu_free_fn f, void *userdata);
+
+/* Frees the cached device workspaces of the current device. */
+int mpsort_gpu_release(void);
+
+/* Number of virtual ranks used by the MPI-shaped entry points
+ * (mpsort_mpi_impl / mpsort_mpi_newarray_impl / radix_sort); default 1, or
+ * the MPSORT_GPU_VRANKS environment variable. */
+void mpsort_gpu_set_vranks(int P);
+int mpsort_gpu_get_vranks(void);
+
+const char *mpsort_gpu_last_error(void);
+const char *mpsort_gpu_version(void);
+
+/* Device-memory helpers for callers without a HIP toolchain (tests, ctypes). */
+void *mpsort_gpu_dmalloc(size_t bytes);
+int mpsort_gpu_dfree(void *p);
+int mpsort_gpu_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int mpsort_gpu_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int mpsort_gpu_synchronize(void);
+int mpsort_gpu_device_count(void);
+
+/* Option bits of the reference (mpsort.h:17-24).  Accepted and recorded: on
+ * one GPU there is no Alltoallv to make sparse and no rank to gather to, so
+ * they do not change the result (nor does any of them in the reference). */
+void mpsort_mpi_set_options(int options);
+int mpsort_mpi_has_options(int options);
+void mpsort_mpi_unset_options(int options);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,451 @@
+// histsort.hip -- the splitter search, layout solve and exchange of the
+// histogram sort (mpsort-mpi.c:333-604), with the P MPI ranks of the
+// reference held as P virtual-rank segments of one GPU's memory.
+//
+//   bisect_kernel   _find_Pmax_Pmin_C (mpsort-mpi.c:606-661) + the piter
+//                   bisection loop (internal-parallel.h:128-248,
+//                   mpsort-mpi.c:381-441) + _histogram counts
+//                   (internal-parallel.h:97-126).  One wave per splitter, one
+//                   lane per segment; every round does two binary searches
+//                   per segment inside an index window that only shrinks.
+//   layout_kernel   _solve_for_layout_mpi (mpsort-mpi.c:663-727), one thread
+//                   per receiving rank.
+//   runs_kernel     SendCount / RecvDispl (mpsort-mpi.c:483-510) as a table of
+//                   (sender, receiver) runs in output order.
+//   exchange_kernel MPIU_Alltoallv (mp-mpiu.c:69-156): every output record is
+//                   copied from its sender run; receiver buffers hold the
+//                   sender runs in sender order.
+#include "device.h"
+
+namespace mpsg {
+
+struct Key {
+    uint64_t w[4];  // little-endian: w[0] least significant
+};
+
+__device__ __forceinline__ Key load_key(const uint32_t* __restrict__ recs, int rw, int64_t idx,
+                                        const KeySpec& ks) {
+    Key k = {{0, 0, 0, 0}};
+    const uint32_t* rec = recs + idx * rw;
+    if (ks.key_bytes == 8 && (ks.key_off & 3) == 0) {
+        const uint32_t lo = rec[ks.key_off >> 2] ^ ((uint32_t)ks.xmask[0] | ((uint32_t)ks.xmask[1] << 8) |
+                                                    ((uint32_t)ks.xmask[2] << 16) | ((uint32_t)ks.xmask[3] << 24));
+        const uint32_t hi = rec[(ks.key_off >> 2) + 1] ^ ((uint32_t)ks.xmask[4] | ((uint32_t)ks.xmask[5] << 8) |
+                                                          ((uint32_t)ks.xmask[6] << 16) | ((uint32_t)ks.xmask[7] << 24));
+        k.w[0] = (uint64_t)lo | ((uint64_t)hi << 32);
+        return k;
+    }
+    if (ks.key_bytes == 4 && (ks.key_off & 3) == 0) {
+        const uint32_t v = rec[ks.key_off >> 2] ^ ((uint32_t)ks.xmask[0] | ((uint32_t)ks.xmask[1] << 8) |
+                                                   ((uint32_t)ks.xmask[2] << 16) | ((uint32_t)ks.xmask[3] << 24));
+        k.w[0] = v;
+        return k;
+    }
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(rec) + ks.key_off;
+    for (int b = 0; b < ks.key_bytes; b++) {
+        const uint64_t byte = (uint64_t)(uint8_t)(p[b] ^ ks.xmask[b]);
+        k.w[b >> 3] |= byte << ((b & 7) * 8);
+    }
+    return k;
+}
+
+__device__ __forceinline__ int key_cmp(const Key& a, const Key& b, int nw) {
+#pragma unroll
+    for (int w = 3; w >= 0; w--) {
+        if (w < nw) {
+            if (a.w[w] < b.w[w]) return -1;
+            if (a.w[w] > b.w[w]) return 1;
+        }
+    }
+    return 0;
+}
+
+// floor((a+b)/2) as (nw*64+1)-bit arithmetic: radixsort.c:55-60, 111-144
+__device__ __forceinline__ Key key_mid(const Key& a, const Key& b, int nw) {
+    Key s;
+    uint64_t carry = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const uint64_t t = a.w[w] + carry;
+        const uint64_t c1 = t < carry;
+        const uint64_t u = t + b.w[w];
+        const uint64_t c2 = u < t;
+        s.w[w] = (w < nw) ? u : 0;
+        carry = (w < nw) ? (c1 | c2) : carry;
+    }
+    Key m;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const uint64_t hi = (w + 1 < nw) ? s.w[w + 1] : ((w + 1 == nw) ? carry : 0);
+        m.w[w] = (w < nw) ? ((s.w[w] >> 1) | (hi << 63)) : 0;
+    }
+    return m;
+}
+
+__device__ __forceinline__ Key shfl_key(const Key& k, int off) {
+    Key o;
+#pragma unroll
+    for (int w = 0; w < 4; w++) o.w[w] = __shfl_xor(k.w[w], off, 64);
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void bisect_kernel(
+    const uint32_t* __restrict__ recs, int rw, KeySpec ks, const int64_t* __restrict__ sbeg,
+    const int64_t* __restrict__ scnt, int P, const int64_t* __restrict__ C,
+    int64_t* __restrict__ win, int64_t* __restrict__ myclt, int64_t* __restrict__ mycle,
+    uint64_t* __restrict__ split_out, uint64_t* __restrict__ pminmax, int* __restrict__ rounds_out,
+    uint32_t* __restrict__ error_flag) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * kWaves + wave;  // splitter index
+    if (i >= P - 1) return;                   // uniform per wave
+    const int nw = (ks.key_bytes + 7) >> 3;
+    const int P1 = P + 1;
+
+    // Pmin / Pmax over the non-empty segments (mpsort-mpi.c:642-660)
+    Key mn = {{~0ull, ~0ull, ~0ull, ~0ull}}, mx = {{0, 0, 0, 0}};
+    bool any = false;
+    for (int j = lane; j < P; j += 64) {
+        const int64_t n = scnt[j];
+        if (n > 0) {
+            const Key f = load_key(recs, rw, sbeg[j], ks);
+            const Key l = load_key(recs, rw, sbeg[j] + n - 1, ks);
+            if (key_cmp(f, mn, nw) < 0) mn = f;
+            if (key_cmp(l, mx, nw) > 0) mx = l;
+            any = true;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const Key a = shfl_key(mn, off), b = shfl_key(mx, off);
+        if (key_cmp(a, mn, nw) < 0) mn = a;
+        if (key_cmp(b, mx, nw) > 0) mx = b;
+    }
+    if (__ballot(any) == 0) {
+        mn = Key{{0, 0, 0, 0}};
+        mx = mn;
+    }
+    if (i == 0 && lane == 0) {
+        for (int w = 0; w < 4; w++) { pminmax[w] = mn.w[w]; pminmax[4 + w] = mx.w[w]; }
+    }
+
+    // piter_init: every bracket starts at [Pmin, Pmax]
+    Key L = mn, R = mx, Pv = mn;
+    bool narrow = false, stable = false;
+    int rounds = 0;
+    const int64_t Ci = C[i + 1];
+    int64_t* wi = win + (int64_t)i * P * 4;
+    for (int j = lane; j < P; j += 64) {
+        wi[4 * j + 0] = 0;
+        wi[4 * j + 1] = scnt[j];
+    }
+    while (!stable) {
+        rounds++;
+        // piter_bisect (internal-parallel.h:168-198)
+        if (narrow) {
+            Pv = R;
+            stable = true;
+        } else {
+            Pv = key_mid(L, R, nw);
+            if (key_cmp(Pv, L, nw) <= 0) narrow = true;
+        }
+        // counts of < Pv and <= Pv per segment, inside the window
+        long long slt = 0, sle = 0;
+        for (int j = lane; j < P; j += 64) {
+            int64_t* wj = wi + 4 * j;
+            int64_t a = wj[0], b = wj[1];
+            const int64_t base = sbeg[j];
+            while (a < b) {
+                const int64_t m = a + ((b - a) >> 1);
+                if (key_cmp(load_key(recs, rw, base + m, ks), Pv, nw) < 0) a = m + 1; else b = m;
+            }
+            const int64_t lt = a;
+            b = wj[1];
+            while (a < b) {
+                const int64_t m = a + ((b - a) >> 1);
+                if (key_cmp(load_key(recs, rw, base + m, ks), Pv, nw) <= 0) a = m + 1; else b = m;
+            }
+            const int64_t le = a;
+            wj[2] = lt;
+            wj[3] = le;
+            slt += lt;
+            sle += le;
+        }
+        slt = wave_sum_ll(slt);
+        sle = wave_sum_ll(sle);
+        // piter_accept (internal-parallel.h:224-248)
+        if (slt < Ci && Ci <= sle) {
+            stable = true;
+        } else if (!stable) {
+            const bool too_big = slt >= Ci;
+            if (too_big) R = Pv; else L = Pv;
+            for (int j = lane; j < P; j += 64) {
+                int64_t* wj = wi + 4 * j;
+                if (too_big) wj[1] = wj[3]; else wj[0] = wj[2];
+            }
+        }
+        if (rounds > 8 * kMaxKeyBytes + 16) {  // cannot happen for a consistent input
+            if (lane == 0) atomicOr(error_flag, 2u);
+            break;
+        }
+    }
+    for (int j = lane; j < P; j += 64) {
+        myclt[(int64_t)j * P1 + i + 1] = wi[4 * j + 2];
+        mycle[(int64_t)j * P1 + i + 1] = wi[4 * j + 3];
+    }
+    if (lane == 0) {
+        for (int w = 0; w < 4; w++) split_out[(int64_t)i * 4 + w] = Pv.w[w];
+        rounds_out[i] = rounds;
+    }
+}
+
+// one thread per receiving rank r (mpsort-mpi.c:663-727); also fills the
+// fixed columns 0 and P of the count matrices
+__global__ void layout_kernel(int P, const int64_t* __restrict__ C, const int64_t* __restrict__ scnt,
+                              int64_t* __restrict__ myclt, int64_t* __restrict__ mycle,
+                              int64_t* __restrict__ myC, uint32_t* __restrict__ error_flag) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P) return;
+    const int P1 = P + 1;
+    const bool last = (r == P - 1);
+    long long sure = 0;
+    for (int j = 0; j < P; j++) sure += last ? scnt[j] : myclt[(int64_t)j * P1 + r + 1];
+    long long deficit = C[r + 1] - sure;
+    if (deficit < 0) atomicOr(error_flag, 4u);
+    for (int j = 0; j < P; j++) {
+        const int64_t lt = last ? scnt[j] : myclt[(int64_t)j * P1 + r + 1];
+        const int64_t le = last ? scnt[j] : mycle[(int64_t)j * P1 + r + 1];
+        const int64_t supply = le - lt;
+        int64_t take = 0;
+        if (deficit > 0) {
+            if (supply < 0) atomicOr(error_flag, 4u);
+            take = supply <= deficit ? supply : deficit;
+            if (take < 0) take = 0;
+            deficit -= take;
+        }
+        myC[(int64_t)j * P1 + r + 1] = lt + take;
+    }
+    if (r == 0) {
+        for (int j = 0; j < P; j++) {
+            myC[(int64_t)j * P1] = 0;
+            myclt[(int64_t)j * P1] = 0;
+            mycle[(int64_t)j * P1] = 0;
+            myclt[(int64_t)j * P1 + P] = scnt[j];
+            mycle[(int64_t)j * P1 + P] = scnt[j];
+        }
+    }
+}
+
+// one thread per receiving rank: send counts, receive displacements, and the
+// run table (receiver-major, i.e. in output order)
+__global__ void runs_kernel(int P, const int64_t* __restrict__ C, const int64_t* __restrict__ sbeg,
+                            const int64_t* __restrict__ myC, int64_t* __restrict__ send,
+                            int64_t* __restrict__ run_dst, int64_t* __restrict__ run_src,
+                            uint32_t* __restrict__ error_flag) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P) return;
+    const int P1 = P + 1;
+    int64_t displ = 0;
+    for (int j = 0; j < P; j++) {
+        const int64_t a = myC[(int64_t)j * P1 + r], b = myC[(int64_t)j * P1 + r + 1];
+        const int64_t cnt = b - a;
+        if (cnt < 0) atomicOr(error_flag, 8u);
+        send[(int64_t)j * P + r] = cnt;
+        run_dst[(int64_t)r * P + j] = C[r] + displ;
+        run_src[(int64_t)r * P + j] = sbeg[j] + a;
+        displ += cnt > 0 ? cnt : 0;
+    }
+    if (displ != C[r + 1] - C[r]) atomicOr(error_flag, 16u);
+}
+
+constexpr int kExchChunk = 4096;
+constexpr int kExchLdsRuns = 512;
+
+template <int RW>
+__global__ __launch_bounds__(kBlock) void exchange_kernel(const uint32_t* __restrict__ src,
+                                                          uint32_t* __restrict__ dst, int64_t ntot,
+                                                          int nruns, const int64_t* __restrict__ run_dst,
+                                                          const int64_t* __restrict__ run_src,
+                                                          uint32_t* __restrict__ error_flag) {
+    __shared__ int64_t s_rd[kExchLdsRuns], s_rs[kExchLdsRuns];
+    __shared__ int s_a, s_n;
+    const int tid = threadIdx.x;
+    const int64_t o0 = (int64_t)blockIdx.x * kExchChunk;
+    const int64_t o1 = min(ntot, o0 + (int64_t)kExchChunk);
+    if (tid == 0) {
+        const int a = upper_index<int64_t>(run_dst, nruns, o0);
+        const int b = upper_index<int64_t>(run_dst, nruns, o1 - 1);
+        s_a = a;
+        s_n = b - a + 1;
+    }
+    __syncthreads();
+    const int a = s_a, n = s_n;
+    const bool local = n <= kExchLdsRuns;
+    if (local) {
+        for (int q = tid; q < n; q += kBlock) {
+            s_rd[q] = run_dst[a + q];
+            s_rs[q] = run_src[a + q];
+        }
+    }
+    __syncthreads();
+    for (int64_t o = o0 + tid; o < o1; o += kBlock) {
+        int64_t sidx;
+        if (local) {
+            const int q = upper_index<int64_t>(s_rd, n, o);
+            sidx = s_rs[q] + (o - s_rd[q]);
+        } else {
+            const int q = upper_index<int64_t>(run_dst, nruns, o);
+            sidx = run_src[q] + (o - run_dst[q]);
+        }
+        if (sidx < 0 || sidx >= ntot) {  // only reachable with an inconsistent layout
+            atomicOr(error_flag, 32u);
+            continue;
+        }
+        uint32_t r[RW];
+        RecIO<RW>::load(src, sidx, r);
+        RecIO<RW>::store(dst, o, r);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// general record path: pack {key bytes, pad, u32 index} sort records from the
+// caller's records, and gather the caller's records by index at the end
+__global__ __launch_bounds__(kBlock) void pack_kernel(const uint8_t* __restrict__ in, int64_t n,
+                                                      int64_t elsize, int64_t key_off, int key_bytes,
+                                                      uint32_t* __restrict__ out, int rw,
+                                                      long long* __restrict__ checksum) {
+    __shared__ long long s_sum[kWaves];
+    long long csum = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        const uint8_t* rec = in + i * elsize;
+        uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int b = 0; b < key_bytes; b++) w[b >> 2] |= (uint32_t)rec[key_off + b] << ((b & 3) * 8);
+        w[rw - 1] = (uint32_t)i;
+        for (int q = 0; q < rw; q++) out[i * rw + q] = w[q];
+        if (checksum)
+            for (int64_t b = 0; b < elsize; b++) csum += (signed char)rec[b];
+    }
+    if (checksum) {
+        csum = wave_sum_ll(csum);
+        if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = csum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long a = 0;
+            for (int q = 0; q < kWaves; q++) a += s_sum[q];
+            atomicAdd((unsigned long long*)checksum, (unsigned long long)a);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void gather_kernel(const uint8_t* __restrict__ in, int64_t n,
+                                                        int64_t elsize, const uint32_t* __restrict__ srec,
+                                                        int rw, uint8_t* __restrict__ out,
+                                                        long long* __restrict__ checksum,
+                                                        uint32_t* __restrict__ error_flag) {
+    __shared__ long long s_sum[kWaves];
+    long long csum = 0;
+    const bool words = (elsize & 3) == 0 && ((uintptr_t)in & 3) == 0 && ((uintptr_t)out & 3) == 0;
+    for (int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x; o < n;
+         o += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = (int64_t)srec[o * rw + rw - 1];
+        if (i >= n) {  // only reachable with corrupted sort records
+            if (error_flag) atomicOr(error_flag, 64u);
+            continue;
+        }
+        if (words) {
+            const uint32_t* s = reinterpret_cast<const uint32_t*>(in + i * elsize);
+            uint32_t* d = reinterpret_cast<uint32_t*>(out + o * elsize);
+            for (int64_t q = 0; q < (elsize >> 2); q++) {
+                const uint32_t v = s[q];
+                d[q] = v;
+                if (checksum) csum += sbyte_sum(v);
+            }
+        } else {
+            for (int64_t b = 0; b < elsize; b++) {
+                const uint8_t v = in[i * elsize + b];
+                out[o * elsize + b] = v;
+                if (checksum) csum += (signed char)v;
+            }
+        }
+    }
+    if (checksum) {
+        csum = wave_sum_ll(csum);
+        if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = csum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long a = 0;
+            for (int q = 0; q < kWaves; q++) a += s_sum[q];
+            atomicAdd((unsigned long long*)checksum, (unsigned long long)a);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sbeg,
+                  const int64_t* scnt, int P, const int64_t* C, int64_t* win, int64_t* myclt,
+                  int64_t* mycle, uint64_t* split, uint64_t* pminmax, int* rounds, uint32_t* err,
+                  hipStream_t st) {
+    if (P < 2) return 0;
+    const int blocks = (P - 1 + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL(bisect_kernel, dim3(blocks), dim3(kBlock), 0, st, (const uint32_t*)recs, rw,
+                       ks, sbeg, scnt, P, C, win, myclt, mycle, split, pminmax, rounds, err);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_layout(int P, const int64_t* C, const int64_t* scnt, const int64_t* sbeg, int64_t* myclt,
+                  int64_t* mycle, int64_t* myC, int64_t* send, int64_t* run_dst, int64_t* run_src,
+                  uint32_t* err, hipStream_t st) {
+    const int blocks = (P + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(layout_kernel, dim3(blocks), dim3(kBlock), 0, st, P, C, scnt, myclt, mycle,
+                       myC, err);
+    MPSG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(runs_kernel, dim3(blocks), dim3(kBlock), 0, st, P, C, sbeg, myC, send, run_dst,
+                       run_src, err);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_exchange(const void* src, void* dst, int64_t ntot, int rw, int nruns,
+                    const int64_t* run_dst, const int64_t* run_src, uint32_t* err, hipStream_t st) {
+    if (ntot <= 0) return 0;
+    const int64_t blocks = (ntot + kExchChunk - 1) / kExchChunk;
+    dim3 g((unsigned)blocks), b(kBlock);
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    switch (rw) {
+        case 1: hipLaunchKernelGGL(exchange_kernel<1>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
+        case 2: hipLaunchKernelGGL(exchange_kernel<2>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
+        case 4: hipLaunchKernelGGL(exchange_kernel<4>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
+        case 8: hipLaunchKernelGGL(exchange_kernel<8>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_pack(const void* in, int64_t n, int64_t elsize, int64_t key_off, int key_bytes,
+                void* out, int rw, long long* checksum, hipStream_t st) {
+    if (n <= 0) return 0;
+    int64_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, (const uint8_t*)in,
+                       n, elsize, key_off, key_bytes, (uint32_t*)out, rw, checksum);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_gather(const void* in, int64_t n, int64_t elsize, const void* srec, int rw, void* out,
+                  long long* checksum, uint32_t* err, hipStream_t st) {
+    if (n <= 0) return 0;
+    int64_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, (const uint8_t*)in,
+                       n, elsize, (const uint32_t*)srec, rw, (uint8_t*)out, checksum, err);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace mpsg

@@ -1,0 +1,134 @@
+// internal.h -- shared declarations of the MI355X histogram-sort library.
+//
+// Data model.  Every kernel moves "sort records": fixed-size records of
+// RW 32-bit words (RW = 1, 2, 4, 8, i.e. 4/8/16/32 bytes).  When the caller's
+// records already have one of these sizes and a 4-byte-aligned key they are
+// sorted directly; otherwise the library packs {key bytes, u32 source index}
+// sort records, sorts those and gathers the caller's records at the end.
+//
+// The sort key is a little-endian unsigned integer of key_bytes bytes at byte
+// key_off of the record, after XOR-ing each byte with a mask (0x80 on the top
+// byte of every signed word).  That integer orders records exactly as the
+// reference orders their radix (radixsort.c:47-110 compares the radix as one
+// little-endian unsigned integer; binding.pyx:81-121 maps signed words by
+// adding 2^63, which is a top-bit flip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace mpsg {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kBins = 256;         // 8-bit digits
+constexpr int kMaxKeyBytes = 32;
+constexpr int kMaxPasses = kMaxKeyBytes;
+
+// One digit pass: digit = ((rec[widx] >> shift) & 0xff) ^ xmask
+struct PassDesc {
+    int32_t widx;
+    int32_t shift;
+    uint32_t xmask;
+    int32_t key_byte;  // which key byte this pass sorts on
+};
+
+struct KeySpec {
+    int32_t rec_words;   // RW
+    int32_t key_off;     // byte offset of the key inside the sort record
+    int32_t key_bytes;   // bytes of key
+    int32_t reserved;
+    uint8_t xmask[kMaxKeyBytes];  // per key byte XOR mask
+};
+
+inline PassDesc make_pass(const KeySpec& ks, int key_byte) {
+    PassDesc p;
+    int b = ks.key_off + key_byte;
+    p.widx = b / 4;
+    p.shift = (b % 4) * 8;
+    p.xmask = ks.xmask[key_byte];
+    p.key_byte = key_byte;
+    return p;
+}
+
+// Segments of records: segment s is records [begin[s], begin[s] + count[s])
+// of the buffer; tiles of one segment are tile_start[s] .. tile_start[s+1]-1.
+struct SegDev {
+    const int64_t* begin;
+    const int64_t* count;
+    const int64_t* tile_start;  // nseg + 1 entries
+    int32_t nseg;
+    int32_t pad;
+};
+
+template <int RW> struct TileShape;
+template <> struct TileShape<1> { static constexpr int kItems = 32; };
+template <> struct TileShape<2> { static constexpr int kItems = 32; };
+template <> struct TileShape<4> { static constexpr int kItems = 16; };
+template <> struct TileShape<8> { static constexpr int kItems = 8; };
+
+inline int tile_records(int rw) {
+    switch (rw) {
+        case 1: return kBlock * TileShape<1>::kItems;
+        case 2: return kBlock * TileShape<2>::kItems;
+        case 4: return kBlock * TileShape<4>::kItems;
+        default: return kBlock * TileShape<8>::kItems;
+    }
+}
+
+// ---- device memory owned by the library (grow-only cache, per device) ----
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// error reporting
+int set_hip_error(hipError_t e, const char* what, const char* file, int line);
+#define MPSG_CHECK(call)                                                        \
+    do {                                                                        \
+        hipError_t _e = (call);                                                 \
+        if (_e != hipSuccess) return ::mpsg::set_hip_error(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+// ---- launchers (onesweep.hip) ----
+struct SortBuffers {
+    void* src;        // input sort records (read only unless == dst)
+    void* dst;        // output
+    void* tmp;        // ping-pong scratch, same size as the data
+};
+
+struct SegSortPlan {
+    int rw;                  // record words
+    KeySpec ks;
+    int64_t nrec;            // total records over all segments
+    int nseg;
+    int64_t ntiles;
+    // device arrays (caller owned)
+    int64_t* d_begin;
+    int64_t* d_count;
+    int64_t* d_tile_start;
+    uint32_t* d_hist;        // nseg * npass * 256
+    int64_t* d_digit_base;   // nseg * npass * 256
+    uint32_t* d_pass_flags;  // kMaxPasses + 1 (the last word is an error flag)
+    uint64_t* d_lookback;    // ntiles * 256
+    uint32_t* d_tickets;     // kMaxPasses
+    long long* d_checksum;   // [0] in, [1] out
+};
+
+// Runs the stable segmented LSD sort of `plan` over bufs.  Returns the number
+// of digit passes executed in *passes_run (may be 0).  checksum_in/out select
+// which signed-byte sums to accumulate into plan.d_checksum.
+// pass_events, when not null, holds 2*kMaxPasses events recorded around
+// every digit pass launched.
+int segmented_sort(const SegSortPlan& plan, const SortBuffers& bufs, hipStream_t st,
+                   bool checksum_in, bool checksum_out, int* passes_run,
+                   hipEvent_t* pass_events);
+
+int launch_copy(const void* src, void* dst, int64_t nrec, int rw, long long* checksum,
+                hipStream_t st);
+int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+
+// epoch counter for the look-back flags (process wide, per device)
+uint32_t next_epoch(int device);
+
+}  // namespace mpsg

@@ -1,0 +1,151 @@
+"""ctypes view of libmpsort_gpu.so (include/mpsort_gpu.h).
+
+The library is the only compute path: importing this module loads it and
+raises ImportError when it is missing (build it with ``make -C mp-sort_amd``).
+There is no CPU fallback.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.dirname(_PKG)
+LIBPATH = os.environ.get("MPSORT_GPU_LIB", os.path.join(LIBDIR, "libmpsort_gpu.so"))
+GENPATH = os.path.join(LIBDIR, "libmpsort_gen.so")
+
+MPSORT_OK = 0
+MPSORT_ERR_ARG = -1
+MPSORT_ERR_SIZE_MISMATCH = -2
+MPSORT_ERR_LAYOUT = -3
+MPSORT_ERR_CHECKSUM = -4
+MPSORT_ERR_HIP = -5
+MPSORT_ERR_NOMEM = -6
+MPSORT_ERR_UNSUPPORTED = -7
+MPSORT_ERR_NODEVICE = -8
+MPSORT_ERR_TIMEOUT = -9
+
+DEVICE_PTRS = 1 << 0
+NO_CHECKSUM = 1 << 1
+TIMING = 1 << 2
+LEARNED = 1 << 3
+FORCE_GENERAL = 1 << 4
+
+# mpsort.h:17-20
+MPSORT_DISABLE_SPARSE_ALLTOALLV = 1 << 1
+MPSORT_DISABLE_GATHER_SORT = 1 << 3
+MPSORT_REQUIRE_GATHER_SORT = 1 << 4
+MPSORT_REQUIRE_SPARSE_ALLTOALLV = 1 << 6
+
+# every symbol include/mpsort_gpu.h and include/mpsort.h declare
+EXPORTS = [
+    "mpsort_gpu_newarray", "mpsort_gpu_newarray_radix", "mpsort_gpu_last_stats",
+    "mpsort_gpu_report_last_run", "mpsort_gpu_last_counts", "mpsort_gpu_set_allocator",
+    "mpsort_gpu_release", "mpsort_gpu_set_vranks", "mpsort_gpu_get_vranks",
+    "mpsort_gpu_last_error", "mpsort_gpu_version", "mpsort_gpu_dmalloc", "mpsort_gpu_dfree",
+    "mpsort_gpu_memcpy_h2d", "mpsort_gpu_memcpy_d2h", "mpsort_gpu_synchronize",
+    "mpsort_gpu_device_count", "mpsort_mpi_set_options", "mpsort_mpi_has_options",
+    "mpsort_mpi_unset_options", "mpsort_mpi_report_last_run", "radix_sort", "mpsort_omp",
+]
+
+
+class KeyDesc(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_int64), ("width", ctypes.c_int32),
+                ("is_signed", ctypes.c_int32), ("nwords", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("t_first_sort_ms", ctypes.c_double), ("t_findp_ms", ctypes.c_double),
+        ("t_laysolve_ms", ctypes.c_double), ("t_exchange_ms", ctypes.c_double),
+        ("t_second_sort_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double),
+        ("t_h2d_ms", ctypes.c_double), ("t_d2h_ms", ctypes.c_double),
+        ("nvranks", ctypes.c_int32), ("record_bytes", ctypes.c_int32),
+        ("first_passes", ctypes.c_int32), ("second_passes", ctypes.c_int32),
+        ("bisect_rounds", ctypes.c_int32), ("general_path", ctypes.c_int32),
+        ("nrecords", ctypes.c_int64), ("checksum_in", ctypes.c_int64),
+        ("checksum_out", ctypes.c_int64), ("npass_ms", ctypes.c_int32),
+        ("learned", ctypes.c_int32), ("pass_ms", ctypes.c_double * 64),
+        ("splitters_exact", ctypes.c_int32), ("splitters_repaired", ctypes.c_int32),
+        ("buckets", ctypes.c_int32), ("buckets_overflow", ctypes.c_int32),
+        ("repair_rounds", ctypes.c_int32), ("reserved2", ctypes.c_int32),
+        ("t_model_ms", ctypes.c_double), ("t_bucket_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if name == "pass_ms":
+                v = [self.pass_ms[i] for i in range(self.npass_ms)]
+            d[name] = v
+        return d
+
+
+RADIX_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+
+
+def _load():
+    if not os.path.exists(LIBPATH):
+        raise ImportError("libmpsort_gpu.so not found at %s; build it with "
+                          "`make -C mp-sort_amd` (hipcc, gfx950)" % LIBPATH)
+    lib = ctypes.CDLL(LIBPATH)
+    P = ctypes.POINTER
+    i64p = P(ctypes.c_int64)
+    lib.mpsort_gpu_newarray.argtypes = [ctypes.c_void_p, i64p, ctypes.c_int, ctypes.c_void_p, i64p,
+                                        ctypes.c_size_t, P(KeyDesc), ctypes.c_uint]
+    lib.mpsort_gpu_newarray.restype = ctypes.c_int
+    lib.mpsort_gpu_newarray_radix.argtypes = [ctypes.c_void_p, i64p, ctypes.c_int, ctypes.c_void_p,
+                                              i64p, ctypes.c_size_t, RADIX_FN, ctypes.c_size_t,
+                                              ctypes.c_void_p, ctypes.c_uint]
+    lib.mpsort_gpu_newarray_radix.restype = ctypes.c_int
+    lib.mpsort_gpu_last_stats.argtypes = [P(Stats)]
+    lib.mpsort_gpu_last_stats.restype = ctypes.c_int
+    lib.mpsort_gpu_last_counts.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           i64p, i64p, i64p, i64p, i64p, i64p]
+    lib.mpsort_gpu_last_counts.restype = ctypes.c_int
+    lib.mpsort_gpu_last_error.restype = ctypes.c_char_p
+    lib.mpsort_gpu_version.restype = ctypes.c_char_p
+    lib.mpsort_gpu_dmalloc.argtypes = [ctypes.c_size_t]
+    lib.mpsort_gpu_dmalloc.restype = ctypes.c_void_p
+    lib.mpsort_gpu_dfree.argtypes = [ctypes.c_void_p]
+    lib.mpsort_gpu_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.mpsort_gpu_memcpy_d2h.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.mpsort_gpu_set_vranks.argtypes = [ctypes.c_int]
+    lib.mpsort_gpu_get_vranks.restype = ctypes.c_int
+    lib.mpsort_mpi_set_options.argtypes = [ctypes.c_int]
+    lib.mpsort_mpi_has_options.argtypes = [ctypes.c_int]
+    lib.mpsort_mpi_has_options.restype = ctypes.c_int
+    lib.mpsort_mpi_unset_options.argtypes = [ctypes.c_int]
+    return lib
+
+
+lib = _load()
+
+
+class MPSortError(RuntimeError):
+    def __init__(self, code, msg):
+        RuntimeError.__init__(self, "mpsort_gpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    """Turn a C return code into the exception the reference binding raises
+    for the same condition (binding.pyx:157-189), or MPSortError."""
+    if rc == MPSORT_OK:
+        return
+    msg = lib.mpsort_gpu_last_error().decode(errors="replace")
+    if rc == MPSORT_ERR_SIZE_MISMATCH or rc == MPSORT_ERR_ARG:
+        raise ValueError(msg)
+    if rc == MPSORT_ERR_UNSUPPORTED:
+        raise TypeError(msg)
+    raise MPSortError(rc, msg)
+
+
+def last_stats():
+    s = Stats()
+    check(lib.mpsort_gpu_last_stats(ctypes.byref(s)))
+    return s
+
+
+def device_count():
+    return int(lib.mpsort_gpu_device_count())
