@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--vranks", type=int, default=64)
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one plain copy kernel of n*16 bytes (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the P virtual ranks of the histogram sort live "
@@ -114,7 +116,13 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     value = n / (ms * 1e-3) / 1e6
 
+    if args.copy_probe:
+        _lib.check(lib.mpsort_gpu_dcopy(dout, din, nb))
+        log("copy probe: copy_kernel<4> moved %d bytes in and %d bytes out" % (nb, nb))
+
     # correctness of the timed output (size-independent properties)
+    if args.copy_probe:
+        step()
     out = numpy.empty_like(recs)
     _lib.check(lib.mpsort_gpu_memcpy_d2h(out.ctypes.data, dout, nb))
     k = out['key']

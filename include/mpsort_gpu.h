@@ -168,6 +168,9 @@ int mpsort_gpu_dfree(void *p);
 int mpsort_gpu_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int mpsort_gpu_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int mpsort_gpu_synchronize(void);
+/* device-to-device copy by the library's own copy kernel (a plain streaming
+ * kernel of known traffic, used to calibrate hardware byte counters) */
+int mpsort_gpu_dcopy(void *dst, const void *src, size_t bytes);
 int mpsort_gpu_device_count(void);
 
 /* Option bits of the reference (mpsort.h:17-24).  Accepted and recorded: on

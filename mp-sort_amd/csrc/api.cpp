@@ -345,6 +345,9 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     MPSG_CHECK(hipMemcpyAsync(h.C, hC, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
     if ((rc = launch_fill_u32(h.err, 0u, 4, st))) return rc;
     if ((rc = launch_fill_u32((uint32_t*)h.checksum, 0u, 4, st))) return rc;
+    // both plans' flag words (the second plan is unused when P == 1)
+    if ((rc = launch_fill_u32(pa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
+    if ((rc = launch_fill_u32(pb.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
 
     auto mkplan = [&](const PlanArrays& a, int64_t nt) {
         SegSortPlan p;
@@ -799,6 +802,20 @@ int mpsort_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 }
 int mpsort_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     MPSG_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+int mpsort_gpu_dcopy(void* dst, const void* src, size_t bytes) {
+    DevCtx* c = nullptr;
+    int rc = get_ctx(&c);
+    if (rc) return rc;
+    if (bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0)
+        rc = launch_copy(src, dst, (int64_t)(bytes / 16), 4, nullptr, c->st);
+    else if (bytes % 4 == 0 && ((uintptr_t)dst % 4) == 0 && ((uintptr_t)src % 4) == 0)
+        rc = launch_copy(src, dst, (int64_t)(bytes / 4), 1, nullptr, c->st);
+    else
+        return fail(MPSORT_ERR_ARG, "dcopy needs 4-byte aligned pointers and sizes");
+    if (rc) return rc;
+    MPSG_CHECK(hipStreamSynchronize(c->st));
     return 0;
 }
 int mpsort_gpu_synchronize(void) {

@@ -41,7 +41,7 @@ EXPORTS = [
     "mpsort_gpu_report_last_run", "mpsort_gpu_last_counts", "mpsort_gpu_set_allocator",
     "mpsort_gpu_release", "mpsort_gpu_set_vranks", "mpsort_gpu_get_vranks",
     "mpsort_gpu_last_error", "mpsort_gpu_version", "mpsort_gpu_dmalloc", "mpsort_gpu_dfree",
-    "mpsort_gpu_memcpy_h2d", "mpsort_gpu_memcpy_d2h", "mpsort_gpu_synchronize",
+    "mpsort_gpu_memcpy_h2d", "mpsort_gpu_memcpy_d2h", "mpsort_gpu_synchronize", "mpsort_gpu_dcopy",
     "mpsort_gpu_device_count", "mpsort_mpi_set_options", "mpsort_mpi_has_options",
     "mpsort_mpi_unset_options", "mpsort_mpi_report_last_run", "radix_sort", "mpsort_omp",
 ]
@@ -110,6 +110,7 @@ def _load():
     lib.mpsort_gpu_dfree.argtypes = [ctypes.c_void_p]
     lib.mpsort_gpu_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.mpsort_gpu_memcpy_d2h.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.mpsort_gpu_dcopy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.mpsort_gpu_set_vranks.argtypes = [ctypes.c_int]
     lib.mpsort_gpu_get_vranks.restype = ctypes.c_int
     lib.mpsort_mpi_set_options.argtypes = [ctypes.c_int]
