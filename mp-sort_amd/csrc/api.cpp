@@ -23,6 +23,9 @@ int launch_layout(int P, const int64_t* C, const int64_t* scnt, const int64_t* s
                   uint32_t* err, hipStream_t st);
 int launch_exchange(const void* src, void* dst, int64_t ntot, int rw, int nruns,
                     const int64_t* run_dst, const int64_t* run_src, uint32_t* err, hipStream_t st);
+int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, int nruns,
+                         const int64_t* run_dst, const int64_t* run_src, uint32_t* err,
+                         const int64_t* C, int P, const KeySpec& ks, uint32_t* hist, hipStream_t st);
 int launch_pack(const void* in, int64_t n, int64_t elsize, int64_t key_off, int key_bytes,
                 void* out, int rw, long long* checksum, hipStream_t st);
 int launch_gather(const void* in, int64_t n, int64_t elsize, const void* srec, int rw, void* out,
@@ -257,8 +260,7 @@ static void fill_keyspec(KeySpec* ks, int rw, int key_off, int key_bytes, int wi
 }
 
 // host description of one segmented sort: segment begins/counts -> tiles
-static int64_t make_tiles(const int64_t* counts, int nseg, int rw, int64_t* begin, int64_t* tile_start) {
-    const int64_t T = tile_records(rw);
+static int64_t make_tiles(const int64_t* counts, int nseg, int64_t T, int64_t* begin, int64_t* tile_start) {
     int64_t b = 0, t = 0;
     for (int s = 0; s < nseg; s++) {
         begin[s] = b;
@@ -329,8 +331,10 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     int64_t *hbB = ht + 3 * P + 1, *hcB = ht + 4 * P + 1, *htB = ht + 5 * P + 1;
     int64_t* hC = ht + 6 * P + 2;
     for (int t = 0; t < P; t++) { hcA[t] = nin[t]; hcB[t] = nout[t]; }
-    const int64_t ntA = make_tiles(hcA, P, rw, hbA, htA);
-    const int64_t ntB = make_tiles(hcB, P, rw, hbB, htB);
+    const TileCfg tile = default_tile(rw);
+    const int64_t trecs = (int64_t)tile.block * tile.items;
+    const int64_t ntA = make_tiles(hcA, P, trecs, hbA, htA);
+    const int64_t ntB = make_tiles(hcB, P, trecs, hbB, htB);
     hC[0] = 0;
     for (int t = 0; t < P; t++) hC[t + 1] = hC[t] + nout[t];
     const int64_t ntiles = ntA > ntB ? ntA : ntB;
@@ -352,6 +356,7 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     auto mkplan = [&](const PlanArrays& a, int64_t nt) {
         SegSortPlan p;
         p.rw = rw;
+        p.tile = tile;
         p.ks = cs.ks;
         p.nrec = n;
         p.nseg = P;
@@ -411,10 +416,15 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
                                 h.run_src, h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
-        if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, P * P, h.run_dst, h.run_src, h.err, st))) return rc;
+        // the exchange also builds the SecondSort's digit histograms
+        if ((rc = seg_clear(planB, st))) return rc;
+        if ((rc = launch_exchange_hist(c.s0.p, c.s1.p, n, rw, P * P, h.run_dst, h.run_src, h.err, h.C, P,
+                                       cs.ks, planB.d_hist, st)))
+            return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
         SortBuffers b2{c.s1.p, final_dst, c.s0.p};
-        if ((rc = segmented_sort(planB, b2, st, false, cs_direct, &second_passes, pevB))) return rc;
+        if ((rc = seg_passes(planB, b2, st, cs_direct ? h.checksum + 1 : nullptr, &second_passes, pevB)))
+            return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvSecond], st));
     }
     if (cs.general) {

@@ -11,6 +11,8 @@ constexpr uint64_t kFlagValueMask = (1ull << kFlagValueBits) - 1;
 constexpr uint32_t kEpochLimit = 1u << 23;
 // a look-back wait gives up after this many polls and raises an error flag
 constexpr uint32_t kSpinLimit = 1u << 24;
+// predecessor flags read per look-back step
+constexpr int kLookback = 4;
 
 __device__ __forceinline__ uint64_t make_flag(uint32_t epoch, uint32_t inclusive, uint64_t v) {
     return ((uint64_t)epoch << 41) | ((uint64_t)inclusive << 40) | (v & kFlagValueMask);

@@ -307,6 +307,77 @@ __global__ __launch_bounds__(kBlock) void exchange_kernel(const uint32_t* __rest
     }
 }
 
+// Exchange fused with the SecondSort's upfront histogram: each block also
+// counts the digits of the records it moves for every digit place of its
+// receiving rank (records are already in their receiver's segment after the
+// move).  hist is [P][npass][256], cleared beforehand.
+template <int RW>
+__global__ __launch_bounds__(kBlock) void exchange_hist_kernel(
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t ntot, int nruns,
+    const int64_t* __restrict__ run_dst, const int64_t* __restrict__ run_src,
+    uint32_t* __restrict__ error_flag, const int64_t* __restrict__ C, int P, int npass,
+    PassTable pt, uint32_t* __restrict__ hist) {
+    __shared__ int64_t s_rd[kExchLdsRuns], s_rs[kExchLdsRuns];
+    __shared__ uint32_t s_hist[kMaxPasses * kBins];
+    __shared__ int s_a, s_n, s_r0, s_r1;
+    const int tid = threadIdx.x;
+    const int nb = npass * kBins;
+    const int64_t o0 = (int64_t)blockIdx.x * kExchChunk;
+    const int64_t o1 = min(ntot, o0 + (int64_t)kExchChunk);
+    if (tid == 0) {
+        const int a = upper_index<int64_t>(run_dst, nruns, o0);
+        const int b = upper_index<int64_t>(run_dst, nruns, o1 - 1);
+        s_a = a;
+        s_n = b - a + 1;
+        s_r0 = upper_index<int64_t>(C, P, o0);
+        s_r1 = upper_index<int64_t>(C, P, o1 - 1);
+    }
+    for (int i = tid; i < nb; i += kBlock) s_hist[i] = 0;
+    __syncthreads();
+    const int a = s_a, n = s_n, r0 = s_r0;
+    const bool one_receiver = (r0 == s_r1);
+    const bool local = n <= kExchLdsRuns;
+    if (local) {
+        for (int q = tid; q < n; q += kBlock) {
+            s_rd[q] = run_dst[a + q];
+            s_rs[q] = run_src[a + q];
+        }
+    }
+    __syncthreads();
+    for (int64_t o = o0 + tid; o < o1; o += kBlock) {
+        int64_t sidx;
+        if (local) {
+            const int q = upper_index<int64_t>(s_rd, n, o);
+            sidx = s_rs[q] + (o - s_rd[q]);
+        } else {
+            const int q = upper_index<int64_t>(run_dst, nruns, o);
+            sidx = run_src[q] + (o - run_dst[q]);
+        }
+        if (sidx < 0 || sidx >= ntot) {
+            atomicOr(error_flag, 32u);
+            continue;
+        }
+        uint32_t r[RW];
+        RecIO<RW>::load(src, sidx, r);
+        RecIO<RW>::store(dst, o, r);
+        if (one_receiver) {
+            for (int p = 0; p < npass; p++) atomicAdd(&s_hist[p * kBins + digit_of<RW>(r, pt.p[p])], 1u);
+        } else {
+            const int rr = upper_index<int64_t>(C, P, o);
+            uint32_t* h = hist + (int64_t)rr * nb;
+            for (int p = 0; p < npass; p++) atomicAdd(&h[p * kBins + digit_of<RW>(r, pt.p[p])], 1u);
+        }
+    }
+    if (one_receiver) {
+        __syncthreads();
+        uint32_t* h = hist + (int64_t)r0 * nb;
+        for (int i = tid; i < nb; i += kBlock) {
+            const uint32_t c = s_hist[i];
+            if (c) atomicAdd(&h[i], c);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // general record path: pack {key bytes, pad, u32 index} sort records from the
 // caller's records, and gather the caller's records by index at the end
@@ -420,6 +491,27 @@ int launch_exchange(const void* src, void* dst, int64_t ntot, int rw, int nruns,
         case 2: hipLaunchKernelGGL(exchange_kernel<2>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
         case 4: hipLaunchKernelGGL(exchange_kernel<4>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
         case 8: hipLaunchKernelGGL(exchange_kernel<8>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, int nruns,
+                         const int64_t* run_dst, const int64_t* run_src, uint32_t* err,
+                         const int64_t* C, int P, const KeySpec& ks, uint32_t* hist, hipStream_t st) {
+    if (ntot <= 0) return 0;
+    const int64_t blocks = (ntot + kExchChunk - 1) / kExchChunk;
+    dim3 g((unsigned)blocks), b(kBlock);
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    const PassTable pt = make_pass_table(ks);
+    const int np = ks.key_bytes;
+    switch (rw) {
+        case 1: hipLaunchKernelGGL(exchange_hist_kernel<1>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err, C, P, np, pt, hist); break;
+        case 2: hipLaunchKernelGGL(exchange_hist_kernel<2>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err, C, P, np, pt, hist); break;
+        case 4: hipLaunchKernelGGL(exchange_hist_kernel<4>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err, C, P, np, pt, hist); break;
+        case 8: hipLaunchKernelGGL(exchange_hist_kernel<8>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err, C, P, np, pt, hist); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());

@@ -33,6 +33,10 @@ struct PassDesc {
     int32_t key_byte;  // which key byte this pass sorts on
 };
 
+struct PassTable {
+    PassDesc p[kMaxPasses];
+};
+
 struct KeySpec {
     int32_t rec_words;   // RW
     int32_t key_off;     // byte offset of the key inside the sort record
@@ -61,20 +65,14 @@ struct SegDev {
     int32_t pad;
 };
 
-template <int RW> struct TileShape;
-template <> struct TileShape<1> { static constexpr int kItems = 32; };
-template <> struct TileShape<2> { static constexpr int kItems = 32; };
-template <> struct TileShape<4> { static constexpr int kItems = 16; };
-template <> struct TileShape<8> { static constexpr int kItems = 8; };
-
-inline int tile_records(int rw) {
-    switch (rw) {
-        case 1: return kBlock * TileShape<1>::kItems;
-        case 2: return kBlock * TileShape<2>::kItems;
-        case 4: return kBlock * TileShape<4>::kItems;
-        default: return kBlock * TileShape<8>::kItems;
-    }
-}
+// Tile shape of the onesweep passes: BLOCK threads x ITEMS records each.
+struct TileCfg {
+    int block;
+    int items;
+};
+bool tile_supported(int rw, TileCfg c);
+// default shape for a record width; MPSORT_GPU_TILE="block:items" overrides
+TileCfg default_tile(int rw);
 
 // ---- device memory owned by the library (grow-only cache, per device) ----
 struct Workspace {
@@ -99,6 +97,7 @@ struct SortBuffers {
 
 struct SegSortPlan {
     int rw;                  // record words
+    TileCfg tile;
     KeySpec ks;
     int64_t nrec;            // total records over all segments
     int nseg;
@@ -114,6 +113,22 @@ struct SegSortPlan {
     uint32_t* d_tickets;     // kMaxPasses
     long long* d_checksum;   // [0] in, [1] out
 };
+
+inline PassTable make_pass_table(const KeySpec& ks) {
+    PassTable pt;
+    for (int b = 0; b < ks.key_bytes && b < kMaxPasses; b++) pt.p[b] = make_pass(ks, b);
+    return pt;
+}
+
+// The two halves of segmented_sort: seg_count clears the plan's counters and
+// histograms every digit place of src; seg_passes scans the histograms and
+// runs the non-trivial digit passes.  A caller that builds the histograms
+// itself (the fused exchange) clears them with seg_clear and calls
+// seg_passes only.
+int seg_clear(const SegSortPlan& plan, hipStream_t st);
+int seg_count(const SegSortPlan& plan, const void* src, long long* checksum_in, hipStream_t st);
+int seg_passes(const SegSortPlan& plan, const SortBuffers& bufs, hipStream_t st,
+               long long* checksum_out, int* passes_run, hipEvent_t* pass_events);
 
 // Runs the stable segmented LSD sort of `plan` over bufs.  Returns the number
 // of digit passes executed in *passes_run (may be 0).  checksum_in/out select

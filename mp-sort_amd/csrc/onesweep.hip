@@ -8,7 +8,7 @@
 // marks digit places that hold a single value in every segment (such a pass
 // would be the identity permutation and is skipped), then one onesweep pass
 // per remaining digit place moves the records:
-//   * tiles of 256 threads x ITEMS records, tile ids from an atomic ticket so
+//   * tiles of BLOCK threads x ITEMS records, tile ids from an atomic ticket so
 //     every predecessor tile is already resident (forward progress for the
 //     look-back);
 //   * wave-striped loads (item k of lane l is record k*64+l of the wave's
@@ -16,43 +16,60 @@
 //   * per-item digit ranking by 8 wave ballots (match-any), per-wave digit
 //     counters in LDS;
 //   * decoupled look-back over the tiles of the same segment, one thread per
-//     digit, 64-bit flag words tagged with a per-launch epoch;
+//     digit, 64-bit flag words tagged with a per-launch epoch, kLookback
+//     predecessor flags read per step;
 //   * records are staged in LDS in digit order and written out in runs that
 //     are contiguous per digit (coalesced).
+// The tile shape (BLOCK, ITEMS) is chosen per call (TileCfg).
+#include <stdlib.h>
+#include <stdio.h>
+
 #include "device.h"
 
 namespace mpsg {
 
-struct PassTable {
-    PassDesc p[kMaxPasses];
-};
-
 // ---------------------------------------------------------------------------
-// upfront histogram of all digit places
+// upfront histogram of all digit places.  Block b counts tiles
+// [b*tpb, (b+1)*tpb) in LDS and flushes to the segment's global histogram
+// whenever the segment changes, so global atomics scale with blocks, not
+// records.
 template <int RW>
 __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict__ src, SegDev seg,
+                                                      int64_t ntiles, int64_t tpb, int tile_recs,
                                                       int npass, PassTable pt,
                                                       uint32_t* __restrict__ hist,
                                                       long long* __restrict__ checksum) {
-    constexpr int ITEMS = TileShape<RW>::kItems;
-    constexpr int TILE = kBlock * ITEMS;
     __shared__ uint32_t s_hist[kMaxPasses * kBins];
     __shared__ long long s_sum[kWaves];
     const int tid = threadIdx.x;
-    for (int i = tid; i < npass * kBins; i += kBlock) s_hist[i] = 0;
+    const int nb = npass * kBins;
+    for (int i = tid; i < nb; i += kBlock) s_hist[i] = 0;
     __syncthreads();
 
-    const int64_t tile = blockIdx.x;
-    const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
-    const int64_t t = tile - seg.tile_start[s];
-    const int64_t sbeg = seg.begin[s];
-    const int64_t begin = sbeg + t * TILE;
-    const int64_t end = min(sbeg + seg.count[s], begin + (int64_t)TILE);
-
+    const int64_t t0 = (int64_t)blockIdx.x * tpb;
+    const int64_t t1 = min(ntiles, t0 + tpb);
     long long csum = 0;
-    for (int k = 0; k < ITEMS; k++) {
-        const int64_t i = begin + (int64_t)k * kBlock + tid;
-        if (i < end) {
+    int cur = -1;
+    for (int64_t tile = t0; tile < t1; tile++) {
+        const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
+        if (s != cur) {
+            if (cur >= 0) {
+                __syncthreads();
+                uint32_t* h = hist + (int64_t)cur * nb;
+                for (int i = tid; i < nb; i += kBlock) {
+                    const uint32_t c = s_hist[i];
+                    if (c) atomicAdd(&h[i], c);
+                    s_hist[i] = 0;
+                }
+                __syncthreads();
+            }
+            cur = s;
+        }
+        const int64_t t = tile - seg.tile_start[s];
+        const int64_t sbeg = seg.begin[s];
+        const int64_t begin = sbeg + t * tile_recs;
+        const int64_t end = min(sbeg + seg.count[s], begin + (int64_t)tile_recs);
+        for (int64_t i = begin + tid; i < end; i += kBlock) {
             uint32_t r[RW];
             RecIO<RW>::load(src, i, r);
             if (checksum) {
@@ -66,10 +83,12 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict
         }
     }
     __syncthreads();
-    uint32_t* h = hist + (int64_t)s * npass * kBins;
-    for (int i = tid; i < npass * kBins; i += kBlock) {
-        const uint32_t c = s_hist[i];
-        if (c) atomicAdd(&h[i], c);
+    if (cur >= 0) {
+        uint32_t* h = hist + (int64_t)cur * nb;
+        for (int i = tid; i < nb; i += kBlock) {
+            const uint32_t c = s_hist[i];
+            if (c) atomicAdd(&h[i], c);
+        }
     }
     if (checksum) {
         csum = wave_sum_ll(csum);
@@ -101,36 +120,57 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const uint32_t* __restrict
     if (tid == 0 && nz > 1) atomicOr(&pass_flags[p], 1u);
 }
 
+// exclusive scan of one value per digit held by threads 0..255 of a block of
+// BLOCK >= 256 threads; every thread of the block must call it
+template <int BLOCK>
+__device__ __forceinline__ uint32_t digit_exclusive_scan(uint32_t v, uint32_t* s_tmp) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63 && wave < 4) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) pre += (w < wave) ? s_tmp[w] : 0u;
+    __syncthreads();
+    return pre + x - v;
+}
+
 // ---------------------------------------------------------------------------
 // one onesweep digit pass
-template <int RW>
-__global__ __launch_bounds__(kBlock) void onesweep_kernel(
+template <int RW, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg,
     const int64_t* __restrict__ digit_base, int npass, int pass_idx, PassDesc pd,
     uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket, uint32_t epoch,
     long long* __restrict__ checksum_out, uint32_t* __restrict__ error_flag) {
-    constexpr int ITEMS = TileShape<RW>::kItems;
-    constexpr int TILE = kBlock * ITEMS;
+    constexpr int NW = BLOCK / 64;
+    constexpr int TILE = BLOCK * ITEMS;
     constexpr int WAVE_TILE = 64 * ITEMS;
     __shared__ __attribute__((aligned(16))) uint32_t s_rec[TILE * RW];
-    __shared__ uint32_t s_whist[kWaves][kBins];
+    __shared__ uint16_t s_whist[NW][kBins];  // counts and offsets < TILE <= 8192
     __shared__ uint32_t s_excl[kBins];
     __shared__ int64_t s_gdelta[kBins];
     __shared__ int64_t s_tile;
-    __shared__ uint32_t s_tmp[kWaves];
-    __shared__ long long s_sum[kWaves];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ long long s_sum[NW];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid == 0) s_tile = (int64_t)atomicAdd(ticket, 1u);
-    for (int i = tid; i < kWaves * kBins; i += kBlock) (&s_whist[0][0])[i] = 0;
+    for (int i = tid; i < NW * kBins; i += BLOCK) (&s_whist[0][0])[i] = 0;
     __syncthreads();
 
     const int64_t tile = s_tile;
     const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
     const int64_t t = tile - seg.tile_start[s];
     const int64_t sbeg = seg.begin[s];
+    const int64_t send = sbeg + seg.count[s];
     const int64_t begin = sbeg + t * TILE;
-    const int64_t end = min(sbeg + seg.count[s], begin + (int64_t)TILE);
+    const int64_t end = min(send, begin + (int64_t)TILE);
     const int count = (int)(end - begin);
 
     uint32_t r[ITEMS][RW];
@@ -150,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     }
 
     // rank inside the wave: items in (k, lane) order are in input order
-    volatile uint32_t* wh = s_whist[wave];
+    volatile uint16_t* wh = s_whist[wave];
     const uint64_t lt_mask = __lanemask_lt();
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
@@ -168,50 +208,73 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
         uint32_t base = 0;
         if (valid) base = wh[d];
         loc[k] = base + rank;
-        if (valid && rank == 0) wh[d] = base + cnt;
+        if (valid && rank == 0) wh[d] = (uint16_t)(base + cnt);
     }
     __syncthreads();
 
-    // thread d owns digit d: wave offsets, tile-local digit start, look-back
+    // threads 0..255 own one digit each: wave offsets, tile-local digit start
     const int d = tid;
     uint32_t total = 0;
-    {
-        uint32_t c[kWaves];
-#pragma unroll
-        for (int w = 0; w < kWaves; w++) c[w] = s_whist[w][d];
+    if (d < kBins) {
         uint32_t acc = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; w++) { s_whist[w][d] = acc; acc += c[w]; }
+        for (int w = 0; w < NW; w++) {
+            const uint32_t c = s_whist[w][d];
+            s_whist[w][d] = (uint16_t)acc;
+            acc += c;
+        }
         total = acc;
     }
-    const uint32_t excl = block_exclusive_scan(total, s_tmp, nullptr);
-    s_excl[d] = excl;
-
-    int64_t prefix = 0;
-    uint64_t* my = lookback + tile * kBins + d;
-    if (t == 0) {
-        store_agent(my, make_flag(epoch, 1, total));
-    } else {
-        store_agent(my, make_flag(epoch, 0, total));
-        int64_t j = tile - 1;
-        uint32_t spins = 0;
-        while (true) {
-            const uint64_t f = load_agent(lookback + j * kBins + d);
-            if ((uint32_t)(f >> 41) != epoch) {
-                if (++spins > kSpinLimit) {
-                    atomicOr(error_flag, 1u);
-                    break;
+    const uint32_t excl = digit_exclusive_scan<BLOCK>(total, s_tmp);
+    if (d < kBins) {
+        s_excl[d] = excl;
+        int64_t prefix = 0;
+        uint64_t* my = lookback + tile * kBins + d;
+        if (t == 0) {
+            store_agent(my, make_flag(epoch, 1, total));
+        } else {
+            store_agent(my, make_flag(epoch, 0, total));
+            // windowed look-back: read kLookback predecessor flags at once,
+            // sum aggregates back to the first inclusive prefix; a flag that
+            // is not ready yet is polled again.  The first tile of a segment
+            // always publishes an inclusive prefix, so the walk never leaves
+            // the segment.
+            const int64_t first = seg.tile_start[s];
+            int64_t j = tile - 1;
+            uint32_t spins = 0;
+            while (true) {
+                uint64_t f[kLookback];
+#pragma unroll
+                for (int q = 0; q < kLookback; q++)
+                    f[q] = (j - q >= first) ? load_agent(lookback + (j - q) * kBins + d) : 0ull;
+                int consumed = 0;
+                bool stop = false, done = false;
+#pragma unroll
+                for (int q = 0; q < kLookback; q++) {
+                    if (!stop && j - q >= first) {
+                        if ((uint32_t)(f[q] >> 41) != epoch) {
+                            stop = true;
+                        } else {
+                            prefix += (int64_t)(f[q] & kFlagValueMask);
+                            consumed = q + 1;
+                            if ((f[q] >> 40) & 1ull) { done = true; stop = true; }
+                        }
+                    }
                 }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+                if (done) break;
+                j -= consumed;
+                if (consumed == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(error_flag, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
             }
-            prefix += (int64_t)(f & kFlagValueMask);
-            if ((f >> 40) & 1ull) break;
-            --j;
+            store_agent(my, make_flag(epoch, 1, (uint64_t)(prefix + total)));
         }
-        store_agent(my, make_flag(epoch, 1, (uint64_t)(prefix + total)));
+        s_gdelta[d] = digit_base[((int64_t)s * npass + pass_idx) * kBins + d] + prefix - (int64_t)excl;
     }
-    s_gdelta[d] = digit_base[((int64_t)s * npass + pass_idx) * kBins + d] + prefix - (int64_t)excl;
     __syncthreads();
 
     // stage records in LDS in (digit, input) order
@@ -228,20 +291,24 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
 
     // write out: consecutive LDS slots of one digit go to consecutive slots
     long long csum = 0;
-    for (int i = tid; i < count; i += kBlock) {
-        uint32_t rr[RW];
 #pragma unroll
-        for (int w = 0; w < RW; w++) rr[w] = s_rec[i * RW + w];
-        const uint32_t dd = digit_of<RW>(rr, pd);
-        const int64_t o = s_gdelta[dd] + i;
-        if (o < sbeg || o >= sbeg + seg.count[s]) {  // only reachable with inconsistent counts
-            atomicOr(error_flag, 4u);
-            continue;
-        }
-        RecIO<RW>::store(dst, o, rr);
-        if (checksum_out) {
+    for (int k = 0; k < ITEMS; k++) {
+        const int i = k * BLOCK + tid;
+        if (i < count) {
+            uint32_t rr[RW];
 #pragma unroll
-            for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
+            for (int w = 0; w < RW; w++) rr[w] = s_rec[i * RW + w];
+            const uint32_t dd = digit_of<RW>(rr, pd);
+            const int64_t o = s_gdelta[dd] + i;
+            if (o < sbeg || o >= send) {  // only reachable with inconsistent counts
+                atomicOr(error_flag, 4u);
+                continue;
+            }
+            RecIO<RW>::store(dst, o, rr);
+            if (checksum_out) {
+#pragma unroll
+                for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
+            }
         }
     }
     if (checksum_out) {
@@ -250,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
         __syncthreads();
         if (tid == 0) {
             long long a = 0;
-            for (int w = 0; w < kWaves; w++) a += s_sum[w];
+            for (int w = 0; w < NW; w++) a += s_sum[w];
             atomicAdd((unsigned long long*)checksum_out, (unsigned long long)a);
         }
     }
@@ -294,6 +361,31 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
 // ---------------------------------------------------------------------------
 // host side
 
+// Tile shapes compiled for every record width: 256 and 512 threads with
+// hi or hi/2 records per thread, 1024 threads with hi/2 or hi/4, where hi
+// fills 64 KB of LDS at 256 threads.
+static int items_hi(int rw) { return rw == 1 ? 32 : (rw == 2 ? 32 : (rw == 4 ? 16 : 8)); }
+
+bool tile_supported(int rw, TileCfg c) {
+    const int hi = items_hi(rw);
+    if (c.block == 256 || c.block == 512) return c.items == hi || c.items == hi / 2;
+    if (c.block == 1024) return c.items == hi / 2 || c.items == hi / 4;
+    return false;
+}
+
+// 1024 threads x hi/2 records (8192-record tiles for 16-byte records) was the
+// fastest shape measured on MI355X for 16-byte records (one workgroup per CU,
+// 16 waves); MPSORT_GPU_TILE="block:items" overrides it.
+TileCfg default_tile(int rw) {
+    TileCfg c{1024, items_hi(rw) / 2};
+    const char* e = getenv("MPSORT_GPU_TILE");
+    if (e) {
+        int b = 0, it = 0;
+        if (sscanf(e, "%d:%d", &b, &it) == 2 && tile_supported(rw, TileCfg{b, it})) c = TileCfg{b, it};
+    }
+    return c;
+}
+
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
     if (n <= 0) return 0;
     int64_t blocks = (n + kBlock - 1) / kBlock;
@@ -326,49 +418,77 @@ template <int RW>
 static void launch_hist(const SegSortPlan& pl, const void* src, int npass, const PassTable& pt,
                         long long* cs, hipStream_t st) {
     SegDev sd{pl.d_begin, pl.d_count, pl.d_tile_start, pl.nseg, 0};
-    hipLaunchKernelGGL(hist_kernel<RW>, dim3((unsigned)pl.ntiles), dim3(kBlock), 0, st,
-                       (const uint32_t*)src, sd, npass, pt, pl.d_hist, cs);
+    // about 2048 blocks (8 per CU), each over a contiguous run of tiles
+    const int64_t tpb = (pl.ntiles + 2047) / 2048;
+    const int64_t blocks = (pl.ntiles + tpb - 1) / tpb;
+    hipLaunchKernelGGL(hist_kernel<RW>, dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                       (const uint32_t*)src, sd, pl.ntiles, tpb, pl.tile.block * pl.tile.items, npass,
+                       pt, pl.d_hist, cs);
 }
 
-template <int RW>
-static void launch_pass(const SegSortPlan& pl, const void* src, void* dst, int npass, int pidx,
-                        const PassDesc& pd, uint32_t* ticket, uint32_t epoch, long long* cs,
-                        uint32_t* err, hipStream_t st) {
+template <int RW, int BLOCK, int ITEMS>
+static void launch_pass_t(const SegSortPlan& pl, const void* src, void* dst, int npass, int pidx,
+                          const PassDesc& pd, uint32_t* ticket, uint32_t epoch, long long* cs,
+                          uint32_t* err, hipStream_t st) {
     SegDev sd{pl.d_begin, pl.d_count, pl.d_tile_start, pl.nseg, 0};
-    hipLaunchKernelGGL(onesweep_kernel<RW>, dim3((unsigned)pl.ntiles), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((onesweep_kernel<RW, BLOCK, ITEMS>), dim3((unsigned)pl.ntiles), dim3(BLOCK), 0, st,
                        (const uint32_t*)src, (uint32_t*)dst, sd, pl.d_digit_base, npass, pidx, pd,
                        pl.d_lookback, ticket, epoch, cs, err);
 }
 
-int segmented_sort(const SegSortPlan& pl, const SortBuffers& bufs, hipStream_t st,
-                   bool checksum_in, bool checksum_out, int* passes_run, hipEvent_t* pass_events) {
-    *passes_run = 0;
+template <int RW>
+static int launch_pass(const SegSortPlan& pl, const void* src, void* dst, int npass, int pidx,
+                       const PassDesc& pd, uint32_t* ticket, uint32_t epoch, long long* cs,
+                       uint32_t* err, hipStream_t st) {
+    constexpr int HI = RW == 1 ? 32 : (RW == 2 ? 32 : (RW == 4 ? 16 : 8));
+    constexpr int LO = HI / 2;
+    const TileCfg c = pl.tile;
+    if (c.block == 256 && c.items == HI) launch_pass_t<RW, 256, HI>(pl, src, dst, npass, pidx, pd, ticket, epoch, cs, err, st);
+    else if (c.block == 256 && c.items == LO) launch_pass_t<RW, 256, LO>(pl, src, dst, npass, pidx, pd, ticket, epoch, cs, err, st);
+    else if (c.block == 512 && c.items == LO) launch_pass_t<RW, 512, LO>(pl, src, dst, npass, pidx, pd, ticket, epoch, cs, err, st);
+    else if (c.block == 512 && c.items == HI) launch_pass_t<RW, 512, HI>(pl, src, dst, npass, pidx, pd, ticket, epoch, cs, err, st);
+    else if (c.block == 1024 && c.items == LO) launch_pass_t<RW, 1024, LO>(pl, src, dst, npass, pidx, pd, ticket, epoch, cs, err, st);
+    else if (c.block == 1024 && c.items == HI / 4) launch_pass_t<RW, 1024, HI / 4>(pl, src, dst, npass, pidx, pd, ticket, epoch, cs, err, st);
+    else return -1;
+    return 0;
+}
+
+int seg_clear(const SegSortPlan& pl, hipStream_t st) {
     const int npass = pl.ks.key_bytes;
-    long long* cs_in = checksum_in ? pl.d_checksum : nullptr;
-    long long* cs_out = checksum_out ? pl.d_checksum + 1 : nullptr;
-    uint32_t* err = (uint32_t*)(pl.d_pass_flags + kMaxPasses);
-    if (pl.nrec == 0 || pl.ntiles == 0) return 0;
-    if (npass <= 0 || npass > kMaxPasses) return -1;
-
-    PassTable pt;
-    for (int b = 0; b < npass; b++) pt.p[b] = make_pass(pl.ks, b);
-
-    // clear counters: hist, pass flags (+ error flag), tickets
     int rc = launch_fill_u32(pl.d_hist, 0u, (int64_t)pl.nseg * npass * kBins, st);
     if (rc) return rc;
     rc = launch_fill_u32(pl.d_pass_flags, 0u, kMaxPasses + 1, st);
     if (rc) return rc;
-    rc = launch_fill_u32(pl.d_tickets, 0u, kMaxPasses, st);
-    if (rc) return rc;
+    return launch_fill_u32(pl.d_tickets, 0u, kMaxPasses, st);
+}
 
+int seg_count(const SegSortPlan& pl, const void* src, long long* cs_in, hipStream_t st) {
+    const int npass = pl.ks.key_bytes;
+    if (npass <= 0 || npass > kMaxPasses) return -1;
+    int rc = seg_clear(pl, st);
+    if (rc) return rc;
+    if (pl.nrec == 0 || pl.ntiles == 0) return 0;
+    const PassTable pt = make_pass_table(pl.ks);
     switch (pl.rw) {
-        case 1: launch_hist<1>(pl, bufs.src, npass, pt, cs_in, st); break;
-        case 2: launch_hist<2>(pl, bufs.src, npass, pt, cs_in, st); break;
-        case 4: launch_hist<4>(pl, bufs.src, npass, pt, cs_in, st); break;
-        case 8: launch_hist<8>(pl, bufs.src, npass, pt, cs_in, st); break;
+        case 1: launch_hist<1>(pl, src, npass, pt, cs_in, st); break;
+        case 2: launch_hist<2>(pl, src, npass, pt, cs_in, st); break;
+        case 4: launch_hist<4>(pl, src, npass, pt, cs_in, st); break;
+        case 8: launch_hist<8>(pl, src, npass, pt, cs_in, st); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int seg_passes(const SegSortPlan& pl, const SortBuffers& bufs, hipStream_t st, long long* cs_out,
+               int* passes_run, hipEvent_t* pass_events) {
+    *passes_run = 0;
+    const int npass = pl.ks.key_bytes;
+    uint32_t* err = (uint32_t*)(pl.d_pass_flags + kMaxPasses);
+    if (pl.nrec == 0 || pl.ntiles == 0) return 0;
+    if (npass <= 0 || npass > kMaxPasses) return -1;
+    if (!tile_supported(pl.rw, pl.tile)) return -1;
+    const PassTable pt = make_pass_table(pl.ks);
     hipLaunchKernelGGL(scan_kernel, dim3((unsigned)((int64_t)pl.nseg * npass)), dim3(kBlock), 0, st,
                        pl.d_hist, pl.d_begin, npass, pl.d_digit_base, pl.d_pass_flags);
     MPSG_CHECK(hipGetLastError());
@@ -384,6 +504,7 @@ int segmented_sort(const SegSortPlan& pl, const SortBuffers& bufs, hipStream_t s
     void* src = bufs.src;
     void* X = bufs.dst;
     void* Y = bufs.tmp;
+    int rc = 0;
     if (k == 0) {
         if (src != X) return launch_copy(src, X, pl.nrec, pl.rw, cs_out, st);
         return 0;
@@ -403,17 +524,27 @@ int segmented_sort(const SegSortPlan& pl, const SortBuffers& bufs, hipStream_t s
         long long* cso = (i == k - 1) ? cs_out : nullptr;
         if (pass_events) MPSG_CHECK(hipEventRecord(pass_events[2 * i], st));
         switch (pl.rw) {
-            case 1: launch_pass<1>(pl, cur, out, npass, todo[i], pd, pl.d_tickets + i, epoch, cso, err, st); break;
-            case 2: launch_pass<2>(pl, cur, out, npass, todo[i], pd, pl.d_tickets + i, epoch, cso, err, st); break;
-            case 4: launch_pass<4>(pl, cur, out, npass, todo[i], pd, pl.d_tickets + i, epoch, cso, err, st); break;
-            case 8: launch_pass<8>(pl, cur, out, npass, todo[i], pd, pl.d_tickets + i, epoch, cso, err, st); break;
+            case 1: rc = launch_pass<1>(pl, cur, out, npass, todo[i], pd, pl.d_tickets + i, epoch, cso, err, st); break;
+            case 2: rc = launch_pass<2>(pl, cur, out, npass, todo[i], pd, pl.d_tickets + i, epoch, cso, err, st); break;
+            case 4: rc = launch_pass<4>(pl, cur, out, npass, todo[i], pd, pl.d_tickets + i, epoch, cso, err, st); break;
+            case 8: rc = launch_pass<8>(pl, cur, out, npass, todo[i], pd, pl.d_tickets + i, epoch, cso, err, st); break;
+            default: rc = -1;
         }
+        if (rc) return rc;
         MPSG_CHECK(hipGetLastError());
         if (pass_events) MPSG_CHECK(hipEventRecord(pass_events[2 * i + 1], st));
         cur = out;
     }
     *passes_run = k;
     return 0;
+}
+
+int segmented_sort(const SegSortPlan& pl, const SortBuffers& bufs, hipStream_t st,
+                   bool checksum_in, bool checksum_out, int* passes_run, hipEvent_t* pass_events) {
+    *passes_run = 0;
+    int rc = seg_count(pl, bufs.src, checksum_in ? pl.d_checksum : nullptr, st);
+    if (rc) return rc;
+    return seg_passes(pl, bufs, st, checksum_out ? pl.d_checksum + 1 : nullptr, passes_run, pass_events);
 }
 
 }  // namespace mpsg
