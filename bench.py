@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrecords/s at 2^28×16B: learned-splitter vs plain GPU sort; % of HBM peak"
 HBM_PEAK_GBS = 8000.0
+PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_second_sort_ms", "t_total_ms")
+# HBM bytes per onesweep launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE +
+# WRITE_SIZE, scaled by the counters' reading for a plain copy kernel of known
+# size (profiles/onesweep_pass_clustered_2e28x16B_pmc.md)
+TRAFFIC_BYTES = None
 
 
 def log(*a):
@@ -64,6 +69,7 @@ def main():
     ap.add_argument("--vranks", type=int, default=64)
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="time the plain path only")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain copy kernel of n*16 bytes (byte-counter calibration)")
     args = ap.parse_args()
@@ -97,74 +103,123 @@ def main():
     key = _lib.KeyDesc(offset=0, width=8, is_signed=0, nwords=1, reserved=0)
     flags = _lib.DEVICE_PTRS | _lib.TIMING
 
-    def step():
+    def plain_step():
         _lib.check(lib.mpsort_gpu_newarray(din, nin, P, dout, nin, 16, ctypes.byref(key), flags))
 
-    for i in range(args.warmup):
-        step()
-        log("warmup %d done" % i)
-    _lib.check(lib.mpsort_gpu_synchronize())
-    pass_ms, phases = [], []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        st = _lib.last_stats()
-        pass_ms.extend(st.pass_ms[j] for j in range(st.npass_ms))
-        phases.append(st.as_dict())
-    _lib.check(lib.mpsort_gpu_synchronize())
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    value = n / (ms * 1e-3) / 1e6
+    def timed(step, label):
+        for i in range(args.warmup):
+            step()
+            log("%s warmup %d done" % (label, i))
+        _lib.check(lib.mpsort_gpu_synchronize())
+        stats = []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step()
+            stats.append(_lib.last_stats().as_dict())
+        _lib.check(lib.mpsort_gpu_synchronize())
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        log("%s: %.3f ms per step, %.1f Mrecords/s" % (label, ms, n / (ms * 1e-3) / 1e6))
+        return ms, stats
 
+    # ---- plain path ------------------------------------------------------
+    ms_plain, st_plain = timed(plain_step, "plain")
     if args.copy_probe:
         _lib.check(lib.mpsort_gpu_dcopy(dout, din, nb))
         log("copy probe: copy_kernel<4> moved %d bytes in and %d bytes out" % (nb, nb))
-
-    # correctness of the timed output (size-independent properties)
-    if args.copy_probe:
-        step()
-    out = numpy.empty_like(recs)
-    _lib.check(lib.mpsort_gpu_memcpy_d2h(out.ctypes.data, dout, nb))
-    k = out['key']
+        plain_step()
+    plain_out = numpy.empty_like(recs)
+    _lib.check(lib.mpsort_gpu_memcpy_d2h(plain_out.ctypes.data, dout, nb))
+    k = plain_out['key']
     ok = bool(numpy.all(k[1:] >= k[:-1]))
     ties = k[1:] == k[:-1]
-    ok = ok and bool(numpy.all(out['idx'][1:][ties] > out['idx'][:-1][ties]))
-    del out
-    lib.mpsort_gpu_dfree(din)
-    lib.mpsort_gpu_dfree(dout)
+    ok = ok and bool(numpy.all(plain_out['idx'][1:][ties] > plain_out['idx'][:-1][ties]))
+    del k, ties
     if not ok:
-        print("bench.py: output is not the stable sort", file=sys.stderr)
+        print("bench.py: plain output is not the stable sort", file=sys.stderr)
         return 5
 
-    last = phases[-1]
+    # ---- learned path ----------------------------------------------------
+    # model: 2^20 stratified samples with exact ranks from the plain pass
+    # output, trained by the GPU kernels (not part of the timed step)
+    result_learned = None
+    if not args.no_learned:
+        from mpsort import learned
+        t0 = time.perf_counter()
+        model = learned.train(None, nsample=1 << 20, seed=1, device_ptr=(dout, n, 16, key))
+        t_train = time.perf_counter() - t0
+        minfo = model.info()
+        log("model: trained in %.3f s (%d Adam steps, %d leaves, err_max %.2e, max leaf %d samples)"
+            % (t_train, minfo['steps'], minfo['leaves'], minfo['err_max'], minfo['max_leaf_samples']))
+
+        def learned_step():
+            _lib.check(lib.mpsort_gpu_newarray_learned(din, nin, P, dout, nin, 16, ctypes.byref(key),
+                                                       model._h, flags))
+
+        ms_learned, st_learned = timed(learned_step, "learned")
+        lout = numpy.empty_like(recs)
+        _lib.check(lib.mpsort_gpu_memcpy_d2h(lout.ctypes.data, dout, nb))
+        same = bool(numpy.array_equal(lout, plain_out))
+        del lout
+        if not same:
+            print("bench.py: learned output differs from the plain path", file=sys.stderr)
+            return 6
+        L = st_learned[-1]
+        result_learned = {"ms_per_step": round(ms_learned, 3),
+                          "value": round(n / (ms_learned * 1e-3) / 1e6, 2),
+                          "bit_identical_to_plain": same,
+                          "train_s": round(t_train, 3), "model": {k2: minfo[k2] for k2 in minfo},
+                          "splitters_exact": L["splitters_exact"],
+                          "splitters_repaired": L["splitters_repaired"],
+                          "repair_rounds": L["repair_rounds"],
+                          "buckets_overflow": L["buckets_overflow"],
+                          "plain_sides": L["plain_sides"],
+                          "buckets_lds_radix": L["buckets_lds_radix"],
+                          "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
+                          "kernels_ms": dict(zip(["hist_A", "scatter_A", "lds_sort_A", "fallback_A",
+                                                  "hist_B", "scatter_B", "lds_sort_B", "fallback_B"],
+                                                 [round(x, 4) for x in L["pass_ms"]]))}
+        log("learned phases: " + json.dumps(result_learned["phases_ms"]))
+        log("learned kernels: " + json.dumps(result_learned["kernels_ms"]))
+        model.close()
+    del plain_out
+    lib.mpsort_gpu_dfree(din)
+    lib.mpsort_gpu_dfree(dout)
+
+    last = st_plain[-1]
     alg_bytes = 2.0 * n * 16
+    pass_ms = [x for st in st_plain for x in st["pass_ms"]]
     avg_pass_ms = sum(pass_ms) / len(pass_ms)
     achieved = alg_bytes / (avg_pass_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "onesweep_kernel<4> (one 8-bit digit pass, 2*n*16 B per launch)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES,
+                "kernel": "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)",
                 "avg_launch_ms": round(avg_pass_ms, 4), "launches": len(pass_ms)}
-    log("phases (ms): first %.3f findP %.3f lay %.3f exch %.3f second %.3f total %.3f; passes %d+%d, "
+    log("plain phases (ms): first %.3f findP %.3f lay %.3f exch %.3f second %.3f total %.3f; passes %d+%d, "
         "bisect rounds %d" % (last["t_first_sort_ms"], last["t_findp_ms"], last["t_laysolve_ms"],
                               last["t_exchange_ms"], last["t_second_sort_ms"], last["t_total_ms"],
                               last["first_passes"], last["second_passes"], last["bisect_rounds"]))
-    log("pass ms: " + " ".join("%.3f" % x for x in last["pass_ms"]))
-    exch_gbs = alg_bytes / (last["t_exchange_ms"] * 1e-3) / 1e9 if last["t_exchange_ms"] > 0 else 0
-    log("exchange %.0f GB/s; sort-equivalent 2ns/t = %.0f GB/s" % (exch_gbs, alg_bytes / (ms * 1e-3) / 1e9))
+    log("plain pass ms: " + " ".join("%.3f" % x for x in last["pass_ms"]))
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(recs, 1 << min(args.cpu_sample_log2, args.log2n))
+    value_plain = n / (ms_plain * 1e-3) / 1e6
+    if result_learned is not None:
+        value, ms, path = result_learned["value"], result_learned["ms_per_step"], "learned"
+    else:
+        value, ms, path = round(value_plain, 2), ms_plain, "plain"
     result = {
-        "metric": METRIC, "value": round(value, 2), "unit": "Mrecords/s", "n_gpus": 1,
+        "metric": METRIC, "value": value, "unit": "Mrecords/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": "plain GPU histogram sort, 2^%d x 16 B {u64 Morton key, i64 index}, "
+        "config": {"workload": "GPU histogram sort, 2^%d x 16 B {u64 Morton key, i64 index}, "
                                "clustered field, %d virtual ranks" % (args.log2n, P),
-                   "records": n, "record_bytes": 16, "vranks": P, "path": "plain"},
-        "phases_ms": {k2: round(last[k2], 4) for k2 in ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms",
-                                                          "t_exchange_ms", "t_second_sort_ms", "t_total_ms")},
+                   "records": n, "record_bytes": 16, "vranks": P, "path": path},
+        "plain": {"value": round(value_plain, 2), "ms_per_step": round(ms_plain, 3),
+                  "phases_ms": {k2: round(last[k2], 4) for k2 in PHASES}},
+        "learned": result_learned,
+        "learned_vs_plain": round(value / value_plain, 3) if result_learned else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -116,9 +116,11 @@ typedef struct {
     int32_t buckets;
     int32_t buckets_overflow;  /* buckets that fell back to the plain sort */
     int32_t repair_rounds;
-    int32_t reserved2;
+    int32_t plain_sides;       /* local sorts that took the plain radix path */
     double t_model_ms;
     double t_bucket_ms;
+    int32_t buckets_lds_radix; /* buckets sorted by the in-LDS radix sort */
+    int32_t reserved3;
 } mpsort_gpu_stats;
 
 int mpsort_gpu_last_stats(mpsort_gpu_stats *stats);
@@ -143,6 +145,50 @@ void mpsort_gpu_report_last_run(void);
 int mpsort_gpu_last_counts(unsigned char *splitters, unsigned char *pmin, unsigned char *pmax,
                            int64_t *C, int64_t *myCLT, int64_t *myCLE, int64_t *myC,
                            int64_t *send, int64_t *rsize_out);
+
+/* ---- learned-splitter path ------------------------------------------------
+ * No reference counterpart: a learned key-CDF model proposes the P-1
+ * splitters (checked with exact counts and repaired by the reference's
+ * bisection inside the proposed bracket) and the bucket boundaries of a
+ * one-pass bucket scatter + in-LDS bucket sort that replaces the radix passes
+ * of both local sorts.  The output is bit-identical to mpsort_gpu_newarray.
+ *
+ * The model is trained on nsample records sampled from `sorted`, the output of
+ * one plain sort (their exact ranks are their positions), on the GPU.  Keys of
+ * at most 8 bytes. */
+typedef struct mpsort_gpu_model_s mpsort_gpu_model;
+#define MPSORT_GPU_MODEL_LINEAR_ROOT (1u << 8) /* no network: linear leaf routing */
+
+typedef struct {
+    int64_t nsample;
+    int32_t leaves;
+    int32_t linear_root;
+    int32_t steps;           /* Adam steps run */
+    int32_t nheavy;          /* duplicate-heavy keys kept as exact boundaries */
+    int32_t max_leaf_samples;/* largest number of samples routed to one leaf */
+    int32_t log_input;       /* 1: root input log2(1 + key - kmin) scaled to [0, 1] */
+    double train_ms;
+    float loss;              /* final mean squared error of the root */
+    float err_max;           /* max |F(key) - rank/n| over the samples */
+    uint64_t kmin, kmax;
+} mpsort_gpu_model_info;
+
+int mpsort_gpu_model_train(mpsort_gpu_model **model, const void *sorted, int64_t n, size_t elsize,
+                           const mpsort_key_desc *key, int64_t nsample, uint64_t seed, unsigned flags);
+int mpsort_gpu_model_free(mpsort_gpu_model *model);
+int mpsort_gpu_model_get_info(const mpsort_gpu_model *model, mpsort_gpu_model_info *info);
+/* F(key) for n keys given as the order-preserving uint64 (host arrays) */
+int mpsort_gpu_model_eval(const mpsort_gpu_model *model, const uint64_t *keys, int64_t n, float *out);
+/* raw MLP parameters (1153 floats: w1[32] b1[32] W2[32][32] b2[32] w3[32] b3)
+ * and the leaves (4 floats each: slope, intercept, lo, hi); either may be NULL */
+int mpsort_gpu_model_params(const mpsort_gpu_model *model, float *raw, float *leaves);
+/* gradient of the root's mean squared error at raw parameters `raw` over n
+ * samples (x, y) computed by the training kernels (host arrays) */
+int mpsort_gpu_mlp_grad(const float *raw, const float *x, const float *y, int64_t n, float *grad,
+                        float *loss);
+int mpsort_gpu_newarray_learned(const void *base, const int64_t *nin, int P, void *out,
+                                const int64_t *nout, size_t elsize, const mpsort_key_desc *key,
+                                const mpsort_gpu_model *model, unsigned flags);
 
 /* Device allocator hook (MPIU_SetMalloc mp-mpiu.h:4-16).  NULL restores
  * hipMalloc/hipFree.  Frees every cached workspace. */

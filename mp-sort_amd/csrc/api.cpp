@@ -8,7 +8,13 @@
 #include <string.h>
 #include <vector>
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
 #include "internal.h"
+#include "lsort.h"
+#include "model.h"
 #include "../../include/mpsort_gpu.h"
 
 namespace mpsg {
@@ -17,7 +23,7 @@ namespace mpsg {
 int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sbeg,
                   const int64_t* scnt, int P, const int64_t* C, int64_t* win, int64_t* myclt,
                   int64_t* mycle, uint64_t* split, uint64_t* pminmax, int* rounds, uint32_t* err,
-                  hipStream_t st);
+                  hipStream_t st, const uint64_t* init = nullptr, int* exact = nullptr);
 int launch_layout(int P, const int64_t* C, const int64_t* scnt, const int64_t* sbeg, int64_t* myclt,
                   int64_t* mycle, int64_t* myC, int64_t* send, int64_t* run_dst, int64_t* run_src,
                   uint32_t* err, hipStream_t st);
@@ -100,6 +106,9 @@ struct DevCtx {
     uint32_t epoch = 0;
     Buf din, dout;      // host staging
     Buf s0, s1, rbuf;   // sort-record buffers
+    Buf s2;             // third record buffer (learned-path bucket fallback)
+    Buf lsmall;         // learned-path tables and counters
+    Buf ovf;            // plans of the learned-path fallback
     Buf small;          // counts, tables
     Buf lookback;       // look-back flags (only ever holds flag words)
     void* pinned = nullptr;
@@ -579,6 +588,473 @@ static void internal_to_radix(const DevCtx& c, const uint64_t* w, unsigned char*
     }
 }
 
+
+// ===========================================================================
+// learned-splitter path
+
+struct LearnedModel {
+    int device = 0;
+    float* raw = nullptr;     // kMlpParams
+    float* eff = nullptr;     // kMlpParams
+    float* leaves = nullptr;  // 4 * L
+    uint64_t* heavy = nullptr;
+    int nheavy = 0;
+    int L = 0;
+    int linear_root = 0;
+    uint64_t kmin = 0, kmax = 0;
+    double inv_range = 0;
+    int log_input = 0;
+    double log_scale = 0;
+    int64_t nsample = 0;
+    float err_max = 0, loss = 0;
+    double train_ms = 0;
+    int steps = 0;
+    int max_leaf = 0;
+    int key_bytes = 8;
+    ModelDev dev() const {
+        ModelDev m;
+        m.eff = eff;
+        m.leaves = leaves;
+        m.L = L;
+        m.linear_root = linear_root;
+        m.kmin = kmin;
+        m.inv_range = inv_range;
+        m.log_input = log_input;
+        m.log_scale = log_scale;
+        return m;
+    }
+};
+
+constexpr int kMaxOverflowList = 1 << 16;  // bucket overflow list capacity
+constexpr int kMaxFallbackSegs = 4096;     // larger lists re-sort the whole side plainly
+
+struct LearnedSide {
+    int64_t *begin, *count, *ltiles;  // segment tables (learned tile size)
+    uint32_t* lhist;                  // nseg * 1024
+    int64_t* bbase;                   // nseg * 1024
+    uint64_t* tables;                 // nseg * 511
+    float *q0, *q1;                   // nseg
+    uint32_t* tickets;                // 4
+    uint32_t* err;                    // 4
+    int64_t* overflow;                // 2 * kMaxOverflowList
+    uint32_t* noverflow;              // 4: [0] oversized buckets, [1] buckets for the LDS radix sort
+    uint32_t* hard;                   // nseg * kLBins
+};
+
+template <class A>
+static void lay_lside(A& a, int P, LearnedSide* ls) {
+    if constexpr (std::is_same<A, Arena>::value) {
+        ls->begin = a.template take<int64_t>(P);
+        ls->count = a.template take<int64_t>(P);
+        ls->ltiles = a.template take<int64_t>(P + 1);
+        ls->lhist = a.template take<uint32_t>((size_t)P * kLBins);
+        ls->bbase = a.template take<int64_t>((size_t)P * kLBins);
+        ls->tables = a.template take<uint64_t>((size_t)P * kLBounds);
+        ls->q0 = a.template take<float>(P);
+        ls->q1 = a.template take<float>(P);
+        ls->tickets = a.template take<uint32_t>(4);
+        ls->err = a.template take<uint32_t>(4);
+        ls->overflow = a.template take<int64_t>(2 * (size_t)kMaxOverflowList);
+        ls->noverflow = a.template take<uint32_t>(4);
+        ls->hard = a.template take<uint32_t>((size_t)P * kLBins);
+    } else {
+        a.template take<int64_t>(P);
+        a.template take<int64_t>(P);
+        a.template take<int64_t>(P + 1);
+        a.template take<uint32_t>((size_t)P * kLBins);
+        a.template take<int64_t>((size_t)P * kLBins);
+        a.template take<uint64_t>((size_t)P * kLBounds);
+        a.template take<float>(P);
+        a.template take<float>(P);
+        a.template take<uint32_t>(4);
+        a.template take<uint32_t>(4);
+        a.template take<int64_t>(2 * (size_t)kMaxOverflowList);
+        a.template take<uint32_t>(4);
+        a.template take<uint32_t>((size_t)P * kLBins);
+    }
+}
+
+struct LearnedCounters {
+    int hard_buckets = 0;
+    int overflow_buckets = 0;
+    int global_fallback = 0;
+};
+
+// model-bucketed local sort of every segment of `side` from src into dst
+// (tmp receives the bucket scatter).  plain is the radix plan of the same
+// segments, used when too many buckets overflow.
+static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64_t n, const LearnedSide& L,
+                              const SegSortPlan& plain, const void* src, void* dst, void* tmp, long long* cs_in,
+                              long long* cs_out, LearnedCounters* lc, hipStream_t st, hipEvent_t* ev) {
+    int rc = 0;
+    // ev (optional): 5 events around bucket histogram, scatter, in-LDS
+    // bucket sorts and the overflow fallback
+    auto mark = [&](int i) -> int {
+        if (ev) MPSG_CHECK(hipEventRecord(ev[i], st));
+        return 0;
+    };
+    const LKey lk = make_lkey(ks);
+    const int64_t trecs = lscatter_tile(rw);
+    if ((rc = launch_fill_u32(L.lhist, 0u, (int64_t)P * kLBins, st))) return rc;
+    if ((rc = launch_fill_u32(L.tickets, 0u, 4, st))) return rc;
+    if ((rc = launch_fill_u32(L.err, 0u, 4, st))) return rc;
+    if ((rc = launch_fill_u32(L.noverflow, 0u, 4, st))) return rc;
+    // tiles of the learned scatter (the look-back buffer was sized by the
+    // caller: it must not move while plans hold its address)
+    std::vector<int64_t> cnt(P), beg(P), lt(P + 1);
+    MPSG_CHECK(hipMemcpyAsync(cnt.data(), L.count, sizeof(int64_t) * P, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    {
+        // segments whose average bucket would exceed half the in-LDS capacity
+        // would mostly overflow: such a side takes the plain radix sort
+        const int64_t big = *std::max_element(cnt.begin(), cnt.end());
+        if (big > (int64_t)kLBins * kChunkCap / 2) {
+            lc->global_fallback++;
+            if ((rc = mark(0))) return rc;
+            int passes = 0;
+            SortBuffers b{const_cast<void*>(src), dst, tmp};
+            if ((rc = segmented_sort(plain, b, st, cs_in != nullptr, cs_out != nullptr, &passes, nullptr))) return rc;
+            for (int e = 1; e <= 4; e++)
+                if ((rc = mark(e))) return rc;
+            return 0;
+        }
+    }
+    const int64_t ntiles = make_tiles(cnt.data(), P, trecs, beg.data(), lt.data());
+    MPSG_CHECK(hipMemcpyAsync(L.ltiles, lt.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if ((size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint64_t) > c.lookback.bytes)
+        return fail(MPSORT_ERR_ARG, "look-back buffer too small for the bucket scatter");
+    SegDev sd{L.begin, L.count, L.ltiles, P, 0};
+    if ((rc = mark(0))) return rc;
+    if ((rc = launch_lhist(rw, src, sd, ntiles, (int)trecs, lk, L.tables, L.lhist, cs_in, st))) return rc;
+    if ((rc = launch_lscan(P, L.lhist, L.begin, L.bbase, st))) return rc;
+    if ((rc = mark(1))) return rc;
+    int dev = 0;
+    MPSG_CHECK(hipGetDevice(&dev));
+    // single-value buckets are written to their final slots by the scatter
+    // unless that would overwrite records the scatter still reads
+    const bool eq_direct = dst != src;
+    if ((rc = launch_lscatter(rw, src, tmp, eq_direct ? dst : tmp, sd, ntiles, lk, L.tables, L.bbase,
+                              (uint64_t*)c.lookback.p, L.tickets, next_epoch(dev), L.err, st)))
+        return rc;
+    if ((rc = mark(2))) return rc;
+    if ((rc = launch_chunk_sort(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, (int64_t)P * kLBins, L.overflow,
+                                L.noverflow, kMaxOverflowList, L.hard, L.noverflow + 1, cs_out, eq_direct, st)))
+        return rc;
+    if ((rc = mark(3))) return rc;
+    uint32_t novh[2] = {0, 0}, errs = 0;
+    MPSG_CHECK(hipMemcpyAsync(novh, L.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipMemcpyAsync(&errs, L.err, sizeof errs, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if (errs) return fail(MPSORT_ERR_TIMEOUT, "learned bucket scatter: device flag 0x%x", errs);
+    const uint32_t nov = novh[0];
+    lc->hard_buckets += (int)novh[1];
+    lc->overflow_buckets += (int)nov;
+    if (nov == 0) return mark(4);
+    const size_t recbytes = (size_t)rw * 4;
+    if ((rc = c.s2.ensure((size_t)n * recbytes))) return rc;
+    int passes = 0;
+    if (nov > (uint32_t)kMaxFallbackSegs) {
+        // too many oversized buckets: the plain radix sort of the whole side
+        lc->global_fallback++;
+        if (cs_out) {
+            if ((rc = launch_fill_u32((uint32_t*)cs_out, 0u, 2, st))) return rc;
+        }
+        // src is intact unless the sort is in place; then tmp holds every
+        // record (no single-value bucket went to dst)
+        SortBuffers b{src == dst ? tmp : const_cast<void*>(src), dst, c.s2.p};
+        if ((rc = segmented_sort(plain, b, st, false, cs_out != nullptr, &passes, nullptr))) return rc;
+        return mark(4);
+    }
+    // radix sort of the oversized buckets only (they are segments of tmp)
+    std::vector<int64_t> ov(2 * (size_t)nov);
+    MPSG_CHECK(hipMemcpy(ov.data(), L.overflow, sizeof(int64_t) * 2 * nov, hipMemcpyDeviceToHost));
+    const int ns = (int)nov;
+    std::vector<int64_t> ob(ns), oc(ns), ot(ns + 1), ob2(ns);
+    // segments must be in ascending position order for the tile tables
+    std::vector<int> ord(ns);
+    for (int i = 0; i < ns; i++) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b2) { return ov[2 * a] < ov[2 * b2]; });
+    for (int i = 0; i < ns; i++) { ob[i] = ov[2 * ord[i]]; oc[i] = ov[2 * ord[i] + 1]; }
+    const TileCfg tile = default_tile(rw);
+    const int64_t ptrecs = (int64_t)tile.block * tile.items;
+    const int64_t ont = make_tiles(oc.data(), ns, ptrecs, ob2.data(), ot.data());
+    Sizer sz;
+    lay_plan(sz, ns, ks.key_bytes, nullptr);
+    if ((rc = c.ovf.ensure(sz.off + 4096))) return rc;
+    Arena ar{(char*)c.ovf.p};
+    PlanArrays pa;
+    lay_plan(ar, ns, ks.key_bytes, &pa);
+    MPSG_CHECK(hipMemcpyAsync(pa.begin, ob.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipMemcpyAsync(pa.count, oc.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipMemcpyAsync(pa.tile_start, ot.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if ((size_t)(ont > 0 ? ont : 1) * kBins * sizeof(uint64_t) > c.lookback.bytes)
+        return fail(MPSORT_ERR_ARG, "look-back buffer too small for the bucket fallback");
+    SegSortPlan op = plain;
+    op.nseg = ns;
+    op.ntiles = ont;
+    op.nrec = 0;
+    for (int i = 0; i < ns; i++) op.nrec += oc[i];
+    op.d_begin = pa.begin;
+    op.d_count = pa.count;
+    op.d_tile_start = pa.tile_start;
+    op.d_hist = pa.hist;
+    op.d_digit_base = pa.digit_base;
+    op.d_pass_flags = pa.pass_flags;
+    op.d_tickets = pa.tickets;
+    op.d_lookback = (uint64_t*)c.lookback.p;
+    op.sparse = true;
+    SortBuffers b{tmp, dst, c.s2.p};
+    if ((rc = seg_count(op, tmp, nullptr, st))) return rc;
+    if ((rc = seg_passes(op, b, st, cs_out, &passes, nullptr))) return rc;
+    uint32_t e2 = 0;
+    MPSG_CHECK(hipMemcpyAsync(&e2, pa.pass_flags + kMaxPasses, sizeof e2, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if (e2) return fail(MPSORT_ERR_TIMEOUT, "bucket fallback sort: device flag 0x%x", e2);
+    return mark(4);
+}
+
+
+// The histogram sort with the learned local sorts and learned splitter
+// proposals (same stages and outputs as run_histsort).
+static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in, void* d_out,
+                                const int64_t* nin, const int64_t* nout, const LearnedModel& M,
+                                mpsort_gpu_stats* stats) {
+    hipStream_t st = c.st;
+    const int P = cs.P;
+    const int rw = cs.rw;
+    const int64_t n = cs.ntot;
+    const size_t recbytes = (size_t)rw * 4;
+    const int npass = cs.ks.key_bytes;
+    int rc = 0;
+
+    Sizer sz;
+    lay_plan(sz, P, npass, nullptr);
+    lay_plan(sz, P, npass, nullptr);
+    lay_hist(sz, P, nullptr);
+    if ((rc = c.small.ensure(sz.off + 4096))) return rc;
+    Arena ar{(char*)c.small.p};
+    PlanArrays pa, pb;
+    HistArrays h;
+    lay_plan(ar, P, npass, &pa);
+    lay_plan(ar, P, npass, &pb);
+    lay_hist(ar, P, &h);
+    Sizer lz;
+    lay_lside(lz, P, nullptr);
+    lay_lside(lz, P, nullptr);
+    lz.take<uint64_t>(3 * (size_t)P + 3);
+    lz.take<int>((size_t)P + 1);
+    if ((rc = c.lsmall.ensure(lz.off + 4096))) return rc;
+    Arena la{(char*)c.lsmall.p};
+    LearnedSide LA, LB;
+    lay_lside(la, P, &LA);
+    lay_lside(la, P, &LB);
+    uint64_t* prop = la.take<uint64_t>(3 * (size_t)P + 3);
+    int* exact = la.take<int>((size_t)P + 1);
+
+    if ((rc = c.s0.ensure(n * recbytes))) return rc;
+    if ((rc = c.s1.ensure(n * recbytes))) return rc;
+    if (cs.general && (rc = c.rbuf.ensure(n * recbytes))) return rc;
+
+    const size_t tab = (size_t)(9 * P + 4);
+    if ((rc = ensure_pinned(c, tab * sizeof(int64_t) + 4096))) return rc;
+    int64_t* ht = (int64_t*)c.pinned;
+    int64_t *hbA = ht, *hcA = ht + P, *htA = ht + 2 * P;
+    int64_t *hbB = ht + 3 * P + 1, *hcB = ht + 4 * P + 1, *htB = ht + 5 * P + 1;
+    int64_t* hC = ht + 6 * P + 2;
+    float* hq = (float*)(ht + 7 * P + 3);  // 4 * P floats
+    for (int t = 0; t < P; t++) { hcA[t] = nin[t]; hcB[t] = nout[t]; }
+    const TileCfg tile = default_tile(rw);
+    const int64_t trecs = (int64_t)tile.block * tile.items;
+    const int64_t ntA = make_tiles(hcA, P, trecs, hbA, htA);
+    const int64_t ntB = make_tiles(hcB, P, trecs, hbB, htB);
+    hC[0] = 0;
+    for (int t = 0; t < P; t++) hC[t + 1] = hC[t] + nout[t];
+    for (int t = 0; t < P; t++) {
+        hq[t] = 0.f;
+        hq[P + t] = 1.f;
+        hq[2 * P + t] = (float)((double)hC[t] / (double)n);
+        hq[3 * P + t] = (float)((double)hC[t + 1] / (double)n);
+    }
+    const int64_t ntiles = ntA > ntB ? ntA : ntB;
+    {
+        // one size for every scatter of this call: radix passes, bucket
+        // scatter (1024 bins) and the bucket fallback
+        const int64_t lt = lscatter_tile(rw);
+        int64_t lA = 0, lB = 0;
+        for (int t = 0; t < P; t++) { lA += (nin[t] + lt - 1) / lt; lB += (nout[t] + lt - 1) / lt; }
+        const size_t need = std::max((size_t)(ntiles + kMaxFallbackSegs + 1) * kBins,
+                                     (size_t)(std::max(lA, lB) + 1) * kLBins) * sizeof(uint64_t);
+        if ((rc = c.lookback.ensure(need))) return rc;
+    }
+    auto up = [&](void* d, const void* hsrc, size_t bytes) {
+        return hipMemcpyAsync(d, hsrc, bytes, hipMemcpyHostToDevice, st);
+    };
+    MPSG_CHECK(up(pa.begin, hbA, sizeof(int64_t) * P));
+    MPSG_CHECK(up(pa.count, hcA, sizeof(int64_t) * P));
+    MPSG_CHECK(up(pa.tile_start, htA, sizeof(int64_t) * (P + 1)));
+    MPSG_CHECK(up(pb.begin, hbB, sizeof(int64_t) * P));
+    MPSG_CHECK(up(pb.count, hcB, sizeof(int64_t) * P));
+    MPSG_CHECK(up(pb.tile_start, htB, sizeof(int64_t) * (P + 1)));
+    MPSG_CHECK(up(h.C, hC, sizeof(int64_t) * (P + 1)));
+    MPSG_CHECK(up(LA.begin, hbA, sizeof(int64_t) * P));
+    MPSG_CHECK(up(LA.count, hcA, sizeof(int64_t) * P));
+    MPSG_CHECK(up(LB.begin, hbB, sizeof(int64_t) * P));
+    MPSG_CHECK(up(LB.count, hcB, sizeof(int64_t) * P));
+    MPSG_CHECK(up(LA.q0, hq, sizeof(float) * P));
+    MPSG_CHECK(up(LA.q1, hq + P, sizeof(float) * P));
+    MPSG_CHECK(up(LB.q0, hq + 2 * P, sizeof(float) * P));
+    MPSG_CHECK(up(LB.q1, hq + 3 * P, sizeof(float) * P));
+    if ((rc = launch_fill_u32(h.err, 0u, 4, st))) return rc;
+    if ((rc = launch_fill_u32((uint32_t*)h.checksum, 0u, 4, st))) return rc;
+    if ((rc = launch_fill_u32(pa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
+    if ((rc = launch_fill_u32(pb.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
+
+    auto mkplan = [&](const PlanArrays& a, int64_t nt) {
+        SegSortPlan p;
+        p.rw = rw;
+        p.tile = tile;
+        p.ks = cs.ks;
+        p.nrec = n;
+        p.nseg = P;
+        p.ntiles = nt;
+        p.d_begin = a.begin;
+        p.d_count = a.count;
+        p.d_tile_start = a.tile_start;
+        p.d_hist = a.hist;
+        p.d_digit_base = a.digit_base;
+        p.d_pass_flags = a.pass_flags;
+        p.d_lookback = (uint64_t*)c.lookback.p;
+        p.d_tickets = a.tickets;
+        p.d_checksum = h.checksum;
+        return p;
+    };
+    SegSortPlan planA = mkplan(pa, ntA);
+    SegSortPlan planB = mkplan(pb, ntB);
+    const ModelDev md = M.dev();
+
+    MPSG_CHECK(hipEventRecord(c.ev[kEvStart], st));
+    void* recs_in = const_cast<void*>(d_in);
+    void* final_dst = d_out;
+    if (cs.general) {
+        if ((rc = launch_pack(d_in, n, cs.elsize, cs.key_off_user, cs.key_bytes_user, c.rbuf.p, rw,
+                              cs.checksum ? h.checksum : nullptr, st)))
+            return rc;
+        recs_in = c.rbuf.p;
+        final_dst = c.rbuf.p;
+    }
+    const bool cs_direct = cs.checksum && !cs.general;
+    LearnedCounters lc;
+    hipEvent_t* evA = cs.timing ? c.pass_ev[0] : nullptr;
+    hipEvent_t* evB = cs.timing && P > 1 ? c.pass_ev[1] : nullptr;
+
+    // FirstSort: every rank's keys span the whole distribution
+    if ((rc = launch_model_tables(md, M.kmax, LA.q0, LA.q1, P, kLBounds, LA.tables, M.heavy, M.nheavy, st)))
+        return rc;
+    if (P == 1) {
+        if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, final_dst, c.s1.p,
+                                     cs_direct ? h.checksum : nullptr, cs_direct ? h.checksum + 1 : nullptr, &lc,
+                                     st, evA)))
+            return rc;
+        MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
+        for (int e = kEvFindP; e <= kEvSecond; e++) MPSG_CHECK(hipEventRecord(c.ev[e], st));
+    } else {
+        if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, c.s0.p, c.s1.p,
+                                     cs_direct ? h.checksum : nullptr, nullptr, &lc, st, evA)))
+            return rc;
+        MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
+        // splitters: model proposals, checked and repaired by bisection
+        const float delta = std::max(2.0f * M.err_max, 8.0f / (float)M.nsample);
+        if ((rc = launch_model_propose(md, M.kmax, h.C, P, n, delta, prop, st))) return rc;
+        if ((rc = launch_bisect(c.s0.p, rw, cs.ks, pa.begin, pa.count, P, h.C, h.win, h.myclt, h.mycle, h.split,
+                                h.pminmax, h.rounds, h.err, st, prop, exact)))
+            return rc;
+        MPSG_CHECK(hipEventRecord(c.ev[kEvFindP], st));
+        if ((rc = launch_layout(P, h.C, pa.count, pa.begin, h.myclt, h.mycle, h.myC, h.send, h.run_dst,
+                                h.run_src, h.err, st)))
+            return rc;
+        MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
+        if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, P * P, h.run_dst, h.run_src, h.err, st))) return rc;
+        MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
+        // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n
+        if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, kLBounds, LB.tables, M.heavy, M.nheavy, st)))
+            return rc;
+        if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LB, planB, c.s1.p, final_dst, c.s0.p, nullptr,
+                                     cs_direct ? h.checksum + 1 : nullptr, &lc, st, evB)))
+            return rc;
+        MPSG_CHECK(hipEventRecord(c.ev[kEvSecond], st));
+    }
+    if (cs.general) {
+        if ((rc = launch_gather(d_in, n, cs.elsize, c.rbuf.p, rw, d_out, cs.checksum ? h.checksum + 1 : nullptr,
+                                h.err, st)))
+            return rc;
+    }
+    MPSG_CHECK(hipEventRecord(c.ev[kEvEnd], st));
+
+    uint32_t err0 = 0;
+    long long sums[2] = {0, 0};
+    std::vector<int> rounds(P > 1 ? P - 1 : 1, 0), ex(P > 1 ? P - 1 : 1, 0);
+    MPSG_CHECK(hipMemcpyAsync(&err0, h.err, sizeof err0, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipMemcpyAsync(sums, h.checksum, sizeof sums, hipMemcpyDeviceToHost, st));
+    if (P > 1) {
+        MPSG_CHECK(hipMemcpyAsync(rounds.data(), h.rounds, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipMemcpyAsync(ex.data(), exact, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
+    }
+    MPSG_CHECK(hipStreamSynchronize(st));
+    int rmax = 0, nexact = 0;
+    for (int i = 0; i + 1 < P; i++) {
+        rmax = std::max(rmax, rounds[i]);
+        nexact += ex[i];
+    }
+    stats->nvranks = P;
+    stats->record_bytes = (int32_t)recbytes;
+    stats->bisect_rounds = rmax;
+    stats->repair_rounds = rmax;
+    stats->general_path = cs.general ? 1 : 0;
+    stats->nrecords = n;
+    stats->checksum_in = sums[0];
+    stats->checksum_out = sums[1];
+    stats->learned = 1;
+    stats->splitters_exact = nexact;
+    stats->splitters_repaired = P > 1 ? P - 1 - nexact : 0;
+    stats->buckets = P * kLBins * (P > 1 ? 2 : 1);
+    stats->buckets_overflow = lc.overflow_buckets;
+    stats->t_first_sort_ms = ev_ms(c.ev[kEvStart], c.ev[kEvFirst]);
+    stats->t_findp_ms = ev_ms(c.ev[kEvFirst], c.ev[kEvFindP]);
+    stats->t_laysolve_ms = ev_ms(c.ev[kEvFindP], c.ev[kEvLayout]);
+    stats->t_exchange_ms = ev_ms(c.ev[kEvLayout], c.ev[kEvExchange]);
+    stats->t_second_sort_ms = ev_ms(c.ev[kEvExchange], c.ev[kEvSecond]);
+    stats->t_total_ms = ev_ms(c.ev[kEvStart], c.ev[kEvEnd]);
+    stats->plain_sides = lc.global_fallback;
+    stats->buckets_lds_radix = lc.hard_buckets;
+    // pass_ms: bucket histogram, scatter, in-LDS sorts, fallback for the
+    // FirstSort and then the SecondSort
+    int np = 0;
+    for (int side = 0; side < 2; side++) {
+        hipEvent_t* e = side == 0 ? evA : evB;
+        if (!e) continue;
+        for (int k = 0; k < 4; k++) stats->pass_ms[np++] = ev_ms(e[k], e[k + 1]);
+    }
+    stats->npass_ms = np;
+
+    c.last_P = P;
+    c.C = h.C;
+    c.myclt = h.myclt;
+    c.mycle = h.mycle;
+    c.myC = h.myC;
+    c.send = h.send;
+    c.split = h.split;
+    c.pminmax = h.pminmax;
+    c.last_ntot = n;
+    if (err0 & 1u) return fail(MPSORT_ERR_TIMEOUT, "a look-back wait exceeded its spin limit");
+    if (err0) return fail(MPSORT_ERR_LAYOUT, "layout check failed (device flags 0x%x)", err0);
+    if (cs.checksum && sums[0] != sums[1])
+        return fail(MPSORT_ERR_CHECKSUM, "MPSort: Data changed after sorting; checksum mismatch (%lld != %lld)",
+                    sums[0], sums[1]);
+    return 0;
+}
+
 }  // namespace mpsg
 
 using namespace mpsg;
@@ -784,6 +1260,9 @@ int mpsort_gpu_release(void) {
     c.s0.release();
     c.s1.release();
     c.rbuf.release();
+    c.s2.release();
+    c.lsmall.release();
+    c.ovf.release();
     c.small.release();
     c.lookback.release();
     c.last_P = 0;
@@ -877,6 +1356,397 @@ void radix_sort(void* base, size_t nmemb, size_t size, void (*radix)(const void*
 void mpsort_omp(void* base, size_t nmemb, size_t size, void (*radix)(const void* ptr, void* radix, void* arg),
                 size_t rsize, void* arg) {
     radix_sort(base, nmemb, size, radix, rsize, arg);
+}
+
+
+// ---- learned-splitter path --------------------------------------------------
+struct mpsort_gpu_model_s {
+    LearnedModel m;
+};
+
+static uint64_t hash64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// deterministic initial parameters: h1_j = tanh(8 (x - c_j)) with centres c_j
+// spread over [0, 1]; small positive W2, spread b2; w3 ~ 1/32; b3 = 0.5
+// Initial root: hidden unit j of the first layer is a soft step centred on
+// the sample quantile (j + 1/2) / 32 of x, as steep as the gap between the
+// neighbouring quantiles (xs: the S training inputs, sorted); the second
+// layer starts as a small positive mixing.
+static void mlp_init(float* raw, uint64_t seed, const float* xs, int64_t S) {
+    auto inv_softplus = [](double v) { return (float)std::log(std::expm1(v)); };
+    auto xq = [&](double f) {
+        int64_t i = (int64_t)(f * (double)S);
+        i = i < 0 ? 0 : (i >= S ? S - 1 : i);
+        return (double)xs[i];
+    };
+    for (int j = 0; j < kH; j++) {
+        const double c = xq((j + 0.5) / kH);
+        const double gap = std::max(xq((j + 1.0) / kH) - xq((double)j / kH), 1e-6);
+        const double w = std::min(std::max(2.0 / gap, 1.0), 1e4);
+        raw[kW1 + j] = inv_softplus(w);
+        raw[kB1 + j] = (float)(-w * c);
+    }
+    for (int q = 0; q < kH * kH; q++) {
+        const double u = (double)(hash64(seed + 7919 * (uint64_t)q) >> 11) * 0x1.0p-53;
+        raw[kW2 + q] = inv_softplus(0.05 + 0.1 * u);
+    }
+    for (int k = 0; k < kH; k++) {
+        const double u = (double)(hash64(seed + 104729 + (uint64_t)k) >> 11) * 0x1.0p-53;
+        raw[kB2 + k] = (float)(-1.6 + 3.2 * u);
+        raw[kW3 + k] = inv_softplus(1.0 / 64.0);
+    }
+    raw[kB3] = 0.5f;
+}
+
+int mpsort_gpu_model_train(mpsort_gpu_model** out, const void* sorted, int64_t n, size_t elsize,
+                           const mpsort_key_desc* key, int64_t nsample, uint64_t seed, unsigned flags) {
+    g_err[0] = 0;
+    if (!out) return fail(MPSORT_ERR_ARG, "model output pointer is NULL");
+    *out = nullptr;
+    if (!key || !sorted || n < 2 || elsize == 0) return fail(MPSORT_ERR_ARG, "bad training arguments");
+    if (key->width != 4 && key->width != 8) return fail(MPSORT_ERR_UNSUPPORTED, "key width %d", key->width);
+    const int64_t kb = (int64_t)key->width * key->nwords;
+    if (kb > 8) return fail(MPSORT_ERR_UNSUPPORTED, "the learned path handles keys of at most 8 bytes");
+    if (key->offset < 0 || key->offset + kb > (int64_t)elsize) return fail(MPSORT_ERR_ARG, "key outside record");
+    DevCtx* c = nullptr;
+    int rc = get_ctx(&c);
+    if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = c->st;
+    const bool devptr = (flags & MPSORT_GPU_DEVICE_PTRS) != 0;
+    const size_t bytes = (size_t)n * elsize;
+    const void* d_sorted = sorted;
+    void* tmpbuf = nullptr;
+    if (!devptr) {
+        tmpbuf = dev_alloc(bytes);
+        if (!tmpbuf) return fail(MPSORT_ERR_NOMEM, "device allocation failed");
+        MPSG_CHECK(hipMemcpy(tmpbuf, sorted, bytes, hipMemcpyHostToDevice));
+        d_sorted = tmpbuf;
+    }
+    uint64_t xm = 0;
+    if (key->is_signed)
+        for (int b = key->width - 1; b < kb; b += key->width) xm |= 0x80ull << (8 * b);
+    auto key_at = [&](int64_t pos, uint64_t* k) -> int {
+        unsigned char buf[8] = {0};
+        MPSG_CHECK(hipMemcpy(buf, (const char*)d_sorted + pos * elsize + key->offset, (size_t)kb,
+                             hipMemcpyDeviceToHost));
+        uint64_t v = 0;
+        for (int b = 0; b < kb; b++) v |= (uint64_t)buf[b] << (8 * b);
+        *k = v ^ xm;
+        return 0;
+    };
+    mpsort_gpu_model* mo = new mpsort_gpu_model_s();
+    LearnedModel& M = mo->m;
+    M.device = c->device;
+    M.key_bytes = (int)kb;
+    if ((rc = key_at(0, &M.kmin)) || (rc = key_at(n - 1, &M.kmax))) { delete mo; dev_free(tmpbuf); return rc; }
+    if (M.kmax < M.kmin) std::swap(M.kmin, M.kmax);
+    M.inv_range = M.kmax > M.kmin ? 1.0 / (double)(M.kmax - M.kmin) : 0.0;
+    M.log_scale = M.kmax > M.kmin ? 1.0 / std::log2((double)(M.kmax - M.kmin) + 1.0) : 0.0;
+    const int64_t S = std::max<int64_t>(16, std::min<int64_t>(nsample > 0 ? nsample : (1 << 20), n));
+    M.nsample = S;
+    M.L = (int)std::max<int64_t>(1, std::min<int64_t>(32768, S / 8));
+    M.linear_root = (flags & MPSORT_GPU_MODEL_LINEAR_ROOT) ? 1 : 0;
+
+    const int grad_blocks = mlp_grad_blocks(S, 1, 0);
+    M.raw = (float*)dev_alloc(sizeof(float) * kMlpParams);
+    M.eff = (float*)dev_alloc(sizeof(float) * kMlpParams);
+    M.leaves = (float*)dev_alloc(sizeof(float) * 4 * (size_t)M.L);
+    float* am = (float*)dev_alloc(sizeof(float) * kMlpParams);
+    float* av = (float*)dev_alloc(sizeof(float) * kMlpParams);
+    float* partial = (float*)dev_alloc(sizeof(float) * kMlpParams * (size_t)grad_blocks);
+    float* lossp = (float*)dev_alloc(sizeof(float) * (size_t)grad_blocks);
+    uint64_t* skeys = (uint64_t*)dev_alloc(sizeof(uint64_t) * S);
+    float* x = (float*)dev_alloc(sizeof(float) * S);
+    float* y = (float*)dev_alloc(sizeof(float) * S);
+    int* leaf = (int*)dev_alloc(sizeof(int) * S);
+    int64_t* start = (int64_t*)dev_alloc(sizeof(int64_t) * (M.L + 1));
+    float* fval = (float*)dev_alloc(sizeof(float) * S);
+    auto cleanup = [&]() {
+        dev_free(am); dev_free(av); dev_free(partial); dev_free(lossp); dev_free(skeys); dev_free(x); dev_free(y);
+        dev_free(leaf); dev_free(start); dev_free(fval); dev_free(tmpbuf);
+    };
+    if (!M.raw || !M.eff || !M.leaves || !am || !av || !partial || !lossp || !skeys || !x || !y || !leaf || !start ||
+        !fval) {
+        cleanup();
+        mpsort_gpu_model_free(mo);
+        return fail(MPSORT_ERR_NOMEM, "device allocation failed (model)");
+    }
+    auto run = [&]() -> int {
+        int r2 = launch_model_sample(d_sorted, (int64_t)elsize, key->offset, (int)kb, xm, n, S, seed, M.dev(),
+                                     skeys, x, y, st);
+        if (r2) return r2;
+        std::vector<uint64_t> hk(S);
+        MPSG_CHECK(hipMemcpyAsync(hk.data(), skeys, sizeof(uint64_t) * S, hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipStreamSynchronize(st));
+        // root input: linear or log-scaled key, whichever spreads the samples
+        // more evenly over L equal cells of x (fewer samples in the fullest)
+        {
+            std::vector<int32_t> cl(M.L), cg(M.L);
+            for (int64_t i = 0; i < S; i++) {
+                const double d = (double)(hk[i] - M.kmin);
+                const double xl = std::min(1.0, d * M.inv_range), xg = std::min(1.0, std::log2(d + 1.0) * M.log_scale);
+                cl[std::min<int64_t>(M.L - 1, (int64_t)(xl * M.L))]++;
+                cg[std::min<int64_t>(M.L - 1, (int64_t)(xg * M.L))]++;
+            }
+            const int32_t ml = *std::max_element(cl.begin(), cl.end()), mg = *std::max_element(cg.begin(), cg.end());
+            if (mg < ml) {
+                M.log_input = 1;
+                if ((r2 = launch_model_sample(d_sorted, (int64_t)elsize, key->offset, (int)kb, xm, n, S, seed,
+                                              M.dev(), skeys, x, y, st)))
+                    return r2;
+            }
+        }
+        // heavy keys: runs of at least max(4, S/4096) equal sampled keys
+        const int64_t hthr = std::max<int64_t>(4, S / 4096);
+        std::vector<std::pair<int64_t, uint64_t>> runs;
+        for (int64_t i = 0; i < S;) {
+            int64_t j = i;
+            while (j < S && hk[j] == hk[i]) j++;
+            if (j - i >= hthr) runs.push_back({j - i, hk[i]});
+            i = j;
+        }
+        std::sort(runs.begin(), runs.end(), [](const std::pair<int64_t, uint64_t>& a, const std::pair<int64_t, uint64_t>& b) { return a.first > b.first; });
+        if (runs.size() > 256) runs.resize(256);
+        std::vector<uint64_t> hv;
+        for (auto& r : runs) hv.push_back(r.second);
+        std::sort(hv.begin(), hv.end());
+        M.nheavy = (int)hv.size();
+        M.heavy = (uint64_t*)dev_alloc(sizeof(uint64_t) * (hv.size() + 1));
+        if (!M.heavy) return fail(MPSORT_ERR_NOMEM, "device allocation failed (heavy keys)");
+        if (!hv.empty()) MPSG_CHECK(hipMemcpy(M.heavy, hv.data(), sizeof(uint64_t) * hv.size(), hipMemcpyHostToDevice));
+        // the root network
+        float init[kMlpParams];
+        {
+            std::vector<float> hx(S);
+            MPSG_CHECK(hipMemcpyAsync(hx.data(), x, sizeof(float) * S, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipStreamSynchronize(st));
+            mlp_init(init, seed, hx.data(), S);
+        }
+        MPSG_CHECK(hipMemcpy(M.raw, init, sizeof init, hipMemcpyHostToDevice));
+        MPSG_CHECK(hipMemset(am, 0, sizeof(float) * kMlpParams));
+        MPSG_CHECK(hipMemset(av, 0, sizeof(float) * kMlpParams));
+        if (!M.linear_root) {
+            // Adam, 2000 steps over 16 interleaved mini-batches, learning rate
+            // cosine-annealed from 1e-2 to 1e-4
+            const int stride = 16, steps = 2000;
+            for (int step = 1; step <= steps; step++) {
+                const int off = (step - 1) % stride;
+                const float lr = 1e-4f + 0.5f * (1e-2f - 1e-4f) * (1.f + std::cos(3.14159265f * step / steps));
+                if ((r2 = launch_mlp_effective(M.raw, M.eff, st))) return r2;
+                if ((r2 = launch_mlp_grad(x, y, S, stride, off, M.raw, M.eff, partial, lossp, st))) return r2;
+                if ((r2 = launch_mlp_adam(M.raw, partial, mlp_grad_blocks(S, stride, off), am, av, lr, step, nullptr,
+                                          st)))
+                    return r2;
+            }
+            M.steps = steps;
+            // loss of the final root over every sample
+            if ((r2 = launch_mlp_effective(M.raw, M.eff, st))) return r2;
+            if ((r2 = launch_mlp_grad(x, y, S, 1, 0, M.raw, M.eff, partial, lossp, st))) return r2;
+            std::vector<float> lp(grad_blocks);
+            MPSG_CHECK(hipMemcpyAsync(lp.data(), lossp, sizeof(float) * grad_blocks, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipStreamSynchronize(st));
+            double L2 = 0;
+            for (float v : lp) L2 += v;
+            M.loss = (float)L2;
+        } else {
+            // the linear root's mean squared error (r = x)
+            std::vector<float> hx(S), hy(S);
+            MPSG_CHECK(hipMemcpyAsync(hx.data(), x, sizeof(float) * S, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipMemcpyAsync(hy.data(), y, sizeof(float) * S, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipStreamSynchronize(st));
+            double L2 = 0;
+            for (int64_t i = 0; i < S; i++) L2 += ((double)hx[i] - hy[i]) * ((double)hx[i] - hy[i]);
+            M.loss = (float)(L2 / (double)S);
+        }
+        if ((r2 = launch_mlp_effective(M.raw, M.eff, st))) return r2;
+        if ((r2 = launch_leaves(x, y, S, M.eff, M.L, M.linear_root, leaf, start, M.leaves, st))) return r2;
+        std::vector<int64_t> hs(M.L + 1);
+        MPSG_CHECK(hipMemcpyAsync(hs.data(), start, sizeof(int64_t) * (M.L + 1), hipMemcpyDeviceToHost, st));
+        // error bound over the samples
+        if ((r2 = launch_model_eval(M.dev(), skeys, S, fval, st))) return r2;
+        std::vector<float> fv(S), yv(S);
+        MPSG_CHECK(hipMemcpyAsync(fv.data(), fval, sizeof(float) * S, hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipMemcpyAsync(yv.data(), y, sizeof(float) * S, hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipStreamSynchronize(st));
+        float e = 0.f;
+        for (int64_t i = 0; i < S; i++) e = std::max(e, std::fabs(fv[i] - yv[i]));
+        M.err_max = e;
+        int64_t ml = 0;
+        for (int l = 0; l < M.L; l++) ml = std::max<int64_t>(ml, hs[l + 1] - hs[l]);
+        M.max_leaf = (int)ml;
+        return 0;
+    };
+    rc = run();
+    cleanup();
+    if (rc) {
+        mpsort_gpu_model_free(mo);
+        return rc;
+    }
+    M.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = mo;
+    return 0;
+}
+
+int mpsort_gpu_model_free(mpsort_gpu_model* mo) {
+    if (!mo) return 0;
+    dev_free(mo->m.raw);
+    dev_free(mo->m.eff);
+    dev_free(mo->m.leaves);
+    dev_free(mo->m.heavy);
+    delete mo;
+    return 0;
+}
+
+int mpsort_gpu_model_get_info(const mpsort_gpu_model* mo, mpsort_gpu_model_info* info) {
+    if (!mo || !info) return fail(MPSORT_ERR_ARG, "NULL argument");
+    const LearnedModel& M = mo->m;
+    memset(info, 0, sizeof *info);
+    info->nsample = M.nsample;
+    info->leaves = M.L;
+    info->linear_root = M.linear_root;
+    info->steps = M.steps;
+    info->nheavy = M.nheavy;
+    info->max_leaf_samples = M.max_leaf;
+    info->log_input = M.log_input;
+    info->train_ms = M.train_ms;
+    info->loss = M.loss;
+    info->err_max = M.err_max;
+    info->kmin = M.kmin;
+    info->kmax = M.kmax;
+    return 0;
+}
+
+int mpsort_gpu_model_eval(const mpsort_gpu_model* mo, const uint64_t* keys, int64_t n, float* out) {
+    if (!mo || (n > 0 && (!keys || !out))) return fail(MPSORT_ERR_ARG, "NULL argument");
+    if (n <= 0) return 0;
+    DevCtx* c = nullptr;
+    int rc = get_ctx(&c);
+    if (rc) return rc;
+    uint64_t* dk = (uint64_t*)dev_alloc(sizeof(uint64_t) * n);
+    float* dv = (float*)dev_alloc(sizeof(float) * n);
+    if (!dk || !dv) { dev_free(dk); dev_free(dv); return fail(MPSORT_ERR_NOMEM, "device allocation failed"); }
+    rc = 0;
+    if (hipMemcpy(dk, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice) != hipSuccess) rc = MPSORT_ERR_HIP;
+    if (!rc) rc = launch_model_eval(mo->m.dev(), dk, n, dv, c->st);
+    if (!rc && hipStreamSynchronize(c->st) != hipSuccess) rc = MPSORT_ERR_HIP;
+    if (!rc && hipMemcpy(out, dv, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess) rc = MPSORT_ERR_HIP;
+    dev_free(dk);
+    dev_free(dv);
+    return rc ? fail(rc, "model evaluation failed") : 0;
+}
+
+int mpsort_gpu_model_params(const mpsort_gpu_model* mo, float* raw, float* leaves) {
+    if (!mo) return fail(MPSORT_ERR_ARG, "NULL model");
+    if (raw) MPSG_CHECK(hipMemcpy(raw, mo->m.raw, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    if (leaves) MPSG_CHECK(hipMemcpy(leaves, mo->m.leaves, sizeof(float) * 4 * (size_t)mo->m.L, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mpsort_gpu_mlp_grad(const float* raw, const float* x, const float* y, int64_t n, float* grad, float* loss) {
+    if (!raw || !x || !y || !grad || n <= 0) return fail(MPSORT_ERR_ARG, "bad arguments");
+    DevCtx* c = nullptr;
+    int rc = get_ctx(&c);
+    if (rc) return rc;
+    const int blocks = mlp_grad_blocks(n, 1, 0);
+    float* d_raw = (float*)dev_alloc(sizeof(float) * kMlpParams);
+    float* d_eff = (float*)dev_alloc(sizeof(float) * kMlpParams);
+    float* d_x = (float*)dev_alloc(sizeof(float) * n);
+    float* d_y = (float*)dev_alloc(sizeof(float) * n);
+    float* d_part = (float*)dev_alloc(sizeof(float) * kMlpParams * (size_t)blocks);
+    float* d_loss = (float*)dev_alloc(sizeof(float) * blocks);
+    float* d_g = (float*)dev_alloc(sizeof(float) * kMlpParams);
+    auto done = [&](int r) {
+        dev_free(d_raw); dev_free(d_eff); dev_free(d_x); dev_free(d_y); dev_free(d_part); dev_free(d_loss);
+        dev_free(d_g);
+        return r;
+    };
+    if (!d_raw || !d_eff || !d_x || !d_y || !d_part || !d_loss || !d_g) return done(fail(MPSORT_ERR_NOMEM, "alloc"));
+    if (hipMemcpy(d_raw, raw, sizeof(float) * kMlpParams, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_x, x, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_y, y, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return done(fail(MPSORT_ERR_HIP, "copy failed"));
+    if ((rc = launch_mlp_effective(d_raw, d_eff, c->st))) return done(rc);
+    if ((rc = launch_mlp_grad(d_x, d_y, n, 1, 0, d_raw, d_eff, d_part, d_loss, c->st))) return done(rc);
+    if ((rc = launch_mlp_adam(d_raw, d_part, blocks, nullptr, nullptr, 0.f, 1, d_g, c->st))) return done(rc);
+    std::vector<float> lp(blocks);
+    if (hipStreamSynchronize(c->st) != hipSuccess ||
+        hipMemcpy(grad, d_g, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(lp.data(), d_loss, sizeof(float) * blocks, hipMemcpyDeviceToHost) != hipSuccess)
+        return done(fail(MPSORT_ERR_HIP, "copy failed"));
+    if (loss) {
+        double a = 0;
+        for (float v : lp) a += v;
+        *loss = (float)a;
+    }
+    return done(0);
+}
+
+int mpsort_gpu_newarray_learned(const void* base, const int64_t* nin, int P, void* out, const int64_t* nout,
+                                size_t elsize, const mpsort_key_desc* key, const mpsort_gpu_model* model,
+                                unsigned flags) {
+    g_err[0] = 0;
+    if (!model) return fail(MPSORT_ERR_ARG, "model is NULL");
+    int64_t ntot = 0;
+    int rc = validate_counts(nin, P, nout, &ntot);
+    if (rc) return rc;
+    CallShape cs;
+    memset(&cs, 0, sizeof cs);
+    const bool devptr = (flags & MPSORT_GPU_DEVICE_PTRS) != 0;
+    rc = shape_for(key, elsize, flags, devptr ? base : nullptr, devptr ? out : nullptr, &cs);
+    if (rc) return rc;
+    const int64_t kb = (int64_t)key->width * key->nwords;
+    if (kb > 8 || kb != model->m.key_bytes)
+        return fail(MPSORT_ERR_UNSUPPORTED, "the model was trained for %d-byte keys; this key has %lld bytes",
+                    model->m.key_bytes, (long long)kb);
+    if (cs.rw != 2 && cs.rw != 4) {
+        // the bucket kernels take 8- and 16-byte sort records: pack {key, index}
+        shape_for(key, elsize, flags | MPSORT_GPU_FORCE_GENERAL, nullptr, nullptr, &cs);
+    }
+    cs.P = P;
+    cs.ntot = ntot;
+    if (cs.general && ntot >= (int64_t)1 << 32)
+        return fail(MPSORT_ERR_UNSUPPORTED, "packed path holds at most 2^32-1 records");
+    DevCtx* c = nullptr;
+    if ((rc = get_ctx(&c))) return rc;
+    mpsort_gpu_stats st;
+    memset(&st, 0, sizeof st);
+    if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");
+    const void* d_in = base;
+    void* d_out = out;
+    const size_t bytes = (size_t)ntot * elsize;
+    hipEvent_t e0 = c->ev[kEvStart], e1 = c->ev[kEvEnd];
+    if (!devptr && ntot > 0) {
+        if ((rc = c->din.ensure(bytes))) return rc;
+        if ((rc = c->dout.ensure(bytes))) return rc;
+        MPSG_CHECK(hipEventRecord(e0, c->st));
+        MPSG_CHECK(hipMemcpyAsync(c->din.p, base, bytes, hipMemcpyHostToDevice, c->st));
+        MPSG_CHECK(hipEventRecord(e1, c->st));
+        MPSG_CHECK(hipStreamSynchronize(c->st));
+        st.t_h2d_ms = ev_ms(e0, e1);
+        d_in = c->din.p;
+        d_out = c->dout.p;
+    }
+    remember_key(*c, key, 0, 0);
+    if (ntot > 0) {
+        rc = run_histsort_learned(*c, cs, d_in, d_out, nin, nout, model->m, &st);
+    } else {
+        st.nvranks = P;
+        c->last_P = 0;
+    }
+    if (!rc && !devptr && ntot > 0) {
+        MPSG_CHECK(hipEventRecord(e0, c->st));
+        MPSG_CHECK(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, c->st));
+        MPSG_CHECK(hipEventRecord(e1, c->st));
+        MPSG_CHECK(hipStreamSynchronize(c->st));
+        st.t_d2h_ms = ev_ms(e0, e1);
+    }
+    g_last = st;
+    return rc;
 }
 
 }  // extern "C"

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
     const int64_t* __restrict__ scnt, int P, const int64_t* __restrict__ C,
     int64_t* __restrict__ win, int64_t* __restrict__ myclt, int64_t* __restrict__ mycle,
     uint64_t* __restrict__ split_out, uint64_t* __restrict__ pminmax, int* __restrict__ rounds_out,
-    uint32_t* __restrict__ error_flag) {
+    uint32_t* __restrict__ error_flag, const uint64_t* __restrict__ init, int* __restrict__ exact_out) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * kWaves + wave;  // splitter index
     if (i >= P - 1) return;                   // uniform per wave
@@ -139,10 +139,58 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
         wi[4 * j + 0] = 0;
         wi[4 * j + 1] = scnt[j];
     }
+    // Learned proposals (single-word keys): init[3i..3i+2] = {lo, proposal,
+    // hi}.  The bracket side that exact counts confirm replaces Pmin / Pmax
+    // (target k[C-1] >= lo iff #{< lo} < C; <= hi iff #{<= hi} >= C), and the
+    // proposal is the first value tested.  Bisection then continues with the
+    // reference's rules inside the bracket.
+    bool test_proposal = false;
+    Key prop = mn;
+    if (init && nw == 1) {
+        Key lo = {{init[3 * i], 0, 0, 0}}, hi = {{init[3 * i + 2], 0, 0, 0}};
+        prop = Key{{init[3 * i + 1], 0, 0, 0}};
+        if (key_cmp(lo, mn, nw) < 0) lo = mn;
+        if (key_cmp(hi, mx, nw) > 0) hi = mx;
+        long long clt_lo = 0, cle_hi = 0;
+        for (int j = lane; j < P; j += 64) {
+            int64_t a = 0, b = scnt[j];
+            const int64_t base = sbeg[j];
+            while (a < b) {
+                const int64_t m = a + ((b - a) >> 1);
+                if (key_cmp(load_key(recs, rw, base + m, ks), lo, nw) < 0) a = m + 1; else b = m;
+            }
+            const int64_t lt = a;
+            a = 0;
+            b = scnt[j];
+            while (a < b) {
+                const int64_t m = a + ((b - a) >> 1);
+                if (key_cmp(load_key(recs, rw, base + m, ks), hi, nw) <= 0) a = m + 1; else b = m;
+            }
+            wi[4 * j + 2] = lt;
+            wi[4 * j + 3] = a;
+            clt_lo += lt;
+            cle_hi += a;
+        }
+        clt_lo = wave_sum_ll(clt_lo);
+        cle_hi = wave_sum_ll(cle_hi);
+        const bool lo_ok = clt_lo < Ci, hi_ok = cle_hi >= Ci;
+        if (lo_ok) L = lo;
+        if (hi_ok) R = hi;
+        for (int j = lane; j < P; j += 64) {
+            if (lo_ok) wi[4 * j + 0] = wi[4 * j + 2];
+            if (hi_ok) wi[4 * j + 1] = wi[4 * j + 3];
+        }
+        rounds++;
+        test_proposal = key_cmp(prop, L, nw) >= 0 && key_cmp(prop, R, nw) <= 0;
+    }
+    int exact = 0;
     while (!stable) {
         rounds++;
         // piter_bisect (internal-parallel.h:168-198)
-        if (narrow) {
+        if (test_proposal) {
+            Pv = prop;
+            test_proposal = false;
+        } else if (narrow) {
             Pv = R;
             stable = true;
         } else {
@@ -175,6 +223,7 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
         sle = wave_sum_ll(sle);
         // piter_accept (internal-parallel.h:224-248)
         if (slt < Ci && Ci <= sle) {
+            if (init && key_cmp(Pv, prop, nw) == 0) exact = 1;
             stable = true;
         } else if (!stable) {
             const bool too_big = slt >= Ci;
@@ -196,6 +245,7 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
     if (lane == 0) {
         for (int w = 0; w < 4; w++) split_out[(int64_t)i * 4 + w] = Pv.w[w];
         rounds_out[i] = rounds;
+        if (exact_out) exact_out[i] = exact;
     }
 }
 
@@ -457,11 +507,11 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const uint8_t* __restric
 int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sbeg,
                   const int64_t* scnt, int P, const int64_t* C, int64_t* win, int64_t* myclt,
                   int64_t* mycle, uint64_t* split, uint64_t* pminmax, int* rounds, uint32_t* err,
-                  hipStream_t st) {
+                  hipStream_t st, const uint64_t* init, int* exact) {
     if (P < 2) return 0;
     const int blocks = (P - 1 + kWaves - 1) / kWaves;
     hipLaunchKernelGGL(bisect_kernel, dim3(blocks), dim3(kBlock), 0, st, (const uint32_t*)recs, rw,
-                       ks, sbeg, scnt, P, C, win, myclt, mycle, split, pminmax, rounds, err);
+                       ks, sbeg, scnt, P, C, win, myclt, mycle, split, pminmax, rounds, err, init, exact);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }

@@ -112,6 +112,7 @@ struct SegSortPlan {
     uint64_t* d_lookback;    // ntiles * 256
     uint32_t* d_tickets;     // kMaxPasses
     long long* d_checksum;   // [0] in, [1] out
+    bool sparse = false;     // segments do not tile [0, nrec): copies go per tile
 };
 
 inline PassTable make_pass_table(const KeySpec& ks) {

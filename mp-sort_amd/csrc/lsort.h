@@ -1,0 +1,49 @@
+// lsort.h -- learned local sort (lsort.hip) declarations.
+#pragma once
+#include "internal.h"
+
+namespace mpsg {
+
+constexpr int kLBins = 1024;    // learned buckets per segment (10-bit digit)
+constexpr int kLBounds = 1023;  // boundaries per segment table -> buckets 0..1023
+constexpr int kChunkCap = 8192; // largest bucket sorted in LDS
+
+// key of a sort record as one uint64 (keys of at most 8 bytes): bytes
+// [sh/8, sh/8 + key_bytes) of words w0..w0+2, masked, XOR-ed with the sign
+// masks
+struct LKey {
+    int32_t w0;
+    int32_t sh;
+    uint64_t mask;
+    uint64_t xmask;
+};
+
+inline LKey make_lkey(const KeySpec& ks) {
+    LKey k;
+    k.w0 = ks.key_off / 4;
+    k.sh = (ks.key_off % 4) * 8;
+    k.mask = ks.key_bytes >= 8 ? ~0ull : ((1ull << (8 * ks.key_bytes)) - 1);
+    k.xmask = 0;
+    for (int b = 0; b < ks.key_bytes && b < 8; b++) k.xmask |= (uint64_t)ks.xmask[b] << (8 * b);
+    return k;
+}
+
+int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
+                 const uint64_t* tables, uint32_t* hist, long long* checksum, hipStream_t st);
+int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st);
+int lscatter_tile(int rw);
+// records of single-value (odd) buckets go to eq_dst, the rest to dst
+int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegDev& sd, int64_t ntiles,
+                    const LKey& lk, const uint64_t* tables, const int64_t* bucket_base, uint64_t* lookback,
+                    uint32_t* ticket, uint32_t epoch, uint32_t* err, hipStream_t st);
+// Sorts every bucket (src -> dst, same slots).  eq_done: the scatter already
+// wrote the single-value buckets to dst.  Buckets above kChunkCap go to the
+// overflow list (*noverflow entries of {base, count}); hard (one uint32 per
+// bucket) and *nhard (zeroed by the caller) list the buckets handed from the
+// interpolation sort to the LSD sort.
+int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
+                      const int64_t* bucket_base, const uint64_t* tables, int64_t nbuckets, int64_t* overflow,
+                      uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, long long* checksum,
+                      bool eq_done, hipStream_t st);
+
+}  // namespace mpsg

@@ -1,0 +1,725 @@
+// lsort.hip -- learned local sort: the model-bucketed replacement of the
+// upper radix passes of the local sorts (FirstSort mpsort-mpi.c:369,
+// SecondSort :597, both radix_sort radixsort.c:35-44, a stable sort).
+//
+//   lhist_kernel     per-segment counts of the learned bucket of every record
+//   lscatter_kernel  one stable onesweep pass whose 10-bit "digit" is the
+//                    learned bucket (same ranking / look-back machinery as the
+//                    radix passes in onesweep.hip, 1024 bins)
+//   interp_sort_kernel one workgroup per bucket of at most kChunkCap records:
+//                    records grouped into 4096 bins interpolated between the
+//                    bucket's smallest and largest key, the rank of every
+//                    record inside its bin, and a direct write to the
+//                    bucket's final slots
+//   lsd_sort_kernel  the buckets whose keys crowd one bin: stable LSD
+//                    radix sort in LDS on the key bits that vary inside the
+//                    bucket
+// Buckets larger than kChunkCap are listed and sorted afterwards by the plain
+// radix path, so the result is the stable sort whatever the model.
+//
+// Bucket of a key against a segment's sorted boundary table t[0..1023):
+//   bucket = #{t[i] < key}, i.e. bucket b holds the keys in (t[b-1], t[b]]
+// -- non-decreasing in the key.  A bucket with t[b-1] + 1 == t[b] (or b == 0
+// with t[0] == 0) holds the single key value t[b]: duplicate-heavy keys get
+// such a bucket of their own, and its records are final after the scatter.
+#include <algorithm>
+
+#include "device.h"
+#include "lsort.h"
+
+namespace mpsg {
+
+template <int RW>
+__device__ __forceinline__ uint64_t lkey(const uint32_t (&r)[RW], const LKey& k) {
+    const uint32_t a = pick_word<RW>(r, k.w0), b = pick_word<RW>(r, k.w0 + 1),
+                   c = pick_word<RW>(r, k.w0 + 2);
+    const uint64_t lo = (uint64_t)a | ((uint64_t)b << 32);
+    const uint64_t v = k.sh ? ((lo >> k.sh) | ((uint64_t)c << (64 - k.sh))) : lo;
+    return (v & k.mask) ^ k.xmask;
+}
+
+__device__ __forceinline__ uint32_t bucket_of(uint64_t key, const uint64_t* __restrict__ t) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 512; step > 0; step >>= 1)
+        if (lo + step <= kLBounds && t[lo + step - 1] < key) lo += step;
+    return (uint32_t)lo;
+}
+
+// bucket b of table t holds one key value only
+__device__ __forceinline__ bool single_bucket(const uint64_t* __restrict__ t, int b) {
+    if (b == 0) return t[0] == 0;
+    if (b >= kLBounds) return false;
+    return t[b - 1] + 1 == t[b];
+}
+
+// ---------------------------------------------------------------------------
+template <int RW>
+__global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restrict__ src, SegDev seg,
+                                                       int64_t ntiles, int64_t tpb, int tile_recs, LKey lk,
+                                                       const uint64_t* __restrict__ tables,
+                                                       uint32_t* __restrict__ hist,
+                                                       long long* __restrict__ checksum) {
+    __shared__ uint32_t s_hist[kLBins];
+    __shared__ uint64_t s_tab[kLBounds];
+    __shared__ long long s_sum[kWaves];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kLBins; i += kBlock) s_hist[i] = 0;
+    const int64_t t0 = (int64_t)blockIdx.x * tpb;
+    const int64_t t1 = min(ntiles, t0 + tpb);
+    long long csum = 0;
+    int cur = -1;
+    for (int64_t tile = t0; tile < t1; tile++) {
+        const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
+        if (s != cur) {
+            __syncthreads();
+            if (cur >= 0) {
+                uint32_t* h = hist + (int64_t)cur * kLBins;
+                for (int i = tid; i < kLBins; i += kBlock) {
+                    const uint32_t c = s_hist[i];
+                    if (c) atomicAdd(&h[i], c);
+                    s_hist[i] = 0;
+                }
+            }
+            for (int i = tid; i < kLBounds; i += kBlock) s_tab[i] = tables[(int64_t)s * kLBounds + i];
+            __syncthreads();
+            cur = s;
+        }
+        const int64_t t = tile - seg.tile_start[s];
+        const int64_t sbeg = seg.begin[s];
+        const int64_t begin = sbeg + t * tile_recs;
+        const int64_t end = min(sbeg + seg.count[s], begin + (int64_t)tile_recs);
+        for (int64_t i = begin + tid; i < end; i += kBlock) {
+            uint32_t r[RW];
+            RecIO<RW>::load(src, i, r);
+            if (checksum) {
+#pragma unroll
+                for (int w = 0; w < RW; w++) csum += sbyte_sum(r[w]);
+            }
+            atomicAdd(&s_hist[bucket_of(lkey<RW>(r, lk), s_tab)], 1u);
+        }
+    }
+    __syncthreads();
+    if (cur >= 0) {
+        uint32_t* h = hist + (int64_t)cur * kLBins;
+        for (int i = tid; i < kLBins; i += kBlock) {
+            const uint32_t c = s_hist[i];
+            if (c) atomicAdd(&h[i], c);
+        }
+    }
+    if (checksum) {
+        csum = wave_sum_ll(csum);
+        if ((tid & 63) == 0) s_sum[tid >> 6] = csum;
+        __syncthreads();
+        if (tid == 0) {
+            long long a = 0;
+            for (int w = 0; w < kWaves; w++) a += s_sum[w];
+            atomicAdd((unsigned long long*)checksum, (unsigned long long)a);
+        }
+    }
+}
+
+// exclusive scan of 1024 values, one per thread of a 1024-thread block
+__device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* s_tmp /*16*/) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+#pragma unroll
+    for (int w = 0; w < 16; w++) pre += (w < wave) ? s_tmp[w] : 0u;
+    __syncthreads();
+    return pre + x - v;
+}
+
+// per segment: absolute base of every bucket
+__global__ __launch_bounds__(1024) void lscan_kernel(const uint32_t* __restrict__ hist,
+                                                     const int64_t* __restrict__ seg_begin,
+                                                     int64_t* __restrict__ base) {
+    __shared__ uint32_t s_tmp[16];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const uint32_t c = hist[(int64_t)s * kLBins + tid];
+    const uint32_t ex = scan1024(c, s_tmp);
+    base[(int64_t)s * kLBins + tid] = seg_begin[s] + (int64_t)ex;
+}
+
+// ---------------------------------------------------------------------------
+// the bucket scatter: a onesweep pass with 1024 bins
+template <int RW, int ITEMS>
+__global__ __launch_bounds__(1024) void lscatter_kernel(
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg, LKey lk,
+    const uint64_t* __restrict__ tables, const int64_t* __restrict__ bucket_base,
+    uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket, uint32_t epoch,
+    uint32_t* __restrict__ error_flag, uint32_t* __restrict__ eq_dst) {
+    constexpr int BLOCK = 1024, NW = 16;
+    constexpr int TILE = BLOCK * ITEMS;
+    constexpr int WAVE_TILE = 64 * ITEMS;
+    __shared__ __attribute__((aligned(16))) uint32_t s_rec[TILE * RW];
+    __shared__ uint16_t s_dig[TILE];
+    __shared__ uint16_t s_whist[NW][kLBins];
+    __shared__ uint32_t s_excl[kLBins];
+    __shared__ int64_t s_gdelta[kLBins];
+    __shared__ uint64_t s_tab[kLBounds];
+    __shared__ uint32_t s_single[kLBins / 32];
+    __shared__ int64_t s_tile;
+    __shared__ uint32_t s_tmp[16];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid == 0) s_tile = (int64_t)atomicAdd(ticket, 1u);
+    for (int i = tid; i < NW * kLBins; i += BLOCK) (&s_whist[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
+    for (int i = tid; i < kLBounds; i += BLOCK) s_tab[i] = tables[(int64_t)s * kLBounds + i];
+    const int64_t t = tile - seg.tile_start[s];
+    const int64_t sbeg = seg.begin[s];
+    const int64_t send = sbeg + seg.count[s];
+    const int64_t begin = sbeg + t * TILE;
+    const int64_t end = min(send, begin + (int64_t)TILE);
+    const int count = (int)(end - begin);
+
+    uint32_t r[ITEMS][RW];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const int64_t i = begin + wave * WAVE_TILE + k * 64 + lane;
+        if (i < end) RecIO<RW>::load(src, i, r[k]);
+    }
+    __syncthreads();  // table in LDS
+    {
+        const uint64_t m = __ballot(eq_dst != dst && single_bucket(s_tab, tid));
+        if (lane == 0) {
+            s_single[2 * wave] = (uint32_t)m;
+            s_single[2 * wave + 1] = (uint32_t)(m >> 32);
+        }
+    }
+
+    uint32_t dl[ITEMS];  // bucket | wave rank << 16
+    volatile uint16_t* wh = s_whist[wave];
+    const uint64_t lt_mask = __lanemask_lt();
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const int64_t i = begin + wave * WAVE_TILE + k * 64 + lane;
+        const bool valid = i < end;
+        const uint32_t d = valid ? bucket_of(lkey<RW>(r[k], lk), s_tab) : (uint32_t)kLBins;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 10; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = __popcll(peers & lt_mask);
+        const uint32_t cnt = __popcll(peers);
+        uint32_t base = 0;
+        if (valid) base = wh[d];
+        dl[k] = d | ((base + rank) << 16);
+        if (valid && rank == 0) wh[d] = (uint16_t)(base + cnt);
+    }
+    __syncthreads();
+
+    const int d = tid;  // one bucket per thread
+    uint32_t total = 0;
+    {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const uint32_t c = s_whist[w][d];
+            s_whist[w][d] = (uint16_t)acc;
+            acc += c;
+        }
+        total = acc;
+    }
+    const uint32_t excl = scan1024(total, s_tmp);
+    s_excl[d] = excl;
+    int64_t prefix = 0;
+    uint64_t* my = lookback + tile * kLBins + d;
+    if (t == 0) {
+        store_agent(my, make_flag(epoch, 1, total));
+    } else {
+        store_agent(my, make_flag(epoch, 0, total));
+        const int64_t first = seg.tile_start[s];
+        int64_t j = tile - 1;
+        uint32_t spins = 0;
+        while (true) {
+            uint64_t f[2];
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+                f[q] = (j - q >= first) ? load_agent(lookback + (j - q) * kLBins + d) : 0ull;
+            int consumed = 0;
+            bool stop = false, done = false;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                if (!stop && j - q >= first) {
+                    if ((uint32_t)(f[q] >> 41) != epoch) {
+                        stop = true;
+                    } else {
+                        prefix += (int64_t)(f[q] & kFlagValueMask);
+                        consumed = q + 1;
+                        if ((f[q] >> 40) & 1ull) { done = true; stop = true; }
+                    }
+                }
+            }
+            if (done) break;
+            j -= consumed;
+            if (consumed == 0) {
+                if (++spins > kSpinLimit) {
+                    atomicOr(error_flag, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        store_agent(my, make_flag(epoch, 1, (uint64_t)(prefix + total)));
+    }
+    s_gdelta[d] = bucket_base[(int64_t)s * kLBins + d] + prefix - (int64_t)excl;
+    __syncthreads();
+
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t dd = dl[k] & 0xffffu;
+        if (dd < kLBins) {
+            const uint32_t pos = s_excl[dd] + s_whist[wave][dd] + (dl[k] >> 16);
+#pragma unroll
+            for (int w = 0; w < RW; w++) s_rec[pos * RW + w] = r[k][w];
+            s_dig[pos] = (uint16_t)dd;
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const int i = k * BLOCK + tid;
+        if (i < count) {
+            uint32_t rr[RW];
+#pragma unroll
+            for (int w = 0; w < RW; w++) rr[w] = s_rec[i * RW + w];
+            const uint32_t dd = s_dig[i];
+            const int64_t o = s_gdelta[dd] + i;
+            if (o < sbeg || o >= send) {
+                atomicOr(error_flag, 4u);
+                continue;
+            }
+            // a bucket of one key value is final after the stable scatter:
+            // it goes straight to the sorted output
+            RecIO<RW>::store(((s_single[dd >> 5] >> (dd & 31)) & 1u) ? eq_dst : dst, o, rr);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// in-LDS sorts of the buckets.  Records of a bucket are numbered wave-major
+// (index = wave * 512 + item * 64 + lane) so that the per-wave counting passes
+// below keep that order among equal digits: both sorts are stable.
+constexpr int kCBlock = 1024, kCWaves = 16, kCItems = kChunkCap / kCBlock, kCWspan = kChunkCap / kCWaves;
+constexpr int kRunMax = 512;  // largest bin ranked directly
+
+template <int RW>
+__device__ __forceinline__ void load_bucket(const uint32_t* __restrict__ src, int64_t base, int count,
+                                            const LKey& lk, uint32_t (&r)[kCItems][RW],
+                                            uint64_t (&kk)[kCItems], uint64_t* s_red, uint64_t& mn,
+                                            uint64_t& mx) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    mn = ~0ull;
+    mx = 0;
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) {
+        const int i = wave * kCWspan + k * 64 + lane;
+        kk[k] = 0;
+        if (i < count) {
+            RecIO<RW>::load(src, base + i, r[k]);
+            kk[k] = lkey<RW>(r[k], lk);
+            mn = kk[k] < mn ? kk[k] : mn;
+            mx = kk[k] > mx ? kk[k] : mx;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t a = __shfl_xor(mn, off, 64), c = __shfl_xor(mx, off, 64);
+        mn = a < mn ? a : mn;
+        mx = c > mx ? c : mx;
+    }
+    if (lane == 0) { s_red[wave] = mn; s_red[kCWaves + wave] = mx; }
+    __syncthreads();
+    mn = s_red[0];
+    mx = s_red[kCWaves];
+#pragma unroll
+    for (int w = 1; w < kCWaves; w++) {
+        mn = s_red[w] < mn ? s_red[w] : mn;
+        mx = s_red[kCWaves + w] > mx ? s_red[kCWaves + w] : mx;
+    }
+}
+
+__device__ __forceinline__ void block_checksum_add(long long csum, long long* s_sum, long long* checksum) {
+    csum = wave_sum_ll(csum);
+    if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = csum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long a = 0;
+        for (int w = 0; w < kCWaves; w++) a += s_sum[w];
+        atomicAdd((unsigned long long*)checksum, (unsigned long long)a);
+    }
+}
+
+// One stable counting pass over 256 digits in LDS.  The records are listed
+// in s_in[0..count) (identity when s_in is null); digit(e) < 256 is the digit
+// of record e; the new order goes to s_out.  Positions are taken wave-major
+// (wave * kCWspan + item * 64 + lane), the order in which the per-wave
+// counters are laid out, so equal digits keep their order.
+template <class Digit>
+__device__ __forceinline__ void lds_count_pass(const uint16_t* s_in, uint16_t* s_out, int count, Digit digit,
+                                               uint16_t (*s_whist)[256], uint32_t* s_excl, uint32_t* s_tmp) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < kCWaves * 256; i += kCBlock) (&s_whist[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t lt_mask = __lanemask_lt();
+    volatile uint16_t* wh = s_whist[wave];
+    uint32_t dl[kCItems];
+    uint16_t el[kCItems];
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) {
+        const int pos = wave * kCWspan + k * 64 + lane;
+        const bool valid = pos < count;
+        const uint16_t e = valid ? (s_in ? s_in[pos] : (uint16_t)pos) : (uint16_t)0;
+        const uint32_t d = valid ? digit(e) : 256u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bb = 0; bb < 8; bb++) {
+            const bool bit = (d >> bb) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll(peers & lt_mask);
+        const uint32_t c = __popcll(peers);
+        uint32_t bs = 0;
+        if (valid) bs = wh[d];
+        dl[k] = d | ((bs + rank) << 16);
+        el[k] = e;
+        if (valid && rank == 0) wh[d] = (uint16_t)(bs + c);
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    if (tid < 256) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < kCWaves; w++) {
+            const uint32_t c = s_whist[w][tid];
+            s_whist[w][tid] = (uint16_t)acc;
+            acc += c;
+        }
+        tot = acc;
+    }
+    {
+        uint32_t x = tot;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63 && wave < 4) s_tmp[wave] = x;
+        __syncthreads();
+        uint32_t pre = 0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) pre += (w < wave) ? s_tmp[w] : 0u;
+        if (tid < 256) s_excl[tid] = pre + x - tot;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) {
+        const uint32_t d = dl[k] & 0xffffu;
+        if (d < 256u) s_out[s_excl[d] + s_whist[wave][d] + (dl[k] >> 16)] = el[k];
+    }
+    __syncthreads();
+}
+
+// Per bucket: single-value buckets already written (eq_done) only add their
+// checksum; oversized ones go to the overflow list; a bucket of one key value
+// is copied.  Otherwise every record gets the bin
+//   d = min(kFine-1, floor(float(key - min) * kFine / (max - min + 1)))
+// (non-decreasing in the key); records are grouped by bin with LDS atomics
+// (in any order inside a bin), and a record's final slot is its bin start +
+// #{records of the bin with a smaller key, or an equal key and a smaller
+// position}: the stable order.  Buckets with a bin above kRunMax records go
+// to the list for lsd_sort_kernel.
+constexpr int kFine = 4096;
+template <int RW>
+__global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
+    const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
+    const uint64_t* __restrict__ tables, int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow,
+    int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
+    long long* __restrict__ checksum_out, int eq_done) {
+    __shared__ uint64_t s_key[kChunkCap];
+    __shared__ uint32_t s_start[kFine];
+    __shared__ uint32_t s_fill[kFine];
+    __shared__ uint16_t s_ord[kChunkCap];
+    __shared__ uint16_t s_pos[kChunkCap];
+    __shared__ uint64_t s_red[2 * kCWaves];
+    __shared__ uint32_t s_tmp[16];
+    __shared__ long long s_sum[kCWaves];
+    __shared__ int s_long;
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t b = blockIdx.x;
+    const uint32_t cnt32 = hist[b];
+    if (cnt32 == 0) return;
+    const int64_t base = bucket_base[b];
+    if (eq_done && single_bucket(tables + (b / kLBins) * kLBounds, (int)(b % kLBins))) {
+        if (checksum_out) {
+            long long csum = 0;
+            for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += kCBlock) {
+                uint32_t rr[RW];
+                RecIO<RW>::load(dst, i, rr);
+#pragma unroll
+                for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
+            }
+            block_checksum_add(csum, s_sum, checksum_out);
+        }
+        return;
+    }
+    if (cnt32 > (uint32_t)kChunkCap) {
+        if (tid == 0) {
+            const uint32_t slot = atomicAdd(noverflow, 1u);
+            if ((int)slot < max_overflow) {
+                overflow[2 * slot] = base;
+                overflow[2 * slot + 1] = (int64_t)cnt32;
+            }
+        }
+        return;
+    }
+    const int count = (int)cnt32;
+    for (int i = tid; i < kFine; i += kCBlock) {
+        s_start[i] = 0;
+        s_fill[i] = 0;
+    }
+    if (tid == 0) s_long = 0;
+    uint32_t r[kCItems][RW];
+    uint64_t kk[kCItems];
+    uint64_t mn, mx;
+    load_bucket<RW>(src, base, count, lk, r, kk, s_red, mn, mx);  // ends with a barrier
+    long long csum = 0;
+    if (mn == mx) {
+        // one key value: the stable order is the current order
+#pragma unroll
+        for (int k = 0; k < kCItems; k++) {
+            const int i = wave * kCWspan + k * 64 + lane;
+            if (i < count) {
+                RecIO<RW>::store(dst, base + i, r[k]);
+                if (checksum_out) {
+#pragma unroll
+                    for (int w = 0; w < RW; w++) csum += sbyte_sum(r[k][w]);
+                }
+            }
+        }
+        if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
+        return;
+    }
+    const float scale = (float)kFine / ((float)(mx - mn) + 1.0f);
+    uint32_t dbin[kCItems];
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) {
+        const int i = wave * kCWspan + k * 64 + lane;
+        uint32_t d = (uint32_t)((float)(kk[k] - mn) * scale);
+        d = d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
+        dbin[k] = d;
+        if (i < count) {
+            s_key[i] = kk[k];
+            atomicAdd(&s_start[d], 1u);
+        }
+    }
+    __syncthreads();
+    {
+        // exclusive scan of the kFine counts, kFine / kCBlock per thread
+        constexpr int PER = kFine / kCBlock;
+        uint32_t c[PER], sum = 0;
+#pragma unroll
+        for (int j = 0; j < PER; j++) { c[j] = s_start[tid * PER + j]; sum += c[j]; }
+        uint32_t pre = scan1024(sum, s_tmp);
+#pragma unroll
+        for (int j = 0; j < PER; j++) { s_start[tid * PER + j] = pre; pre += c[j]; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) {
+        const int i = wave * kCWspan + k * 64 + lane;
+        if (i < count) s_ord[s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u)] = (uint16_t)i;
+    }
+    __syncthreads();
+    // final slots, one record per thread in bin order (threads of a wave
+    // mostly share a bin, so the loops below stay coherent)
+    for (int o = tid; o < count; o += kCBlock) {
+        const uint32_t i = s_ord[o];
+        const uint64_t key = s_key[i];
+        uint32_t d = (uint32_t)((float)(key - mn) * scale);
+        d = d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
+        const uint32_t s0 = s_start[d], nb = s_fill[d];
+        if (nb == 1) {
+            s_pos[i] = (uint16_t)s0;
+        } else if (nb > (uint32_t)kRunMax) {
+            s_long = 1;
+        } else {
+            uint32_t rank = 0;
+            for (uint32_t j = s0; j < s0 + nb; j++) {
+                const uint32_t ij = s_ord[j];
+                const uint64_t kj = s_key[ij];
+                rank += (kj < key || (kj == key && ij < i)) ? 1u : 0u;
+            }
+            s_pos[i] = (uint16_t)(s0 + rank);
+        }
+    }
+    __syncthreads();
+    if (s_long) {
+        if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) {
+        const int i = wave * kCWspan + k * 64 + lane;
+        if (i < count) {
+            RecIO<RW>::store(dst, base + s_pos[i], r[k]);
+            if (checksum_out) {
+#pragma unroll
+                for (int w = 0; w < RW; w++) csum += sbyte_sum(r[k][w]);
+            }
+        }
+    }
+    if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
+}
+
+// The listed buckets: stable LSD radix sort in LDS over the bytes of
+// (key - min) that vary.  Persistent blocks walk the list.
+template <int RW>
+__global__ __launch_bounds__(kCBlock) void lsd_sort_kernel(
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
+    const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
+    const uint32_t* __restrict__ hard, const uint32_t* __restrict__ nhard, long long* __restrict__ checksum_out) {
+    __shared__ uint64_t s_key[kChunkCap];
+    __shared__ uint16_t s_idx[2][kChunkCap];
+    __shared__ uint16_t s_whist[kCWaves][256];
+    __shared__ uint32_t s_excl[256];
+    __shared__ uint64_t s_red[2 * kCWaves];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ long long s_sum[kCWaves];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t nh = *nhard;
+    long long csum = 0;
+    for (uint32_t q = blockIdx.x; q < nh; q += gridDim.x) {
+        const int64_t b = hard[q];
+        const int64_t base = bucket_base[b];
+        const int count = (int)hist[b];
+        uint32_t r[kCItems][RW];
+        uint64_t kk[kCItems];
+        uint64_t mn, mx;
+        __syncthreads();  // LDS of the previous bucket is free
+        load_bucket<RW>(src, base, count, lk, r, kk, s_red, mn, mx);
+        const uint64_t span = mx - mn;
+        const int nbytes = (64 - __clzll((long long)span) + 7) >> 3;
+#pragma unroll
+        for (int k = 0; k < kCItems; k++) {
+            const int i = wave * kCWspan + k * 64 + lane;
+            if (i < count) s_key[i] = kk[k] - mn;
+        }
+        __syncthreads();
+        int cur = 0;
+        for (int p = 0; p < nbytes; p++) {
+            const int sh = 8 * p;
+            lds_count_pass(p == 0 ? nullptr : s_idx[cur], s_idx[cur ^ (p == 0 ? 0 : 1)], count,
+                           [&](uint16_t e) { return (uint32_t)((s_key[e] >> sh) & 0xffu); }, s_whist, s_excl,
+                           s_tmp);
+            cur ^= (p == 0 ? 0 : 1);
+        }
+        // rank of every original record, then a direct write
+        for (int o = tid; o < count; o += kCBlock) s_idx[cur ^ 1][s_idx[cur][o]] = (uint16_t)o;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kCItems; k++) {
+            const int i = wave * kCWspan + k * 64 + lane;
+            if (i < count) {
+                RecIO<RW>::store(dst, base + s_idx[cur ^ 1][i], r[k]);
+                if (checksum_out) {
+#pragma unroll
+                    for (int w = 0; w < RW; w++) csum += sbyte_sum(r[k][w]);
+                }
+            }
+        }
+    }
+    if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
+                 const uint64_t* tables, uint32_t* hist, long long* checksum, hipStream_t st) {
+    if (ntiles <= 0) return 0;
+    const int64_t tpb = (ntiles + 2047) / 2048;
+    const int64_t blocks = (ntiles + tpb - 1) / tpb;
+    const uint32_t* s = (const uint32_t*)src;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(lhist_kernel<2>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, hist, checksum); break;
+        case 4: hipLaunchKernelGGL(lhist_kernel<4>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, hist, checksum); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st) {
+    hipLaunchKernelGGL(lscan_kernel, dim3(nseg), dim3(1024), 0, st, hist, seg_begin, base);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int lscatter_tile(int rw) { return rw == 2 ? 1024 * 8 : 1024 * 4; }
+
+int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegDev& sd, int64_t ntiles,
+                    const LKey& lk, const uint64_t* tables, const int64_t* bucket_base, uint64_t* lookback,
+                    uint32_t* ticket, uint32_t epoch, uint32_t* err, hipStream_t st) {
+    if (ntiles <= 0) return 0;
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    uint32_t* e = (uint32_t*)eq_dst;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, bucket_base, lookback, ticket, epoch, err, e); break;
+        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, bucket_base, lookback, ticket, epoch, err, e); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
+                      const int64_t* bucket_base, const uint64_t* tables, int64_t nbuckets, int64_t* overflow,
+                      uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, long long* checksum,
+                      bool eq_done, hipStream_t st) {
+    if (nbuckets <= 0) return 0;
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    const int ed = eq_done ? 1 : 0;
+    const unsigned persistent = (unsigned)std::min<int64_t>(nbuckets, 512);
+    switch (rw) {
+        case 2:
+            hipLaunchKernelGGL(interp_sort_kernel<2>, dim3((unsigned)nbuckets), dim3(kCBlock), 0, st, s, d, lk, hist,
+                               bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
+            MPSG_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
+                               (const uint32_t*)hard, (const uint32_t*)nhard, checksum);
+            break;
+        case 4:
+            hipLaunchKernelGGL(interp_sort_kernel<4>, dim3((unsigned)nbuckets), dim3(kCBlock), 0, st, s, d, lk, hist,
+                               bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
+            MPSG_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
+                               (const uint32_t*)hard, (const uint32_t*)nhard, checksum);
+            break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace mpsg

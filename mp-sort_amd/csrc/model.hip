@@ -1,0 +1,489 @@
+// model.hip -- the learned key-CDF model of the learned-splitter path.
+//
+// F(key) in [0, 1] approximates the fraction of records with radix <= key.
+// It is a two-level learned index:
+//   root   a monotone MLP 1 -> 32 -> 32 -> 1 on x = (key - kmin) / (kmax - kmin)
+//          or, for keys spread over many orders of magnitude,
+//          x = log2(1 + key - kmin) / log2(1 + kmax - kmin) (both monotone):
+//              h1 = tanh(w1 * x + b1), h2 = tanh(W2 h1 + b2), r = w3 . h2 + b3
+//          with w1, W2, w3 = softplus(raw) >= 0 and monotone activations, so r
+//          is non-decreasing in x.  Trained here on the GPU (forward/backward
+//          kernels below, Adam) on sampled keys with their exact ranks.
+//   leaves L linear pieces; leaf = clamp(floor(r * L), 0, L-1) and
+//          F = clamp(a_leaf * x + b_leaf, lo_leaf, hi_leaf) with a_leaf >= 0 and
+//          [lo, hi] the range of sampled ranks routed to the leaf.  Samples are
+//          sorted and routing is monotone, so hi_leaf <= lo_leaf+1 and F is
+//          non-decreasing across leaves.
+// Everything is FP32 on the vector ALUs.  The model never decides an output
+// position: it only proposes splitter values (checked and repaired with exact
+// counts) and bucket boundaries (integer keys, compared exactly).
+#include "device.h"
+#include "model.h"
+
+namespace mpsg {
+
+__device__ __forceinline__ float softplus_f(float v) {
+    return v > 20.f ? v : log1pf(expf(v));
+}
+__device__ __forceinline__ float sigmoid_f(float v) { return 1.f / (1.f + expf(-v)); }
+
+// effective (non-negative) weights from raw parameters
+__global__ void mlp_effective_kernel(const float* __restrict__ raw, float* __restrict__ eff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kMlpParams) return;
+    const float v = raw[i];
+    const bool positive = (i >= kW1 && i < kW1 + kH) || (i >= kW2 && i < kW2 + kH * kH) ||
+                          (i >= kW3 && i < kW3 + kH);
+    eff[i] = positive ? softplus_f(v) : v;
+}
+
+// forward pass with effective weights held in LDS
+__device__ __forceinline__ float mlp_forward(const float* __restrict__ e, float x, float* h1, float* h2) {
+#pragma unroll
+    for (int j = 0; j < kH; j++) h1[j] = tanhf(e[kW1 + j] * x + e[kB1 + j]);
+    float r = e[kB3];
+#pragma unroll 4
+    for (int k = 0; k < kH; k++) {
+        float z = e[kB2 + k];
+#pragma unroll
+        for (int j = 0; j < kH; j++) z += e[kW2 + k * kH + j] * h1[j];
+        h2[k] = tanhf(z);
+        r += e[kW3 + k] * h2[k];
+    }
+    return r;
+}
+
+__device__ __forceinline__ float mlp_forward_only(const float* __restrict__ e, float x) {
+    float h1[kH], h2[kH];
+    return mlp_forward(e, x, h1, h2);
+}
+
+// Loss = mean over the batch of (r(x_i) - y_i)^2.  Block b computes the
+// gradient of its samples and writes it to partial[b][kMlpParams] (no float
+// atomics: the reduction order is fixed, so training is deterministic).
+constexpr int kGradBlock = 256;
+__global__ __launch_bounds__(kGradBlock) void mlp_grad_kernel(const float* __restrict__ x,
+                                                              const float* __restrict__ y, int64_t n,
+                                                              int64_t stride, int64_t offset,
+                                                              const float* __restrict__ raw,
+                                                              const float* __restrict__ eff_g,
+                                                              float* __restrict__ partial,
+                                                              float* __restrict__ loss_partial) {
+    __shared__ float e[kMlpParams];
+    __shared__ float s_h1[kGradBlock][kH + 1];
+    __shared__ float s_dz2[kGradBlock][kH + 1];
+    __shared__ float s_red[(kGradBlock / 64) * (3 * kH + 2)];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kMlpParams; i += kGradBlock) e[i] = eff_g[i];
+    __syncthreads();
+    const int64_t si = offset + ((int64_t)blockIdx.x * kGradBlock + tid) * stride;
+    const bool valid = si < n;
+    const float inv_n = 1.f / (float)((n - offset + stride - 1) / stride);
+    float h1[kH], h2[kH];
+    float xv = valid ? x[si] : 0.f;
+    float r = mlp_forward(e, xv, h1, h2);
+    const float dy = valid ? 2.f * (r - y[si]) * inv_n : 0.f;
+    const float lv = valid ? (r - y[si]) * (r - y[si]) * inv_n : 0.f;
+    // backward
+    float dz1[kH];
+#pragma unroll
+    for (int j = 0; j < kH; j++) dz1[j] = 0.f;
+#pragma unroll 4
+    for (int k = 0; k < kH; k++) {
+        const float dz2 = dy * e[kW3 + k] * (1.f - h2[k] * h2[k]);
+        s_dz2[tid][k] = dz2;
+#pragma unroll
+        for (int j = 0; j < kH; j++) dz1[j] += dz2 * e[kW2 + k * kH + j];
+    }
+#pragma unroll
+    for (int j = 0; j < kH; j++) {
+        dz1[j] *= (1.f - h1[j] * h1[j]);
+        s_h1[tid][j] = h1[j];
+    }
+    // per-sample small gradients (w1, b1, w3: 3 x 32, b3, loss): wave
+    // shuffle sums, then a fixed-order sum over the block's waves
+    float* out = partial + (int64_t)blockIdx.x * kMlpParams;
+    const int lane = tid & 63, wv = tid >> 6;
+    constexpr int kSmall = 3 * kH + 2;
+    for (int q = 0; q < kSmall; q++) {
+        float v;
+        if (q < kH) v = dz1[q] * xv;
+        else if (q < 2 * kH) v = dz1[q - kH];
+        else if (q < 3 * kH) v = dy * h2[q - 2 * kH];
+        else if (q == 3 * kH) v = dy;
+        else v = lv;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) s_red[wv * kSmall + q] = v;
+    }
+    __syncthreads();
+    if (tid < kSmall) {
+        float acc = 0.f;
+        for (int w = 0; w < kGradBlock / 64; w++) acc += s_red[w * kSmall + tid];
+        if (tid < kH) out[kW1 + tid] = acc * sigmoid_f(raw[kW1 + tid]);
+        else if (tid < 2 * kH) out[kB1 + tid - kH] = acc;
+        else if (tid < 3 * kH) out[kW3 + tid - 2 * kH] = acc * sigmoid_f(raw[kW3 + tid - 2 * kH]);
+        else if (tid == 3 * kH) out[kB3] = acc;
+        else loss_partial[blockIdx.x] = acc;
+    }
+    // b2[k] = sum dz2[:,k]; W2[k][j] = sum dz2[:,k] * h1[:,j]  (1024 + 32 outputs)
+    __syncthreads();
+    for (int o = tid; o < kH * kH + kH; o += kGradBlock) {
+        float acc = 0.f;
+        if (o < kH * kH) {
+            const int k = o / kH, j = o % kH;
+            for (int s = 0; s < kGradBlock; s++) acc += s_dz2[s][k] * s_h1[s][j];
+            out[kW2 + o] = acc * sigmoid_f(raw[kW2 + o]);
+        } else {
+            const int k = o - kH * kH;
+            for (int s = 0; s < kGradBlock; s++) acc += s_dz2[s][k];
+            out[kB2 + k] = acc;
+        }
+    }
+}
+
+// sum the per-block partial gradients (fixed order) and apply one Adam step
+__global__ void mlp_adam_kernel(float* __restrict__ raw, const float* __restrict__ partial, int nblocks,
+                                float* __restrict__ m, float* __restrict__ v, float lr, float beta1,
+                                float beta2, float eps, int step, float* __restrict__ grad_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kMlpParams) return;
+    float g = 0.f;
+    for (int b = 0; b < nblocks; b++) g += partial[(int64_t)b * kMlpParams + i];
+    if (grad_out) grad_out[i] = g;
+    if (lr <= 0.f) return;  // gradient only
+    const float mi = beta1 * m[i] + (1.f - beta1) * g;
+    const float vi = beta2 * v[i] + (1.f - beta2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.f - powf(beta1, (float)step));
+    const float vh = vi / (1.f - powf(beta2, (float)step));
+    raw[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// root prediction of every sample -> leaf index, made monotone over the sorted
+// samples by a running maximum (done by the caller's scan kernel below)
+__global__ void mlp_leaf_kernel(const float* __restrict__ x, int64_t n, const float* __restrict__ eff_g,
+                                int L, int linear_root, int* __restrict__ leaf) {
+    __shared__ float e[kMlpParams];
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) e[i] = eff_g[i];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float r = linear_root ? x[i] : mlp_forward_only(e, x[i]);
+    int l = (int)floorf(r * (float)L);
+    l = l < 0 ? 0 : (l >= L ? L - 1 : l);
+    leaf[i] = l;
+}
+
+// single-block: running max of leaf[] (monotone routing) and leaf start offsets
+__global__ void leaf_starts_kernel(int* __restrict__ leaf, int64_t n, int L, int64_t* __restrict__ start) {
+    // serial over samples in chunks: a single block of 1024 threads does a
+    // blocked running max then fills starts
+    __shared__ int s_max[1024];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int64_t per = (n + nt - 1) / nt;
+    const int64_t a = tid * per, b = min(n, a + per);
+    int m = -1;
+    for (int64_t i = a; i < b; i++) m = max(m, leaf[i]);
+    s_max[tid] = m;
+    __syncthreads();
+    if (tid == 0) {
+        int run = -1;
+        for (int t = 0; t < nt; t++) {
+            const int v = s_max[t];
+            s_max[t] = run;  // exclusive running max
+            run = max(run, v);
+        }
+    }
+    __syncthreads();
+    m = s_max[tid];
+    for (int64_t i = a; i < b; i++) {
+        m = max(m, leaf[i]);
+        leaf[i] = m;
+    }
+    __syncthreads();
+    // start[l] = first sample with leaf >= l; start[L] = n
+    for (int l = tid; l <= L; l += nt) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (leaf[mid] < l) lo = mid + 1; else hi = mid;
+        }
+        start[l] = lo;
+    }
+}
+
+// least-squares line per leaf, slope clamped to >= 0, output range [lo, hi]
+__global__ void leaf_fit_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                const int64_t* __restrict__ start, int L, float* __restrict__ leaves,
+                                float ylast_default) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L) return;
+    const int64_t a = start[l], b = start[l + 1];
+    float* o = leaves + 4 * (int64_t)l;
+    if (b <= a) {
+        // empty leaf: constant at the last rank before it
+        const float c = a > 0 ? y[a - 1] : 0.f;
+        o[0] = 0.f; o[1] = c; o[2] = c; o[3] = c;
+        return;
+    }
+    double sx = 0, sy = 0, sxx = 0, sxy = 0;
+    const double cnt = (double)(b - a);
+    const double x0 = x[a];
+    for (int64_t i = a; i < b; i++) {
+        const double dx = (double)x[i] - x0, yy = y[i];
+        sx += dx; sy += yy; sxx += dx * dx; sxy += dx * yy;
+    }
+    const double den = cnt * sxx - sx * sx;
+    double slope = den > 0 ? (cnt * sxy - sx * sy) / den : 0.0;
+    if (!(slope >= 0)) slope = 0.0;
+    const double icpt = (sy - slope * sx) / cnt - slope * x0;
+    o[0] = (float)slope;
+    o[1] = (float)icpt;
+    o[2] = y[a];
+    o[3] = y[b - 1];
+    (void)ylast_default;
+}
+
+// ---- evaluation ----------------------------------------------------------
+__device__ __forceinline__ float key_to_x(uint64_t key, const ModelDev& m) {
+    const uint64_t d = key > m.kmin ? key - m.kmin : 0ull;
+    float x = m.log_input ? (float)(log2((double)d + 1.0) * m.log_scale) : (float)((double)d * m.inv_range);
+    return x > 1.f ? 1.f : x;
+}
+
+__device__ __forceinline__ float model_eval(const float* __restrict__ e, const ModelDev& m, uint64_t key) {
+    const float x = key_to_x(key, m);
+    const float r = m.linear_root ? x : mlp_forward_only(e, x);
+    int l = (int)floorf(r * (float)m.L);
+    l = l < 0 ? 0 : (l >= m.L ? m.L - 1 : l);
+    const float* lf = m.leaves + 4 * (int64_t)l;
+    float f = lf[0] * x + lf[1];
+    f = f < lf[2] ? lf[2] : (f > lf[3] ? lf[3] : f);
+    return f;
+}
+
+__global__ void model_eval_kernel(ModelDev m, const uint64_t* __restrict__ keys, int64_t n,
+                                  float* __restrict__ out) {
+    __shared__ float e[kMlpParams];
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) e[i] = m.eff[i];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = model_eval(e, m, keys[i]);
+}
+
+// smallest key in [kmin, kmax] with F(key) >= q  (bisection on the integer key)
+__device__ uint64_t model_inverse(const float* __restrict__ e, const ModelDev& m, uint64_t kmax, float q) {
+    uint64_t lo = m.kmin, hi = kmax;
+    if (model_eval(e, m, hi) < q) return hi;
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if (model_eval(e, m, mid) >= q) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Boundary tables: table t has nb boundaries for quantile targets
+// q = q0[t] + (q1[t] - q0[t]) * (i + 1) / (nb + 1); tables are made
+// non-decreasing.  One thread per boundary.
+__global__ void model_tables_kernel(ModelDev m, uint64_t kmax, const float* __restrict__ q0,
+                                    const float* __restrict__ q1, int ntables, int nb,
+                                    uint64_t* __restrict__ tables) {
+    __shared__ float e[kMlpParams];
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) e[i] = m.eff[i];
+    __syncthreads();
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)ntables * nb) return;
+    const int t = (int)(g / nb), i = (int)(g % nb);
+    const float q = q0[t] + (q1[t] - q0[t]) * (float)(i + 1) / (float)(nb + 1);
+    tables[g] = model_inverse(e, m, kmax, q);
+}
+
+// per table: running max so the table is sorted even if FP32 rounding made F
+// locally non-monotone; then every heavy key v (ascending) gets the boundary
+// pair {v - 1, v}, i.e. a bucket holding v alone.  Overwriting the two
+// boundaries around v's position keeps the table sorted; a pair that would
+// overwrite the previous heavy key's pair moves just after it, or is dropped
+// when the table has no room left.
+__global__ void tables_fix_kernel(uint64_t* __restrict__ tables, int ntables, int nb,
+                                  const uint64_t* __restrict__ heavy, int nheavy) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntables) return;
+    uint64_t* tb = tables + (int64_t)t * nb;
+    for (int i = 1; i < nb; i++)
+        if (tb[i] < tb[i - 1]) tb[i] = tb[i - 1];
+    if (nb < 2) return;
+    int last = -1;  // slot of the previous heavy key
+    for (int h = 0; h < nheavy; h++) {
+        const uint64_t v = heavy[h];
+        if (v == 0) {
+            tb[0] = 0;  // bucket 0 = keys <= 0
+            last = 0;
+            continue;
+        }
+        int a = 0, b = nb;  // a = first slot with tb[a] >= v
+        while (a < b) {
+            const int mid = (a + b) >> 1;
+            if (tb[mid] < v) a = mid + 1; else b = mid;
+        }
+        if (a < nb && tb[a] == v && a > 0 && tb[a - 1] == v - 1) {
+            last = a;
+            continue;
+        }
+        int j = a < 1 ? 1 : (a >= nb ? nb - 1 : a);
+        if (j - 1 <= last) {
+            // tb[last] holds the previous heavy key u < v
+            j = (tb[last] == v - 1) ? last + 1 : last + 2;
+            if (j >= nb) break;
+        }
+        tb[j - 1] = v - 1;
+        tb[j] = v;
+        last = j;
+    }
+}
+
+// smallest key in [kmin, kmax] with F(key) >= q for a monotone F, found by
+// one wave with 64 probes per step (a 64-ary search, ~11 steps for 64-bit
+// keys).  Every lane returns the result.
+__device__ uint64_t model_inverse_wave(const float* __restrict__ e, const ModelDev& m, uint64_t kmax, float q) {
+    const int lane = threadIdx.x & 63;
+    uint64_t lo = m.kmin, hi = kmax;
+    if (model_eval(e, m, hi) < q) return hi;
+    while (lo < hi) {
+        // probe l at lo + floor(span * (l + 1) / 64); probe 63 is hi
+        const uint64_t span = hi - lo;
+        const uint64_t l1 = (uint64_t)(lane + 1);
+        const uint64_t p = lo + (span >> 6) * l1 + (((span & 63) * l1) >> 6);
+        const bool ok = model_eval(e, m, p) >= q;
+        const uint64_t mask = __ballot(ok) | (1ull << 63);
+        const int first = __builtin_ctzll(mask);
+        const uint64_t pf = __shfl(p, first, 64);
+        const uint64_t pb = __shfl(p, first > 0 ? first - 1 : 0, 64);
+        hi = pf;
+        lo = first > 0 ? pb + 1 : lo;
+    }
+    return lo;
+}
+
+// splitter proposals: for target counts c (fraction q = c / n), the proposal
+// F^-1(q) and a bracket F^-1(q -+ delta).  One block per splitter, one wave
+// per value.
+__global__ __launch_bounds__(192) void model_propose_kernel(ModelDev m, uint64_t kmax, const int64_t* __restrict__ C,
+                                                            int P, int64_t n, float delta,
+                                                            uint64_t* __restrict__ prop) {
+    __shared__ float e[kMlpParams];
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) e[i] = m.eff[i];
+    __syncthreads();
+    const int i = blockIdx.x, w = threadIdx.x >> 6;
+    if (i >= P - 1) return;
+    // F approximates the fraction of keys <= key; the splitter is k[C-1]
+    const float q = (float)((double)C[i + 1] / (double)n) + (float)(w - 1) * delta;
+    const uint64_t v = model_inverse_wave(e, m, kmax, q);
+    if ((threadIdx.x & 63) == 0) prop[3 * i + w] = v;
+}
+
+// training samples from a sorted record array: S stratified positions with
+// exact ranks; x = normalised key, y = (rank + 1) / n
+__global__ void model_sample_kernel(const uint8_t* __restrict__ recs, int64_t elsize, int64_t key_off,
+                                    int key_bytes, uint64_t key_xmask, int64_t n, int64_t S, uint64_t seed,
+                                    ModelDev m, uint64_t* __restrict__ skeys,
+                                    float* __restrict__ x, float* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    uint64_t h = seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(i + 1));
+    h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+    h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+    h ^= h >> 31;
+    const double u = (double)(h >> 11) * 0x1.0p-53;
+    int64_t pos = (int64_t)(((double)i + u) * (double)n / (double)S);
+    if (pos >= n) pos = n - 1;
+    const uint8_t* p = recs + pos * elsize + key_off;
+    uint64_t k = 0;
+    for (int b = 0; b < key_bytes; b++) k |= (uint64_t)p[b] << (8 * b);
+    k ^= key_xmask;
+    skeys[i] = k;
+    x[i] = key_to_x(k, m);
+    y[i] = (float)((double)(pos + 1) / (double)n);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int launch_mlp_effective(const float* raw, float* eff, hipStream_t st) {
+    hipLaunchKernelGGL(mlp_effective_kernel, dim3(nblk(kMlpParams, 256)), dim3(256), 0, st, raw, eff);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int mlp_grad_blocks(int64_t n, int64_t stride, int64_t offset) {
+    const int64_t m = (n - offset + stride - 1) / stride;
+    return (int)nblk(m, kGradBlock);
+}
+
+int launch_mlp_grad(const float* x, const float* y, int64_t n, int64_t stride, int64_t offset,
+                    const float* raw, const float* eff, float* partial, float* loss_partial, hipStream_t st) {
+    const int blocks = mlp_grad_blocks(n, stride, offset);
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(kGradBlock), 0, st, x, y, n, stride, offset, raw,
+                       eff, partial, loss_partial);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_mlp_adam(float* raw, const float* partial, int nblocks, float* m, float* v, float lr, int step,
+                    float* grad_out, hipStream_t st) {
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3(nblk(kMlpParams, 256)), dim3(256), 0, st, raw, partial, nblocks,
+                       m, v, lr, 0.9f, 0.999f, 1e-8f, step, grad_out);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_leaves(const float* x, const float* y, int64_t S, const float* eff, int L, int linear_root,
+                  int* leaf, int64_t* start, float* leaves, hipStream_t st) {
+    hipLaunchKernelGGL(mlp_leaf_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, x, S, eff, L, linear_root, leaf);
+    MPSG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(leaf_starts_kernel, dim3(1), dim3(1024), 0, st, leaf, S, L, start);
+    MPSG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(leaf_fit_kernel, dim3(nblk(L, 256)), dim3(256), 0, st, x, y, start, L, leaves, 0.f);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_model_eval(const ModelDev& m, const uint64_t* keys, int64_t n, float* out, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(model_eval_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, m, keys, n, out);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_model_tables(const ModelDev& m, uint64_t kmax, const float* q0, const float* q1, int ntables,
+                        int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st) {
+    const int64_t tot = (int64_t)ntables * nb;
+    hipLaunchKernelGGL(model_tables_kernel, dim3(nblk(tot, 128)), dim3(128), 0, st, m, kmax, q0, q1, ntables, nb,
+                       tables);
+    MPSG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(tables_fix_kernel, dim3(nblk(ntables, 64)), dim3(64), 0, st, tables, ntables, nb, heavy,
+                       nheavy);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_model_propose(const ModelDev& m, uint64_t kmax, const int64_t* C, int P, int64_t n, float delta,
+                         uint64_t* prop, hipStream_t st) {
+    if (P < 2) return 0;
+    hipLaunchKernelGGL(model_propose_kernel, dim3(P - 1), dim3(192), 0, st, m, kmax, C, P, n, delta, prop);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_model_sample(const void* recs, int64_t elsize, int64_t key_off, int key_bytes, uint64_t key_xmask,
+                        int64_t n, int64_t S, uint64_t seed, const ModelDev& m, uint64_t* skeys, float* x,
+                        float* y, hipStream_t st) {
+    hipLaunchKernelGGL(model_sample_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, (const uint8_t*)recs, elsize,
+                       key_off, key_bytes, key_xmask, n, S, seed, m, skeys, x, y);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace mpsg

@@ -352,6 +352,40 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const uint32_t* __restrict
     }
 }
 
+// copy of the records inside a plan's segments only (one block per tile);
+// used when the segments are scattered through the buffer
+template <int RW>
+__global__ __launch_bounds__(kBlock) void seg_copy_kernel(const uint32_t* __restrict__ src,
+                                                          uint32_t* __restrict__ dst, SegDev seg,
+                                                          int tile_recs, long long* __restrict__ checksum) {
+    __shared__ long long s_sum[kWaves];
+    const int64_t tile = blockIdx.x;
+    const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
+    const int64_t sbeg = seg.begin[s];
+    const int64_t begin = sbeg + (tile - seg.tile_start[s]) * tile_recs;
+    const int64_t end = min(sbeg + seg.count[s], begin + (int64_t)tile_recs);
+    long long csum = 0;
+    for (int64_t i = begin + threadIdx.x; i < end; i += kBlock) {
+        uint32_t r[RW];
+        RecIO<RW>::load(src, i, r);
+        RecIO<RW>::store(dst, i, r);
+        if (checksum) {
+#pragma unroll
+            for (int w = 0; w < RW; w++) csum += sbyte_sum(r[w]);
+        }
+    }
+    if (checksum) {
+        csum = wave_sum_ll(csum);
+        if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = csum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long a = 0;
+            for (int w = 0; w < kWaves; w++) a += s_sum[w];
+            atomicAdd((unsigned long long*)checksum, (unsigned long long)a);
+        }
+    }
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -408,6 +442,28 @@ int launch_copy(const void* src, void* dst, int64_t nrec, int rw, long long* che
         case 2: hipLaunchKernelGGL(copy_kernel<2>, g, b, 0, st, s, d, nrec, checksum); break;
         case 4: hipLaunchKernelGGL(copy_kernel<4>, g, b, 0, st, s, d, nrec, checksum); break;
         case 8: hipLaunchKernelGGL(copy_kernel<8>, g, b, 0, st, s, d, nrec, checksum); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+// copy src -> dst over the plan's segments: the whole range when they tile
+// [0, nrec), tile by tile otherwise
+static int plan_copy(const SegSortPlan& pl, const void* src, void* dst, long long* checksum,
+                     hipStream_t st) {
+    if (!pl.sparse) return launch_copy(src, dst, pl.nrec, pl.rw, checksum, st);
+    if (pl.ntiles <= 0) return 0;
+    SegDev sd{pl.d_begin, pl.d_count, pl.d_tile_start, pl.nseg, 0};
+    dim3 g((unsigned)pl.ntiles), b(kBlock);
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    const int tr = pl.tile.block * pl.tile.items;
+    switch (pl.rw) {
+        case 1: hipLaunchKernelGGL(seg_copy_kernel<1>, g, b, 0, st, s, d, sd, tr, checksum); break;
+        case 2: hipLaunchKernelGGL(seg_copy_kernel<2>, g, b, 0, st, s, d, sd, tr, checksum); break;
+        case 4: hipLaunchKernelGGL(seg_copy_kernel<4>, g, b, 0, st, s, d, sd, tr, checksum); break;
+        case 8: hipLaunchKernelGGL(seg_copy_kernel<8>, g, b, 0, st, s, d, sd, tr, checksum); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
@@ -506,12 +562,12 @@ int seg_passes(const SegSortPlan& pl, const SortBuffers& bufs, hipStream_t st, l
     void* Y = bufs.tmp;
     int rc = 0;
     if (k == 0) {
-        if (src != X) return launch_copy(src, X, pl.nrec, pl.rw, cs_out, st);
+        if (src != X) return plan_copy(pl, src, X, cs_out, st);
         return 0;
     }
     const void* cur = src;
     if ((k & 1) && src == X) {
-        rc = launch_copy(src, Y, pl.nrec, pl.rw, nullptr, st);
+        rc = plan_copy(pl, src, Y, nullptr, st);
         if (rc) return rc;
         cur = Y;
     }

@@ -44,7 +44,13 @@ EXPORTS = [
     "mpsort_gpu_memcpy_h2d", "mpsort_gpu_memcpy_d2h", "mpsort_gpu_synchronize", "mpsort_gpu_dcopy",
     "mpsort_gpu_device_count", "mpsort_mpi_set_options", "mpsort_mpi_has_options",
     "mpsort_mpi_unset_options", "mpsort_mpi_report_last_run", "radix_sort", "mpsort_omp",
+    "mpsort_gpu_model_train", "mpsort_gpu_model_free", "mpsort_gpu_model_get_info",
+    "mpsort_gpu_model_eval", "mpsort_gpu_model_params", "mpsort_gpu_mlp_grad",
+    "mpsort_gpu_newarray_learned",
 ]
+
+MODEL_LINEAR_ROOT = 1 << 8
+MLP_PARAMS = 1153
 
 
 class KeyDesc(ctypes.Structure):
@@ -67,8 +73,9 @@ class Stats(ctypes.Structure):
         ("learned", ctypes.c_int32), ("pass_ms", ctypes.c_double * 64),
         ("splitters_exact", ctypes.c_int32), ("splitters_repaired", ctypes.c_int32),
         ("buckets", ctypes.c_int32), ("buckets_overflow", ctypes.c_int32),
-        ("repair_rounds", ctypes.c_int32), ("reserved2", ctypes.c_int32),
+        ("repair_rounds", ctypes.c_int32), ("plain_sides", ctypes.c_int32),
         ("t_model_ms", ctypes.c_double), ("t_bucket_ms", ctypes.c_double),
+        ("buckets_lds_radix", ctypes.c_int32), ("reserved3", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -79,6 +86,16 @@ class Stats(ctypes.Structure):
                 v = [self.pass_ms[i] for i in range(self.npass_ms)]
             d[name] = v
         return d
+
+
+class ModelInfo(ctypes.Structure):
+    _fields_ = [("nsample", ctypes.c_int64), ("leaves", ctypes.c_int32), ("linear_root", ctypes.c_int32),
+                ("steps", ctypes.c_int32), ("nheavy", ctypes.c_int32), ("max_leaf_samples", ctypes.c_int32),
+                ("log_input", ctypes.c_int32), ("train_ms", ctypes.c_double), ("loss", ctypes.c_float),
+                ("err_max", ctypes.c_float), ("kmin", ctypes.c_uint64), ("kmax", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 RADIX_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
@@ -111,6 +128,16 @@ def _load():
     lib.mpsort_gpu_memcpy_h2d.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.mpsort_gpu_memcpy_d2h.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.mpsort_gpu_dcopy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    vp = ctypes.c_void_p
+    lib.mpsort_gpu_model_train.argtypes = [P(vp), vp, ctypes.c_int64, ctypes.c_size_t, P(KeyDesc),
+                                           ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint]
+    lib.mpsort_gpu_model_free.argtypes = [vp]
+    lib.mpsort_gpu_model_get_info.argtypes = [vp, P(ModelInfo)]
+    lib.mpsort_gpu_model_eval.argtypes = [vp, vp, ctypes.c_int64, vp]
+    lib.mpsort_gpu_model_params.argtypes = [vp, vp, vp]
+    lib.mpsort_gpu_mlp_grad.argtypes = [vp, vp, vp, ctypes.c_int64, vp, P(ctypes.c_float)]
+    lib.mpsort_gpu_newarray_learned.argtypes = [vp, i64p, ctypes.c_int, vp, i64p, ctypes.c_size_t,
+                                                P(KeyDesc), vp, ctypes.c_uint]
     lib.mpsort_gpu_set_vranks.argtypes = [ctypes.c_int]
     lib.mpsort_gpu_get_vranks.restype = ctypes.c_int
     lib.mpsort_mpi_set_options.argtypes = [ctypes.c_int]

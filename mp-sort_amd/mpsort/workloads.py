@@ -82,9 +82,11 @@ def clustered_records(n, seed=2, nclump=None, median_r=None, sigma_ln=None, frac
     return out
 
 
-def clustered_keys(n, seed=2):
+def clustered_keys(n, seed=2, nclump=None):
     out = numpy.empty(n, dtype=numpy.uint64)
-    c = CLUSTER
+    c = dict(CLUSTER)
+    if nclump is not None:
+        c["nclump"] = nclump
     _gen().mpsort_gen_clustered(_p(out), None, SZ(n), U64(seed), ctypes.c_int(c["nclump"]),
                                 ctypes.c_double(c["median_r"]), ctypes.c_double(c["sigma_ln"]),
                                 ctypes.c_double(c["frac_uniform"]))

@@ -20,6 +20,7 @@
 #include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 
 #define OR_MAXR 64 /* max radix bytes the oracle handles */
 
@@ -449,4 +450,84 @@ int oracle_histogram_sort(const void *base, const int64_t *nin, int NTask,
     free(Pleft); free(Pright); free(stable); free(narrow);
     free(CLT); free(CLE); free(mCLT); free(mCLE);
     return rc;
+}
+
+/* ---- the learned path's root network (no reference counterpart) ---------
+ * Double-precision restatement of the monotone MLP 1 -> 32 -> 32 -> 1 the GPU
+ * trains: w1, W2, w3 = softplus(raw) >= 0, tanh activations, loss = mean of
+ * (r(x_i) - y_i)^2.  Parameter layout: w1[32] b1[32] W2[32][32] b2[32] w3[32]
+ * b3.  Pinned by central finite differences (tests/test_oracle.py). */
+#include <math.h>
+#define OM_H 32
+#define OM_W1 0
+#define OM_B1 (OM_W1 + OM_H)
+#define OM_W2 (OM_B1 + OM_H)
+#define OM_B2 (OM_W2 + OM_H * OM_H)
+#define OM_W3 (OM_B2 + OM_H)
+#define OM_B3 (OM_W3 + OM_H)
+#define OM_N (OM_B3 + 1)
+
+static double om_softplus(double v) { return v > 30 ? v : log1p(exp(v)); }
+static double om_sigmoid(double v) { return 1.0 / (1.0 + exp(-v)); }
+
+double oracle_mlp_forward(const double *raw, double x)
+{
+    double h1[OM_H], r = raw[OM_B3];
+    for (int j = 0; j < OM_H; j++) h1[j] = tanh(om_softplus(raw[OM_W1 + j]) * x + raw[OM_B1 + j]);
+    for (int k = 0; k < OM_H; k++) {
+        double z = raw[OM_B2 + k];
+        for (int j = 0; j < OM_H; j++) z += om_softplus(raw[OM_W2 + k * OM_H + j]) * h1[j];
+        r += om_softplus(raw[OM_W3 + k]) * tanh(z);
+    }
+    return r;
+}
+
+/* loss and its gradient with respect to the raw parameters */
+double oracle_mlp_grad(const double *raw, const float *x, const float *y, int64_t n, double *grad)
+{
+    double w1[OM_H], W2[OM_H * OM_H], w3[OM_H];
+    for (int j = 0; j < OM_H; j++) { w1[j] = om_softplus(raw[OM_W1 + j]); w3[j] = om_softplus(raw[OM_W3 + j]); }
+    for (int q = 0; q < OM_H * OM_H; q++) W2[q] = om_softplus(raw[OM_W2 + q]);
+    for (int q = 0; q < OM_N; q++) grad[q] = 0;
+    double loss = 0;
+    for (int64_t i = 0; i < n; i++) {
+        double h1[OM_H], h2[OM_H], r = raw[OM_B3];
+        for (int j = 0; j < OM_H; j++) h1[j] = tanh(w1[j] * x[i] + raw[OM_B1 + j]);
+        for (int k = 0; k < OM_H; k++) {
+            double z = raw[OM_B2 + k];
+            for (int j = 0; j < OM_H; j++) z += W2[k * OM_H + j] * h1[j];
+            h2[k] = tanh(z);
+            r += w3[k] * h2[k];
+        }
+        double e = r - y[i];
+        loss += e * e / n;
+        double dy = 2 * e / n, dz1[OM_H];
+        for (int j = 0; j < OM_H; j++) dz1[j] = 0;
+        grad[OM_B3] += dy;
+        for (int k = 0; k < OM_H; k++) {
+            grad[OM_W3 + k] += dy * h2[k] * om_sigmoid(raw[OM_W3 + k]);
+            double dz2 = dy * w3[k] * (1 - h2[k] * h2[k]);
+            grad[OM_B2 + k] += dz2;
+            for (int j = 0; j < OM_H; j++) {
+                grad[OM_W2 + k * OM_H + j] += dz2 * h1[j] * om_sigmoid(raw[OM_W2 + k * OM_H + j]);
+                dz1[j] += dz2 * W2[k * OM_H + j];
+            }
+        }
+        for (int j = 0; j < OM_H; j++) {
+            double d1 = dz1[j] * (1 - h1[j] * h1[j]);
+            grad[OM_B1 + j] += d1;
+            grad[OM_W1 + j] += d1 * x[i] * om_sigmoid(raw[OM_W1 + j]);
+        }
+    }
+    return loss;
+}
+
+double oracle_mlp_loss(const double *raw, const float *x, const float *y, int64_t n)
+{
+    double loss = 0;
+    for (int64_t i = 0; i < n; i++) {
+        double e = oracle_mlp_forward(raw, x[i]) - y[i];
+        loss += e * e / n;
+    }
+    return loss;
 }

@@ -262,3 +262,40 @@ def ref_mpi_sort(records, nin, nout, key, repeat=1, timeout=600):
                 t = float(line.split()[1])
         out = numpy.fromfile(fout, dtype=records.dtype)
     return out, t
+
+
+# ---- the learned path's root network ---------------------------------------
+MLP_PARAMS = 1153
+
+
+def _mlp_lib():
+    l = _oracle()
+    l.oracle_mlp_grad.restype = ctypes.c_double
+    l.oracle_mlp_loss.restype = ctypes.c_double
+    l.oracle_mlp_forward.restype = ctypes.c_double
+    return l
+
+
+def mlp_grad(raw, x, y):
+    """Double-precision loss and gradient (oracle.c oracle_mlp_grad)."""
+    raw = numpy.ascontiguousarray(raw, dtype=numpy.float64)
+    x = numpy.ascontiguousarray(x, dtype=numpy.float32)
+    y = numpy.ascontiguousarray(y, dtype=numpy.float32)
+    g = numpy.zeros(MLP_PARAMS, dtype=numpy.float64)
+    loss = _mlp_lib().oracle_mlp_grad(_ptr(raw), _ptr(x), _ptr(y), ctypes.c_int64(len(x)), _ptr(g))
+    return loss, g
+
+
+def mlp_loss(raw, x, y):
+    raw = numpy.ascontiguousarray(raw, dtype=numpy.float64)
+    x = numpy.ascontiguousarray(x, dtype=numpy.float32)
+    y = numpy.ascontiguousarray(y, dtype=numpy.float32)
+    return _mlp_lib().oracle_mlp_loss(_ptr(raw), _ptr(x), _ptr(y), ctypes.c_int64(len(x)))
+
+
+def mlp_forward(raw, x):
+    raw = numpy.ascontiguousarray(raw, dtype=numpy.float64)
+    return _mlp_lib().oracle_mlp_forward(_ptr(raw), ctypes.c_double(x))
+
+
+__all__ += ["mlp_grad", "mlp_loss", "mlp_forward", "MLP_PARAMS"]

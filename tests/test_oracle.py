@@ -218,3 +218,29 @@ def test_reference_mpsort_mpi_signed_and_multiword():
     out, info = oracle.histogram_sort(a, nin, nin, key)
     ref, _ = oracle.ref_mpi_sort(a, nin, nin, key)
     assert numpy.array_equal(out.view(numpy.uint8), ref.view(numpy.uint8))
+
+
+# ---- the learned path's root network (oracle_mlp_grad) ---------------------
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """oracle_mlp_grad is pinned by central differences of oracle_mlp_loss;
+    a dropped chain-rule factor (softplus'/tanh') fails this."""
+    g = rng(0)
+    raw = g.normal(0, 1, oracle.MLP_PARAMS)
+    x = g.random(64).astype('f4')
+    y = numpy.sort(g.random(64)).astype('f4')
+    loss, grad = oracle.mlp_grad(raw, x, y)
+    assert abs(loss - oracle.mlp_loss(raw, x, y)) < 1e-12
+    for i in list(range(0, oracle.MLP_PARAMS, 37)) + [oracle.MLP_PARAMS - 1]:
+        h = 1e-6
+        a = raw.copy(); a[i] += h
+        b = raw.copy(); b[i] -= h
+        fd = (oracle.mlp_loss(a, x, y) - oracle.mlp_loss(b, x, y)) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, grad[i])
+
+
+def test_mlp_oracle_is_monotone():
+    g = rng(1)
+    raw = g.normal(0, 2, oracle.MLP_PARAMS)
+    xs = numpy.linspace(0, 1, 2001)
+    ys = [oracle.mlp_forward(raw, float(v)) for v in xs]
+    assert all(b >= a - 1e-12 for a, b in zip(ys, ys[1:]))
