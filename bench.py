@@ -27,10 +27,14 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrecords/s at 2^28×16B: learned-splitter vs plain GPU sort; % of HBM peak"
 HBM_PEAK_GBS = 8000.0
 PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_second_sort_ms", "t_total_ms")
-# HBM bytes per onesweep launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE +
-# WRITE_SIZE, scaled by the counters' reading for a plain copy kernel of known
-# size (profiles/onesweep_pass_clustered_2e28x16B_pmc.md)
-TRAFFIC_BYTES = None
+# HBM bytes per launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE + WRITE_SIZE,
+# each scaled by the counter's reading for a plain copy kernel of known size
+# (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
+TRAFFIC_ONESWEEP = 9069913293
+TRAFFIC_BUCKET_SORT = 8754310656
+# per-side stages of the learned local sort, as timed by the library
+LEARNED_KERNELS = ["hist_A", "scatter_A", "bucket_sort_A", "lds_radix_A", "fallback_A",
+                   "hist_B", "scatter_B", "bucket_sort_B", "lds_radix_B", "fallback_B"]
 
 
 def log(*a):
@@ -175,9 +179,7 @@ def main():
                           "plain_sides": L["plain_sides"],
                           "buckets_lds_radix": L["buckets_lds_radix"],
                           "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
-                          "kernels_ms": dict(zip(["hist_A", "scatter_A", "lds_sort_A", "fallback_A",
-                                                  "hist_B", "scatter_B", "lds_sort_B", "fallback_B"],
-                                                 [round(x, 4) for x in L["pass_ms"]]))}
+                          "kernels_ms": dict(zip(LEARNED_KERNELS, [round(x, 4) for x in L["pass_ms"]]))}
         log("learned phases: " + json.dumps(result_learned["phases_ms"]))
         log("learned kernels: " + json.dumps(result_learned["kernels_ms"]))
         model.close()
@@ -186,14 +188,31 @@ def main():
     lib.mpsort_gpu_dfree(dout)
 
     last = st_plain[-1]
-    alg_bytes = 2.0 * n * 16
-    pass_ms = [x for st in st_plain for x in st["pass_ms"]]
-    avg_pass_ms = sum(pass_ms) / len(pass_ms)
-    achieved = alg_bytes / (avg_pass_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES,
-                "kernel": "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)",
-                "avg_launch_ms": round(avg_pass_ms, 4), "launches": len(pass_ms)}
+    alg_bytes = 2.0 * n * 16  # one read and one write of every record
+
+    def roof(launch_ms, kernel, traffic):
+        avg = sum(launch_ms) / len(launch_ms)
+        achieved = alg_bytes / (avg * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+                "avg_launch_ms": round(avg, 4), "launches": len(launch_ms)}
+
+    plain_roof = roof([x for st in st_plain for x in st["pass_ms"]],
+                      "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP)
+    if result_learned is not None:
+        # the learned step's dominant kernel: the in-LDS bucket sort, one
+        # launch per local sort over all n records
+        ib = LEARNED_KERNELS.index
+        launches = [st["pass_ms"][ib(k)] for st in st_learned for k in ("bucket_sort_A", "bucket_sort_B")]
+        roofline = roof(launches, "interp_sort_kernel<4> (in-LDS bucket sort, 2*n*16 B per launch)",
+                        TRAFFIC_BUCKET_SORT)
+        result_learned["roofline_by_stage"] = {
+            k: {"avg_ms": round(sum(st["pass_ms"][ib(k)] for st in st_learned) / len(st_learned), 4),
+                "GB/s": round((alg_bytes / 2 if k.startswith("hist") else alg_bytes) /
+                              (sum(st["pass_ms"][ib(k)] for st in st_learned) / len(st_learned) * 1e-3) / 1e9, 1)}
+            for k in ("hist_A", "scatter_A", "bucket_sort_A", "hist_B", "scatter_B", "bucket_sort_B")}
+    else:
+        roofline = plain_roof
     log("plain phases (ms): first %.3f findP %.3f lay %.3f exch %.3f second %.3f total %.3f; passes %d+%d, "
         "bisect rounds %d" % (last["t_first_sort_ms"], last["t_findp_ms"], last["t_laysolve_ms"],
                               last["t_exchange_ms"], last["t_second_sort_ms"], last["t_total_ms"],
@@ -221,6 +240,7 @@ def main():
         "learned": result_learned,
         "learned_vs_plain": round(value / value_plain, 3) if result_learned else None,
         "roofline": roofline,
+        "plain_roofline": plain_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

@@ -687,8 +687,9 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
                               const SegSortPlan& plain, const void* src, void* dst, void* tmp, long long* cs_in,
                               long long* cs_out, LearnedCounters* lc, hipStream_t st, hipEvent_t* ev) {
     int rc = 0;
-    // ev (optional): 5 events around bucket histogram, scatter, in-LDS
-    // bucket sorts and the overflow fallback
+    // ev (optional): 6 events around bucket histogram, scatter, in-LDS
+    // bucket sort, in-LDS radix sort of crowded buckets and the overflow
+    // fallback
     auto mark = [&](int i) -> int {
         if (ev) MPSG_CHECK(hipEventRecord(ev[i], st));
         return 0;
@@ -714,7 +715,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
             int passes = 0;
             SortBuffers b{const_cast<void*>(src), dst, tmp};
             if ((rc = segmented_sort(plain, b, st, cs_in != nullptr, cs_out != nullptr, &passes, nullptr))) return rc;
-            for (int e = 1; e <= 4; e++)
+            for (int e = 1; e <= 5; e++)
                 if ((rc = mark(e))) return rc;
             return 0;
         }
@@ -739,9 +740,10 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         return rc;
     if ((rc = mark(2))) return rc;
     if ((rc = launch_chunk_sort(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, (int64_t)P * kLBins, L.overflow,
-                                L.noverflow, kMaxOverflowList, L.hard, L.noverflow + 1, cs_out, eq_direct, st)))
+                                L.noverflow, kMaxOverflowList, L.hard, L.noverflow + 1, cs_out, eq_direct, st,
+                                ev ? ev[3] : nullptr)))
         return rc;
-    if ((rc = mark(3))) return rc;
+    if ((rc = mark(4))) return rc;
     uint32_t novh[2] = {0, 0}, errs = 0;
     MPSG_CHECK(hipMemcpyAsync(novh, L.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(&errs, L.err, sizeof errs, hipMemcpyDeviceToHost, st));
@@ -750,7 +752,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     const uint32_t nov = novh[0];
     lc->hard_buckets += (int)novh[1];
     lc->overflow_buckets += (int)nov;
-    if (nov == 0) return mark(4);
+    if (nov == 0) return mark(5);
     const size_t recbytes = (size_t)rw * 4;
     if ((rc = c.s2.ensure((size_t)n * recbytes))) return rc;
     int passes = 0;
@@ -764,7 +766,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         // record (no single-value bucket went to dst)
         SortBuffers b{src == dst ? tmp : const_cast<void*>(src), dst, c.s2.p};
         if ((rc = segmented_sort(plain, b, st, false, cs_out != nullptr, &passes, nullptr))) return rc;
-        return mark(4);
+        return mark(5);
     }
     // radix sort of the oversized buckets only (they are segments of tmp)
     std::vector<int64_t> ov(2 * (size_t)nov);
@@ -812,7 +814,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     MPSG_CHECK(hipMemcpyAsync(&e2, pa.pass_flags + kMaxPasses, sizeof e2, hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipStreamSynchronize(st));
     if (e2) return fail(MPSORT_ERR_TIMEOUT, "bucket fallback sort: device flag 0x%x", e2);
-    return mark(4);
+    return mark(5);
 }
 
 
@@ -1028,13 +1030,13 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     stats->t_total_ms = ev_ms(c.ev[kEvStart], c.ev[kEvEnd]);
     stats->plain_sides = lc.global_fallback;
     stats->buckets_lds_radix = lc.hard_buckets;
-    // pass_ms: bucket histogram, scatter, in-LDS sorts, fallback for the
-    // FirstSort and then the SecondSort
+    // pass_ms: bucket histogram, scatter, in-LDS bucket sort, in-LDS radix
+    // sort, overflow fallback for the FirstSort and then the SecondSort
     int np = 0;
     for (int side = 0; side < 2; side++) {
         hipEvent_t* e = side == 0 ? evA : evB;
         if (!e) continue;
-        for (int k = 0; k < 4; k++) stats->pass_ms[np++] = ev_ms(e[k], e[k + 1]);
+        for (int k = 0; k < 5; k++) stats->pass_ms[np++] = ev_ms(e[k], e[k + 1]);
     }
     stats->npass_ms = np;
 

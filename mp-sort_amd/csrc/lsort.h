@@ -44,6 +44,6 @@ int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegD
 int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                       const int64_t* bucket_base, const uint64_t* tables, int64_t nbuckets, int64_t* overflow,
                       uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, long long* checksum,
-                      bool eq_done, hipStream_t st);
+                      bool eq_done, hipStream_t st, hipEvent_t mid = nullptr);
 
 }  // namespace mpsg

@@ -782,8 +782,11 @@ int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegD
 int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                       const int64_t* bucket_base, const uint64_t* tables, int64_t nbuckets, int64_t* overflow,
                       uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, long long* checksum,
-                      bool eq_done, hipStream_t st) {
-    if (nbuckets <= 0) return 0;
+                      bool eq_done, hipStream_t st, hipEvent_t mid) {
+    if (nbuckets <= 0) {
+        if (mid) MPSG_CHECK(hipEventRecord(mid, st));
+        return 0;
+    }
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
     const int ed = eq_done ? 1 : 0;
@@ -793,6 +796,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
             hipLaunchKernelGGL(interp_sort_kernel<2>, dim3((unsigned)nbuckets), dim3(kCBlock), 0, st, s, d, lk, hist,
                                bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
             MPSG_CHECK(hipGetLastError());
+            if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                (const uint32_t*)hard, (const uint32_t*)nhard, checksum);
             break;
@@ -800,6 +804,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
             hipLaunchKernelGGL(interp_sort_kernel<4>, dim3((unsigned)nbuckets), dim3(kCBlock), 0, st, s, d, lk, hist,
                                bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
             MPSG_CHECK(hipGetLastError());
+            if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                (const uint32_t*)hard, (const uint32_t*)nhard, checksum);
             break;
