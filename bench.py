@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain path only")
+    ap.add_argument("--host-leg", action="store_true",
+                    help="also time one call on host buffers (PCIe transfers included)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain copy kernel of n*16 bytes (byte-counter calibration)")
     args = ap.parse_args()
@@ -183,9 +185,22 @@ def main():
         log("learned phases: " + json.dumps(result_learned["phases_ms"]))
         log("learned kernels: " + json.dumps(result_learned["kernels_ms"]))
         model.close()
-    del plain_out
     lib.mpsort_gpu_dfree(din)
     lib.mpsort_gpu_dfree(dout)
+    # one call with host buffers (PCIe copies in and out included): reported,
+    # never the headline value
+    host_leg = None
+    if args.host_leg:
+        t0 = time.perf_counter()
+        _lib.check(lib.mpsort_gpu_newarray(recs.ctypes.data, nin, P, plain_out.ctypes.data, nin, 16,
+                                           ctypes.byref(key), _lib.TIMING))
+        t_host = time.perf_counter() - t0
+        hs = _lib.last_stats().as_dict()
+        host_leg = {"ms": round(t_host * 1e3, 3), "value": round(n / t_host / 1e6, 2), "unit": "Mrecords/s",
+                    "h2d_ms": round(hs["t_h2d_ms"], 3), "d2h_ms": round(hs["t_d2h_ms"], 3),
+                    "path": "plain, host pointers (pageable numpy buffers)"}
+        log("host-pointer call: %.3f ms (h2d %.3f, d2h %.3f)" % (t_host * 1e3, hs["t_h2d_ms"], hs["t_d2h_ms"]))
+    del plain_out
 
     last = st_plain[-1]
     alg_bytes = 2.0 * n * 16  # one read and one write of every record
@@ -238,6 +253,7 @@ def main():
         "plain": {"value": round(value_plain, 2), "ms_per_step": round(ms_plain, 3),
                   "phases_ms": {k2: round(last[k2], 4) for k2 in PHASES}},
         "learned": result_learned,
+        "host_pointer_call": host_leg,
         "learned_vs_plain": round(value / value_plain, 3) if result_learned else None,
         "roofline": roofline,
         "plain_roofline": plain_roof,
