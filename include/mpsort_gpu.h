@@ -190,6 +190,33 @@ int mpsort_gpu_newarray_learned(const void *base, const int64_t *nin, int P, voi
                                 const int64_t *nout, size_t elsize, const mpsort_key_desc *key,
                                 const mpsort_gpu_model *model, unsigned flags);
 
+/* Batched segmented sort: many independent stable sorts (the reference's
+ * counterpart is a loop of radix_sort, radixsort.c:35-44, one call per
+ * segment).  A plan holds groups; a group is nseg segments of one record type
+ * laid out back to back in device memory (d_in), sorted into d_out (may equal
+ * d_in).  Records of 4/8/16/32 bytes, key 4-byte aligned inside the record.
+ * MPSORT_GPU_SEGPLAN_GRAPH captures the whole plan once in a hipGraph and
+ * replays it on every run (the segment sizes and pointers are fixed by the
+ * plan); MPSORT_GPU_SEGPLAN_TUNE times tile shapes and the in-LDS size
+ * threshold on the first run and caches the choice per (record size, key
+ * width, size class) in memory and in $MPSORT_GPU_TUNE_CACHE (default
+ * ~/.cache/mpsort_gpu_segsort.tune). */
+typedef struct mpsort_gpu_segplan_s mpsort_gpu_segplan;
+#define MPSORT_GPU_SEGPLAN_GRAPH (1u << 0)
+#define MPSORT_GPU_SEGPLAN_TUNE (1u << 1)
+int mpsort_gpu_segplan_create(mpsort_gpu_segplan **plan, unsigned flags);
+int mpsort_gpu_segplan_add(mpsort_gpu_segplan *plan, const void *d_in, void *d_out, const int64_t *seg_count,
+                           int nseg, size_t elsize, const mpsort_key_desc *key);
+/* stats (may be NULL): t_total_ms and nrecords are filled */
+int mpsort_gpu_segplan_run(mpsort_gpu_segplan *plan, mpsort_gpu_stats *stats);
+/* the tuning a group runs with: segments sorted in LDS / by onesweep, tile */
+int mpsort_gpu_segplan_info(const mpsort_gpu_segplan *plan, int group, int *nsmall, int *nlarge, int *block,
+                            int *items, int *threshold);
+int mpsort_gpu_segplan_free(mpsort_gpu_segplan *plan);
+/* one group, no graph: create + add + run + free */
+int mpsort_gpu_sort_segments(const void *d_in, void *d_out, const int64_t *seg_count, int nseg, size_t elsize,
+                             const mpsort_key_desc *key, unsigned flags);
+
 /* Device allocator hook (MPIU_SetMalloc mp-mpiu.h:4-16).  NULL restores
  * hipMalloc/hipFree.  Frees every cached workspace. */
 typedef void *(*mpsort_gpu_malloc_fn)(size_t bytes, void *userdata);

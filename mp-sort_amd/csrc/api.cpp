@@ -9,6 +9,8 @@
 #include <vector>
 
 #include <algorithm>
+#include <map>
+#include <string>
 #include <chrono>
 #include <cmath>
 
@@ -1061,6 +1063,283 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
 
 using namespace mpsg;
 
+// ---- batched segmented sort ------------------------------------------------
+// Thousands of independent stable sorts of one record type, segments back to
+// back in one buffer.  Segments of at most `threshold` records are sorted by
+// one workgroup each in LDS (the bucket-sort kernels of the learned path, with
+// the segment as the bucket); the larger ones by the segmented onesweep over
+// just those segments.  Every launch of a group depends only on device data,
+// so the whole batch can be captured once in a hipGraph and replayed.
+struct SegTune {
+    TileCfg tile;
+    int threshold;
+};
+
+struct SegGroup {
+    const void* in = nullptr;
+    void* out = nullptr;
+    size_t elsize = 0;
+    int rw = 0;
+    KeySpec ks;
+    LKey lk;
+    int nseg = 0;
+    int64_t nrec = 0;
+    std::vector<int64_t> count, begin;
+    SegTune tune{{1024, 8}, kChunkCap};
+    bool tuned = false;
+    // device side
+    Buf small;     // small-segment counts, bases, hard list, counters; epoch
+    Buf plan_buf;  // arrays of the large-segment plan
+    Buf tmp;       // record buffer for the digit passes
+    Buf lookback;
+    int nsmall = 0, nlarge = 0;
+    uint32_t* d_cnt = nullptr;
+    int64_t* d_base = nullptr;
+    uint32_t* d_hard = nullptr;
+    uint32_t* d_counters = nullptr;  // [0] oversized, [1] hard, [2] epoch
+    int64_t* d_ovf = nullptr;
+    SegSortPlan plan;
+    uint64_t epoch_used = 0;  // epoch steps consumed since the last reset
+};
+
+static std::map<std::string, SegTune>& seg_tune_cache() {
+    static std::map<std::string, SegTune> m;
+    return m;
+}
+
+// size class: log2 of the median segment length
+static std::string seg_tune_key(const SegGroup& G) {
+    std::vector<int64_t> c(G.count);
+    std::sort(c.begin(), c.end());
+    const int64_t med = c.empty() ? 0 : c[c.size() / 2];
+    int cls = 0;
+    while ((int64_t(1) << (cls + 1)) <= med) cls++;
+    char k[96];
+    snprintf(k, sizeof k, "rw=%d keybytes=%d class=%d", G.rw, G.ks.key_bytes, cls);
+    return k;
+}
+
+static std::string seg_tune_file() {
+    const char* e = getenv("MPSORT_GPU_TUNE_CACHE");
+    if (e) return e;
+    const char* h = getenv("HOME");
+    return h ? std::string(h) + "/.cache/mpsort_gpu_segsort.tune" : std::string();
+}
+
+static void seg_tune_load() {
+    static bool loaded = false;
+    if (loaded) return;
+    loaded = true;
+    const std::string f = seg_tune_file();
+    if (f.empty()) return;
+    FILE* fp = fopen(f.c_str(), "r");
+    if (!fp) return;
+    char line[256];
+    while (fgets(line, sizeof line, fp)) {
+        int rw, kb, cls, blk, items, thr;
+        if (sscanf(line, "rw=%d keybytes=%d class=%d block=%d items=%d threshold=%d", &rw, &kb, &cls, &blk, &items,
+                   &thr) == 6) {
+            char k[96];
+            snprintf(k, sizeof k, "rw=%d keybytes=%d class=%d", rw, kb, cls);
+            SegTune t{{blk, items}, thr};
+            if (tile_supported(rw, t.tile) && thr >= 0 && thr <= kChunkCap) seg_tune_cache()[k] = t;
+        }
+    }
+    fclose(fp);
+}
+
+static void seg_tune_store() {
+    const std::string f = seg_tune_file();
+    if (f.empty()) return;
+    FILE* fp = fopen(f.c_str(), "w");
+    if (!fp) return;
+    for (auto& kv : seg_tune_cache())
+        fprintf(fp, "%s block=%d items=%d threshold=%d\n", kv.first.c_str(), kv.second.tile.block,
+                kv.second.tile.items, kv.second.threshold);
+    fclose(fp);
+}
+
+// lay out the group's device arrays for its current tuning
+static int seg_group_prepare(SegGroup& G, hipStream_t st) {
+    int rc = 0;
+    const int npass = G.ks.key_bytes;
+    const bool lds_ok = (G.rw == 2 || G.rw == 4) && G.ks.key_bytes <= 8;
+    const int thr = lds_ok ? G.tune.threshold : 0;
+    std::vector<uint32_t> scnt;
+    std::vector<int64_t> sbase, lcnt, lbeg;
+    for (int i = 0; i < G.nseg; i++) {
+        if (G.count[i] == 0) continue;
+        if (G.count[i] <= thr) {
+            scnt.push_back((uint32_t)G.count[i]);
+            sbase.push_back(G.begin[i]);
+        } else {
+            lcnt.push_back(G.count[i]);
+            lbeg.push_back(G.begin[i]);
+        }
+    }
+    G.nsmall = (int)scnt.size();
+    G.nlarge = (int)lcnt.size();
+    Sizer sz;
+    sz.take<uint32_t>((size_t)G.nsmall + 1);
+    sz.take<int64_t>((size_t)G.nsmall + 1);
+    sz.take<uint32_t>((size_t)G.nsmall + 1);
+    sz.take<uint32_t>(4);
+    sz.take<int64_t>(2);
+    if ((rc = G.small.ensure(sz.off + 4096))) return rc;
+    Arena a{(char*)G.small.p};
+    G.d_cnt = a.take<uint32_t>((size_t)G.nsmall + 1);
+    G.d_base = a.take<int64_t>((size_t)G.nsmall + 1);
+    G.d_hard = a.take<uint32_t>((size_t)G.nsmall + 1);
+    G.d_counters = a.take<uint32_t>(4);
+    G.d_ovf = a.take<int64_t>(2);
+    if (G.nsmall) {
+        MPSG_CHECK(hipMemcpyAsync(G.d_cnt, scnt.data(), sizeof(uint32_t) * G.nsmall, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(G.d_base, sbase.data(), sizeof(int64_t) * G.nsmall, hipMemcpyHostToDevice, st));
+    }
+    if ((rc = launch_fill_u32(G.d_counters, 0u, 4, st))) return rc;
+    G.epoch_used = 0;
+
+    SegSortPlan& p = G.plan;
+    p = SegSortPlan();
+    p.rw = G.rw;
+    p.tile = G.tune.tile;
+    p.ks = G.ks;
+    p.nseg = G.nlarge;
+    p.nrec = 0;
+    p.ntiles = 0;
+    if (G.nlarge) {
+        const int64_t T = (int64_t)p.tile.block * p.tile.items;
+        std::vector<int64_t> ts(G.nlarge + 1);
+        int64_t t = 0;
+        for (int i = 0; i < G.nlarge; i++) {
+            ts[i] = t;
+            t += (lcnt[i] + T - 1) / T;
+            p.nrec += lcnt[i];
+        }
+        ts[G.nlarge] = t;
+        p.ntiles = t;
+        Sizer pz;
+        lay_plan(pz, G.nlarge, npass, nullptr);
+        if ((rc = G.plan_buf.ensure(pz.off + 4096))) return rc;
+        Arena pa_a{(char*)G.plan_buf.p};
+        PlanArrays pa;
+        lay_plan(pa_a, G.nlarge, npass, &pa);
+        MPSG_CHECK(hipMemcpyAsync(pa.begin, lbeg.data(), sizeof(int64_t) * G.nlarge, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(pa.count, lcnt.data(), sizeof(int64_t) * G.nlarge, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(pa.tile_start, ts.data(), sizeof(int64_t) * (G.nlarge + 1), hipMemcpyHostToDevice,
+                                  st));
+        if ((rc = G.tmp.ensure((size_t)G.nrec * G.elsize))) return rc;
+        G.lookback.zero_on_grow = true;
+        if ((rc = G.lookback.ensure((size_t)(t + 1) * kBins * sizeof(uint64_t)))) return rc;
+        MPSG_CHECK(hipMemsetAsync(G.lookback.p, 0, G.lookback.bytes, st));
+        p.d_begin = pa.begin;
+        p.d_count = pa.count;
+        p.d_tile_start = pa.tile_start;
+        p.d_hist = pa.hist;
+        p.d_digit_base = pa.digit_base;
+        p.d_pass_flags = pa.pass_flags;
+        p.d_tickets = pa.tickets;
+        p.d_lookback = (uint64_t*)G.lookback.p;
+        p.d_checksum = nullptr;
+        p.sparse = true;
+        p.all_passes = true;
+        p.d_epoch_base = G.d_counters + 2;
+    }
+    MPSG_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// the device work of one group; records nothing but kernels, so it can be
+// captured (look-back epochs come from the counter bumped here)
+static int seg_group_enqueue(SegGroup& G, hipStream_t st) {
+    int rc = 0;
+    if ((rc = launch_fill_u32(G.d_counters, 0u, 2, st))) return rc;
+    if (G.nsmall) {
+        if ((rc = launch_chunk_sort(G.rw, G.in, G.out, G.lk, G.d_cnt, G.d_base, nullptr, G.nsmall, G.d_ovf,
+                                    G.d_counters, 1, G.d_hard, G.d_counters + 1, nullptr, false, st)))
+            return rc;
+    }
+    if (G.nlarge) {
+        if ((rc = launch_add_u32(G.d_counters + 2, (uint32_t)kMaxPasses, st))) return rc;
+        if ((rc = seg_count(G.plan, G.in, nullptr, st))) return rc;
+        SortBuffers b{const_cast<void*>(G.in), G.out, G.tmp.p};
+        int passes = 0;
+        if ((rc = seg_passes(G.plan, b, st, nullptr, &passes, nullptr))) return rc;
+    }
+    return 0;
+}
+
+// look-back epochs must stay below 2^23: reset the flags and the counter
+// (outside any capture) before a run that could pass the limit
+static int seg_group_epoch_guard(SegGroup& G, hipStream_t st) {
+    if (!G.nlarge) return 0;
+    if (G.epoch_used + 2 * kMaxPasses < (1u << 23)) {
+        G.epoch_used += kMaxPasses;
+        return 0;
+    }
+    int rc = launch_fill_u32((uint32_t*)G.lookback.p, 0u, (int64_t)(G.lookback.bytes / 4), st);
+    if (rc) return rc;
+    if ((rc = launch_fill_u32(G.d_counters + 2, 0u, 1, st))) return rc;
+    G.epoch_used = kMaxPasses;
+    return 0;
+}
+
+static int seg_group_check(SegGroup& G, hipStream_t st) {
+    uint32_t c[4] = {0, 0, 0, 0}, e = 0;
+    MPSG_CHECK(hipMemcpyAsync(c, G.d_counters, sizeof c, hipMemcpyDeviceToHost, st));
+    if (G.nlarge)
+        MPSG_CHECK(hipMemcpyAsync(&e, G.plan.d_pass_flags + kMaxPasses, sizeof e, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if (c[0]) return fail(MPSORT_ERR_LAYOUT, "a small segment exceeded the in-LDS capacity");
+    if (e & 1u) return fail(MPSORT_ERR_TIMEOUT, "a look-back wait exceeded its spin limit");
+    if (e) return fail(MPSORT_ERR_LAYOUT, "segmented sort: device flag 0x%x", e);
+    return 0;
+}
+
+// pick the tile shape and the in-LDS threshold by timing the candidates on
+// this group (cached per record size, key width and size class)
+static int seg_group_tune(SegGroup& G, DevCtx& c) {
+    seg_tune_load();
+    const std::string key = seg_tune_key(G);
+    auto it = seg_tune_cache().find(key);
+    if (it != seg_tune_cache().end()) {
+        G.tune = it->second;
+        G.tuned = true;
+        return seg_group_prepare(G, c.st);
+    }
+    const int hi = G.rw == 1 ? 32 : (G.rw == 2 ? 32 : (G.rw == 4 ? 16 : 8));
+    const TileCfg tiles[] = {{1024, hi / 2}, {1024, hi / 4}, {512, hi}, {256, hi}};
+    const int thresholds[] = {kChunkCap, kChunkCap / 4, 0};
+    float best = 1e30f;
+    SegTune pick = G.tune;
+    int rc = 0;
+    for (const TileCfg& t : tiles) {
+        if (!tile_supported(G.rw, t)) continue;
+        for (int thr : thresholds) {
+            G.tune = SegTune{t, thr};
+            if ((rc = seg_group_prepare(G, c.st))) return rc;
+            float best_here = 1e30f;
+            for (int rep = 0; rep < 2; rep++) {
+                if ((rc = seg_group_epoch_guard(G, c.st))) return rc;
+                MPSG_CHECK(hipEventRecord(c.ev[kEvStart], c.st));
+                if ((rc = seg_group_enqueue(G, c.st))) return rc;
+                MPSG_CHECK(hipEventRecord(c.ev[kEvEnd], c.st));
+                if ((rc = seg_group_check(G, c.st))) return rc;
+                best_here = std::min(best_here, (float)ev_ms(c.ev[kEvStart], c.ev[kEvEnd]));
+            }
+            if (best_here < best) {
+                best = best_here;
+                pick = G.tune;
+            }
+        }
+    }
+    G.tune = pick;
+    G.tuned = true;
+    seg_tune_cache()[key] = pick;
+    seg_tune_store();
+    return seg_group_prepare(G, c.st);
+}
+
 extern "C" {
 
 const char* mpsort_gpu_last_error(void) { return g_err; }
@@ -1748,6 +2027,153 @@ int mpsort_gpu_newarray_learned(const void* base, const int64_t* nin, int P, voi
         st.t_d2h_ms = ev_ms(e0, e1);
     }
     g_last = st;
+    return rc;
+}
+
+
+// ---- batched segmented sort -------------------------------------------------
+struct mpsort_gpu_segplan_s {
+    int device = 0;
+    unsigned flags = 0;
+    std::vector<SegGroup*> groups;
+    bool prepared = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int mpsort_gpu_segplan_create(mpsort_gpu_segplan** out, unsigned flags) {
+    g_err[0] = 0;
+    if (!out) return fail(MPSORT_ERR_ARG, "plan output pointer is NULL");
+    DevCtx* c = nullptr;
+    int rc = get_ctx(&c);
+    if (rc) return rc;
+    mpsort_gpu_segplan* p = new mpsort_gpu_segplan_s();
+    p->device = c->device;
+    p->flags = flags;
+    *out = p;
+    return 0;
+}
+
+int mpsort_gpu_segplan_add(mpsort_gpu_segplan* p, const void* d_in, void* d_out, const int64_t* seg_count, int nseg,
+                           size_t elsize, const mpsort_key_desc* key) {
+    g_err[0] = 0;
+    if (!p || !key || (nseg > 0 && (!seg_count || !d_in || !d_out)) || nseg < 0)
+        return fail(MPSORT_ERR_ARG, "NULL or negative argument");
+    if (p->prepared) return fail(MPSORT_ERR_ARG, "the plan has already run; create a new plan to add segments");
+    const int64_t kb = (int64_t)key->width * key->nwords;
+    if (key->width != 4 && key->width != 8) return fail(MPSORT_ERR_UNSUPPORTED, "key width %d", key->width);
+    if (elsize != 4 && elsize != 8 && elsize != 16 && elsize != 32)
+        return fail(MPSORT_ERR_UNSUPPORTED, "batched sort: records of 4, 8, 16 or 32 bytes");
+    if (key->offset < 0 || (key->offset & 3) || key->offset + kb > (int64_t)elsize || kb > kMaxKeyBytes)
+        return fail(MPSORT_ERR_UNSUPPORTED, "batched sort: the key must be 4-byte aligned inside the record");
+    SegGroup* G = new SegGroup();
+    G->in = d_in;
+    G->out = d_out;
+    G->elsize = elsize;
+    G->rw = (int)(elsize / 4);
+    fill_keyspec(&G->ks, G->rw, (int)key->offset, (int)kb, key->width, key->is_signed);
+    G->lk = make_lkey(G->ks);
+    G->nseg = nseg;
+    G->count.assign(seg_count, seg_count + nseg);
+    G->begin.resize(nseg);
+    for (int i = 0; i < nseg; i++) {
+        if (seg_count[i] < 0) {
+            delete G;
+            return fail(MPSORT_ERR_ARG, "negative segment size");
+        }
+        G->begin[i] = G->nrec;
+        G->nrec += seg_count[i];
+    }
+    p->groups.push_back(G);
+    return 0;
+}
+
+int mpsort_gpu_segplan_run(mpsort_gpu_segplan* p, mpsort_gpu_stats* stats) {
+    g_err[0] = 0;
+    if (!p) return fail(MPSORT_ERR_ARG, "NULL plan");
+    DevCtx* c = nullptr;
+    int rc = get_ctx(&c);
+    if (rc) return rc;
+    if (c->device != p->device) return fail(MPSORT_ERR_ARG, "the plan belongs to device %d", p->device);
+    hipStream_t st = c->st;
+    if (!p->prepared) {
+        for (SegGroup* G : p->groups) {
+            if (p->flags & MPSORT_GPU_SEGPLAN_TUNE) rc = seg_group_tune(*G, *c);
+            else rc = seg_group_prepare(*G, st);
+            if (rc) return rc;
+        }
+        p->prepared = true;
+    }
+    for (SegGroup* G : p->groups)
+        if ((rc = seg_group_epoch_guard(*G, st))) return rc;
+    MPSG_CHECK(hipEventRecord(c->ev[kEvStart], st));
+    if (p->flags & MPSORT_GPU_SEGPLAN_GRAPH) {
+        if (!p->exec) {
+            MPSG_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (SegGroup* G : p->groups) {
+                if ((rc = seg_group_enqueue(*G, st))) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+            }
+            MPSG_CHECK(hipStreamEndCapture(st, &p->graph));
+            MPSG_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+        }
+        MPSG_CHECK(hipGraphLaunch(p->exec, st));
+    } else {
+        for (SegGroup* G : p->groups)
+            if ((rc = seg_group_enqueue(*G, st))) return rc;
+    }
+    MPSG_CHECK(hipEventRecord(c->ev[kEvEnd], st));
+    for (SegGroup* G : p->groups)
+        if ((rc = seg_group_check(*G, st))) return rc;
+    if (stats) {
+        memset(stats, 0, sizeof *stats);
+        stats->t_total_ms = ev_ms(c->ev[kEvStart], c->ev[kEvEnd]);
+        int64_t n = 0;
+        for (SegGroup* G : p->groups) n += G->nrec;
+        stats->nrecords = n;
+        stats->nvranks = 0;
+    }
+    return 0;
+}
+
+int mpsort_gpu_segplan_info(const mpsort_gpu_segplan* p, int group, int* nsmall, int* nlarge, int* block, int* items,
+                            int* threshold) {
+    if (!p || group < 0 || group >= (int)p->groups.size()) return fail(MPSORT_ERR_ARG, "no such group");
+    const SegGroup* G = p->groups[group];
+    if (nsmall) *nsmall = G->nsmall;
+    if (nlarge) *nlarge = G->nlarge;
+    if (block) *block = G->tune.tile.block;
+    if (items) *items = G->tune.tile.items;
+    if (threshold) *threshold = G->tune.threshold;
+    return 0;
+}
+
+int mpsort_gpu_segplan_free(mpsort_gpu_segplan* p) {
+    if (!p) return 0;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (SegGroup* G : p->groups) {
+        G->small.release();
+        G->plan_buf.release();
+        G->tmp.release();
+        G->lookback.release();
+        delete G;
+    }
+    delete p;
+    return 0;
+}
+
+int mpsort_gpu_sort_segments(const void* d_in, void* d_out, const int64_t* seg_count, int nseg, size_t elsize,
+                             const mpsort_key_desc* key, unsigned flags) {
+    mpsort_gpu_segplan* p = nullptr;
+    int rc = mpsort_gpu_segplan_create(&p, flags);
+    if (rc) return rc;
+    if (!(rc = mpsort_gpu_segplan_add(p, d_in, d_out, seg_count, nseg, elsize, key))) rc = mpsort_gpu_segplan_run(p, nullptr);
+    mpsort_gpu_segplan_free(p);
     return rc;
 }
 

@@ -113,6 +113,10 @@ struct SegSortPlan {
     uint32_t* d_tickets;     // kMaxPasses
     long long* d_checksum;   // [0] in, [1] out
     bool sparse = false;     // segments do not tile [0, nrec): copies go per tile
+    // graph-captured use: every digit pass runs (no read-back of the pass
+    // flags) and pass i uses look-back epoch *d_epoch_base + i + 1
+    bool all_passes = false;
+    const uint32_t* d_epoch_base = nullptr;
 };
 
 inline PassTable make_pass_table(const KeySpec& ks) {
@@ -143,6 +147,7 @@ int segmented_sort(const SegSortPlan& plan, const SortBuffers& bufs, hipStream_t
 int launch_copy(const void* src, void* dst, int64_t nrec, int rw, long long* checksum,
                 hipStream_t st);
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+int launch_add_u32(uint32_t* p, uint32_t v, hipStream_t st);  // *p += v on the device
 
 // epoch counter for the look-back flags (process wide, per device)
 uint32_t next_epoch(int device);

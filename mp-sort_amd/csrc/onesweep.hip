@@ -147,7 +147,8 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg,
     const int64_t* __restrict__ digit_base, int npass, int pass_idx, PassDesc pd,
     uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket, uint32_t epoch,
-    long long* __restrict__ checksum_out, uint32_t* __restrict__ error_flag) {
+    const uint32_t* __restrict__ epoch_base, long long* __restrict__ checksum_out,
+    uint32_t* __restrict__ error_flag) {
     constexpr int NW = BLOCK / 64;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int WAVE_TILE = 64 * ITEMS;
@@ -160,6 +161,8 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     __shared__ long long s_sum[NW];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // replayed launches (graphs) take their epoch from a device counter
+    if (epoch_base) epoch += *epoch_base;
     if (tid == 0) s_tile = (int64_t)atomicAdd(ticket, 1u);
     for (int i = tid; i < NW * kBins; i += BLOCK) (&s_whist[0][0])[i] = 0;
     __syncthreads();
@@ -386,6 +389,10 @@ __global__ __launch_bounds__(kBlock) void seg_copy_kernel(const uint32_t* __rest
     }
 }
 
+__global__ void add_u32_kernel(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -418,6 +425,12 @@ TileCfg default_tile(int rw) {
         if (sscanf(e, "%d:%d", &b, &it) == 2 && tile_supported(rw, TileCfg{b, it})) c = TileCfg{b, it};
     }
     return c;
+}
+
+int launch_add_u32(uint32_t* p, uint32_t v, hipStream_t st) {
+    hipLaunchKernelGGL(add_u32_kernel, dim3(1), dim3(64), 0, st, p, v);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
 }
 
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
@@ -489,7 +502,7 @@ static void launch_pass_t(const SegSortPlan& pl, const void* src, void* dst, int
     SegDev sd{pl.d_begin, pl.d_count, pl.d_tile_start, pl.nseg, 0};
     hipLaunchKernelGGL((onesweep_kernel<RW, BLOCK, ITEMS>), dim3((unsigned)pl.ntiles), dim3(BLOCK), 0, st,
                        (const uint32_t*)src, (uint32_t*)dst, sd, pl.d_digit_base, npass, pidx, pd,
-                       pl.d_lookback, ticket, epoch, cs, err);
+                       pl.d_lookback, ticket, epoch, pl.d_epoch_base, cs, err);
 }
 
 template <int RW>
@@ -549,13 +562,18 @@ int seg_passes(const SegSortPlan& pl, const SortBuffers& bufs, hipStream_t st, l
                        pl.d_hist, pl.d_begin, npass, pl.d_digit_base, pl.d_pass_flags);
     MPSG_CHECK(hipGetLastError());
 
-    uint32_t flags[kMaxPasses];
-    MPSG_CHECK(hipMemcpyAsync(flags, pl.d_pass_flags, sizeof(uint32_t) * npass,
-                              hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipStreamSynchronize(st));
     int todo[kMaxPasses], k = 0;
-    for (int b = 0; b < npass; b++)
-        if (flags[b]) todo[k++] = b;
+    if (pl.all_passes) {
+        // no read-back (the launch sequence may be captured in a graph)
+        for (int b = 0; b < npass; b++) todo[k++] = b;
+    } else {
+        uint32_t flags[kMaxPasses];
+        MPSG_CHECK(hipMemcpyAsync(flags, pl.d_pass_flags, sizeof(uint32_t) * npass,
+                                  hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipStreamSynchronize(st));
+        for (int b = 0; b < npass; b++)
+            if (flags[b]) todo[k++] = b;
+    }
 
     void* src = bufs.src;
     void* X = bufs.dst;
@@ -575,7 +593,7 @@ int seg_passes(const SegSortPlan& pl, const SortBuffers& bufs, hipStream_t st, l
     MPSG_CHECK(hipGetDevice(&dev));
     for (int i = 0; i < k; i++) {
         void* out = ((k - 1 - i) % 2 == 0) ? X : Y;
-        const uint32_t epoch = next_epoch(dev);
+        const uint32_t epoch = pl.d_epoch_base ? (uint32_t)(i + 1) : next_epoch(dev);
         const PassDesc pd = pt.p[todo[i]];
         long long* cso = (i == k - 1) ? cs_out : nullptr;
         if (pass_events) MPSG_CHECK(hipEventRecord(pass_events[2 * i], st));

@@ -46,8 +46,13 @@ EXPORTS = [
     "mpsort_mpi_unset_options", "mpsort_mpi_report_last_run", "radix_sort", "mpsort_omp",
     "mpsort_gpu_model_train", "mpsort_gpu_model_free", "mpsort_gpu_model_get_info",
     "mpsort_gpu_model_eval", "mpsort_gpu_model_params", "mpsort_gpu_mlp_grad",
-    "mpsort_gpu_newarray_learned",
+    "mpsort_gpu_newarray_learned", "mpsort_gpu_segplan_create", "mpsort_gpu_segplan_add",
+    "mpsort_gpu_segplan_run", "mpsort_gpu_segplan_info", "mpsort_gpu_segplan_free",
+    "mpsort_gpu_sort_segments",
 ]
+
+SEGPLAN_GRAPH = 1 << 0
+SEGPLAN_TUNE = 1 << 1
 
 MODEL_LINEAR_ROOT = 1 << 8
 MLP_PARAMS = 1153
@@ -138,6 +143,13 @@ def _load():
     lib.mpsort_gpu_mlp_grad.argtypes = [vp, vp, vp, ctypes.c_int64, vp, P(ctypes.c_float)]
     lib.mpsort_gpu_newarray_learned.argtypes = [vp, i64p, ctypes.c_int, vp, i64p, ctypes.c_size_t,
                                                 P(KeyDesc), vp, ctypes.c_uint]
+    lib.mpsort_gpu_segplan_create.argtypes = [P(ctypes.c_void_p), ctypes.c_uint]
+    lib.mpsort_gpu_segplan_add.argtypes = [vp, vp, vp, i64p, ctypes.c_int, ctypes.c_size_t, P(KeyDesc)]
+    lib.mpsort_gpu_segplan_run.argtypes = [vp, P(Stats)]
+    lib.mpsort_gpu_segplan_info.argtypes = [vp, ctypes.c_int] + [P(ctypes.c_int)] * 5
+    lib.mpsort_gpu_segplan_free.argtypes = [vp]
+    lib.mpsort_gpu_sort_segments.argtypes = [vp, vp, i64p, ctypes.c_int, ctypes.c_size_t, P(KeyDesc),
+                                             ctypes.c_uint]
     lib.mpsort_gpu_set_vranks.argtypes = [ctypes.c_int]
     lib.mpsort_gpu_get_vranks.restype = ctypes.c_int
     lib.mpsort_mpi_set_options.argtypes = [ctypes.c_int]

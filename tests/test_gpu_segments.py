@@ -1,0 +1,129 @@
+"""GPU tests of the batched segmented sort (many independent stable sorts):
+every segment equals its own stable sort, for the two record types of the
+batched workload ({i32 key, f32 value}, {i64 key, i64 value}), small
+segments (in LDS) and large ones (onesweep), duplicate-heavy keys, empty
+segments, graph replays against eager runs, and the autotuned plan."""
+import ctypes
+
+import numpy
+import pytest
+
+from mpsort import _lib, segments, workloads
+from mpsort._lib import lib
+
+pytestmark = pytest.mark.gpu
+
+
+def note(*a):
+    print(*a, flush=True)
+
+
+def make(dtype, sizes, seed, key_bits=None):
+    n = int(sum(sizes))
+    a = numpy.zeros(n, dtype)
+    raw = workloads.uniform(n, seed)
+    if key_bits is not None:
+        raw = raw >> numpy.uint64(64 - key_bits)
+    if a['key'].dtype == numpy.int32:
+        a['key'] = (raw >> numpy.uint64(32)).astype(numpy.uint32).view(numpy.int32)
+        a['value'] = numpy.arange(n, dtype=numpy.float32)
+    else:
+        a['key'] = raw.view(numpy.int64)
+        a['value'] = numpy.arange(n, dtype=numpy.int64)
+    return a
+
+
+def expected(a, sizes):
+    out = numpy.empty_like(a)
+    o = 0
+    for s in sizes:
+        s = int(s)
+        seg = a[o:o + s]
+        out[o:o + s] = seg[numpy.argsort(seg['key'], kind='stable')]
+        o += s
+    return out
+
+
+def sizes_mixed(nseg, seed, total):
+    s = workloads.segment_sizes(nseg, seed, lo=1e1, hi=1e5, total=total).astype(numpy.int64)
+    s[::17] = 0  # some empty segments
+    return s
+
+
+@pytest.mark.parametrize("dtype", [workloads.REC8, workloads.REC16I], ids=["8B", "16B"])
+def test_segments_equal_per_segment_stable_sort(dtype):
+    sizes = sizes_mixed(300, 3, 1 << 21)
+    a = make(dtype, sizes, 11)
+    note("segments: %d records %d" % (len(sizes), int(sizes.sum())))
+    out = segments.sort_segments(a, sizes, 'key')
+    assert numpy.array_equal(out, expected(a, sizes))
+
+
+@pytest.mark.parametrize("dtype", [workloads.REC8, workloads.REC16I], ids=["8B", "16B"])
+def test_segments_duplicate_heavy(dtype):
+    """keys drawn from 2^6 values: crowded bins, single-value segments"""
+    sizes = sizes_mixed(120, 5, 1 << 20)
+    a = make(dtype, sizes, 12, key_bits=6)
+    out = segments.sort_segments(a, sizes, 'key')
+    assert numpy.array_equal(out, expected(a, sizes))
+
+
+def test_segments_graph_replays_match_eager():
+    """A plan of both record types captured once and replayed three times,
+    with eager work and synchronisation in between; every replay equals the
+    eager result."""
+    s8 = sizes_mixed(200, 7, 1 << 20)
+    s16 = sizes_mixed(150, 8, 1 << 20)
+    a8 = make(workloads.REC8, s8, 21)
+    a16 = make(workloads.REC16I, s16, 22)
+    e8, e16 = expected(a8, s8), expected(a16, s16)
+    d8 = lib.mpsort_gpu_dmalloc(a8.nbytes)
+    d16 = lib.mpsort_gpu_dmalloc(a16.nbytes)
+    o16 = lib.mpsort_gpu_dmalloc(a16.nbytes)
+    try:
+        plan = segments.SegmentPlan(graph=True)
+        plan.add(d8, d8, s8, workloads.REC8, 'key')      # in place
+        plan.add(d16, o16, s16, workloads.REC16I, 'key')  # out of place
+        for rep in range(3):
+            _lib.check(lib.mpsort_gpu_memcpy_h2d(d8, a8.ctypes.data, a8.nbytes))
+            _lib.check(lib.mpsort_gpu_memcpy_h2d(d16, a16.ctypes.data, a16.nbytes))
+            # eager work of the program between replays
+            eager = segments.sort_segments(a16[:100000], [100000], 'key')
+            assert numpy.array_equal(eager, expected(a16[:100000], [100000]))
+            ms = plan.run()
+            _lib.check(lib.mpsort_gpu_synchronize())
+            g8 = numpy.empty_like(a8)
+            g16 = numpy.empty_like(a16)
+            _lib.check(lib.mpsort_gpu_memcpy_d2h(g8.ctypes.data, d8, a8.nbytes))
+            _lib.check(lib.mpsort_gpu_memcpy_d2h(g16.ctypes.data, o16, a16.nbytes))
+            note("replay %d: %.3f ms" % (rep, ms))
+            assert numpy.array_equal(g8, e8), rep
+            assert numpy.array_equal(g16, e16), rep
+        plan.close()
+    finally:
+        for p in (d8, d16, o16):
+            lib.mpsort_gpu_dfree(p)
+
+
+def test_segments_tuned_plan(tmp_path, monkeypatch):
+    monkeypatch.setenv("MPSORT_GPU_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    sizes = sizes_mixed(100, 9, 1 << 20)
+    a = make(workloads.REC16I, sizes, 31)
+    d = lib.mpsort_gpu_dmalloc(a.nbytes)
+    try:
+        _lib.check(lib.mpsort_gpu_memcpy_h2d(d, a.ctypes.data, a.nbytes))
+        plan = segments.SegmentPlan(tune=True)
+        plan.add(d, d, sizes, a.dtype, 'key')
+        # tuning runs candidates in place on the same buffer: start again
+        plan.run()
+        _lib.check(lib.mpsort_gpu_memcpy_h2d(d, a.ctypes.data, a.nbytes))
+        plan.run()
+        info = plan.info()
+        note("tuned: %s" % info)
+        assert info['block'] in (256, 512, 1024) and 0 <= info['threshold'] <= 8192
+        g = numpy.empty_like(a)
+        _lib.check(lib.mpsort_gpu_memcpy_d2h(g.ctypes.data, d, a.nbytes))
+        plan.close()
+    finally:
+        lib.mpsort_gpu_dfree(d)
+    assert numpy.array_equal(g, expected(a, sizes))
