@@ -64,6 +64,117 @@ def cpu_baseline(recs, nsample):
                       % (nsample, t)}
 
 
+def bench_batched(args):
+    """The batched workload: 10,000 independent stable sorts, sizes
+    log-uniform on [1e3, 1e7] rescaled to 2^30 records in total, each segment
+    of one of two record types chosen by a seeded draw ({i32 key, f32 value}
+    or {i64 key, i64 value}), keys uniform over the full signed range.  One
+    plan (both types) captured in a hipGraph, tuned on its first run, against
+    a loop of one plain sort call per segment."""
+    import numpy
+    from mpsort import _lib, segments, workloads
+    lib = _lib.lib
+    nseg = args.segments
+    total = 1 << args.log2n
+    sizes = workloads.segment_sizes(nseg, 4, lo=1e3, hi=1e7, total=total).astype(numpy.int64)
+    pick = numpy.array([workloads.mix64(0xB0A7 + i) & 1 for i in range(nseg)], dtype=bool)
+    groups = []
+    for gi, (dt, sel) in enumerate(((workloads.REC8, ~pick), (workloads.REC16I, pick))):
+        sz = sizes[sel]
+        n = int(sz.sum())
+        a = numpy.zeros(n, dt)
+        raw = workloads.uniform(n, 40 + gi)
+        if dt == workloads.REC8:
+            a['key'] = (raw >> numpy.uint64(32)).astype(numpy.uint32).view(numpy.int32)
+            # the f32 value field carries the record's position as its bit pattern
+            a['value'] = numpy.arange(n, dtype=numpy.uint32).view(numpy.float32)
+        else:
+            a['key'] = raw.view(numpy.int64)
+            a['value'] = numpy.arange(n, dtype=numpy.int64)
+        del raw
+        src = lib.mpsort_gpu_dmalloc(a.nbytes)
+        dst = lib.mpsort_gpu_dmalloc(a.nbytes)
+        _lib.check(lib.mpsort_gpu_memcpy_h2d(src, a.ctypes.data, a.nbytes))
+        groups.append(dict(dtype=dt, sizes=sz, n=n, host=a, src=src, dst=dst))
+        log("group %d: %d segments, %d records of %d B" % (gi, len(sz), n, dt.itemsize))
+    nrec = sum(g["n"] for g in groups)
+    nbytes = sum(g["n"] * g["dtype"].itemsize for g in groups)
+
+    plan = segments.SegmentPlan(graph=True, tune=True)
+    for g in groups:
+        plan.add(g["src"], g["dst"], g["sizes"], g["dtype"], 'key')
+    t0 = time.perf_counter()
+    plan.run()  # tunes, captures
+    log("plan tuned and captured in %.2f s: %s" % (time.perf_counter() - t0, [plan.info(i) for i in range(2)]))
+    for i in range(args.warmup):
+        plan.run()
+    _lib.check(lib.mpsort_gpu_synchronize())
+    t0 = time.perf_counter()
+    dev_ms = []
+    for i in range(args.steps):
+        dev_ms.append(plan.run())
+    _lib.check(lib.mpsort_gpu_synchronize())
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log("batched: %.3f ms per run (device %.3f ms)" % (ms, sum(dev_ms) / len(dev_ms)))
+
+    # correctness: sorted, stable (values are the input positions) and a
+    # permutation of each segment
+    for g in groups:
+        out = numpy.empty_like(g["host"])
+        _lib.check(lib.mpsort_gpu_memcpy_d2h(out.ctypes.data, g["dst"], out.nbytes))
+        starts = numpy.concatenate([[0], numpy.cumsum(g["sizes"])[:-1]])
+        seg_of = numpy.repeat(numpy.arange(len(g["sizes"])), g["sizes"])
+        k = out['key']
+        v = (out['value'].view(numpy.uint32) if g["dtype"] == workloads.REC8 else out['value']).astype(numpy.int64)
+        same = seg_of[1:] == seg_of[:-1]
+        ok = bool(numpy.all(~same | (k[1:] > k[:-1]) | ((k[1:] == k[:-1]) & (v[1:] > v[:-1]))))
+        ok = ok and bool(numpy.all((v >= starts[seg_of]) & (v < (starts + g["sizes"])[seg_of])))
+        seen = numpy.zeros(g["n"], dtype=bool)
+        seen[v] = True
+        ok = ok and bool(seen.all())
+        del out, seg_of, seen
+        if not ok:
+            print("bench.py: batched output is not the per-segment stable sort", file=sys.stderr)
+            return 5
+    plan.close()
+
+    # the per-segment loop: one plain sort call per segment
+    t0 = time.perf_counter()
+    for g in groups:
+        key = _lib.KeyDesc(offset=0, width=g["dtype"]['key'].itemsize, is_signed=1, nwords=1, reserved=0)
+        es = g["dtype"].itemsize
+        off = 0
+        for c in g["sizes"]:
+            c = int(c)
+            cnt = (ctypes.c_int64 * 1)(c)
+            _lib.check(lib.mpsort_gpu_newarray(g["src"] + off * es, cnt, 1, g["dst"] + off * es, cnt, es,
+                                               ctypes.byref(key), _lib.DEVICE_PTRS | _lib.NO_CHECKSUM))
+            off += c
+    _lib.check(lib.mpsort_gpu_synchronize())
+    loop_s = time.perf_counter() - t0
+    log("per-segment loop: %.3f s" % loop_s)
+    for g in groups:
+        lib.mpsort_gpu_dfree(g["src"])
+        lib.mpsort_gpu_dfree(g["dst"])
+
+    value = nrec / (ms * 1e-3) / 1e6
+    loop_value = nrec / loop_s / 1e6
+    result = {
+        "metric": "Mrecords/s, batched segmented sort (10,000 segments, 2^30 records)", "value": round(value, 2),
+        "unit": "Mrecords/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "i32/i64", "data": "synthetic",
+        "config": {"workload": "batched: %d segments log-uniform [1e3, 1e7] rescaled to %d records, "
+                               "{i32,f32} and {i64,i64} records" % (nseg, nrec),
+                   "segments": nseg, "records": nrec, "bytes": nbytes},
+        "per_segment_loop": {"value": round(loop_value, 2), "s": round(loop_s, 3)},
+        "speedup_vs_loop": round(value / loop_value, 2),
+        "sort_equivalent_GBps": round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1),
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +187,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="time the plain path only")
     ap.add_argument("--host-leg", action="store_true",
                     help="also time one call on host buffers (PCIe transfers included)")
+    ap.add_argument("--workload", default="histsort", choices=["histsort", "batched"],
+                    help="histsort: the 2^28 x 16 B histogram sort (default); batched: 10,000 independent sorts")
+    ap.add_argument("--segments", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain copy kernel of n*16 bytes (byte-counter calibration)")
     args = ap.parse_args()
@@ -90,6 +204,10 @@ def main():
     if _lib.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 3
+    if args.workload == "batched":
+        if args.log2n == 28:
+            args.log2n = 30
+        return bench_batched(args)
 
     n = 1 << args.log2n
     P = args.vranks
