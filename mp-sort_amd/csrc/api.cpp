@@ -599,6 +599,7 @@ struct LearnedModel {
     float* raw = nullptr;     // kMlpParams
     float* eff = nullptr;     // kMlpParams
     float* leaves = nullptr;  // 4 * L
+    float* xstart = nullptr;  // L + 1
     uint64_t* heavy = nullptr;
     int nheavy = 0;
     int L = 0;
@@ -617,6 +618,8 @@ struct LearnedModel {
         ModelDev m;
         m.eff = eff;
         m.leaves = leaves;
+        m.xstart = xstart;
+        m.kmax = kmax;
         m.L = L;
         m.linear_root = linear_root;
         m.kmin = kmin;
@@ -641,6 +644,7 @@ struct LearnedSide {
     int64_t* overflow;                // 2 * kMaxOverflowList
     uint32_t* noverflow;              // 4: [0] oversized buckets, [1] buckets for the LDS radix sort
     uint32_t* hard;                   // nseg * kLBins
+    int64_t tstride = kLBounds;       // segment s uses tables + s * tstride (0: one shared table)
 };
 
 template <class A>
@@ -729,7 +733,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         return fail(MPSORT_ERR_ARG, "look-back buffer too small for the bucket scatter");
     SegDev sd{L.begin, L.count, L.ltiles, P, 0};
     if ((rc = mark(0))) return rc;
-    if ((rc = launch_lhist(rw, src, sd, ntiles, (int)trecs, lk, L.tables, L.lhist, cs_in, st))) return rc;
+    if ((rc = launch_lhist(rw, src, sd, ntiles, (int)trecs, lk, L.tables, L.tstride, L.lhist, cs_in, st))) return rc;
     if ((rc = launch_lscan(P, L.lhist, L.begin, L.bbase, st))) return rc;
     if ((rc = mark(1))) return rc;
     int dev = 0;
@@ -737,11 +741,11 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     // single-value buckets are written to their final slots by the scatter
     // unless that would overwrite records the scatter still reads
     const bool eq_direct = dst != src;
-    if ((rc = launch_lscatter(rw, src, tmp, eq_direct ? dst : tmp, sd, ntiles, lk, L.tables, L.bbase,
+    if ((rc = launch_lscatter(rw, src, tmp, eq_direct ? dst : tmp, sd, ntiles, lk, L.tables, L.tstride, L.bbase,
                               (uint64_t*)c.lookback.p, L.tickets, next_epoch(dev), L.err, st)))
         return rc;
     if ((rc = mark(2))) return rc;
-    if ((rc = launch_chunk_sort(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, (int64_t)P * kLBins, L.overflow,
+    if ((rc = launch_chunk_sort(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, L.tstride, (int64_t)P * kLBins, L.overflow,
                                 L.noverflow, kMaxOverflowList, L.hard, L.noverflow + 1, cs_out, eq_direct, st,
                                 ev ? ev[3] : nullptr)))
         return rc;
@@ -953,8 +957,11 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     hipEvent_t* evA = cs.timing ? c.pass_ev[0] : nullptr;
     hipEvent_t* evB = cs.timing && P > 1 ? c.pass_ev[1] : nullptr;
 
-    // FirstSort: every rank's keys span the whole distribution
-    if ((rc = launch_model_tables(md, M.kmax, LA.q0, LA.q1, P, kLBounds, LA.tables, M.heavy, M.nheavy, st)))
+    // FirstSort: every rank's keys span the whole distribution, so all
+    // segments share one table
+    LA.tstride = 0;
+    LB.tstride = kLBounds;
+    if ((rc = launch_model_tables(md, M.kmax, LA.q0, LA.q1, 1, kLBounds, LA.tables, M.heavy, M.nheavy, st)))
         return rc;
     if (P == 1) {
         if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, final_dst, c.s1.p,
@@ -1255,7 +1262,7 @@ static int seg_group_enqueue(SegGroup& G, hipStream_t st) {
     int rc = 0;
     if ((rc = launch_fill_u32(G.d_counters, 0u, 2, st))) return rc;
     if (G.nsmall) {
-        if ((rc = launch_chunk_sort(G.rw, G.in, G.out, G.lk, G.d_cnt, G.d_base, nullptr, G.nsmall, G.d_ovf,
+        if ((rc = launch_chunk_sort(G.rw, G.in, G.out, G.lk, G.d_cnt, G.d_base, nullptr, 0, G.nsmall, G.d_ovf,
                                     G.d_counters, 1, G.d_hard, G.d_counters + 1, nullptr, false, st)))
             return rc;
     }
@@ -1737,6 +1744,7 @@ int mpsort_gpu_model_train(mpsort_gpu_model** out, const void* sorted, int64_t n
     M.raw = (float*)dev_alloc(sizeof(float) * kMlpParams);
     M.eff = (float*)dev_alloc(sizeof(float) * kMlpParams);
     M.leaves = (float*)dev_alloc(sizeof(float) * 4 * (size_t)M.L);
+    M.xstart = (float*)dev_alloc(sizeof(float) * ((size_t)M.L + 1));
     float* am = (float*)dev_alloc(sizeof(float) * kMlpParams);
     float* av = (float*)dev_alloc(sizeof(float) * kMlpParams);
     float* partial = (float*)dev_alloc(sizeof(float) * kMlpParams * (size_t)grad_blocks);
@@ -1751,7 +1759,7 @@ int mpsort_gpu_model_train(mpsort_gpu_model** out, const void* sorted, int64_t n
         dev_free(am); dev_free(av); dev_free(partial); dev_free(lossp); dev_free(skeys); dev_free(x); dev_free(y);
         dev_free(leaf); dev_free(start); dev_free(fval); dev_free(tmpbuf);
     };
-    if (!M.raw || !M.eff || !M.leaves || !am || !av || !partial || !lossp || !skeys || !x || !y || !leaf || !start ||
+    if (!M.raw || !M.eff || !M.leaves || !M.xstart || !am || !av || !partial || !lossp || !skeys || !x || !y || !leaf || !start ||
         !fval) {
         cleanup();
         mpsort_gpu_model_free(mo);
@@ -1846,6 +1854,7 @@ int mpsort_gpu_model_train(mpsort_gpu_model** out, const void* sorted, int64_t n
         }
         if ((r2 = launch_mlp_effective(M.raw, M.eff, st))) return r2;
         if ((r2 = launch_leaves(x, y, S, M.eff, M.L, M.linear_root, leaf, start, M.leaves, st))) return r2;
+        if ((r2 = launch_leaf_xstart(M.eff, M.L, M.linear_root, M.xstart, st))) return r2;
         std::vector<int64_t> hs(M.L + 1);
         MPSG_CHECK(hipMemcpyAsync(hs.data(), start, sizeof(int64_t) * (M.L + 1), hipMemcpyDeviceToHost, st));
         // error bound over the samples
@@ -1878,6 +1887,7 @@ int mpsort_gpu_model_free(mpsort_gpu_model* mo) {
     dev_free(mo->m.raw);
     dev_free(mo->m.eff);
     dev_free(mo->m.leaves);
+    dev_free(mo->m.xstart);
     dev_free(mo->m.heavy);
     delete mo;
     return 0;

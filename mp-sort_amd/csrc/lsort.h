@@ -28,22 +28,23 @@ inline LKey make_lkey(const KeySpec& ks) {
     return k;
 }
 
+// tables: segment s uses tables + s * tstride (tstride 0: one table shared)
 int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
-                 const uint64_t* tables, uint32_t* hist, long long* checksum, hipStream_t st);
+                 const uint64_t* tables, int64_t tstride, uint32_t* hist, long long* checksum, hipStream_t st);
 int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st);
 int lscatter_tile(int rw);
 // records of single-value (odd) buckets go to eq_dst, the rest to dst
 int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegDev& sd, int64_t ntiles,
-                    const LKey& lk, const uint64_t* tables, const int64_t* bucket_base, uint64_t* lookback,
-                    uint32_t* ticket, uint32_t epoch, uint32_t* err, hipStream_t st);
+                    const LKey& lk, const uint64_t* tables, int64_t tstride, const int64_t* bucket_base,
+                    uint64_t* lookback, uint32_t* ticket, uint32_t epoch, uint32_t* err, hipStream_t st);
 // Sorts every bucket (src -> dst, same slots).  eq_done: the scatter already
 // wrote the single-value buckets to dst.  Buckets above kChunkCap go to the
 // overflow list (*noverflow entries of {base, count}); hard (one uint32 per
 // bucket) and *nhard (zeroed by the caller) list the buckets handed from the
 // interpolation sort to the LSD sort.
 int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
-                      const int64_t* bucket_base, const uint64_t* tables, int64_t nbuckets, int64_t* overflow,
-                      uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, long long* checksum,
-                      bool eq_done, hipStream_t st, hipEvent_t mid = nullptr);
+                      const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
+                      int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
+                      long long* checksum, bool eq_done, hipStream_t st, hipEvent_t mid = nullptr);
 
 }  // namespace mpsg

@@ -57,7 +57,7 @@ __device__ __forceinline__ bool single_bucket(const uint64_t* __restrict__ t, in
 template <int RW>
 __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restrict__ src, SegDev seg,
                                                        int64_t ntiles, int64_t tpb, int tile_recs, LKey lk,
-                                                       const uint64_t* __restrict__ tables,
+                                                       const uint64_t* __restrict__ tables, int64_t tstride,
                                                        uint32_t* __restrict__ hist,
                                                        long long* __restrict__ checksum) {
     __shared__ uint32_t s_hist[kLBins];
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
                     s_hist[i] = 0;
                 }
             }
-            for (int i = tid; i < kLBounds; i += kBlock) s_tab[i] = tables[(int64_t)s * kLBounds + i];
+            for (int i = tid; i < kLBounds; i += kBlock) s_tab[i] = tables[(int64_t)s * tstride + i];
             __syncthreads();
             cur = s;
         }
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(1024) void lscan_kernel(const uint32_t* __restrict_
 template <int RW, int ITEMS>
 __global__ __launch_bounds__(1024) void lscatter_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg, LKey lk,
-    const uint64_t* __restrict__ tables, const int64_t* __restrict__ bucket_base,
+    const uint64_t* __restrict__ tables, int64_t tstride, const int64_t* __restrict__ bucket_base,
     uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket, uint32_t epoch,
     uint32_t* __restrict__ error_flag, uint32_t* __restrict__ eq_dst) {
     constexpr int BLOCK = 1024, NW = 16;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
     __syncthreads();
     const int64_t tile = s_tile;
     const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
-    for (int i = tid; i < kLBounds; i += BLOCK) s_tab[i] = tables[(int64_t)s * kLBounds + i];
+    for (int i = tid; i < kLBounds; i += BLOCK) s_tab[i] = tables[(int64_t)s * tstride + i];
     const int64_t t = tile - seg.tile_start[s];
     const int64_t sbeg = seg.begin[s];
     const int64_t send = sbeg + seg.count[s];
@@ -458,8 +458,8 @@ template <int RW>
 __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
-    const uint64_t* __restrict__ tables, int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow,
-    int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
+    const uint64_t* __restrict__ tables, int64_t tstride, int64_t* __restrict__ overflow,
+    uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
     long long* __restrict__ checksum_out, int eq_done) {
     __shared__ uint64_t s_key[kChunkCap];
     __shared__ uint32_t s_start[kFine + 1];
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     const uint32_t cnt32 = hist[b];
     if (cnt32 == 0) return;
     const int64_t base = bucket_base[b];
-    if (eq_done && single_bucket(tables + (b / kLBins) * kLBounds, (int)(b % kLBins))) {
+    if (eq_done && single_bucket(tables + (b / kLBins) * tstride, (int)(b % kLBins))) {
         if (checksum_out) {
             long long csum = 0;
             for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += kCBlock) {
@@ -741,14 +741,14 @@ __global__ __launch_bounds__(kCBlock) void lsd_sort_kernel(
 // ---------------------------------------------------------------------------
 // host launchers
 int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
-                 const uint64_t* tables, uint32_t* hist, long long* checksum, hipStream_t st) {
+                 const uint64_t* tables, int64_t tstride, uint32_t* hist, long long* checksum, hipStream_t st) {
     if (ntiles <= 0) return 0;
     const int64_t tpb = (ntiles + 2047) / 2048;
     const int64_t blocks = (ntiles + tpb - 1) / tpb;
     const uint32_t* s = (const uint32_t*)src;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(lhist_kernel<2>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, hist, checksum); break;
-        case 4: hipLaunchKernelGGL(lhist_kernel<4>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, hist, checksum); break;
+        case 2: hipLaunchKernelGGL(lhist_kernel<2>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, checksum); break;
+        case 4: hipLaunchKernelGGL(lhist_kernel<4>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, checksum); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
@@ -764,15 +764,15 @@ int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64
 int lscatter_tile(int rw) { return rw == 2 ? 1024 * 8 : 1024 * 4; }
 
 int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegDev& sd, int64_t ntiles,
-                    const LKey& lk, const uint64_t* tables, const int64_t* bucket_base, uint64_t* lookback,
-                    uint32_t* ticket, uint32_t epoch, uint32_t* err, hipStream_t st) {
+                    const LKey& lk, const uint64_t* tables, int64_t tstride, const int64_t* bucket_base,
+                    uint64_t* lookback, uint32_t* ticket, uint32_t epoch, uint32_t* err, hipStream_t st) {
     if (ntiles <= 0) return 0;
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
     uint32_t* e = (uint32_t*)eq_dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, bucket_base, lookback, ticket, epoch, err, e); break;
-        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, bucket_base, lookback, ticket, epoch, err, e); break;
+        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, tstride, bucket_base, lookback, ticket, epoch, err, e); break;
+        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, tstride, bucket_base, lookback, ticket, epoch, err, e); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
@@ -780,9 +780,9 @@ int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegD
 }
 
 int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
-                      const int64_t* bucket_base, const uint64_t* tables, int64_t nbuckets, int64_t* overflow,
-                      uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, long long* checksum,
-                      bool eq_done, hipStream_t st, hipEvent_t mid) {
+                      const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
+                      int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
+                      long long* checksum, bool eq_done, hipStream_t st, hipEvent_t mid) {
     if (nbuckets <= 0) {
         if (mid) MPSG_CHECK(hipEventRecord(mid, st));
         return 0;
@@ -794,7 +794,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
     switch (rw) {
         case 2:
             hipLaunchKernelGGL(interp_sort_kernel<2>, dim3((unsigned)nbuckets), dim3(kCBlock), 0, st, s, d, lk, hist,
-                               bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
+                               bucket_base, tables, tstride, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
@@ -802,7 +802,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
             break;
         case 4:
             hipLaunchKernelGGL(interp_sort_kernel<4>, dim3((unsigned)nbuckets), dim3(kCBlock), 0, st, s, d, lk, hist,
-                               bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
+                               bucket_base, tables, tstride, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,

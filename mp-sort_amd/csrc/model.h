@@ -26,6 +26,8 @@ struct ModelDev {
     double inv_range;     // 1 / (kmax - kmin)
     int log_input;        // 1: x = log2(1 + key - kmin) * log_scale
     double log_scale;     // 1 / log2(1 + kmax - kmin)
+    const float* xstart;  // L + 1: smallest x routed to each leaf (xstart[L] = 2)
+    uint64_t kmax;
 };
 
 int launch_mlp_effective(const float* raw, float* eff, hipStream_t st);
@@ -36,6 +38,8 @@ int launch_mlp_adam(float* raw, const float* partial, int nblocks, float* m, flo
                     float* grad_out, hipStream_t st);
 int launch_leaves(const float* x, const float* y, int64_t S, const float* eff, int L, int linear_root,
                   int* leaf, int64_t* start, float* leaves, hipStream_t st);
+// xstart[l] = smallest x the root routes to leaf l or beyond
+int launch_leaf_xstart(const float* eff, int L, int linear_root, float* xstart, hipStream_t st);
 int launch_model_eval(const ModelDev& m, const uint64_t* keys, int64_t n, float* out, hipStream_t st);
 int launch_model_tables(const ModelDev& m, uint64_t kmax, const float* q0, const float* q1, int ntables,
                         int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st);

@@ -284,63 +284,130 @@ __device__ uint64_t model_inverse(const float* __restrict__ e, const ModelDev& m
     return lo;
 }
 
-// Boundary tables: table t has nb boundaries for quantile targets
-// q = q0[t] + (q1[t] - q0[t]) * (i + 1) / (nb + 1); tables are made
-// non-decreasing.  One thread per boundary.
-__global__ void model_tables_kernel(ModelDev m, uint64_t kmax, const float* __restrict__ q0,
-                                    const float* __restrict__ q1, int ntables, int nb,
-                                    uint64_t* __restrict__ tables) {
+// leaf of x under the root (as model_eval routes it)
+__device__ __forceinline__ int leaf_of_x(const float* __restrict__ e, const ModelDev& m, float x) {
+    const float r = m.linear_root ? x : mlp_forward_only(e, x);
+    int l = (int)floorf(r * (float)m.L);
+    return l < 0 ? 0 : (l >= m.L ? m.L - 1 : l);
+}
+
+// xstart[l]: the smallest float x in [0, 1] with leaf_of_x(x) >= l, by
+// bisection on the bit pattern (non-negative floats order like their bits)
+__global__ void leaf_xstart_kernel(ModelDev m, float* __restrict__ xstart) {
     __shared__ float e[kMlpParams];
     for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) e[i] = m.eff[i];
     __syncthreads();
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > m.L) return;
+    if (l == 0) { xstart[0] = 0.f; return; }
+    if (l == m.L) { xstart[l] = 2.f; return; }
+    uint32_t lo = 0, hi = 0x3f800000u;  // [0, 1.0]
+    if (leaf_of_x(e, m, 1.0f) < l) { xstart[l] = 2.f; return; }
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (leaf_of_x(e, m, __uint_as_float(mid)) >= l) hi = mid; else lo = mid + 1;
+    }
+    xstart[l] = __uint_as_float(lo);
+}
+
+// key with x(key) ~ x (the inverse of key_to_x), clamped to [kmin, kmax]
+__device__ __forceinline__ uint64_t x_to_key(const ModelDev& m, float x) {
+    const double span = (double)(m.kmax - m.kmin);
+    double d = m.log_input ? exp2((double)x / m.log_scale) - 1.0 : (double)x * span;
+    d = d < 0.0 ? 0.0 : (d > span ? span : d);
+    const uint64_t k = m.kmin + (uint64_t)d;
+    return k > m.kmax ? m.kmax : k;
+}
+
+// F^-1(q) from the leaves alone: the first leaf whose range reaches q, then
+// the inverse of its line, kept inside the leaf's x interval.  Monotone in q;
+// the boundary tables need no more (bucket membership is an exact key
+// comparison whatever the boundaries are).
+__device__ uint64_t model_inverse_leaves(const ModelDev& m, float q) {
+    int lo = 0, hi = m.L;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (m.leaves[4 * (int64_t)mid + 3] < q) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= m.L) return m.kmax;
+    const float* lf = m.leaves + 4 * (int64_t)lo;
+    const float x0 = m.xstart[lo], x1 = fminf(m.xstart[lo + 1], 1.0f);
+    float x = x0;
+    if (lf[2] < q && lf[0] > 0.f) {
+        x = (q - lf[1]) / lf[0];
+        x = x < x0 ? x0 : (x > x1 ? x1 : x);
+    }
+    return x_to_key(m, x);
+}
+
+// Boundary tables: table t has nb boundaries for quantile targets
+// q = q0[t] + (q1[t] - q0[t]) * (i + 1) / (nb + 1); tables are made
+// non-decreasing.  One thread per boundary.
+__global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, const float* __restrict__ q1,
+                                    int ntables, int nb, uint64_t* __restrict__ tables) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (int64_t)ntables * nb) return;
     const int t = (int)(g / nb), i = (int)(g % nb);
     const float q = q0[t] + (q1[t] - q0[t]) * (float)(i + 1) / (float)(nb + 1);
-    tables[g] = model_inverse(e, m, kmax, q);
+    tables[g] = model_inverse_leaves(m, q);
 }
 
-// per table: running max so the table is sorted even if FP32 rounding made F
-// locally non-monotone; then every heavy key v (ascending) gets the boundary
-// pair {v - 1, v}, i.e. a bucket holding v alone.  Overwriting the two
-// boundaries around v's position keeps the table sorted; a pair that would
-// overwrite the previous heavy key's pair moves just after it, or is dropped
-// when the table has no room left.
-__global__ void tables_fix_kernel(uint64_t* __restrict__ tables, int ntables, int nb,
-                                  const uint64_t* __restrict__ heavy, int nheavy) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntables) return;
-    uint64_t* tb = tables + (int64_t)t * nb;
-    for (int i = 1; i < nb; i++)
-        if (tb[i] < tb[i - 1]) tb[i] = tb[i - 1];
-    if (nb < 2) return;
-    int last = -1;  // slot of the previous heavy key
-    for (int h = 0; h < nheavy; h++) {
-        const uint64_t v = heavy[h];
-        if (v == 0) {
-            tb[0] = 0;  // bucket 0 = keys <= 0
-            last = 0;
-            continue;
-        }
-        int a = 0, b = nb;  // a = first slot with tb[a] >= v
-        while (a < b) {
-            const int mid = (a + b) >> 1;
-            if (tb[mid] < v) a = mid + 1; else b = mid;
-        }
-        if (a < nb && tb[a] == v && a > 0 && tb[a - 1] == v - 1) {
-            last = a;
-            continue;
-        }
-        int j = a < 1 ? 1 : (a >= nb ? nb - 1 : a);
-        if (j - 1 <= last) {
-            // tb[last] holds the previous heavy key u < v
-            j = (tb[last] == v - 1) ? last + 1 : last + 2;
-            if (j >= nb) break;
-        }
-        tb[j - 1] = v - 1;
-        tb[j] = v;
-        last = j;
+// per table (one block each, nb <= 1024): running max so the table is sorted
+// even if FP32 rounding made F locally non-monotone; then every heavy key v
+// (ascending) gets the boundary pair {v - 1, v}, i.e. a bucket holding v
+// alone.  Overwriting the two boundaries around v's position keeps the table
+// sorted; a pair that would overwrite the previous heavy key's pair moves
+// just after it, or is dropped when the table has no room left.
+__global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__ tables, int ntables, int nb,
+                                                          const uint64_t* __restrict__ heavy, int nheavy) {
+    __shared__ uint64_t s_t[1024];
+    __shared__ uint64_t s_w[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint64_t* tb = tables + (int64_t)blockIdx.x * nb;
+    uint64_t v = tid < nb ? tb[tid] : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t u = __shfl_up(v, off, 64);
+        if (lane >= off && u > v) v = u;
     }
+    if (lane == 63) s_w[wave] = v;
+    __syncthreads();
+    uint64_t pre = 0;
+    for (int w = 0; w < wave; w++) pre = s_w[w] > pre ? s_w[w] : pre;
+    v = pre > v ? pre : v;
+    if (tid < nb) s_t[tid] = v;
+    __syncthreads();
+    if (tid == 0 && nb >= 2) {
+        int last = -1;  // slot of the previous heavy key
+        for (int h = 0; h < nheavy; h++) {
+            const uint64_t hv = heavy[h];
+            if (hv == 0) {
+                s_t[0] = 0;  // bucket 0 = keys <= 0
+                last = 0;
+                continue;
+            }
+            int a = 0, b = nb;  // a = first slot with s_t[a] >= hv
+            while (a < b) {
+                const int mid = (a + b) >> 1;
+                if (s_t[mid] < hv) a = mid + 1; else b = mid;
+            }
+            if (a < nb && s_t[a] == hv && a > 0 && s_t[a - 1] == hv - 1) {
+                last = a;
+                continue;
+            }
+            int j = a < 1 ? 1 : (a >= nb ? nb - 1 : a);
+            if (j - 1 <= last) {
+                // s_t[last] holds the previous heavy key u < hv
+                j = (s_t[last] == hv - 1) ? last + 1 : last + 2;
+                if (j >= nb) break;
+            }
+            s_t[j - 1] = hv - 1;
+            s_t[j] = hv;
+            last = j;
+        }
+    }
+    __syncthreads();
+    if (tid < nb) tb[tid] = s_t[tid];
 }
 
 // smallest key in [kmin, kmax] with F(key) >= q for a monotone F, found by
@@ -450,6 +517,16 @@ int launch_leaves(const float* x, const float* y, int64_t S, const float* eff, i
     return 0;
 }
 
+int launch_leaf_xstart(const float* eff, int L, int linear_root, float* xstart, hipStream_t st) {
+    ModelDev m{};
+    m.eff = eff;
+    m.L = L;
+    m.linear_root = linear_root;
+    hipLaunchKernelGGL(leaf_xstart_kernel, dim3(nblk(L + 1, 128)), dim3(128), 0, st, m, xstart);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_model_eval(const ModelDev& m, const uint64_t* keys, int64_t n, float* out, hipStream_t st) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(model_eval_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, m, keys, n, out);
@@ -460,11 +537,11 @@ int launch_model_eval(const ModelDev& m, const uint64_t* keys, int64_t n, float*
 int launch_model_tables(const ModelDev& m, uint64_t kmax, const float* q0, const float* q1, int ntables,
                         int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st) {
     const int64_t tot = (int64_t)ntables * nb;
-    hipLaunchKernelGGL(model_tables_kernel, dim3(nblk(tot, 128)), dim3(128), 0, st, m, kmax, q0, q1, ntables, nb,
-                       tables);
+    (void)kmax;
+    hipLaunchKernelGGL(model_tables_kernel, dim3(nblk(tot, 128)), dim3(128), 0, st, m, q0, q1, ntables, nb, tables);
     MPSG_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(tables_fix_kernel, dim3(nblk(ntables, 64)), dim3(64), 0, st, tables, ntables, nb, heavy,
-                       nheavy);
+    if (nb > 1024) return -1;
+    hipLaunchKernelGGL(tables_fix_kernel, dim3(ntables), dim3(1024), 0, st, tables, ntables, nb, heavy, nheavy);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
