@@ -113,6 +113,7 @@ struct DevCtx {
     Buf ovf;            // plans of the learned-path fallback
     Buf small;          // counts, tables
     Buf lookback;       // look-back flags (only ever holds flag words)
+    Buf ltile;          // learned scatter: per-tile bucket counts and offsets
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     hipEvent_t ev[kEvPhase];
@@ -729,20 +730,23 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     const int64_t ntiles = make_tiles(cnt.data(), P, trecs, beg.data(), lt.data());
     MPSG_CHECK(hipMemcpyAsync(L.ltiles, lt.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
     MPSG_CHECK(hipStreamSynchronize(st));
-    if ((size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint64_t) > c.lookback.bytes)
-        return fail(MPSORT_ERR_ARG, "look-back buffer too small for the bucket scatter");
+    // per-tile bucket counts (u16) and per-tile offsets (u32)
+    const size_t tc_bytes = ((size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint16_t) + 255) & ~(size_t)255;
+    if ((rc = c.ltile.ensure(tc_bytes + (size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint32_t)))) return rc;
+    uint16_t* tile_cnt = (uint16_t*)c.ltile.p;
+    uint32_t* tile_off = (uint32_t*)((char*)c.ltile.p + tc_bytes);
     SegDev sd{L.begin, L.count, L.ltiles, P, 0};
     if ((rc = mark(0))) return rc;
-    if ((rc = launch_lhist(rw, src, sd, ntiles, (int)trecs, lk, L.tables, L.tstride, L.lhist, cs_in, st))) return rc;
+    if ((rc = launch_lhist(rw, src, sd, ntiles, (int)trecs, lk, L.tables, L.tstride, L.lhist, tile_cnt, cs_in, st)))
+        return rc;
     if ((rc = launch_lscan(P, L.lhist, L.begin, L.bbase, st))) return rc;
+    if ((rc = launch_ltile_scan(tile_cnt, sd, L.bbase, tile_off, st))) return rc;
     if ((rc = mark(1))) return rc;
-    int dev = 0;
-    MPSG_CHECK(hipGetDevice(&dev));
     // single-value buckets are written to their final slots by the scatter
     // unless that would overwrite records the scatter still reads
     const bool eq_direct = dst != src;
-    if ((rc = launch_lscatter(rw, src, tmp, eq_direct ? dst : tmp, sd, ntiles, lk, L.tables, L.tstride, L.bbase,
-                              (uint64_t*)c.lookback.p, L.tickets, next_epoch(dev), L.err, st)))
+    if ((rc = launch_lscatter(rw, src, tmp, eq_direct ? dst : tmp, sd, ntiles, lk, L.tables, L.tstride, tile_off,
+                              L.err, st)))
         return rc;
     if ((rc = mark(2))) return rc;
     if ((rc = launch_chunk_sort(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, L.tstride, (int64_t)P * kLBins, L.overflow,
@@ -887,13 +891,9 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     }
     const int64_t ntiles = ntA > ntB ? ntA : ntB;
     {
-        // one size for every scatter of this call: radix passes, bucket
-        // scatter (1024 bins) and the bucket fallback
-        const int64_t lt = lscatter_tile(rw);
-        int64_t lA = 0, lB = 0;
-        for (int t = 0; t < P; t++) { lA += (nin[t] + lt - 1) / lt; lB += (nout[t] + lt - 1) / lt; }
-        const size_t need = std::max((size_t)(ntiles + kMaxFallbackSegs + 1) * kBins,
-                                     (size_t)(std::max(lA, lB) + 1) * kLBins) * sizeof(uint64_t);
+        // one size for every radix pass of this call (whole sides and the
+        // bucket fallback)
+        const size_t need = (size_t)(ntiles + kMaxFallbackSegs + 1) * kBins * sizeof(uint64_t);
         if ((rc = c.lookback.ensure(need))) return rc;
     }
     auto up = [&](void* d, const void* hsrc, size_t bytes) {
@@ -1553,6 +1553,7 @@ int mpsort_gpu_release(void) {
     c.ovf.release();
     c.small.release();
     c.lookback.release();
+    c.ltile.release();
     c.last_P = 0;
     return 0;
 }

@@ -59,12 +59,17 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
                                                        int64_t ntiles, int64_t tpb, int tile_recs, LKey lk,
                                                        const uint64_t* __restrict__ tables, int64_t tstride,
                                                        uint32_t* __restrict__ hist,
+                                                       uint16_t* __restrict__ tile_cnt,
                                                        long long* __restrict__ checksum) {
-    __shared__ uint32_t s_hist[kLBins];
+    __shared__ uint32_t s_hist[kLBins];  // the block's counts for its current segment
+    __shared__ uint32_t s_th[kLBins];    // counts of the current tile
     __shared__ uint64_t s_tab[kLBounds];
     __shared__ long long s_sum[kWaves];
     const int tid = threadIdx.x;
-    for (int i = tid; i < kLBins; i += kBlock) s_hist[i] = 0;
+    for (int i = tid; i < kLBins; i += kBlock) {
+        s_hist[i] = 0;
+        s_th[i] = 0;
+    }
     const int64_t t0 = (int64_t)blockIdx.x * tpb;
     const int64_t t1 = min(ntiles, t0 + tpb);
     long long csum = 0;
@@ -96,10 +101,18 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
 #pragma unroll
                 for (int w = 0; w < RW; w++) csum += sbyte_sum(r[w]);
             }
-            atomicAdd(&s_hist[bucket_of(lkey<RW>(r, lk), s_tab)], 1u);
+            atomicAdd(&s_th[bucket_of(lkey<RW>(r, lk), s_tab)], 1u);
         }
+        __syncthreads();
+        // the tile's counts (for the scatter's offsets), added to the segment
+        for (int i = tid; i < kLBins; i += kBlock) {
+            const uint32_t c = s_th[i];
+            tile_cnt[tile * kLBins + i] = (uint16_t)c;
+            s_hist[i] += c;
+            s_th[i] = 0;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (cur >= 0) {
         uint32_t* h = hist + (int64_t)cur * kLBins;
         for (int i = tid; i < kLBins; i += kBlock) {
@@ -148,13 +161,41 @@ __global__ __launch_bounds__(1024) void lscan_kernel(const uint32_t* __restrict_
     base[(int64_t)s * kLBins + tid] = seg_begin[s] + (int64_t)ex;
 }
 
+// offset of every (tile, bucket) run inside its segment: the bucket's start
+// plus the counts of the segment's earlier tiles.  Thread per (segment,
+// bucket); the segment's tiles in order.
+__global__ __launch_bounds__(256) void ltile_scan_kernel(const uint16_t* __restrict__ tile_cnt, SegDev seg,
+                                                         const int64_t* __restrict__ bucket_base,
+                                                         uint32_t* __restrict__ tile_off) {
+    constexpr int PARTS = kLBins / 256;
+    const int s = blockIdx.x / PARTS;
+    const int b = (blockIdx.x % PARTS) * 256 + threadIdx.x;
+    const int64_t t1 = seg.tile_start[s + 1];
+    uint32_t run = (uint32_t)(bucket_base[(int64_t)s * kLBins + b] - seg.begin[s]);
+    int64_t t = seg.tile_start[s];
+    for (; t + 8 <= t1; t += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) c[k] = tile_cnt[(t + k) * kLBins + b];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            tile_off[(t + k) * kLBins + b] = run;
+            run += c[k];
+        }
+    }
+    for (; t < t1; t++) {
+        const uint32_t c = tile_cnt[t * kLBins + b];
+        tile_off[t * kLBins + b] = run;
+        run += c;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // the bucket scatter: a onesweep pass with 1024 bins
 template <int RW, int ITEMS>
 __global__ __launch_bounds__(1024) void lscatter_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg, LKey lk,
-    const uint64_t* __restrict__ tables, int64_t tstride, const int64_t* __restrict__ bucket_base,
-    uint64_t* __restrict__ lookback, uint32_t* __restrict__ ticket, uint32_t epoch,
+    const uint64_t* __restrict__ tables, int64_t tstride, const uint32_t* __restrict__ tile_off,
     uint32_t* __restrict__ error_flag, uint32_t* __restrict__ eq_dst) {
     constexpr int BLOCK = 1024, NW = 16;
     constexpr int TILE = BLOCK * ITEMS;
@@ -166,14 +207,11 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
     __shared__ int64_t s_gdelta[kLBins];
     __shared__ uint64_t s_tab[kLBounds];
     __shared__ uint32_t s_single[kLBins / 32];
-    __shared__ int64_t s_tile;
     __shared__ uint32_t s_tmp[16];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    if (tid == 0) s_tile = (int64_t)atomicAdd(ticket, 1u);
     for (int i = tid; i < NW * kLBins; i += BLOCK) (&s_whist[0][0])[i] = 0;
-    __syncthreads();
-    const int64_t tile = s_tile;
+    const int64_t tile = blockIdx.x;
     const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
     for (int i = tid; i < kLBounds; i += BLOCK) s_tab[i] = tables[(int64_t)s * tstride + i];
     const int64_t t = tile - seg.tile_start[s];
@@ -236,47 +274,7 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
     }
     const uint32_t excl = scan1024(total, s_tmp);
     s_excl[d] = excl;
-    int64_t prefix = 0;
-    uint64_t* my = lookback + tile * kLBins + d;
-    if (t == 0) {
-        store_agent(my, make_flag(epoch, 1, total));
-    } else {
-        store_agent(my, make_flag(epoch, 0, total));
-        const int64_t first = seg.tile_start[s];
-        int64_t j = tile - 1;
-        uint32_t spins = 0;
-        while (true) {
-            uint64_t f[2];
-#pragma unroll
-            for (int q = 0; q < 2; q++)
-                f[q] = (j - q >= first) ? load_agent(lookback + (j - q) * kLBins + d) : 0ull;
-            int consumed = 0;
-            bool stop = false, done = false;
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                if (!stop && j - q >= first) {
-                    if ((uint32_t)(f[q] >> 41) != epoch) {
-                        stop = true;
-                    } else {
-                        prefix += (int64_t)(f[q] & kFlagValueMask);
-                        consumed = q + 1;
-                        if ((f[q] >> 40) & 1ull) { done = true; stop = true; }
-                    }
-                }
-            }
-            if (done) break;
-            j -= consumed;
-            if (consumed == 0) {
-                if (++spins > kSpinLimit) {
-                    atomicOr(error_flag, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        store_agent(my, make_flag(epoch, 1, (uint64_t)(prefix + total)));
-    }
-    s_gdelta[d] = bucket_base[(int64_t)s * kLBins + d] + prefix - (int64_t)excl;
+    s_gdelta[d] = sbeg + (int64_t)tile_off[tile * kLBins + d] - (int64_t)excl;
     __syncthreads();
 
 #pragma unroll
@@ -741,14 +739,15 @@ __global__ __launch_bounds__(kCBlock) void lsd_sort_kernel(
 // ---------------------------------------------------------------------------
 // host launchers
 int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
-                 const uint64_t* tables, int64_t tstride, uint32_t* hist, long long* checksum, hipStream_t st) {
+                 const uint64_t* tables, int64_t tstride, uint32_t* hist, uint16_t* tile_cnt, long long* checksum,
+                 hipStream_t st) {
     if (ntiles <= 0) return 0;
     const int64_t tpb = (ntiles + 2047) / 2048;
     const int64_t blocks = (ntiles + tpb - 1) / tpb;
     const uint32_t* s = (const uint32_t*)src;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(lhist_kernel<2>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, checksum); break;
-        case 4: hipLaunchKernelGGL(lhist_kernel<4>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, checksum); break;
+        case 2: hipLaunchKernelGGL(lhist_kernel<2>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, tile_cnt, checksum); break;
+        case 4: hipLaunchKernelGGL(lhist_kernel<4>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, tile_cnt, checksum); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
@@ -763,16 +762,25 @@ int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64
 
 int lscatter_tile(int rw) { return rw == 2 ? 1024 * 8 : 1024 * 4; }
 
+int launch_ltile_scan(const uint16_t* tile_cnt, const SegDev& sd, const int64_t* bucket_base, uint32_t* tile_off,
+                      hipStream_t st) {
+    if (sd.nseg <= 0) return 0;
+    hipLaunchKernelGGL(ltile_scan_kernel, dim3((unsigned)sd.nseg * (kLBins / 256)), dim3(256), 0, st, tile_cnt, sd,
+                       bucket_base, tile_off);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegDev& sd, int64_t ntiles,
-                    const LKey& lk, const uint64_t* tables, int64_t tstride, const int64_t* bucket_base,
-                    uint64_t* lookback, uint32_t* ticket, uint32_t epoch, uint32_t* err, hipStream_t st) {
+                    const LKey& lk, const uint64_t* tables, int64_t tstride, const uint32_t* tile_off, uint32_t* err,
+                    hipStream_t st) {
     if (ntiles <= 0) return 0;
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
     uint32_t* e = (uint32_t*)eq_dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, tstride, bucket_base, lookback, ticket, epoch, err, e); break;
-        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, tstride, bucket_base, lookback, ticket, epoch, err, e); break;
+        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, tstride, tile_off, err, e); break;
+        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, tstride, tile_off, err, e); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
