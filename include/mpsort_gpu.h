@@ -67,6 +67,9 @@ typedef struct {
 #define MPSORT_GPU_TIMING (1u << 2)      /* record per-pass kernel times */
 #define MPSORT_GPU_LEARNED (1u << 3)     /* learned-CDF splitters + bucket sort */
 #define MPSORT_GPU_FORCE_GENERAL (1u << 4) /* always sort packed {key, index} records */
+#define MPSORT_GPU_LEARNED_EXCHANGE (1u << 5) /* learned path: explicit exchange, then the
+                                                 SecondSort (default: each receiver bucket is
+                                                 gathered from the senders' sorted runs) */
 
 /*
  * Sort the P virtual ranks of `base` (rank t holds nin[t] records, in rank

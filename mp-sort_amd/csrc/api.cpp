@@ -114,6 +114,7 @@ struct DevCtx {
     Buf small;          // counts, tables
     Buf lookback;       // look-back flags (only ever holds flag words)
     Buf ltile;          // learned scatter: per-tile bucket counts and offsets
+    Buf mbnd;           // merged SecondSort: sub-run bounds per (receiver, sender, bucket)
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     hipEvent_t ev[kEvPhase];
@@ -295,6 +296,7 @@ struct CallShape {
     int key_bytes_user;
     bool checksum;
     bool timing;
+    bool exchange;  // learned path: keep the explicit exchange before the SecondSort
 };
 
 static double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -554,6 +556,7 @@ static int shape_for(const mpsort_key_desc* key, size_t elsize, unsigned flags, 
     }
     cs->checksum = !(flags & MPSORT_GPU_NO_CHECKSUM);
     cs->timing = (flags & MPSORT_GPU_TIMING) != 0;
+    cs->exchange = (flags & MPSORT_GPU_LEARNED_EXCHANGE) != 0;
     return 0;
 }
 
@@ -687,6 +690,64 @@ struct LearnedCounters {
     int global_fallback = 0;
 };
 
+// Radix sort of the listed segments (device list of nov {begin, count}
+// pairs) from src into dst at the same positions; tmp is the third buffer
+// of the digit passes.  plain supplies the record shape and key.
+static int sort_listed_segments(DevCtx& c, const SegSortPlan& plain, int rw, const int64_t* d_list, uint32_t nov,
+                                const void* src, void* dst, void* tmp, long long* cs_out, hipStream_t st) {
+    int rc = 0, passes = 0;
+    std::vector<int64_t> ov(2 * (size_t)nov);
+    MPSG_CHECK(hipMemcpyAsync(ov.data(), d_list, sizeof(int64_t) * 2 * nov, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    const int ns = (int)nov;
+    std::vector<int64_t> ob(ns), oc(ns), ot(ns + 1), ob2(ns);
+    // segments must be in ascending position order for the tile tables
+    std::vector<int> ord(ns);
+    for (int i = 0; i < ns; i++) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b2) { return ov[2 * a] < ov[2 * b2]; });
+    for (int i = 0; i < ns; i++) { ob[i] = ov[2 * ord[i]]; oc[i] = ov[2 * ord[i] + 1]; }
+    const TileCfg tile = default_tile(rw);
+    const int64_t ptrecs = (int64_t)tile.block * tile.items;
+    const int64_t ont = make_tiles(oc.data(), ns, ptrecs, ob2.data(), ot.data());
+    Sizer sz;
+    lay_plan(sz, ns, plain.ks.key_bytes, nullptr);
+    if ((rc = c.ovf.ensure(sz.off + 4096))) return rc;
+    Arena ar{(char*)c.ovf.p};
+    PlanArrays pa;
+    lay_plan(ar, ns, plain.ks.key_bytes, &pa);
+    MPSG_CHECK(hipMemcpyAsync(pa.begin, ob.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipMemcpyAsync(pa.count, oc.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipMemcpyAsync(pa.tile_start, ot.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if ((size_t)(ont > 0 ? ont : 1) * kBins * sizeof(uint64_t) > c.lookback.bytes)
+        return fail(MPSORT_ERR_ARG, "look-back buffer too small for the bucket fallback");
+    SegSortPlan op = plain;
+    op.tile = tile;
+    op.nseg = ns;
+    op.ntiles = ont;
+    op.nrec = 0;
+    for (int i = 0; i < ns; i++) op.nrec += oc[i];
+    op.d_begin = pa.begin;
+    op.d_count = pa.count;
+    op.d_tile_start = pa.tile_start;
+    op.d_hist = pa.hist;
+    op.d_digit_base = pa.digit_base;
+    op.d_pass_flags = pa.pass_flags;
+    op.d_tickets = pa.tickets;
+    op.d_lookback = (uint64_t*)c.lookback.p;
+    op.sparse = true;
+    op.all_passes = false;
+    op.d_epoch_base = nullptr;
+    SortBuffers b{const_cast<void*>(src), dst, tmp};
+    if ((rc = seg_count(op, src, nullptr, st))) return rc;
+    if ((rc = seg_passes(op, b, st, cs_out, &passes, nullptr))) return rc;
+    uint32_t e2 = 0;
+    MPSG_CHECK(hipMemcpyAsync(&e2, pa.pass_flags + kMaxPasses, sizeof e2, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if (e2) return fail(MPSORT_ERR_TIMEOUT, "bucket fallback sort: device flag 0x%x", e2);
+    return 0;
+}
+
 // model-bucketed local sort of every segment of `side` from src into dst
 // (tmp receives the bucket scatter).  plain is the radix plan of the same
 // segments, used when too many buckets overflow.
@@ -779,51 +840,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         return mark(5);
     }
     // radix sort of the oversized buckets only (they are segments of tmp)
-    std::vector<int64_t> ov(2 * (size_t)nov);
-    MPSG_CHECK(hipMemcpy(ov.data(), L.overflow, sizeof(int64_t) * 2 * nov, hipMemcpyDeviceToHost));
-    const int ns = (int)nov;
-    std::vector<int64_t> ob(ns), oc(ns), ot(ns + 1), ob2(ns);
-    // segments must be in ascending position order for the tile tables
-    std::vector<int> ord(ns);
-    for (int i = 0; i < ns; i++) ord[i] = i;
-    std::sort(ord.begin(), ord.end(), [&](int a, int b2) { return ov[2 * a] < ov[2 * b2]; });
-    for (int i = 0; i < ns; i++) { ob[i] = ov[2 * ord[i]]; oc[i] = ov[2 * ord[i] + 1]; }
-    const TileCfg tile = default_tile(rw);
-    const int64_t ptrecs = (int64_t)tile.block * tile.items;
-    const int64_t ont = make_tiles(oc.data(), ns, ptrecs, ob2.data(), ot.data());
-    Sizer sz;
-    lay_plan(sz, ns, ks.key_bytes, nullptr);
-    if ((rc = c.ovf.ensure(sz.off + 4096))) return rc;
-    Arena ar{(char*)c.ovf.p};
-    PlanArrays pa;
-    lay_plan(ar, ns, ks.key_bytes, &pa);
-    MPSG_CHECK(hipMemcpyAsync(pa.begin, ob.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(pa.count, oc.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(pa.tile_start, ot.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipStreamSynchronize(st));
-    if ((size_t)(ont > 0 ? ont : 1) * kBins * sizeof(uint64_t) > c.lookback.bytes)
-        return fail(MPSORT_ERR_ARG, "look-back buffer too small for the bucket fallback");
-    SegSortPlan op = plain;
-    op.nseg = ns;
-    op.ntiles = ont;
-    op.nrec = 0;
-    for (int i = 0; i < ns; i++) op.nrec += oc[i];
-    op.d_begin = pa.begin;
-    op.d_count = pa.count;
-    op.d_tile_start = pa.tile_start;
-    op.d_hist = pa.hist;
-    op.d_digit_base = pa.digit_base;
-    op.d_pass_flags = pa.pass_flags;
-    op.d_tickets = pa.tickets;
-    op.d_lookback = (uint64_t*)c.lookback.p;
-    op.sparse = true;
-    SortBuffers b{tmp, dst, c.s2.p};
-    if ((rc = seg_count(op, tmp, nullptr, st))) return rc;
-    if ((rc = seg_passes(op, b, st, cs_out, &passes, nullptr))) return rc;
-    uint32_t e2 = 0;
-    MPSG_CHECK(hipMemcpyAsync(&e2, pa.pass_flags + kMaxPasses, sizeof e2, hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipStreamSynchronize(st));
-    if (e2) return fail(MPSORT_ERR_TIMEOUT, "bucket fallback sort: device flag 0x%x", e2);
+    if ((rc = sort_listed_segments(c, plain, rw, L.overflow, nov, tmp, dst, c.s2.p, cs_out, st))) return rc;
     return mark(5);
 }
 
@@ -986,14 +1003,54 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                                 h.run_src, h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
-        if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, P * P, h.run_dst, h.run_src, h.err, st))) return rc;
-        MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
         // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n
         if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, kLBounds, LB.tables, M.heavy, M.nheavy, st)))
             return rc;
-        if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LB, planB, c.s1.p, final_dst, c.s0.p, nullptr,
-                                     cs_direct ? h.checksum + 1 : nullptr, &lc, st, evB)))
-            return rc;
+        const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4);
+        if (merged) {
+            // no exchange: every receiver bucket is gathered from the senders'
+            // sorted segments and sorted in LDS
+            auto markB = [&](int i) -> int {
+                if (evB) MPSG_CHECK(hipEventRecord(evB[i], st));
+                return 0;
+            };
+            const LKey lk = make_lkey(cs.ks);
+            if ((rc = c.mbnd.ensure((size_t)P * P * (kLBins + 1) * sizeof(int32_t)))) return rc;
+            int32_t* bnd = (int32_t*)c.mbnd.p;
+            if ((rc = markB(0))) return rc;
+            if ((rc = launch_merge_bounds(rw, c.s0.p, lk, pa.begin, h.myC, P, LB.tables, bnd, st))) return rc;
+            if ((rc = launch_merge_count(bnd, P, LB.lhist, st))) return rc;
+            if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st))) return rc;
+            if ((rc = launch_fill_u32(LB.noverflow, 0u, 4, st))) return rc;
+            MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
+            if ((rc = markB(1)) || (rc = markB(2))) return rc;
+            if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase, LB.tables,
+                                          LB.overflow, LB.noverflow, kMaxOverflowList, LB.hard, LB.noverflow + 1,
+                                          cs_direct ? h.checksum + 1 : nullptr, st)))
+                return rc;
+            if ((rc = markB(3))) return rc;
+            // crowded buckets were copied in place in sender order
+            if ((rc = launch_lsd_sort(rw, final_dst, final_dst, lk, LB.lhist, LB.bbase, LB.hard, LB.noverflow + 1,
+                                      (int64_t)P * kLBins, nullptr, st)))
+                return rc;
+            if ((rc = markB(4))) return rc;
+            uint32_t novh[2] = {0, 0};
+            MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipStreamSynchronize(st));
+            const uint32_t nov = novh[0];
+            lc.overflow_buckets += (int)nov;
+            lc.hard_buckets += (int)novh[1];
+            if (nov && (rc = sort_listed_segments(c, planB, rw, LB.overflow, nov, final_dst, final_dst, c.s1.p,
+                                                  nullptr, st)))
+                return rc;
+            if ((rc = markB(5))) return rc;
+        } else {
+            if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, P * P, h.run_dst, h.run_src, h.err, st))) return rc;
+            MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
+            if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LB, planB, c.s1.p, final_dst, c.s0.p, nullptr,
+                                         cs_direct ? h.checksum + 1 : nullptr, &lc, st, evB)))
+                return rc;
+        }
         MPSG_CHECK(hipEventRecord(c.ev[kEvSecond], st));
     }
     if (cs.general) {
@@ -1554,6 +1611,7 @@ int mpsort_gpu_release(void) {
     c.small.release();
     c.lookback.release();
     c.ltile.release();
+    c.mbnd.release();
     c.last_P = 0;
     return 0;
 }

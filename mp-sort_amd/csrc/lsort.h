@@ -52,4 +52,20 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
                       int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
                       long long* checksum, bool eq_done, hipStream_t st, hipEvent_t mid = nullptr);
 
+// SecondSort straight from the senders' sorted segments (no exchange), for
+// P <= 64 receivers with 1023-boundary tables each:
+//   bnd: P * P * 1025 int32; hist / bucket_base: P * 1024
+int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* myC, int P,
+                        const uint64_t* tables, int32_t* bnd, hipStream_t st);
+int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st);
+int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
+                        const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
+                        const uint64_t* tables, int64_t* overflow, uint32_t* noverflow, int max_overflow,
+                        uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st);
+// LDS radix sort of the listed buckets (src may equal dst)
+int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
+                    const int64_t* bucket_base, const uint32_t* hard, const uint32_t* nhard, int64_t max_buckets,
+                    long long* checksum, hipStream_t st);
+constexpr int kMergeMaxPH = 64;  // largest P of the merged SecondSort
+
 }  // namespace mpsg

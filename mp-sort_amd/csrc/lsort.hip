@@ -452,13 +452,36 @@ __device__ __forceinline__ void lds_count_pass(const uint16_t* s_in, uint16_t* s
 constexpr int kSamp = 64, kSub = 63, kFine = (kSamp + 1) * kSub;  // 4095 bins
 constexpr int kInterpMax = 32;
 
+// the records of one bucket, contiguous in src from base
 template <int RW>
-__global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
-    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
-    const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
-    const uint64_t* __restrict__ tables, int64_t tstride, int64_t* __restrict__ overflow,
-    uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
-    long long* __restrict__ checksum_out, int eq_done) {
+struct RunSrc {
+    const uint32_t* recs;
+    int64_t base;
+    __device__ __forceinline__ void load(int i, uint32_t (&r)[RW]) const { RecIO<RW>::load(recs, base + i, r); }
+};
+
+// the records of one receiver bucket gathered from the senders' sorted
+// segments, in sender order: sub-run j starts at begin[j], and records
+// [pref[j], pref[j+1]) of the bucket come from it (LDS tables, nrun runs)
+template <int RW>
+struct GatherSrc {
+    const uint32_t* recs;
+    const int64_t* begin;
+    const int* pref;
+    int nrun;
+    __device__ __forceinline__ void load(int i, uint32_t (&r)[RW]) const {
+        const int j = upper_index<int>(pref, nrun, i);
+        RecIO<RW>::load(recs, begin[j] + (i - pref[j]), r);
+    }
+};
+
+// Sorts one bucket of count (1 .. kChunkCap) records, given by src, into
+// dst[base .. base + count).  Returns false (and writes nothing) when a bin
+// is crowded beyond kRunMax; the records are then in r, in bucket order.
+template <int RW, class Src>
+__device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
+                                                const LKey& lk, uint32_t (&r)[kCItems][RW],
+                                                long long* __restrict__ checksum_out) {
     __shared__ uint64_t s_key[kChunkCap];
     __shared__ uint32_t s_start[kFine + 1];
     __shared__ uint32_t s_fill[kFine + 1];
@@ -472,39 +495,10 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     __shared__ int s_long;
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int64_t b = blockIdx.x;
-    const uint32_t cnt32 = hist[b];
-    if (cnt32 == 0) return;
-    const int64_t base = bucket_base[b];
-    if (eq_done && single_bucket(tables + (b / kLBins) * tstride, (int)(b % kLBins))) {
-        if (checksum_out) {
-            long long csum = 0;
-            for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += kCBlock) {
-                uint32_t rr[RW];
-                RecIO<RW>::load(dst, i, rr);
-#pragma unroll
-                for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
-            }
-            block_checksum_add(csum, s_sum, checksum_out);
-        }
-        return;
-    }
-    if (cnt32 > (uint32_t)kChunkCap) {
-        if (tid == 0) {
-            const uint32_t slot = atomicAdd(noverflow, 1u);
-            if ((int)slot < max_overflow) {
-                overflow[2 * slot] = base;
-                overflow[2 * slot + 1] = (int64_t)cnt32;
-            }
-        }
-        return;
-    }
-    const int count = (int)cnt32;
     for (int i = tid; i <= kFine; i += kCBlock) s_fill[i] = 0;
     if (tid == 0) s_long = 0;
 
     // load (item k of thread t = record k * kCBlock + t) with min / max
-    uint32_t r[kCItems][RW];
     uint64_t kk[kCItems];
     uint64_t mn = ~0ull, mx = 0;
 #pragma unroll
@@ -512,7 +506,7 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
         const int i = k * kCBlock + tid;
         kk[k] = 0;
         if (i < count) {
-            RecIO<RW>::load(src, base + i, r[k]);
+            src.load(i, r[k]);
             kk[k] = lkey<RW>(r[k], lk);
             mn = kk[k] < mn ? kk[k] : mn;
             mx = kk[k] > mx ? kk[k] : mx;
@@ -548,7 +542,7 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
             }
         }
         if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
-        return;
+        return true;
     }
 
     const float scale = (float)kFine / ((float)(mx - mn) + 1.0f);
@@ -610,7 +604,7 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
         mode = 1;
         if (wave == 0) {
             uint32_t rr[RW];
-            RecIO<RW>::load(src, base + (int64_t)lane * count / kSamp, rr);
+            src.load((int)((int64_t)lane * count / kSamp), rr);
             uint64_t v = lkey<RW>(rr, lk);
 #pragma unroll
             for (int k = 2; k <= kSamp; k <<= 1) {
@@ -657,10 +651,7 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
         }
     }
     __syncthreads();
-    if (s_long) {
-        if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
-        return;
-    }
+    if (s_long) return false;
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
         const int i = k * kCBlock + tid;
@@ -673,6 +664,184 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
         }
     }
     if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
+    return true;
+}
+
+template <int RW>
+__global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
+    const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
+    const uint64_t* __restrict__ tables, int64_t tstride, int64_t* __restrict__ overflow,
+    uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
+    long long* __restrict__ checksum_out, int eq_done) {
+    __shared__ long long s_sum[kCWaves];
+    const int tid = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    const uint32_t cnt32 = hist[b];
+    if (cnt32 == 0) return;
+    const int64_t base = bucket_base[b];
+    if (eq_done && single_bucket(tables + (b / kLBins) * tstride, (int)(b % kLBins))) {
+        if (checksum_out) {
+            long long csum = 0;
+            for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += kCBlock) {
+                uint32_t rr[RW];
+                RecIO<RW>::load(dst, i, rr);
+#pragma unroll
+                for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
+            }
+            block_checksum_add(csum, s_sum, checksum_out);
+        }
+        return;
+    }
+    if (cnt32 > (uint32_t)kChunkCap) {
+        if (tid == 0) {
+            const uint32_t slot = atomicAdd(noverflow, 1u);
+            if ((int)slot < max_overflow) {
+                overflow[2 * slot] = base;
+                overflow[2 * slot + 1] = (int64_t)cnt32;
+            }
+        }
+        return;
+    }
+    uint32_t r[kCItems][RW];
+    const RunSrc<RW> rs{src, base};
+    if (!sort_bucket_lds<RW>(rs, dst, base, (int)cnt32, lk, r, checksum_out))
+        if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
+}
+
+// ---------------------------------------------------------------------------
+// SecondSort without the exchange.  Receiver r's records from sender j are
+// positions [myC[j][r], myC[j][r+1]) of sender j's sorted segment; bucket k of
+// r holds the keys in (t_r[k-1], t_r[k]], i.e. one sub-run per sender.
+//   merge_bounds_kernel  bnd[r][j][k] = start of bucket k's sub-run in sender
+//                        j (relative to j's segment), k = 0..1024
+//   merge_count_kernel   bucket sizes (sum over senders)
+//   merge_bucket_kernel  one workgroup per (receiver, bucket): gathers the
+//                        sub-runs in sender order (the order the exchange
+//                        would have left them in) and sorts them in LDS;
+//                        single-value buckets are copied; buckets above the
+//                        LDS capacity or with a crowded bin are copied in
+//                        that order and listed for the LDS radix sort or the
+//                        radix path
+constexpr int kMergeMaxP = 64;
+
+template <int RW>
+__global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __restrict__ recs, LKey lk,
+                                                           const int64_t* __restrict__ seg_begin,
+                                                           const int64_t* __restrict__ myC, int P,
+                                                           const uint64_t* __restrict__ tables,
+                                                           int32_t* __restrict__ bnd) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, j, k), k = 0..1024
+    if (g >= (int64_t)P * P * (kLBins + 1)) return;
+    const int k = (int)(g % (kLBins + 1));
+    const int64_t rj = g / (kLBins + 1);
+    const int j = (int)(rj % P), r = (int)(rj / P);
+    const int64_t* cj = myC + (int64_t)j * (P + 1);
+    const int64_t lo0 = cj[r], hi0 = cj[r + 1];
+    int64_t res;
+    if (k == 0) {
+        res = lo0;
+    } else if (k == kLBins) {
+        res = hi0;
+    } else {
+        // first position of the window with key > t_r[k-1]
+        const uint64_t t = tables[(int64_t)r * kLBounds + (k - 1)];
+        const int64_t sb = seg_begin[j];
+        int64_t a = lo0, e = hi0;
+        while (a < e) {
+            const int64_t m = a + ((e - a) >> 1);
+            uint32_t rr[RW];
+            RecIO<RW>::load(recs, sb + m, rr);
+            if (lkey<RW>(rr, lk) <= t) a = m + 1; else e = m;
+        }
+        res = a;
+    }
+    bnd[g] = (int32_t)res;
+}
+
+__global__ __launch_bounds__(256) void merge_count_kernel(const int32_t* __restrict__ bnd, int P,
+                                                          uint32_t* __restrict__ hist) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, k)
+    if (g >= (int64_t)P * kLBins) return;
+    const int k = (int)(g % kLBins), r = (int)(g / kLBins);
+    uint32_t c = 0;
+    for (int j = 0; j < P; j++) {
+        const int32_t* bj = bnd + ((int64_t)r * P + j) * (kLBins + 1);
+        c += (uint32_t)(bj[k + 1] - bj[k]);
+    }
+    hist[g] = c;
+}
+
+template <int RW>
+__global__ __launch_bounds__(kCBlock) void merge_bucket_kernel(
+    const uint32_t* __restrict__ recs, uint32_t* __restrict__ dst, LKey lk,
+    const int64_t* __restrict__ seg_begin, const int32_t* __restrict__ bnd, int P,
+    const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
+    const uint64_t* __restrict__ tables, int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow,
+    int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
+    long long* __restrict__ checksum_out) {
+    __shared__ int64_t s_beg[kMergeMaxP];
+    __shared__ int s_pref[kMergeMaxP + 1];
+    __shared__ long long s_sum2[kCWaves];
+    const int tid = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    const uint32_t cnt32 = hist[b];
+    if (cnt32 == 0) return;
+    const int r = (int)(b / kLBins), k = (int)(b % kLBins);
+    const int64_t base = bucket_base[b];
+    if (tid < 64) {
+        // the sub-runs of this bucket, sender by sender
+        int len = 0;
+        int64_t beg = 0;
+        if (tid < P) {
+            const int32_t* bj = bnd + ((int64_t)r * P + tid) * (kLBins + 1);
+            beg = seg_begin[tid] + bj[k];
+            len = bj[k + 1] - bj[k];
+        }
+        int x = len;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (tid >= off) x += y;
+        }
+        if (tid < P) {
+            s_beg[tid] = beg;
+            s_pref[tid] = x - len;
+        }
+        if (tid == P - 1) s_pref[P] = x;
+    }
+    __syncthreads();
+    const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
+    const int count = (int)cnt32;
+    const bool single = single_bucket(tables + (int64_t)r * kLBounds, k);
+    bool done = false;
+    uint32_t rr[kCItems][RW];
+    if (!single && count <= kChunkCap) done = sort_bucket_lds<RW>(gs, dst, base, count, lk, rr, checksum_out);
+    if (done) return;
+    // copy in sender order: final for a single-value bucket, otherwise the
+    // input of the radix path
+    long long csum = 0;
+    for (int i = tid; i < count; i += kCBlock) {
+        uint32_t x[RW];
+        gs.load(i, x);
+        RecIO<RW>::store(dst, base + i, x);
+        if (checksum_out) {
+#pragma unroll
+            for (int w = 0; w < RW; w++) csum += sbyte_sum(x[w]);
+        }
+    }
+    if (!single && tid == 0) {
+        if (count <= kChunkCap) {
+            hard[atomicAdd(nhard, 1u)] = (uint32_t)b;  // a crowded bin: the LDS radix sort
+        } else {
+            const uint32_t slot = atomicAdd(noverflow, 1u);
+            if ((int)slot < max_overflow) {
+                overflow[2 * slot] = base;
+                overflow[2 * slot + 1] = (int64_t)cnt32;
+            }
+        }
+    }
+    if (checksum_out) block_checksum_add(csum, s_sum2, checksum_out);
 }
 
 // The listed buckets: stable LSD radix sort in LDS over the bytes of
@@ -816,6 +985,60 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
             hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                (const uint32_t*)hard, (const uint32_t*)nhard, checksum);
             break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* myC, int P,
+                        const uint64_t* tables, int32_t* bnd, hipStream_t st) {
+    const int64_t n = (int64_t)P * P * (kLBins + 1);
+    const unsigned g = (unsigned)((n + 255) / 256);
+    const uint32_t* r = (const uint32_t*)recs;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, myC, P, tables, bnd); break;
+        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, myC, P, tables, bnd); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st) {
+    const int64_t n = (int64_t)P * kLBins;
+    hipLaunchKernelGGL(merge_count_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bnd, P, hist);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
+                        const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
+                        const uint64_t* tables, int64_t* overflow, uint32_t* noverflow, int max_overflow,
+                        uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st) {
+    if (P > kMergeMaxP) return -1;
+    const unsigned g = (unsigned)((int64_t)P * kLBins);
+    const uint32_t* r = (const uint32_t*)recs;
+    uint32_t* d = (uint32_t*)dst;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(g), dim3(kCBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum); break;
+        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(g), dim3(kCBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
+                    const int64_t* bucket_base, const uint32_t* hard, const uint32_t* nhard, int64_t max_buckets,
+                    long long* checksum, hipStream_t st) {
+    if (max_buckets <= 0) return 0;
+    const unsigned persistent = (unsigned)std::min<int64_t>(max_buckets, 512);
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base, hard, nhard, checksum); break;
+        case 4: hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base, hard, nhard, checksum); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());

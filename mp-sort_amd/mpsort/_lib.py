@@ -28,6 +28,7 @@ NO_CHECKSUM = 1 << 1
 TIMING = 1 << 2
 LEARNED = 1 << 3
 FORCE_GENERAL = 1 << 4
+LEARNED_EXCHANGE = 1 << 5
 
 # mpsort.h:17-20
 MPSORT_DISABLE_SPARSE_ALLTOALLV = 1 << 1

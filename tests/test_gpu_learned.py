@@ -43,15 +43,19 @@ def trained(dist, n, seed=101, linear_root=False, nsample=1 << 16):
     return learned.train(ts, 'key', nsample=nsample, seed=7, linear_root=linear_root)
 
 
+@pytest.mark.parametrize("mode", ["merged", "exchange"])
 @pytest.mark.parametrize("dist", ["uniform", "clustered", "lognormal", "zipf", "nearly_sorted"])
-def test_learned_equals_plain_and_counts(dist):
+def test_learned_equals_plain_and_counts(dist, mode):
+    """Both SecondSort forms: buckets gathered from the senders' sorted runs
+    (default) and the explicit exchange followed by the bucket sort."""
     n, P = 1 << 20, 16
-    note("learned dist=%s n=%d P=%d" % (dist, n, P))
+    note("learned dist=%s n=%d P=%d mode=%s" % (dist, n, P, mode))
     model = trained(dist, n)
     a = workloads.records16(keys_of(dist, n, 5))
     ref, _ = gpu_sort(a, even(n, P), even(n, P), 'key')
     ref_counts = last_counts(P, 8)
-    out = learned.sort_learned(a, model, 'key', nin=even(n, P))
+    flags = _lib.LEARNED_EXCHANGE if mode == "exchange" else 0
+    out = learned.sort_learned(a, model, 'key', nin=even(n, P), flags=flags)
     assert numpy.array_equal(out, ref), "learned output differs from the plain path"
     c = last_counts(P, 8)
     for k in ("C", "P", "myCLT", "myCLE", "myC", "send"):
@@ -66,7 +70,7 @@ def test_learned_equals_plain_and_counts(dist):
     model.close()
 
 
-@pytest.mark.parametrize("P", [1, 3, 8])
+@pytest.mark.parametrize("P", [1, 3, 8, 64])
 def test_learned_against_oracle(P):
     n = 1 << 15
     model = trained("clustered", n, nsample=4096)
@@ -84,14 +88,16 @@ def test_learned_against_oracle(P):
             assert numpy.array_equal(c[k], info[k]), k
 
 
-def test_mismatched_model_overflows_and_stays_exact():
+@pytest.mark.parametrize("mode", ["merged", "exchange"])
+def test_mismatched_model_overflows_and_stays_exact(mode):
     """A model of uniform keys applied to Zipf keys: buckets overflow and fall
     back to the radix path; the output is still the stable sort."""
     n, P = 1 << 20, 4
     model = trained("uniform", n)
     a = workloads.records16(workloads.zipf(n, 3))
     ref, _ = gpu_sort(a, even(n, P), even(n, P), 'key')
-    out = learned.sort_learned(a, model, 'key', nin=even(n, P))
+    flags = _lib.LEARNED_EXCHANGE if mode == "exchange" else 0
+    out = learned.sort_learned(a, model, 'key', nin=even(n, P), flags=flags)
     from mpsort import last_stats
     s = last_stats()
     note("  overflow buckets=%d plain sides=%d" % (s['buckets_overflow'], s['plain_sides']))
