@@ -1006,7 +1006,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n
         if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, kLBounds, LB.tables, M.heavy, M.nheavy, st)))
             return rc;
-        const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4);
+        const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 32);
         if (merged) {
             // no exchange: every receiver bucket is gathered from the senders'
             // sorted segments and sorted in LDS

@@ -475,32 +475,42 @@ struct GatherSrc {
     }
 };
 
+// the same records through a table of their indices (LDS)
+template <int RW>
+struct IndexSrc {
+    const uint32_t* recs;
+    const uint32_t* index;
+    __device__ __forceinline__ void load(int i, uint32_t (&r)[RW]) const { RecIO<RW>::load(recs, index[i], r); }
+};
+
 // Sorts one bucket of count (1 .. kChunkCap) records, given by src, into
 // dst[base .. base + count).  Returns false (and writes nothing) when a bin
-// is crowded beyond kRunMax; the records are then in r, in bucket order.
+// is crowded beyond kRunMax.  s_bins: 2 * (kFine + 1) words of LDS owned by
+// the caller, who may keep data there that src reads while loading.
 template <int RW, class Src>
 __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
-                                                const LKey& lk, uint32_t (&r)[kCItems][RW],
+                                                const LKey& lk, uint32_t* s_bins,
                                                 long long* __restrict__ checksum_out) {
-    __shared__ uint64_t s_key[kChunkCap];
-    __shared__ uint32_t s_start[kFine + 1];
-    __shared__ uint32_t s_fill[kFine + 1];
-    __shared__ uint16_t s_ord[kChunkCap];
-    __shared__ uint16_t s_pos[kChunkCap];
+    __shared__ uint64_t s_key[kChunkCap];   // key of each bucket position
+    __shared__ uint16_t s_ord[kChunkCap];   // bucket position of each bin-order slot
+    __shared__ uint16_t s_pos[kChunkCap];   // final slot of each bucket position
     __shared__ uint64_t s_samp[kSamp];
     __shared__ uint64_t s_red[2 * kCWaves];
     __shared__ uint32_t s_tmp[16];
     __shared__ uint32_t s_max[kCWaves];
     __shared__ long long s_sum[kCWaves];
     __shared__ int s_long;
+    uint32_t* s_start = s_bins;
+    uint32_t* s_fill = s_bins + kFine + 1;
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    for (int i = tid; i <= kFine; i += kCBlock) s_fill[i] = 0;
-    if (tid == 0) s_long = 0;
 
-    // load (item k of thread t = record k * kCBlock + t) with min / max
+    // load (item k of thread t = record k * kCBlock + t) with min / max; the
+    // keys at positions s * count / 64 are the sample of the sampled bins
+    uint32_t r[kCItems][RW];
     uint64_t kk[kCItems];
     uint64_t mn = ~0ull, mx = 0;
+    const float qscale = (float)kSamp / (float)count;
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
         const int i = k * kCBlock + tid;
@@ -510,6 +520,11 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             kk[k] = lkey<RW>(r[k], lk);
             mn = kk[k] < mn ? kk[k] : mn;
             mx = kk[k] > mx ? kk[k] : mx;
+            // sample q is the record at floor(q * count / 64): the q near
+            // i * 64 / count, checked exactly
+            const int q0 = (int)((float)i * qscale);
+            for (int q = q0 > 0 ? q0 - 1 : 0; q <= q0 + 1 + (count < kSamp ? kSamp : 0) && q < kSamp; q++)
+                if (((q * count) >> 6) == i) s_samp[q] = kk[k];
         }
     }
 #pragma unroll
@@ -519,7 +534,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         mx = c > mx ? c : mx;
     }
     if (lane == 0) { s_red[wave] = mn; s_red[kCWaves + wave] = mx; }
-    __syncthreads();
+    __syncthreads();  // every load of src is done: s_bins is free
     mn = s_red[0];
     mx = s_red[kCWaves];
 #pragma unroll
@@ -544,6 +559,9 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
         return true;
     }
+    for (int i = tid; i <= kFine; i += kCBlock) s_fill[i] = 0;
+    if (tid == 0) s_long = 0;
+    __syncthreads();
 
     const float scale = (float)kFine / ((float)(mx - mn) + 1.0f);
     int mode = 0;  // 0: plain bins, 1: sampled bins (uniform over the block)
@@ -600,12 +618,10 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
 #pragma unroll
         for (int w = 0; w < kCWaves; w++) bmax = s_max[w] > bmax ? s_max[w] : bmax;
         if (mode == 1 || bmax <= (uint32_t)kInterpMax) break;
-        // clustered keys: bins from a sorted sample of the bucket instead
+        // clustered keys: bins from the sorted sample of the bucket instead
         mode = 1;
         if (wave == 0) {
-            uint32_t rr[RW];
-            src.load((int)((int64_t)lane * count / kSamp), rr);
-            uint64_t v = lkey<RW>(rr, lk);
+            uint64_t v = s_samp[lane];
 #pragma unroll
             for (int k = 2; k <= kSamp; k <<= 1) {
 #pragma unroll
@@ -619,7 +635,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         }
         __syncthreads();
     }
-    // group the records by bin; keys by record index
+    // group by bin (keys stay at their bucket position)
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
         const int i = k * kCBlock + tid;
@@ -629,8 +645,8 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         }
     }
     __syncthreads();
-    // final slots, one record per thread in bin order (threads of a wave
-    // mostly share a bin, so the loops below stay coherent)
+    // final slots, one bin-order slot per thread (threads of a wave mostly
+    // share a bin, so the loops below stay coherent)
     for (int o = tid; o < count; o += kCBlock) {
         const uint32_t i = s_ord[o];
         const uint64_t key = s_key[i];
@@ -703,9 +719,9 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
         }
         return;
     }
-    uint32_t r[kCItems][RW];
+    __shared__ uint32_t s_bins[2 * (kFine + 1)];
     const RunSrc<RW> rs{src, base};
-    if (!sort_bucket_lds<RW>(rs, dst, base, (int)cnt32, lk, r, checksum_out))
+    if (!sort_bucket_lds<RW>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out))
         if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
 }
 
@@ -783,6 +799,7 @@ __global__ __launch_bounds__(kCBlock) void merge_bucket_kernel(
     __shared__ int64_t s_beg[kMergeMaxP];
     __shared__ int s_pref[kMergeMaxP + 1];
     __shared__ long long s_sum2[kCWaves];
+    __shared__ uint32_t s_bins[2 * (kFine + 1)];  // the gather index first, then the bins
     const int tid = threadIdx.x;
     const int64_t b = blockIdx.x;
     const uint32_t cnt32 = hist[b];
@@ -814,10 +831,18 @@ __global__ __launch_bounds__(kCBlock) void merge_bucket_kernel(
     const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
     const int count = (int)cnt32;
     const bool single = single_bucket(tables + (int64_t)r * kLBounds, k);
-    bool done = false;
-    uint32_t rr[kCItems][RW];
-    if (!single && count <= kChunkCap) done = sort_bucket_lds<RW>(gs, dst, base, count, lk, rr, checksum_out);
-    if (done) return;
+    if (!single && count <= kChunkCap) {
+        // index of every record of the bucket, sub-run by sub-run (one wave
+        // per sub-run at a time)
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int j = wave; j < P; j += kCWaves) {
+            const int a = s_pref[j], len = s_pref[j + 1] - a;
+            for (int l = lane; l < len; l += 64) s_bins[a + l] = (uint32_t)(s_beg[j] + l);
+        }
+        __syncthreads();
+        const IndexSrc<RW> is{recs, s_bins};
+        if (sort_bucket_lds<RW>(is, dst, base, count, lk, s_bins, checksum_out)) return;
+    }
     // copy in sender order: final for a single-value bucket, otherwise the
     // input of the radix path
     long long csum = 0;
