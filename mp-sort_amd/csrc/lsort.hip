@@ -486,11 +486,13 @@ struct IndexSrc {
 // Sorts one bucket of count (1 .. kChunkCap) records, given by src, into
 // dst[base .. base + count).  Returns false (and writes nothing) when a bin
 // is crowded beyond kRunMax.  s_bins: 2 * (kFine + 1) words of LDS owned by
-// the caller, who may keep data there that src reads while loading.
+// the caller, who may keep data there that src reads while loading.  When
+// bounded, every key lies in [lo, hi] (the bucket's boundaries) and the
+// bucket's minimum and maximum are not computed.
 template <int RW, class Src>
 __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
-                                                const LKey& lk, uint32_t* s_bins,
-                                                long long* __restrict__ checksum_out) {
+                                                const LKey& lk, uint32_t* s_bins, long long* __restrict__ checksum_out,
+                                                bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull) {
     __shared__ uint64_t s_key[kChunkCap];   // key of each bucket position
     __shared__ uint16_t s_ord[kChunkCap];   // bucket position of each bin-order slot
     __shared__ uint16_t s_pos[kChunkCap];   // final slot of each bucket position
@@ -527,20 +529,26 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
                 if (((q * count) >> 6) == i) s_samp[q] = kk[k];
         }
     }
+    if (bounded) {
+        mn = lo;
+        mx = hi;
+        __syncthreads();  // every load of src is done: s_bins is free
+    } else {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const uint64_t a = __shfl_xor(mn, off, 64), c = __shfl_xor(mx, off, 64);
-        mn = a < mn ? a : mn;
-        mx = c > mx ? c : mx;
-    }
-    if (lane == 0) { s_red[wave] = mn; s_red[kCWaves + wave] = mx; }
-    __syncthreads();  // every load of src is done: s_bins is free
-    mn = s_red[0];
-    mx = s_red[kCWaves];
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t a = __shfl_xor(mn, off, 64), c = __shfl_xor(mx, off, 64);
+            mn = a < mn ? a : mn;
+            mx = c > mx ? c : mx;
+        }
+        if (lane == 0) { s_red[wave] = mn; s_red[kCWaves + wave] = mx; }
+        __syncthreads();  // every load of src is done: s_bins is free
+        mn = s_red[0];
+        mx = s_red[kCWaves];
 #pragma unroll
-    for (int w = 1; w < kCWaves; w++) {
-        mn = s_red[w] < mn ? s_red[w] : mn;
-        mx = s_red[kCWaves + w] > mx ? s_red[kCWaves + w] : mx;
+        for (int w = 1; w < kCWaves; w++) {
+            mn = s_red[w] < mn ? s_red[w] : mn;
+            mx = s_red[kCWaves + w] > mx ? s_red[kCWaves + w] : mx;
+        }
     }
     long long csum = 0;
     if (mn == mx) {
@@ -620,19 +628,18 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         if (mode == 1 || bmax <= (uint32_t)kInterpMax) break;
         // clustered keys: bins from the sorted sample of the bucket instead
         mode = 1;
+        // sort the sample: each lane's rank among the 64 (ties by lane)
+        uint64_t v = 0;
+        uint32_t rank = 0;
         if (wave == 0) {
-            uint64_t v = s_samp[lane];
-#pragma unroll
-            for (int k = 2; k <= kSamp; k <<= 1) {
-#pragma unroll
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    const uint64_t o = __shfl_xor(v, j, 64);
-                    const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-                    v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
-                }
+            v = s_samp[lane];
+            for (int j = 0; j < kSamp; j++) {
+                const uint64_t o = s_samp[j];
+                rank += (o < v || (o == v && j < lane)) ? 1u : 0u;
             }
-            s_samp[lane] = v;
         }
+        __syncthreads();
+        if (wave == 0) s_samp[rank] = v;
         __syncthreads();
     }
     // group by bin (keys stay at their bucket position)
@@ -721,7 +728,16 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     }
     __shared__ uint32_t s_bins[2 * (kFine + 1)];
     const RunSrc<RW> rs{src, base};
-    if (!sort_bucket_lds<RW>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out))
+    // an interior bucket's keys lie in (t[k-1], t[k]]
+    const int kb = (int)(b % kLBins);
+    const bool bounded = tables && kb > 0 && kb < kLBins - 1;
+    uint64_t lo = 0, hi = ~0ull;
+    if (bounded) {
+        const uint64_t* t = tables + (b / kLBins) * tstride;
+        lo = t[kb - 1] + 1;
+        hi = t[kb];
+    }
+    if (!sort_bucket_lds<RW>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded && lo <= hi, lo, hi))
         if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
 }
 
@@ -841,7 +857,11 @@ __global__ __launch_bounds__(kCBlock) void merge_bucket_kernel(
         }
         __syncthreads();
         const IndexSrc<RW> is{recs, s_bins};
-        if (sort_bucket_lds<RW>(is, dst, base, count, lk, s_bins, checksum_out)) return;
+        const uint64_t* t = tables + (int64_t)r * kLBounds;
+        const bool bounded = k > 0 && k < kLBins - 1 && t[k - 1] < t[k];
+        if (sort_bucket_lds<RW>(is, dst, base, count, lk, s_bins, checksum_out, bounded,
+                                bounded ? t[k - 1] + 1 : 0, bounded ? t[k] : ~0ull))
+            return;
     }
     // copy in sender order: final for a single-value bucket, otherwise the
     // input of the radix path
