@@ -494,6 +494,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
                                                 const LKey& lk, uint32_t* s_bins, long long* __restrict__ checksum_out,
                                                 bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull) {
     __shared__ uint64_t s_key[kChunkCap];   // key of each bucket position
+    __shared__ uint16_t s_bin[kChunkCap];   // bin of each bucket position
     __shared__ uint16_t s_ord[kChunkCap];   // bucket position of each bin-order slot
     __shared__ uint16_t s_pos[kChunkCap];   // final slot of each bucket position
     __shared__ uint64_t s_samp[kSamp];
@@ -642,12 +643,13 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         if (wave == 0) s_samp[rank] = v;
         __syncthreads();
     }
-    // group by bin (keys stay at their bucket position)
+    // group by bin (keys and bins stay at their bucket position)
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
         const int i = k * kCBlock + tid;
         if (i < count) {
             s_key[i] = kk[k];
+            s_bin[i] = (uint16_t)dbin[k];
             s_ord[s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u)] = (uint16_t)i;
         }
     }
@@ -657,7 +659,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     for (int o = tid; o < count; o += kCBlock) {
         const uint32_t i = s_ord[o];
         const uint64_t key = s_key[i];
-        const uint32_t d = bin_of(key);
+        const uint32_t d = s_bin[i];
         const uint32_t s0 = s_start[d], nb = s_start[d + 1] - s0;
         if (nb == 1) {
             s_pos[i] = (uint16_t)s0;
