@@ -451,6 +451,7 @@ __device__ __forceinline__ void lds_count_pass(const uint16_t* s_in, uint16_t* s
 // i % kCBlock.
 constexpr int kSamp = 64, kSub = 63, kFine = (kSamp + 1) * kSub;  // 4095 bins
 constexpr int kInterpMax = 32;
+static_assert(kCBlock == 16 * kSamp, "the sample sort takes 16 threads per sample");
 
 // the records of one bucket, contiguous in src from base
 template <int RW>
@@ -493,11 +494,16 @@ template <int RW, class Src>
 __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
                                                 const LKey& lk, uint32_t* s_bins, long long* __restrict__ checksum_out,
                                                 bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull) {
-    __shared__ uint64_t s_key[kChunkCap];   // key of each bucket position
-    __shared__ uint16_t s_bin[kChunkCap];   // bin of each bucket position
-    __shared__ uint16_t s_ord[kChunkCap];   // bucket position of each bin-order slot
-    __shared__ uint16_t s_pos[kChunkCap];   // final slot of each bucket position
-    __shared__ uint64_t s_samp[kSamp];
+    __shared__ uint64_t s_gkey[kChunkCap + 4];  // key of each bin-order slot (+4: unrolled reads)
+    __shared__ uint16_t s_sbin[kChunkCap];      // bin of each bin-order slot
+    __shared__ uint16_t s_ord[kChunkCap];       // bucket position of each bin-order slot
+    __shared__ uint16_t s_pos[kChunkCap];       // final slot of each bucket position
+    __shared__ uint64_t s_samp[kSamp];    // the sample, in bucket order
+    __shared__ uint64_t s_sorted[kSamp];  // the sample, sorted
+    __shared__ int s_crowd;
+    __shared__ uint64_t s_ilo[kSamp + 1];  // sampled bins: interval j starts at key s_ilo[j],
+    __shared__ uint32_t s_ish[kSamp + 1];  // its offsets are shifted right by s_ish[j]
+    __shared__ float s_isc[kSamp + 1];     // and scaled by s_isc[j] into kSub bins
     __shared__ uint64_t s_red[2 * kCWaves];
     __shared__ uint32_t s_tmp[16];
     __shared__ uint32_t s_max[kCWaves];
@@ -508,12 +514,18 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
-    // load (item k of thread t = record k * kCBlock + t) with min / max; the
-    // keys at positions s * count / 64 are the sample of the sampled bins
+    // the sample of the sampled bins: thread q < 64 reads the key of record
+    // floor(q * count / 64), alongside the loads below
+    uint64_t skey = 0;
+    if (tid < kSamp) {
+        uint32_t rs[RW];
+        src.load((tid * count) >> 6, rs);
+        skey = lkey<RW>(rs, lk);
+    }
+    // load (item k of thread t = record k * kCBlock + t) with min / max
     uint32_t r[kCItems][RW];
     uint64_t kk[kCItems];
     uint64_t mn = ~0ull, mx = 0;
-    const float qscale = (float)kSamp / (float)count;
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
         const int i = k * kCBlock + tid;
@@ -523,12 +535,12 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             kk[k] = lkey<RW>(r[k], lk);
             mn = kk[k] < mn ? kk[k] : mn;
             mx = kk[k] > mx ? kk[k] : mx;
-            // sample q is the record at floor(q * count / 64): the q near
-            // i * 64 / count, checked exactly
-            const int q0 = (int)((float)i * qscale);
-            for (int q = q0 > 0 ? q0 - 1 : 0; q <= q0 + 1 + (count < kSamp ? kSamp : 0) && q < kSamp; q++)
-                if (((q * count) >> 6) == i) s_samp[q] = kk[k];
         }
+    }
+    if (tid < kSamp) s_samp[tid] = skey;
+    if (tid == 0) {
+        s_crowd = 0;
+        s_long = 0;
     }
     if (bounded) {
         mn = lo;
@@ -569,24 +581,57 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         return true;
     }
     for (int i = tid; i <= kFine; i += kCBlock) s_fill[i] = 0;
-    if (tid == 0) s_long = 0;
-    __syncthreads();
-
     const float scale = (float)kFine / ((float)(mx - mn) + 1.0f);
-    int mode = 0;  // 0: plain bins, 1: sampled bins (uniform over the block)
-    auto bin_of = [&](uint64_t key) -> uint32_t {
-        if (mode == 0) {
-            const uint32_t d = (uint32_t)((float)(key - mn) * scale);
-            return d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
+    auto plain_bin = [&](uint64_t key) -> uint32_t {
+        const uint32_t d = (uint32_t)((float)(key - mn) * scale);
+        return d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
+    };
+    {
+        // sort the sample: thread t ranks sample t / 16 against samples
+        // 4 (t % 16) .. 4 (t % 16) + 3 (ties by index), sums over 16 lanes
+        const int a = tid >> 4, part = tid & 15;
+        const uint64_t v = s_samp[a];
+        uint32_t c = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = part * 4 + u;
+            const uint64_t o = s_samp[q];
+            c += (o < v || (o == v && q < a)) ? 1u : 0u;
         }
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) c += __shfl_xor(c, off, 16);
+        if (part == 0) s_sorted[c] = v;
+    }
+    __syncthreads();
+    if (tid <= kSamp) {
+        // interval j (keys k with s_sorted[j-1] <= k < s_sorted[j]) spans the
+        // offsets 0 .. span from its first key; offsets shifted to 32 bits
+        const int j = tid;
+        const uint64_t lo = j == 0 ? mn : s_sorted[j - 1];
+        const uint64_t hi = j == kSamp ? mx : (s_sorted[j] > lo ? s_sorted[j] - 1 : lo);
+        const uint64_t span = hi - lo;
+        const int bits = 64 - __clzll((long long)(span | 1));
+        const int sh = bits > 32 ? bits - 32 : 0;
+        s_ilo[j] = lo;
+        s_ish[j] = (uint32_t)sh;
+        s_isc[j] = (float)kSub / ((float)(uint32_t)(span >> sh) + 1.0f);
+        // two neighbouring samples in one plain bin put about count / 64
+        // records there: the sampled bins are used
+        if (j + 1 < kSamp && plain_bin(s_sorted[j]) == plain_bin(s_sorted[j + 1])) s_crowd = 1;
+    }
+    __syncthreads();
+    int mode = count >= kInterpMax * kSamp && s_crowd ? 1 : 0;  // 0: plain bins, 1: sampled bins
+    auto bin_of = [&](uint64_t key) -> uint32_t {
+        if (mode == 0) return plain_bin(key);
         int j = 0;
 #pragma unroll
         for (int step = kSamp / 2; step > 0; step >>= 1)
-            if (s_samp[j + step - 1] <= key) j += step;
-        if (j == kSamp - 1 && s_samp[kSamp - 1] <= key) j = kSamp;
-        const uint64_t lo = j == 0 ? mn : s_samp[j - 1];
-        const float width = j == kSamp ? (float)(mx - lo) + 1.0f : (float)(s_samp[j] - lo);
-        uint32_t sub = (uint32_t)((float)(key - lo) * ((float)kSub / width));
+            if (s_sorted[j + step - 1] <= key) j += step;
+        if (j == kSamp - 1 && s_sorted[kSamp - 1] <= key) j = kSamp;
+        // non-decreasing in the key inside the interval: shift, conversion
+        // and scaling are all monotone
+        const uint32_t x = (uint32_t)((key - s_ilo[j]) >> s_ish[j]);
+        uint32_t sub = (uint32_t)((float)x * s_isc[j]);
         sub = sub > (uint32_t)(kSub - 1) ? (uint32_t)(kSub - 1) : sub;
         return (uint32_t)j * kSub + sub;
     };
@@ -626,53 +671,58 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         uint32_t bmax = 0;
 #pragma unroll
         for (int w = 0; w < kCWaves; w++) bmax = s_max[w] > bmax ? s_max[w] : bmax;
+        // clustered keys the sample did not reveal: sampled bins instead
         if (mode == 1 || bmax <= (uint32_t)kInterpMax) break;
-        // clustered keys: bins from the sorted sample of the bucket instead
         mode = 1;
-        // sort the sample: each lane's rank among the 64 (ties by lane)
-        uint64_t v = 0;
-        uint32_t rank = 0;
-        if (wave == 0) {
-            v = s_samp[lane];
-            for (int j = 0; j < kSamp; j++) {
-                const uint64_t o = s_samp[j];
-                rank += (o < v || (o == v && j < lane)) ? 1u : 0u;
-            }
-        }
-        __syncthreads();
-        if (wave == 0) s_samp[rank] = v;
-        __syncthreads();
     }
-    // group by bin (keys and bins stay at their bucket position)
+    // group by bin: key, bin and bucket position of every bin-order slot
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
         const int i = k * kCBlock + tid;
         if (i < count) {
-            s_key[i] = kk[k];
-            s_bin[i] = (uint16_t)dbin[k];
-            s_ord[s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u)] = (uint16_t)i;
+            const uint32_t o = s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u);
+            s_gkey[o] = kk[k];
+            s_sbin[o] = (uint16_t)dbin[k];
+            s_ord[o] = (uint16_t)i;
         }
     }
     __syncthreads();
     // final slots, one bin-order slot per thread (threads of a wave mostly
-    // share a bin, so the loops below stay coherent)
+    // share a bin, so the loops below stay coherent).  The keys of a bin are
+    // read four at a time; the up to three read past its end are keys of
+    // later bins (larger) or the padding (all ones), which add no smaller
+    // key.  Equal keys keep their bucket order.
+    if (tid < 4) s_gkey[count + tid] = ~0ull;
+    __syncthreads();
     for (int o = tid; o < count; o += kCBlock) {
+        const uint32_t d = s_sbin[o];
         const uint32_t i = s_ord[o];
-        const uint64_t key = s_key[i];
-        const uint32_t d = s_bin[i];
-        const uint32_t s0 = s_start[d], nb = s_start[d + 1] - s0;
-        if (nb == 1) {
+        const uint32_t s0 = s_start[d], e = s_start[d + 1];
+        if (e - s0 == 1) {
             s_pos[i] = (uint16_t)s0;
-        } else if (nb > (uint32_t)kRunMax) {
+        } else if (e - s0 > (uint32_t)kRunMax) {
             s_long = 1;
         } else {
-            uint32_t rank = 0;
-            for (uint32_t j = s0; j < s0 + nb; j++) {
-                const uint32_t ij = s_ord[j];
-                const uint64_t kj = s_key[ij];
-                rank += (kj < key || (kj == key && ij < i)) ? 1u : 0u;
+            const uint64_t key = s_gkey[o];
+            uint32_t lt = 0, eq = 0;
+            for (uint32_t j = s0; j < e; j += 4) {
+                uint64_t k4[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) k4[u] = s_gkey[j + u];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    lt += k4[u] < key ? 1u : 0u;
+                    eq += k4[u] == key ? 1u : 0u;
+                }
             }
-            s_pos[i] = (uint16_t)(s0 + rank);
+            if (eq > 1) {
+                // equal keys (or the padding): the exact count of the equal
+                // keys earlier in the bucket
+                eq = 0;
+                for (uint32_t j = s0; j < e; j++) eq += (s_gkey[j] == key && s_ord[j] < i) ? 1u : 0u;
+                lt += eq;
+            }
+            s_pos[i] = (uint16_t)(s0 + lt);
         }
     }
     __syncthreads();
