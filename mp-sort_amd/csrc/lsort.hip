@@ -581,9 +581,13 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         return true;
     }
     for (int i = tid; i <= kFine; i += kCBlock) s_fill[i] = 0;
-    const float scale = (float)kFine / ((float)(mx - mn) + 1.0f);
+    // plain bins: offsets from the minimum shifted to 32 bits, then scaled
+    // (shift, conversion and scaling are all monotone)
+    const int pbits = 64 - __clzll((long long)((mx - mn) | 1));
+    const int psh = pbits > 32 ? pbits - 32 : 0;
+    const float scale = (float)kFine / ((float)(uint32_t)((mx - mn) >> psh) + 1.0f);
     auto plain_bin = [&](uint64_t key) -> uint32_t {
-        const uint32_t d = (uint32_t)((float)(key - mn) * scale);
+        const uint32_t d = (uint32_t)((float)(uint32_t)((key - mn) >> psh) * scale);
         return d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
     };
     {
@@ -623,13 +627,13 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     int mode = count >= kInterpMax * kSamp && s_crowd ? 1 : 0;  // 0: plain bins, 1: sampled bins
     auto bin_of = [&](uint64_t key) -> uint32_t {
         if (mode == 0) return plain_bin(key);
+        // sampled: j = #{samples <= key}, then the interval's offset shifted
+        // and scaled into kSub bins
         int j = 0;
 #pragma unroll
         for (int step = kSamp / 2; step > 0; step >>= 1)
             if (s_sorted[j + step - 1] <= key) j += step;
         if (j == kSamp - 1 && s_sorted[kSamp - 1] <= key) j = kSamp;
-        // non-decreasing in the key inside the interval: shift, conversion
-        // and scaling are all monotone
         const uint32_t x = (uint32_t)((key - s_ilo[j]) >> s_ish[j]);
         uint32_t sub = (uint32_t)((float)x * s_isc[j]);
         sub = sub > (uint32_t)(kSub - 1) ? (uint32_t)(kSub - 1) : sub;
@@ -742,63 +746,81 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     return true;
 }
 
+// One workgroup per bucket; the bucket's count, base and table bounds are
+// read together.
 template <int RW>
 __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
-    const uint64_t* __restrict__ tables, int64_t tstride, int64_t* __restrict__ overflow,
+    const uint64_t* __restrict__ tables, int64_t tstride, int64_t nbuckets, int64_t* __restrict__ overflow,
     uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
     long long* __restrict__ checksum_out, int eq_done) {
     __shared__ long long s_sum[kCWaves];
-    const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
-    const uint32_t cnt32 = hist[b];
-    if (cnt32 == 0) return;
-    const int64_t base = bucket_base[b];
-    if (eq_done && single_bucket(tables + (b / kLBins) * tstride, (int)(b % kLBins))) {
-        if (checksum_out) {
-            long long csum = 0;
-            for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += kCBlock) {
-                uint32_t rr[RW];
-                RecIO<RW>::load(dst, i, rr);
-#pragma unroll
-                for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
-            }
-            block_checksum_add(csum, s_sum, checksum_out);
-        }
-        return;
-    }
-    if (cnt32 > (uint32_t)kChunkCap) {
-        if (tid == 0) {
-            const uint32_t slot = atomicAdd(noverflow, 1u);
-            if ((int)slot < max_overflow) {
-                overflow[2 * slot] = base;
-                overflow[2 * slot + 1] = (int64_t)cnt32;
-            }
-        }
-        return;
-    }
     __shared__ uint32_t s_bins[2 * (kFine + 1)];
-    const RunSrc<RW> rs{src, base};
-    // an interior bucket's keys lie in (t[k-1], t[k]]
-    const int kb = (int)(b % kLBins);
-    const bool bounded = tables && kb > 0 && kb < kLBins - 1;
-    uint64_t lo = 0, hi = ~0ull;
-    if (bounded) {
-        const uint64_t* t = tables + (b / kLBins) * tstride;
-        lo = t[kb - 1] + 1;
-        hi = t[kb];
+    const int tid = threadIdx.x;
+    // count, base and the table entries t[k-1], t[k] around bucket k
+    struct Meta {
+        uint32_t cnt;
+        int64_t base;
+        uint64_t tlo, thi;
+    };
+    auto meta = [&](int64_t bb) -> Meta {
+        Meta m{hist[bb], bucket_base[bb], 0, ~0ull};
+        if (tables) {
+            const int kb = (int)(bb % kLBins);
+            const uint64_t* t = tables + (bb / kLBins) * tstride;
+            if (kb > 0) m.tlo = t[kb - 1];
+            if (kb < kLBounds) m.thi = t[kb];
+        }
+        return m;
+    };
+    {
+        const int64_t b = blockIdx.x;
+        if (b >= nbuckets) return;
+        const Meta cur = meta(b);
+        const uint32_t cnt32 = cur.cnt;
+        const int64_t base = cur.base;
+        const int kb = (int)(b % kLBins);
+        // bucket k holds one key value (see single_bucket)
+        const bool single = tables && (kb == 0 ? cur.thi == 0 : (kb < kLBounds && cur.tlo + 1 == cur.thi));
+        if (cnt32 == 0) {
+        } else if (eq_done && single) {
+            if (checksum_out) {
+                long long csum = 0;
+                for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += kCBlock) {
+                    uint32_t rr[RW];
+                    RecIO<RW>::load(dst, i, rr);
+#pragma unroll
+                    for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
+                }
+                block_checksum_add(csum, s_sum, checksum_out);
+            }
+        } else if (cnt32 > (uint32_t)kChunkCap) {
+            if (tid == 0) {
+                const uint32_t slot = atomicAdd(noverflow, 1u);
+                if ((int)slot < max_overflow) {
+                    overflow[2 * slot] = base;
+                    overflow[2 * slot + 1] = (int64_t)cnt32;
+                }
+            }
+        } else {
+            const RunSrc<RW> rs{src, base};
+            // an interior bucket's keys lie in (t[k-1], t[k]]
+            const bool bounded = tables && kb > 0 && kb < kLBins - 1 && cur.tlo < cur.thi;
+            if (!sort_bucket_lds<RW>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
+                                     bounded ? cur.tlo + 1 : 0, bounded ? cur.thi : ~0ull))
+                if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
+        }
     }
-    if (!sort_bucket_lds<RW>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded && lo <= hi, lo, hi))
-        if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
 }
 
 // ---------------------------------------------------------------------------
 // SecondSort without the exchange.  Receiver r's records from sender j are
 // positions [myC[j][r], myC[j][r+1]) of sender j's sorted segment; bucket k of
 // r holds the keys in (t_r[k-1], t_r[k]], i.e. one sub-run per sender.
-//   merge_bounds_kernel  bnd[r][j][k] = start of bucket k's sub-run in sender
-//                        j (relative to j's segment), k = 0..1024
+//   merge_bounds_kernel  bnd[r][k][j] = start of bucket k's sub-run in sender
+//                        j (relative to j's segment), k = 0..1024 (the P
+//                        senders of one bucket side by side)
 //   merge_count_kernel   bucket sizes (sum over senders)
 //   merge_bucket_kernel  one workgroup per (receiver, bucket): gathers the
 //                        sub-runs in sender order (the order the exchange
@@ -840,7 +862,7 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
         }
         res = a;
     }
-    bnd[g] = (int32_t)res;
+    bnd[((int64_t)r * (kLBins + 1) + k) * P + j] = (int32_t)res;
 }
 
 __global__ __launch_bounds__(256) void merge_count_kernel(const int32_t* __restrict__ bnd, int P,
@@ -848,11 +870,9 @@ __global__ __launch_bounds__(256) void merge_count_kernel(const int32_t* __restr
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, k)
     if (g >= (int64_t)P * kLBins) return;
     const int k = (int)(g % kLBins), r = (int)(g / kLBins);
+    const int32_t* bk = bnd + ((int64_t)r * (kLBins + 1) + k) * P;
     uint32_t c = 0;
-    for (int j = 0; j < P; j++) {
-        const int32_t* bj = bnd + ((int64_t)r * P + j) * (kLBins + 1);
-        c += (uint32_t)(bj[k + 1] - bj[k]);
-    }
+    for (int j = 0; j < P; j++) c += (uint32_t)(bk[P + j] - bk[j]);
     hist[g] = c;
 }
 
@@ -868,77 +888,83 @@ __global__ __launch_bounds__(kCBlock) void merge_bucket_kernel(
     __shared__ int s_pref[kMergeMaxP + 1];
     __shared__ long long s_sum2[kCWaves];
     __shared__ uint32_t s_bins[2 * (kFine + 1)];  // the gather index first, then the bins
-    const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
-    const uint32_t cnt32 = hist[b];
-    if (cnt32 == 0) return;
-    const int r = (int)(b / kLBins), k = (int)(b % kLBins);
-    const int64_t base = bucket_base[b];
-    if (tid < 64) {
-        // the sub-runs of this bucket, sender by sender
-        int len = 0;
-        int64_t beg = 0;
-        if (tid < P) {
-            const int32_t* bj = bnd + ((int64_t)r * P + tid) * (kLBins + 1);
-            beg = seg_begin[tid] + bj[k];
-            len = bj[k + 1] - bj[k];
-        }
-        int x = len;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    {
+        const int64_t b = blockIdx.x;
+        const uint32_t cnt32 = hist[b];
+        const int r = (int)(b / kLBins), k = (int)(b % kLBins);
+        const int64_t base = bucket_base[b];
+        const uint64_t* t = tables + (int64_t)r * kLBounds;
+        const uint64_t tlo = k > 0 ? t[k - 1] : 0, thi = k < kLBounds ? t[k] : ~0ull;
+        if (cnt32 == 0) return;
+        if (wave == 0) {
+            // the sub-runs of this bucket, sender by sender
+            int len = 0;
+            int64_t beg = 0;
+            if (lane < P) {
+                const int32_t* bk = bnd + ((int64_t)r * (kLBins + 1) + k) * P;
+                beg = seg_begin[lane] + bk[lane];
+                len = bk[P + lane] - bk[lane];
+            }
+            int x = len;
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int y = __shfl_up(x, off, 64);
-            if (tid >= off) x += y;
-        }
-        if (tid < P) {
-            s_beg[tid] = beg;
-            s_pref[tid] = x - len;
-        }
-        if (tid == P - 1) s_pref[P] = x;
-    }
-    __syncthreads();
-    const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
-    const int count = (int)cnt32;
-    const bool single = single_bucket(tables + (int64_t)r * kLBounds, k);
-    if (!single && count <= kChunkCap) {
-        // index of every record of the bucket, sub-run by sub-run (one wave
-        // per sub-run at a time)
-        const int wave = tid >> 6, lane = tid & 63;
-        for (int j = wave; j < P; j += kCWaves) {
-            const int a = s_pref[j], len = s_pref[j + 1] - a;
-            for (int l = lane; l < len; l += 64) s_bins[a + l] = (uint32_t)(s_beg[j] + l);
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(x, off, 64);
+                if (lane >= off) x += y;
+            }
+            if (lane < P) {
+                s_beg[lane] = beg;
+                s_pref[lane] = x - len;
+            }
+            if (lane == P - 1) s_pref[P] = x;
         }
         __syncthreads();
-        const IndexSrc<RW> is{recs, s_bins};
-        const uint64_t* t = tables + (int64_t)r * kLBounds;
-        const bool bounded = k > 0 && k < kLBins - 1 && t[k - 1] < t[k];
-        if (sort_bucket_lds<RW>(is, dst, base, count, lk, s_bins, checksum_out, bounded,
-                                bounded ? t[k - 1] + 1 : 0, bounded ? t[k] : ~0ull))
-            return;
-    }
-    // copy in sender order: final for a single-value bucket, otherwise the
-    // input of the radix path
-    long long csum = 0;
-    for (int i = tid; i < count; i += kCBlock) {
-        uint32_t x[RW];
-        gs.load(i, x);
-        RecIO<RW>::store(dst, base + i, x);
-        if (checksum_out) {
-#pragma unroll
-            for (int w = 0; w < RW; w++) csum += sbyte_sum(x[w]);
-        }
-    }
-    if (!single && tid == 0) {
-        if (count <= kChunkCap) {
-            hard[atomicAdd(nhard, 1u)] = (uint32_t)b;  // a crowded bin: the LDS radix sort
-        } else {
-            const uint32_t slot = atomicAdd(noverflow, 1u);
-            if ((int)slot < max_overflow) {
-                overflow[2 * slot] = base;
-                overflow[2 * slot + 1] = (int64_t)cnt32;
+        const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
+        const int count = (int)cnt32;
+        // bucket k holds one key value (see single_bucket)
+        const bool single = k == 0 ? thi == 0 : (k < kLBounds && tlo + 1 == thi);
+        bool done = false;
+        if (!single && count <= kChunkCap) {
+            // index of every record of the bucket, sub-run by sub-run (one
+            // wave per sub-run at a time)
+            for (int j = wave; j < P; j += kCWaves) {
+                const int a = s_pref[j], len = s_pref[j + 1] - a;
+                for (int l = lane; l < len; l += 64) s_bins[a + l] = (uint32_t)(s_beg[j] + l);
             }
+            __syncthreads();
+            const IndexSrc<RW> is{recs, s_bins};
+            // an interior bucket's keys lie in (t[k-1], t[k]]
+            const bool bounded = k > 0 && k < kLBins - 1 && tlo < thi;
+            done = sort_bucket_lds<RW>(is, dst, base, count, lk, s_bins, checksum_out, bounded,
+                                       bounded ? tlo + 1 : 0, bounded ? thi : ~0ull);
+        }
+        if (!done) {
+            // copy in sender order: final for a single-value bucket,
+            // otherwise the input of the radix path
+            long long csum = 0;
+            for (int i = tid; i < count; i += kCBlock) {
+                uint32_t x[RW];
+                gs.load(i, x);
+                RecIO<RW>::store(dst, base + i, x);
+                if (checksum_out) {
+#pragma unroll
+                    for (int w = 0; w < RW; w++) csum += sbyte_sum(x[w]);
+                }
+            }
+            if (!single && tid == 0) {
+                if (count <= kChunkCap) {
+                    hard[atomicAdd(nhard, 1u)] = (uint32_t)b;  // a crowded bin: the LDS radix sort
+                } else {
+                    const uint32_t slot = atomicAdd(noverflow, 1u);
+                    if ((int)slot < max_overflow) {
+                        overflow[2 * slot] = base;
+                        overflow[2 * slot + 1] = (int64_t)cnt32;
+                    }
+                }
+            }
+            if (checksum_out) block_checksum_add(csum, s_sum2, checksum_out);
         }
     }
-    if (checksum_out) block_checksum_add(csum, s_sum2, checksum_out);
 }
 
 // The listed buckets: stable LSD radix sort in LDS over the bytes of
@@ -1065,18 +1091,19 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
     uint32_t* d = (uint32_t*)dst;
     const int ed = eq_done ? 1 : 0;
     const unsigned persistent = (unsigned)std::min<int64_t>(nbuckets, 512);
+    const unsigned grid = (unsigned)nbuckets;
     switch (rw) {
         case 2:
-            hipLaunchKernelGGL(interp_sort_kernel<2>, dim3((unsigned)nbuckets), dim3(kCBlock), 0, st, s, d, lk, hist,
-                               bucket_base, tables, tstride, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
+            hipLaunchKernelGGL(interp_sort_kernel<2>, dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
+                               tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                (const uint32_t*)hard, (const uint32_t*)nhard, checksum);
             break;
         case 4:
-            hipLaunchKernelGGL(interp_sort_kernel<4>, dim3((unsigned)nbuckets), dim3(kCBlock), 0, st, s, d, lk, hist,
-                               bucket_base, tables, tstride, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
+            hipLaunchKernelGGL(interp_sort_kernel<4>, dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
+                               tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
