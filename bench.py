@@ -6,12 +6,14 @@ clustered 3-D field (2^16 Gaussian clumps, log-normal radii sigma_ln=1,
 median 2^-8, 30% uniform background, 21 bits per axis), on 64 virtual ranks
 of 4,194,304 records each.  One step = one mpsort_gpu_newarray call on
 device-resident input (the whole histogram sort: FirstSort, Pmin/Pmax,
-splitter bisection, layout, exchange, SecondSort, checksums).
+splitter bisection, layout, exchange, SecondSort, checksums), timed for the
+plain radix path and for the learned-splitter path (model trained on the GPU
+beforehand, outside the timed steps); `value` is the learned path's rate.
 
-Prints one JSON line: Mrecords/s, the roofline of the dominant kernel (the
-onesweep digit pass, 2*n*16 algorithmic bytes per launch), and the reference
-CPU sort (mpsort_omp compiled from the reference sources, oracle/_ref) timed
-on a bounded sample of the same workload on this host.
+Prints one JSON line: Mrecords/s, the roofline of the dominant kernel of the
+reported path (2*n*16 algorithmic bytes per launch), and the reference CPU
+sort (mpsort_omp compiled from the reference sources, oracle/_ref) timed on a
+bounded sample of the same workload on this host.
 """
 import argparse
 import ctypes
@@ -30,8 +32,9 @@ PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_
 # HBM bytes per launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # each scaled by the counter's reading for a plain copy kernel of known size
 # (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
-TRAFFIC_ONESWEEP = 9069913293
-TRAFFIC_BUCKET_SORT = 8754310656
+TRAFFIC_ONESWEEP = 9069400576
+TRAFFIC_BUCKET_SORT = 10241614131
+TRAFFIC_MERGE_BUCKET = 9206444442
 # per-side stages of the learned local sort, as timed by the library
 LEARNED_KERNELS = ["hist_A", "scatter_A", "bucket_sort_A", "lds_radix_A", "fallback_A",
                    "hist_B", "scatter_B", "bucket_sort_B", "lds_radix_B", "fallback_B"]
@@ -182,7 +185,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--vranks", type=int, default=64)
-    ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    ap.add_argument("--cpu-sample-log2", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain path only")
     ap.add_argument("--host-leg", action="store_true",
@@ -333,12 +336,16 @@ def main():
     plain_roof = roof([x for st in st_plain for x in st["pass_ms"]],
                       "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP)
     if result_learned is not None:
-        # the learned step's dominant kernel: the in-LDS bucket sort, one
-        # launch per local sort over all n records
+        # the learned step's dominant kernel, one launch per local sort over
+        # all n records: the FirstSort's in-LDS bucket sort or the merged
+        # SecondSort's gather-and-sort, whichever takes longer
         ib = LEARNED_KERNELS.index
-        launches = [st["pass_ms"][ib(k)] for st in st_learned for k in ("bucket_sort_A", "bucket_sort_B")]
-        roofline = roof(launches, "interp_sort_kernel<4> (in-LDS bucket sort, 2*n*16 B per launch)",
-                        TRAFFIC_BUCKET_SORT)
+        sides = {"bucket_sort_A": ("interp_sort_kernel<4> (FirstSort in-LDS bucket sort, 2*n*16 B per launch)",
+                                   TRAFFIC_BUCKET_SORT),
+                 "bucket_sort_B": ("merge_bucket_kernel<4> (SecondSort gather of the senders' sub-runs and "
+                                   "in-LDS bucket sort, 2*n*16 B per launch)", TRAFFIC_MERGE_BUCKET)}
+        dom = max(sides, key=lambda k: sum(st["pass_ms"][ib(k)] for st in st_learned))
+        roofline = roof([st["pass_ms"][ib(dom)] for st in st_learned], *sides[dom])
         result_learned["roofline_by_stage"] = {
             k: {"avg_ms": round(sum(st["pass_ms"][ib(k)] for st in st_learned) / len(st_learned), 4),
                 "GB/s": round((alg_bytes / 2 if k.startswith("hist") else alg_bytes) /
