@@ -514,30 +514,26 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
-    // the sample of the sampled bins: thread q < 64 reads the key of record
-    // floor(q * count / 64), alongside the loads below
-    uint64_t skey = 0;
-    if (tid < kSamp) {
-        uint32_t rs[RW];
-        src.load((tid * count) >> 6, rs);
-        skey = lkey<RW>(rs, lk);
-    }
-    // load (item k of thread t = record k * kCBlock + t) with min / max
+    // Every load is issued before any of them is used, so they are all in
+    // flight at once: item k of thread t is record k * kCBlock + t (indices
+    // past the bucket read its last record again and are ignored); thread
+    // q < 64 also reads the sample key of record floor(q * count / 64).
+    uint32_t rs[RW];
+    src.load(((tid & (kSamp - 1)) * count) >> 6, rs);
     uint32_t r[kCItems][RW];
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) src.load(min(k * kCBlock + tid, count - 1), r[k]);
     uint64_t kk[kCItems];
     uint64_t mn = ~0ull, mx = 0;
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
-        const int i = k * kCBlock + tid;
-        kk[k] = 0;
-        if (i < count) {
-            src.load(i, r[k]);
-            kk[k] = lkey<RW>(r[k], lk);
-            mn = kk[k] < mn ? kk[k] : mn;
-            mx = kk[k] > mx ? kk[k] : mx;
-        }
+        const bool valid = k * kCBlock + tid < count;
+        const uint64_t key = lkey<RW>(r[k], lk);
+        kk[k] = valid ? key : 0;
+        mn = valid && key < mn ? key : mn;
+        mx = valid && key > mx ? key : mx;
     }
-    if (tid < kSamp) s_samp[tid] = skey;
+    if (tid < kSamp) s_samp[tid] = lkey<RW>(rs, lk);
     if (tid == 0) {
         s_crowd = 0;
         s_long = 0;
