@@ -178,6 +178,95 @@ def bench_batched(args):
     return 0
 
 
+def bench_snapshot(args):
+    """The snapshot-matching workload: two arrays of 2^27 particles
+    ({u64 id, f32 pos[3], f32 vel[3]}, 32 B), A with the ids a seeded
+    permutation, B a copy of A with 0.1% of the records displaced; both
+    sorted by id on 64 virtual ranks, learned path with one model (trained
+    on A's plain sort) for both.  One step = the sorts of A and B."""
+    import numpy
+    from mpsort import _lib, learned, workloads
+    lib = _lib.lib
+    n, P = 1 << args.log2n, args.vranks
+    t0 = time.perf_counter()
+    a, b = workloads.snapshot_pair(n, 27)
+    log("generated 2 x %d particles in %.2f s" % (n, time.perf_counter() - t0))
+    key = learned._key_desc(a.dtype, 'id')
+    nin = (ctypes.c_int64 * P)(*[n * (t + 1) // P - n * t // P for t in range(P)])
+    da, db, dout = (lib.mpsort_gpu_dmalloc(a.nbytes) for _ in range(3))
+    if not (da and db and dout):
+        print("bench.py: device allocation failed", file=sys.stderr)
+        return 4
+    _lib.check(lib.mpsort_gpu_memcpy_h2d(da, a.ctypes.data, a.nbytes))
+    _lib.check(lib.mpsort_gpu_memcpy_h2d(db, b.ctypes.data, b.nbytes))
+    flags = _lib.DEVICE_PTRS | _lib.TIMING
+
+    def plain_step():
+        for src in (da, db):
+            _lib.check(lib.mpsort_gpu_newarray(src, nin, P, dout, nin, 32, ctypes.byref(key), flags))
+
+    def timed(step, label):
+        for i in range(args.warmup):
+            step()
+        _lib.check(lib.mpsort_gpu_synchronize())
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step()
+        _lib.check(lib.mpsort_gpu_synchronize())
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        log("%s: %.3f ms per step (2 sorts), %.1f Mrecords/s" % (label, ms, 2 * n / (ms * 1e-3) / 1e6))
+        return ms
+
+    ms_plain = timed(plain_step, "plain")
+    ref = numpy.empty_like(a)
+    _lib.check(lib.mpsort_gpu_newarray(da, nin, P, dout, nin, 32, ctypes.byref(key), flags))
+    _lib.check(lib.mpsort_gpu_memcpy_d2h(ref.ctypes.data, dout, ref.nbytes))
+    if not numpy.array_equal(ref['id'], numpy.arange(n, dtype=numpy.uint64)):
+        print("bench.py: plain output ids are not 0..n-1", file=sys.stderr)
+        return 5
+    t0 = time.perf_counter()
+    model = learned.train(None, nsample=1 << 20, seed=1, device_ptr=(dout, n, 32, key))
+    t_train = time.perf_counter() - t0
+
+    def learned_step():
+        for src in (da, db):
+            _lib.check(lib.mpsort_gpu_newarray_learned(src, nin, P, dout, nin, 32, ctypes.byref(key), model._h,
+                                                       flags))
+
+    ms = timed(learned_step, "learned")
+    st = _lib.last_stats().as_dict()
+    ok = True
+    out = numpy.empty_like(a)
+    for src in (da, db):
+        _lib.check(lib.mpsort_gpu_newarray_learned(src, nin, P, dout, nin, 32, ctypes.byref(key), model._h, flags))
+        _lib.check(lib.mpsort_gpu_memcpy_d2h(out.ctypes.data, dout, out.nbytes))
+        ok = ok and bool(numpy.array_equal(out.view(numpy.uint8), ref.view(numpy.uint8)))
+    model.close()
+    for p in (da, db, dout):
+        lib.mpsort_gpu_dfree(p)
+    if not ok:
+        print("bench.py: learned snapshot outputs differ from the plain sort of A", file=sys.stderr)
+        return 6
+    value = 2 * n / (ms * 1e-3) / 1e6
+    result = {
+        "metric": "Mrecords/s, snapshot matching (2 x 2^%d x 32 B particles sorted by id, learned splitters)"
+                  % args.log2n,
+        "value": round(value, 2), "unit": "Mrecords/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "snapshot matching: A = seeded permutation of ids with uniform pos/vel, "
+                               "B = A with 0.1%% of records displaced, %d virtual ranks" % P,
+                   "records_per_array": n, "record_bytes": 32},
+        "ids_equal_and_sorted": ok, "train_s": round(t_train, 3),
+        "plain": {"value": round(2 * n / (ms_plain * 1e-3) / 1e6, 2), "ms_per_step": round(ms_plain, 3)},
+        "learned_vs_plain": round(ms_plain / ms, 3),
+        "last_sort_phases_ms": {k: round(st[k], 4) for k in PHASES},
+        "sort_equivalent_GBps": round(2 * 2.0 * n * 32 / (ms * 1e-3) / 1e9, 1),
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,8 +279,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="time the plain path only")
     ap.add_argument("--host-leg", action="store_true",
                     help="also time one call on host buffers (PCIe transfers included)")
-    ap.add_argument("--workload", default="histsort", choices=["histsort", "batched"],
-                    help="histsort: the 2^28 x 16 B histogram sort (default); batched: 10,000 independent sorts")
+    ap.add_argument("--workload", default="histsort", choices=["histsort", "batched", "snapshot"],
+                    help="histsort: the 2^28 x 16 B histogram sort (default); batched: 10,000 independent "
+                         "sorts; snapshot: two 2^27 x 32 B particle snapshots sorted by id")
     ap.add_argument("--segments", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain copy kernel of n*16 bytes (byte-counter calibration)")
@@ -211,6 +301,10 @@ def main():
         if args.log2n == 28:
             args.log2n = 30
         return bench_batched(args)
+    if args.workload == "snapshot":
+        if args.log2n == 28:
+            args.log2n = 27
+        return bench_snapshot(args)
 
     n = 1 << args.log2n
     P = args.vranks

@@ -431,21 +431,45 @@ __global__ __launch_bounds__(kBlock) void exchange_hist_kernel(
 // ---------------------------------------------------------------------------
 // general record path: pack {key bytes, pad, u32 index} sort records from the
 // caller's records, and gather the caller's records by index at the end
+// {key bytes, u32 index} sort records of rw words from the caller's records.
+// vec: records of a multiple of 16 bytes, 16-byte aligned (read as 16-byte
+// vectors for the checksum; the key words come from the same lines again);
+// kwords: key offset and length are whole words.
 __global__ __launch_bounds__(kBlock) void pack_kernel(const uint8_t* __restrict__ in, int64_t n,
                                                       int64_t elsize, int64_t key_off, int key_bytes,
                                                       uint32_t* __restrict__ out, int rw,
-                                                      long long* __restrict__ checksum) {
+                                                      long long* __restrict__ checksum, int vec, int kwords) {
     __shared__ long long s_sum[kWaves];
     long long csum = 0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock) {
         const uint8_t* rec = in + i * elsize;
         uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int b = 0; b < key_bytes; b++) w[b >> 2] |= (uint32_t)rec[key_off + b] << ((b & 3) * 8);
+        if (checksum) {
+            if (vec) {
+                const uint4* v = reinterpret_cast<const uint4*>(rec);
+                for (int64_t q = 0; q < (elsize >> 4); q++) {
+                    const uint4 x = v[q];
+                    csum += sbyte_sum(x.x) + sbyte_sum(x.y) + sbyte_sum(x.z) + sbyte_sum(x.w);
+                }
+            } else {
+                for (int64_t b = 0; b < elsize; b++) csum += (signed char)rec[b];
+            }
+        }
+        if (kwords) {
+            const uint32_t* kw = reinterpret_cast<const uint32_t*>(rec + key_off);
+            for (int q = 0; q < (key_bytes >> 2); q++) w[q] = kw[q];
+        } else {
+            for (int b = 0; b < key_bytes; b++) w[b >> 2] |= (uint32_t)rec[key_off + b] << ((b & 3) * 8);
+        }
         w[rw - 1] = (uint32_t)i;
-        for (int q = 0; q < rw; q++) out[i * rw + q] = w[q];
-        if (checksum)
-            for (int64_t b = 0; b < elsize; b++) csum += (signed char)rec[b];
+        if (rw == 4) {
+            reinterpret_cast<uint4*>(out)[i] = make_uint4(w[0], w[1], w[2], w[3]);
+        } else if (rw == 2) {
+            reinterpret_cast<uint2*>(out)[i] = make_uint2(w[0], w[1]);
+        } else {
+            for (int q = 0; q < rw; q++) out[i * rw + q] = w[q];
+        }
     }
     if (checksum) {
         csum = wave_sum_ll(csum);
@@ -467,6 +491,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const uint8_t* __restric
     __shared__ long long s_sum[kWaves];
     long long csum = 0;
     const bool words = (elsize & 3) == 0 && ((uintptr_t)in & 3) == 0 && ((uintptr_t)out & 3) == 0;
+    const bool vec = (elsize & 15) == 0 && ((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0;
     for (int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x; o < n;
          o += (int64_t)gridDim.x * kBlock) {
         const int64_t i = (int64_t)srec[o * rw + rw - 1];
@@ -474,7 +499,15 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const uint8_t* __restric
             if (error_flag) atomicOr(error_flag, 64u);
             continue;
         }
-        if (words) {
+        if (vec) {
+            const uint4* s = reinterpret_cast<const uint4*>(in + i * elsize);
+            uint4* d = reinterpret_cast<uint4*>(out + o * elsize);
+            for (int64_t q = 0; q < (elsize >> 4); q++) {
+                const uint4 v = s[q];
+                d[q] = v;
+                if (checksum) csum += sbyte_sum(v.x) + sbyte_sum(v.y) + sbyte_sum(v.z) + sbyte_sum(v.w);
+            }
+        } else if (words) {
             const uint32_t* s = reinterpret_cast<const uint32_t*>(in + i * elsize);
             uint32_t* d = reinterpret_cast<uint32_t*>(out + o * elsize);
             for (int64_t q = 0; q < (elsize >> 2); q++) {
@@ -573,8 +606,10 @@ int launch_pack(const void* in, int64_t n, int64_t elsize, int64_t key_off, int 
     if (n <= 0) return 0;
     int64_t blocks = (n + kBlock - 1) / kBlock;
     if (blocks > 8192) blocks = 8192;
+    const int vec = (elsize % 16 == 0 && (uintptr_t)in % 16 == 0) ? 1 : 0;
+    const int kwords = (key_off % 4 == 0 && key_bytes % 4 == 0 && elsize % 4 == 0 && (uintptr_t)in % 4 == 0) ? 1 : 0;
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, (const uint8_t*)in,
-                       n, elsize, key_off, key_bytes, (uint32_t*)out, rw, checksum);
+                       n, elsize, key_off, key_bytes, (uint32_t*)out, rw, checksum, vec, kwords);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }

@@ -130,3 +130,21 @@ def segment_sizes(nseg, seed, lo=1e3, hi=1e7, total=1 << 30):
 
 def mix64(z):
     return int(_gen().mpsort_gen_mix64(z))
+
+
+def snapshot_pair(n, seed, displaced=0.001):
+    """Two snapshots of n particles ({u64 id, f32 pos[3], f32 vel[3]}, 32 B):
+    in A the ids are a seeded permutation of 0..n-1 and pos/vel uniform in
+    [0, 1); B is A with round(displaced * n) records moved to other chosen
+    positions (a cyclic shift among those positions)."""
+    a = numpy.empty(n, PARTICLE32)
+    a['id'] = permutation(n, seed)
+    g = numpy.random.default_rng(seed)
+    a['pos'] = g.random((n, 3), dtype=numpy.float32)
+    a['vel'] = g.random((n, 3), dtype=numpy.float32)
+    m = int(round(displaced * n))
+    b = a.copy()
+    if m > 1:
+        pos = numpy.sort(g.choice(n, m, replace=False))
+        b[pos] = a[numpy.roll(pos, 1)]
+    return a, b
