@@ -639,48 +639,52 @@ constexpr int kMaxFallbackSegs = 4096;     // larger lists re-sort the whole sid
 
 struct LearnedSide {
     int64_t *begin, *count, *ltiles;  // segment tables (learned tile size)
-    uint32_t* lhist;                  // nseg * 1024
-    int64_t* bbase;                   // nseg * 1024
-    uint64_t* tables;                 // nseg * 511
+    uint32_t* lhist;                  // nseg * buckets per segment
+    int64_t* bbase;                   // nseg * buckets per segment
+    uint64_t* tables;                 // nseg * boundaries per segment
     float *q0, *q1;                   // nseg
     uint32_t* tickets;                // 4
     uint32_t* err;                    // 4
     int64_t* overflow;                // 2 * kMaxOverflowList
     uint32_t* noverflow;              // 4: [0] oversized buckets, [1] buckets for the LDS radix sort
-    uint32_t* hard;                   // nseg * kLBins
+    uint32_t* hard;                   // nseg * buckets per segment
+    uint32_t* medium;                 // nseg * buckets per segment (merged SecondSort)
     int64_t tstride = kLBounds;       // segment s uses tables + s * tstride (0: one shared table)
 };
 
+// sized for the larger of the two bucket counts (kLBins, kMergeBins per segment)
 template <class A>
 static void lay_lside(A& a, int P, LearnedSide* ls) {
     if constexpr (std::is_same<A, Arena>::value) {
         ls->begin = a.template take<int64_t>(P);
         ls->count = a.template take<int64_t>(P);
         ls->ltiles = a.template take<int64_t>(P + 1);
-        ls->lhist = a.template take<uint32_t>((size_t)P * kLBins);
-        ls->bbase = a.template take<int64_t>((size_t)P * kLBins);
-        ls->tables = a.template take<uint64_t>((size_t)P * kLBounds);
+        ls->lhist = a.template take<uint32_t>((size_t)P * kMergeBins);
+        ls->bbase = a.template take<int64_t>((size_t)P * kMergeBins);
+        ls->tables = a.template take<uint64_t>((size_t)P * kMergeBounds);
         ls->q0 = a.template take<float>(P);
         ls->q1 = a.template take<float>(P);
         ls->tickets = a.template take<uint32_t>(4);
         ls->err = a.template take<uint32_t>(4);
         ls->overflow = a.template take<int64_t>(2 * (size_t)kMaxOverflowList);
         ls->noverflow = a.template take<uint32_t>(4);
-        ls->hard = a.template take<uint32_t>((size_t)P * kLBins);
+        ls->hard = a.template take<uint32_t>((size_t)P * kMergeBins);
+        ls->medium = a.template take<uint32_t>((size_t)P * kMergeBins);
     } else {
         a.template take<int64_t>(P);
         a.template take<int64_t>(P);
         a.template take<int64_t>(P + 1);
-        a.template take<uint32_t>((size_t)P * kLBins);
-        a.template take<int64_t>((size_t)P * kLBins);
-        a.template take<uint64_t>((size_t)P * kLBounds);
+        a.template take<uint32_t>((size_t)P * kMergeBins);
+        a.template take<int64_t>((size_t)P * kMergeBins);
+        a.template take<uint64_t>((size_t)P * kMergeBounds);
         a.template take<float>(P);
         a.template take<float>(P);
         a.template take<uint32_t>(4);
         a.template take<uint32_t>(4);
         a.template take<int64_t>(2 * (size_t)kMaxOverflowList);
         a.template take<uint32_t>(4);
-        a.template take<uint32_t>((size_t)P * kLBins);
+        a.template take<uint32_t>((size_t)P * kMergeBins);
+        a.template take<uint32_t>((size_t)P * kMergeBins);
     }
 }
 
@@ -1003,10 +1007,13 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                                 h.run_src, h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
-        // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n
-        if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, kLBounds, LB.tables, M.heavy, M.nheavy, st)))
-            return rc;
+        // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n;
+        // the merged form uses kMergeBins buckets per receiver
         const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 32);
+        const int nbB = merged ? kMergeBounds : kLBounds;
+        LB.tstride = nbB;
+        if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, st)))
+            return rc;
         if (merged) {
             // no exchange: every receiver bucket is gathered from the senders'
             // sorted segments and sorted in LDS
@@ -1015,28 +1022,33 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                 return 0;
             };
             const LKey lk = make_lkey(cs.ks);
-            if ((rc = c.mbnd.ensure((size_t)P * P * (kLBins + 1) * sizeof(int32_t)))) return rc;
+            if ((rc = c.mbnd.ensure((size_t)P * P * (kMergeBins + 1) * sizeof(int32_t)))) return rc;
             int32_t* bnd = (int32_t*)c.mbnd.p;
             if ((rc = markB(0))) return rc;
             if ((rc = launch_merge_bounds(rw, c.s0.p, lk, pa.begin, h.myC, P, LB.tables, bnd, st))) return rc;
             if ((rc = launch_merge_count(bnd, P, LB.lhist, st))) return rc;
-            if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st))) return rc;
+            if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st, kMergeBins))) return rc;
             if ((rc = launch_fill_u32(LB.noverflow, 0u, 4, st))) return rc;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
             if ((rc = markB(1)) || (rc = markB(2))) return rc;
             if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase, LB.tables,
                                           LB.overflow, LB.noverflow, kMaxOverflowList, LB.hard, LB.noverflow + 1,
-                                          cs_direct ? h.checksum + 1 : nullptr, st)))
+                                          LB.medium, LB.noverflow + 2, cs_direct ? h.checksum + 1 : nullptr, st)))
                 return rc;
             if ((rc = markB(3))) return rc;
-            // crowded buckets were copied in place in sender order
-            if ((rc = launch_lsd_sort(rw, final_dst, final_dst, lk, LB.lhist, LB.bbase, LB.hard, LB.noverflow + 1,
-                                      (int64_t)P * kLBins, nullptr, st)))
-                return rc;
-            if ((rc = markB(4))) return rc;
-            uint32_t novh[2] = {0, 0};
+            // buckets above the workgroup's capacity and crowded ones were
+            // copied in place in sender order: the 1024-thread in-LDS sort,
+            // then the LDS radix sort
+            uint32_t novh[3] = {0, 0, 0};
             MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
             MPSG_CHECK(hipStreamSynchronize(st));
+            if ((rc = launch_listed_sort(rw, final_dst, lk, LB.medium, novh[2], LB.lhist, LB.bbase, LB.tables,
+                                         kMergeBins, kMergeBounds, LB.hard, LB.noverflow + 1, st)))
+                return rc;
+            if ((rc = launch_lsd_sort(rw, final_dst, final_dst, lk, LB.lhist, LB.bbase, LB.hard, LB.noverflow + 1,
+                                      (int64_t)P * kMergeBins, nullptr, st)))
+                return rc;
+            if ((rc = markB(4))) return rc;
             const uint32_t nov = novh[0];
             lc.overflow_buckets += (int)nov;
             lc.hard_buckets += (int)novh[1];
@@ -1086,7 +1098,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     stats->learned = 1;
     stats->splitters_exact = nexact;
     stats->splitters_repaired = P > 1 ? P - 1 - nexact : 0;
-    stats->buckets = P * kLBins * (P > 1 ? 2 : 1);
+    stats->buckets = P * kLBins + (P > 1 ? P * (LB.tstride == kMergeBounds ? kMergeBins : kLBins) : 0);
     stats->buckets_overflow = lc.overflow_buckets;
     stats->t_first_sort_ms = ev_ms(c.ev[kEvStart], c.ev[kEvFirst]);
     stats->t_findp_ms = ev_ms(c.ev[kEvFirst], c.ev[kEvFindP]);

@@ -7,6 +7,10 @@ namespace mpsg {
 constexpr int kLBins = 1024;    // learned buckets per segment (10-bit digit)
 constexpr int kLBounds = 1023;  // boundaries per segment table -> buckets 0..1023
 constexpr int kChunkCap = 8192; // largest bucket sorted in LDS
+// merged SecondSort: buckets per receiver, table boundaries, in-LDS capacity
+constexpr int kMergeBins = 2048;
+constexpr int kMergeBounds = 2047;
+constexpr int kMergeCap = 4096;
 
 // key of a sort record as one uint64 (keys of at most 8 bytes): bytes
 // [sh/8, sh/8 + key_bytes) of words w0..w0+2, masked, XOR-ed with the sign
@@ -33,7 +37,9 @@ inline LKey make_lkey(const KeySpec& ks) {
 int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
                  const uint64_t* tables, int64_t tstride, uint32_t* hist, uint16_t* tile_cnt, long long* checksum,
                  hipStream_t st);
-int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st);
+// nb buckets per segment: 1024 or 2048
+int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st,
+                 int nb = kLBins);
 int lscatter_tile(int rw);
 // tile_off[tile][b]: offset of the tile's bucket-b run inside its segment
 int launch_ltile_scan(const uint16_t* tile_cnt, const SegDev& sd, const int64_t* bucket_base, uint32_t* tile_off,
@@ -53,15 +59,23 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
                       long long* checksum, bool eq_done, hipStream_t st, hipEvent_t mid = nullptr);
 
 // SecondSort straight from the senders' sorted segments (no exchange), for
-// P <= 64 receivers with 1023-boundary tables each:
-//   bnd: P * P * 1025 int32; hist / bucket_base: P * 1024
+// P <= 64 receivers with kMergeBounds-boundary tables each:
+//   bnd: P * P * (kMergeBins + 1) int32; hist / bucket_base: P * kMergeBins
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* myC, int P,
                         const uint64_t* tables, int32_t* bnd, hipStream_t st);
 int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st);
+// buckets above kMergeCap (up to kChunkCap) are copied in sender order and
+// listed in medium (*nmedium entries) for launch_listed_sort
 int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
                         const uint64_t* tables, int64_t* overflow, uint32_t* noverflow, int max_overflow,
-                        uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st);
+                        uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum,
+                        hipStream_t st);
+// in-place in-LDS sort of the nlist listed buckets (nbins buckets and nbounds
+// table boundaries per segment); crowded ones are appended to hard
+int launch_listed_sort(int rw, void* recs, const LKey& lk, const uint32_t* list, uint32_t nlist,
+                       const uint32_t* hist, const int64_t* bucket_base, const uint64_t* tables, int nbins,
+                       int nbounds, uint32_t* hard, uint32_t* nhard, hipStream_t st);
 // LDS radix sort of the listed buckets (src may equal dst)
 int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                     const int64_t* bucket_base, const uint32_t* hard, const uint32_t* nhard, int64_t max_buckets,

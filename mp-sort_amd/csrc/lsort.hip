@@ -132,8 +132,9 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
     }
 }
 
-// exclusive scan of 1024 values, one per thread of a 1024-thread block
-__device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* s_tmp /*16*/) {
+// exclusive scan of B values, one per thread of a B-thread block
+template <int B>
+__device__ __forceinline__ uint32_t scan_block(uint32_t v, uint32_t* s_tmp /*B / 64*/) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -145,20 +146,25 @@ __device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* s_tmp /*16*/)
     __syncthreads();
     uint32_t pre = 0;
 #pragma unroll
-    for (int w = 0; w < 16; w++) pre += (w < wave) ? s_tmp[w] : 0u;
+    for (int w = 0; w < B / 64; w++) pre += (w < wave) ? s_tmp[w] : 0u;
     __syncthreads();
     return pre + x - v;
 }
+__device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* s_tmp /*16*/) { return scan_block<1024>(v, s_tmp); }
 
-// per segment: absolute base of every bucket
+// per segment: absolute base of every bucket (nb buckets per segment, a
+// multiple of 1024: nb / 1024 consecutive buckets per thread)
 __global__ __launch_bounds__(1024) void lscan_kernel(const uint32_t* __restrict__ hist,
                                                      const int64_t* __restrict__ seg_begin,
-                                                     int64_t* __restrict__ base) {
+                                                     int64_t* __restrict__ base, int nb) {
     __shared__ uint32_t s_tmp[16];
-    const int s = blockIdx.x, tid = threadIdx.x;
-    const uint32_t c = hist[(int64_t)s * kLBins + tid];
-    const uint32_t ex = scan1024(c, s_tmp);
-    base[(int64_t)s * kLBins + tid] = seg_begin[s] + (int64_t)ex;
+    const int s = blockIdx.x, tid = threadIdx.x, per = nb / 1024;
+    const uint32_t* h = hist + (int64_t)s * nb + (int64_t)tid * per;
+    uint32_t c[2] = {h[0], per > 1 ? h[1] : 0u};
+    uint32_t ex = scan1024(c[0] + c[1], s_tmp);
+    int64_t* bs = base + (int64_t)s * nb + (int64_t)tid * per;
+    bs[0] = seg_begin[s] + (int64_t)ex;
+    if (per > 1) bs[1] = seg_begin[s] + (int64_t)(ex + c[0]);
 }
 
 // offset of every (tile, bucket) run inside its segment: the bucket's start
@@ -352,13 +358,14 @@ __device__ __forceinline__ void load_bucket(const uint32_t* __restrict__ src, in
     }
 }
 
+template <int W = kCWaves>
 __device__ __forceinline__ void block_checksum_add(long long csum, long long* s_sum, long long* checksum) {
     csum = wave_sum_ll(csum);
     if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = csum;
     __syncthreads();
     if (threadIdx.x == 0) {
         long long a = 0;
-        for (int w = 0; w < kCWaves; w++) a += s_sum[w];
+        for (int w = 0; w < W; w++) a += s_sum[w];
         atomicAdd((unsigned long long*)checksum, (unsigned long long)a);
     }
 }
@@ -437,21 +444,33 @@ __device__ __forceinline__ void lds_count_pass(const uint16_t* s_in, uint16_t* s
 // Per bucket: single-value buckets already written (eq_done) only add their
 // checksum; oversized ones go to the overflow list; a bucket of one key value
 // is copied.  Otherwise each record gets a bin, non-decreasing in the key:
-//   plain:    d = min(kFine-1, floor(float(key - min) * kFine / (max - min + 1)))
+//   plain:    d = min(FINE-1, floor(float((key - min) >> sh) * FINE / (((max - min) >> sh) + 1)))
+//             (sh brings the offsets to 32 bits)
 // and, when a plain bin holds more than kInterpMax records (clustered keys),
 //   sampled:  64 keys sampled from the bucket and sorted split the key range
-//             into 65 intervals of about equal counts, each cut into 63 bins:
-//             j = #{samples <= key}, [lo, hi) = interval j,
-//             d = 63 j + min(62, floor(float(key - lo) * (63 / float(hi - lo))))
+//             into 65 intervals of about equal counts, each cut into SUB bins:
+//             j = #{samples <= key}, [lo, hi] = interval j,
+//             d = SUB j + min(SUB-1, floor(float((key - lo) >> sh_j) * SUB / (((hi - lo) >> sh_j) + 1)))
 // Records are grouped by bin with LDS atomics (in any order inside a bin),
 // and a record's final slot is its bin start + #{records of the bin with a
 // smaller key, or an equal key and a smaller position}: the stable order.
 // Buckets with a bin above kRunMax records go to the list for
-// lsd_sort_kernel.  Record i of the bucket is item i / kCBlock of thread
-// i % kCBlock.
-constexpr int kSamp = 64, kSub = 63, kFine = (kSamp + 1) * kSub;  // 4095 bins
+// lsd_sort_kernel.  Record i of the bucket is item i / BLOCK of thread
+// i % BLOCK.
+constexpr int kSamp = 64;
 constexpr int kInterpMax = 32;
-static_assert(kCBlock == 16 * kSamp, "the sample sort takes 16 threads per sample");
+// Workgroup shapes of the in-LDS sort: 1024 threads sort up to 8192 records
+// in 4095 bins (65 x 63), 512 threads up to 4096 records in 2015 bins
+// (65 x 31) with half the LDS, so that two workgroups share a CU.
+template <int BLOCK>
+struct BSort {
+    static constexpr int B = BLOCK, W = BLOCK / 64, CAP = BLOCK * kCItems;
+    static constexpr int SUB = BLOCK == 1024 ? 63 : 31;  // bins per sample interval
+    static constexpr int FINE = (kSamp + 1) * SUB;       // bins
+    static constexpr int NB = CAP / 2;                   // bin counters (FINE + 1 rounded up)
+    static constexpr int TPS = BLOCK / kSamp;            // threads per sample in the sample sort
+    static_assert(NB >= FINE + 1 && NB % BLOCK == 0 && kSamp % TPS == 0, "in-LDS sort shape");
+};
 
 // the records of one bucket, contiguous in src from base
 template <int RW>
@@ -484,50 +503,51 @@ struct IndexSrc {
     __device__ __forceinline__ void load(int i, uint32_t (&r)[RW]) const { RecIO<RW>::load(recs, index[i], r); }
 };
 
-// Sorts one bucket of count (1 .. kChunkCap) records, given by src, into
-// dst[base .. base + count).  Returns false (and writes nothing) when a bin
-// is crowded beyond kRunMax.  s_bins: 2 * (kFine + 1) words of LDS owned by
+// Sorts one bucket of count (1 .. BSort<BLOCK>::CAP) records, given by src,
+// into dst[base .. base + count) with a BLOCK-thread workgroup.  Returns false (and writes nothing) when a bin
+// is crowded beyond kRunMax.  s_bins: 2 * BSort<BLOCK>::NB words of LDS owned by
 // the caller, who may keep data there that src reads while loading.  When
 // bounded, every key lies in [lo, hi] (the bucket's boundaries) and the
 // bucket's minimum and maximum are not computed.
-template <int RW, class Src>
+template <int RW, int BLOCK, class Src>
 __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
                                                 const LKey& lk, uint32_t* s_bins, long long* __restrict__ checksum_out,
                                                 bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull) {
-    __shared__ uint64_t s_gkey[kChunkCap + 4];  // key of each bin-order slot (+4: unrolled reads)
-    __shared__ uint16_t s_sbin[kChunkCap];      // bin of each bin-order slot
-    __shared__ uint16_t s_ord[kChunkCap];       // bucket position of each bin-order slot
-    __shared__ uint16_t s_pos[kChunkCap];       // final slot of each bucket position
+    using C = BSort<BLOCK>;
+    __shared__ uint64_t s_gkey[C::CAP + 4];  // key of each bin-order slot (+4: unrolled reads)
+    __shared__ uint16_t s_sbin[C::CAP];      // bin of each bin-order slot
+    __shared__ uint16_t s_ord[C::CAP];       // bucket position of each bin-order slot
+    __shared__ uint16_t s_pos[C::CAP];       // final slot of each bucket position
     __shared__ uint64_t s_samp[kSamp];    // the sample, in bucket order
     __shared__ uint64_t s_sorted[kSamp];  // the sample, sorted
     __shared__ int s_crowd;
     __shared__ uint64_t s_ilo[kSamp + 1];  // sampled bins: interval j starts at key s_ilo[j],
     __shared__ uint32_t s_ish[kSamp + 1];  // its offsets are shifted right by s_ish[j]
-    __shared__ float s_isc[kSamp + 1];     // and scaled by s_isc[j] into kSub bins
-    __shared__ uint64_t s_red[2 * kCWaves];
-    __shared__ uint32_t s_tmp[16];
-    __shared__ uint32_t s_max[kCWaves];
-    __shared__ long long s_sum[kCWaves];
+    __shared__ float s_isc[kSamp + 1];     // and scaled by s_isc[j] into SUB bins
+    __shared__ uint64_t s_red[2 * C::W];
+    __shared__ uint32_t s_tmp[C::W];
+    __shared__ uint32_t s_max[C::W];
+    __shared__ long long s_sum[C::W];
     __shared__ int s_long;
     uint32_t* s_start = s_bins;
-    uint32_t* s_fill = s_bins + kFine + 1;
+    uint32_t* s_fill = s_bins + C::NB;
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
     // Every load is issued before any of them is used, so they are all in
-    // flight at once: item k of thread t is record k * kCBlock + t (indices
+    // flight at once: item k of thread t is record k * BLOCK + t (indices
     // past the bucket read its last record again and are ignored); thread
     // q < 64 also reads the sample key of record floor(q * count / 64).
     uint32_t rs[RW];
     src.load(((tid & (kSamp - 1)) * count) >> 6, rs);
     uint32_t r[kCItems][RW];
 #pragma unroll
-    for (int k = 0; k < kCItems; k++) src.load(min(k * kCBlock + tid, count - 1), r[k]);
+    for (int k = 0; k < kCItems; k++) src.load(min(k * C::B + tid, count - 1), r[k]);
     uint64_t kk[kCItems];
     uint64_t mn = ~0ull, mx = 0;
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
-        const bool valid = k * kCBlock + tid < count;
+        const bool valid = k * C::B + tid < count;
         const uint64_t key = lkey<RW>(r[k], lk);
         kk[k] = valid ? key : 0;
         mn = valid && key < mn ? key : mn;
@@ -549,14 +569,14 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             mn = a < mn ? a : mn;
             mx = c > mx ? c : mx;
         }
-        if (lane == 0) { s_red[wave] = mn; s_red[kCWaves + wave] = mx; }
+        if (lane == 0) { s_red[wave] = mn; s_red[C::W + wave] = mx; }
         __syncthreads();  // every load of src is done: s_bins is free
         mn = s_red[0];
-        mx = s_red[kCWaves];
+        mx = s_red[C::W];
 #pragma unroll
-        for (int w = 1; w < kCWaves; w++) {
+        for (int w = 1; w < C::W; w++) {
             mn = s_red[w] < mn ? s_red[w] : mn;
-            mx = s_red[kCWaves + w] > mx ? s_red[kCWaves + w] : mx;
+            mx = s_red[C::W + w] > mx ? s_red[C::W + w] : mx;
         }
     }
     long long csum = 0;
@@ -564,7 +584,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         // one key value: the stable order is the current order
 #pragma unroll
         for (int k = 0; k < kCItems; k++) {
-            const int i = k * kCBlock + tid;
+            const int i = k * C::B + tid;
             if (i < count) {
                 RecIO<RW>::store(dst, base + i, r[k]);
                 if (checksum_out) {
@@ -573,33 +593,34 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
                 }
             }
         }
-        if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
+        if (checksum_out) block_checksum_add<C::W>(csum, s_sum, checksum_out);
         return true;
     }
-    for (int i = tid; i <= kFine; i += kCBlock) s_fill[i] = 0;
+    for (int i = tid; i < C::NB; i += C::B) s_fill[i] = 0;
     // plain bins: offsets from the minimum shifted to 32 bits, then scaled
     // (shift, conversion and scaling are all monotone)
     const int pbits = 64 - __clzll((long long)((mx - mn) | 1));
     const int psh = pbits > 32 ? pbits - 32 : 0;
-    const float scale = (float)kFine / ((float)(uint32_t)((mx - mn) >> psh) + 1.0f);
+    const float scale = (float)C::FINE / ((float)(uint32_t)((mx - mn) >> psh) + 1.0f);
     auto plain_bin = [&](uint64_t key) -> uint32_t {
         const uint32_t d = (uint32_t)((float)(uint32_t)((key - mn) >> psh) * scale);
-        return d > (uint32_t)(kFine - 1) ? (uint32_t)(kFine - 1) : d;
+        return d > (uint32_t)(C::FINE - 1) ? (uint32_t)(C::FINE - 1) : d;
     };
     {
-        // sort the sample: thread t ranks sample t / 16 against samples
-        // 4 (t % 16) .. 4 (t % 16) + 3 (ties by index), sums over 16 lanes
-        const int a = tid >> 4, part = tid & 15;
+        // sort the sample: TPS threads rank one sample, each against
+        // kSamp / TPS others (ties by index), sums over the TPS lanes
+        constexpr int PER = kSamp / C::TPS;
+        const int a = tid / C::TPS, part = tid % C::TPS;
         const uint64_t v = s_samp[a];
         uint32_t c = 0;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int q = part * 4 + u;
+        for (int u = 0; u < PER; u++) {
+            const int q = part * PER + u;
             const uint64_t o = s_samp[q];
             c += (o < v || (o == v && q < a)) ? 1u : 0u;
         }
 #pragma unroll
-        for (int off = 1; off < 16; off <<= 1) c += __shfl_xor(c, off, 16);
+        for (int off = 1; off < C::TPS; off <<= 1) c += __shfl_xor(c, off, C::TPS);
         if (part == 0) s_sorted[c] = v;
     }
     __syncthreads();
@@ -614,7 +635,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         const int sh = bits > 32 ? bits - 32 : 0;
         s_ilo[j] = lo;
         s_ish[j] = (uint32_t)sh;
-        s_isc[j] = (float)kSub / ((float)(uint32_t)(span >> sh) + 1.0f);
+        s_isc[j] = (float)C::SUB / ((float)(uint32_t)(span >> sh) + 1.0f);
         // two neighbouring samples in one plain bin put about count / 64
         // records there: the sampled bins are used
         if (j + 1 < kSamp && plain_bin(s_sorted[j]) == plain_bin(s_sorted[j + 1])) s_crowd = 1;
@@ -624,7 +645,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     auto bin_of = [&](uint64_t key) -> uint32_t {
         if (mode == 0) return plain_bin(key);
         // sampled: j = #{samples <= key}, then the interval's offset shifted
-        // and scaled into kSub bins
+        // and scaled into SUB bins
         int j = 0;
 #pragma unroll
         for (int step = kSamp / 2; step > 0; step >>= 1)
@@ -632,14 +653,14 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         if (j == kSamp - 1 && s_sorted[kSamp - 1] <= key) j = kSamp;
         const uint32_t x = (uint32_t)((key - s_ilo[j]) >> s_ish[j]);
         uint32_t sub = (uint32_t)((float)x * s_isc[j]);
-        sub = sub > (uint32_t)(kSub - 1) ? (uint32_t)(kSub - 1) : sub;
-        return (uint32_t)j * kSub + sub;
+        sub = sub > (uint32_t)(C::SUB - 1) ? (uint32_t)(C::SUB - 1) : sub;
+        return (uint32_t)j * C::SUB + sub;
     };
     uint32_t dbin[kCItems];
     for (;;) {
 #pragma unroll
         for (int k = 0; k < kCItems; k++) {
-            const int i = k * kCBlock + tid;
+            const int i = k * C::B + tid;
             dbin[k] = 0;
             if (i < count) {
                 dbin[k] = bin_of(kk[k]);
@@ -647,9 +668,9 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             }
         }
         __syncthreads();
-        // exclusive scan of the kFine + 1 counts (the last one zero), four
+        // exclusive scan of the NB counts (those past FINE zero), NB / BLOCK
         // per thread; the counters are reset for the grouping below
-        constexpr int PER = (kFine + 1) / kCBlock;
+        constexpr int PER = C::NB / C::B;
         uint32_t c[PER], sum = 0, big = 0;
 #pragma unroll
         for (int j = 0; j < PER; j++) {
@@ -658,7 +679,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             sum += c[j];
             big = c[j] > big ? c[j] : big;
         }
-        uint32_t pre = scan1024(sum, s_tmp);
+        uint32_t pre = scan_block<C::B>(sum, s_tmp);
 #pragma unroll
         for (int j = 0; j < PER; j++) { s_start[tid * PER + j] = pre; pre += c[j]; }
 #pragma unroll
@@ -670,7 +691,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         __syncthreads();
         uint32_t bmax = 0;
 #pragma unroll
-        for (int w = 0; w < kCWaves; w++) bmax = s_max[w] > bmax ? s_max[w] : bmax;
+        for (int w = 0; w < C::W; w++) bmax = s_max[w] > bmax ? s_max[w] : bmax;
         // clustered keys the sample did not reveal: sampled bins instead
         if (mode == 1 || bmax <= (uint32_t)kInterpMax) break;
         mode = 1;
@@ -678,7 +699,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     // group by bin: key, bin and bucket position of every bin-order slot
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
-        const int i = k * kCBlock + tid;
+        const int i = k * C::B + tid;
         if (i < count) {
             const uint32_t o = s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u);
             s_gkey[o] = kk[k];
@@ -694,7 +715,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     // key.  Equal keys keep their bucket order.
     if (tid < 4) s_gkey[count + tid] = ~0ull;
     __syncthreads();
-    for (int o = tid; o < count; o += kCBlock) {
+    for (int o = tid; o < count; o += C::B) {
         const uint32_t d = s_sbin[o];
         const uint32_t i = s_ord[o];
         const uint32_t s0 = s_start[d], e = s_start[d + 1];
@@ -729,7 +750,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     if (s_long) return false;
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
-        const int i = k * kCBlock + tid;
+        const int i = k * C::B + tid;
         if (i < count) {
             RecIO<RW>::store(dst, base + s_pos[i], r[k]);
             if (checksum_out) {
@@ -738,7 +759,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             }
         }
     }
-    if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
+    if (checksum_out) block_checksum_add<C::W>(csum, s_sum, checksum_out);
     return true;
 }
 
@@ -752,7 +773,7 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
     uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
     long long* __restrict__ checksum_out, int eq_done) {
     __shared__ long long s_sum[kCWaves];
-    __shared__ uint32_t s_bins[2 * (kFine + 1)];
+    __shared__ uint32_t s_bins[2 * BSort<kCBlock>::NB];
     const int tid = threadIdx.x;
     // count, base and the table entries t[k-1], t[k] around bucket k
     struct Meta {
@@ -803,20 +824,49 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
             const RunSrc<RW> rs{src, base};
             // an interior bucket's keys lie in (t[k-1], t[k]]
             const bool bounded = tables && kb > 0 && kb < kLBins - 1 && cur.tlo < cur.thi;
-            if (!sort_bucket_lds<RW>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
+            if (!sort_bucket_lds<RW, kCBlock>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
                                      bounded ? cur.tlo + 1 : 0, bounded ? cur.thi : ~0ull))
                 if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
         }
     }
 }
 
+// The listed buckets (ids into hist / bucket_base, nbins buckets and nbounds
+// table boundaries per segment), sorted in place by 1024-thread workgroups:
+// the merged SecondSort's buckets above its own in-LDS capacity, already
+// copied in sender order.  A crowded bin sends the bucket on to the LDS radix
+// sort's list.
+template <int RW>
+__global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
+    uint32_t* __restrict__ recs, LKey lk, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hist,
+    const int64_t* __restrict__ bucket_base, const uint64_t* __restrict__ tables, int nbins, int nbounds,
+    uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard) {
+    __shared__ uint32_t s_bins[2 * BSort<kCBlock>::NB];
+    const uint32_t b = list[blockIdx.x];
+    const uint32_t cnt32 = hist[b];
+    const int64_t base = bucket_base[b];
+    const int k = (int)(b % (uint32_t)nbins);
+    const uint64_t* t = tables + (int64_t)(b / (uint32_t)nbins) * nbounds;
+    const uint64_t tlo = k > 0 ? t[k - 1] : 0, thi = k < nbounds ? t[k] : ~0ull;
+    if (cnt32 == 0 || cnt32 > (uint32_t)kChunkCap) return;
+    // every record is in registers before the first store: in place is safe
+    const RunSrc<RW> rs{recs, base};
+    const bool bounded = k > 0 && k < nbins - 1 && tlo < thi;
+    if (!sort_bucket_lds<RW, kCBlock>(rs, recs, base, (int)cnt32, lk, s_bins, nullptr, bounded,
+                                      bounded ? tlo + 1 : 0, bounded ? thi : ~0ull))
+        if (threadIdx.x == 0) hard[atomicAdd(nhard, 1u)] = b;
+}
+
 // ---------------------------------------------------------------------------
 // SecondSort without the exchange.  Receiver r's records from sender j are
 // positions [myC[j][r], myC[j][r+1]) of sender j's sorted segment; bucket k of
-// r holds the keys in (t_r[k-1], t_r[k]], i.e. one sub-run per sender.
+// r holds the keys in (t_r[k-1], t_r[k]], i.e. one sub-run per sender.  Every
+// receiver has kMergeBins buckets (tables of kMergeBounds boundaries), so a
+// bucket averages half the records of a FirstSort bucket and is sorted by a
+// kMergeBlock-thread workgroup with half the LDS: two share a CU.
 //   merge_bounds_kernel  bnd[r][k][j] = start of bucket k's sub-run in sender
-//                        j (relative to j's segment), k = 0..1024 (the P
-//                        senders of one bucket side by side)
+//                        j (relative to j's segment), k = 0..kMergeBins (the
+//                        P senders of one bucket side by side)
 //   merge_count_kernel   bucket sizes (sum over senders)
 //   merge_bucket_kernel  one workgroup per (receiver, bucket): gathers the
 //                        sub-runs in sender order (the order the exchange
@@ -826,6 +876,8 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
 //                        that order and listed for the LDS radix sort or the
 //                        radix path
 constexpr int kMergeMaxP = 64;
+constexpr int kMergeBlock = 512;
+static_assert(BSort<kMergeBlock>::CAP == kMergeCap, "merged SecondSort capacity");
 
 template <int RW>
 __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __restrict__ recs, LKey lk,
@@ -833,21 +885,21 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
                                                            const int64_t* __restrict__ myC, int P,
                                                            const uint64_t* __restrict__ tables,
                                                            int32_t* __restrict__ bnd) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, j, k), k = 0..1024
-    if (g >= (int64_t)P * P * (kLBins + 1)) return;
-    const int k = (int)(g % (kLBins + 1));
-    const int64_t rj = g / (kLBins + 1);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, j, k), k = 0..kMergeBins
+    if (g >= (int64_t)P * P * (kMergeBins + 1)) return;
+    const int k = (int)(g % (kMergeBins + 1));
+    const int64_t rj = g / (kMergeBins + 1);
     const int j = (int)(rj % P), r = (int)(rj / P);
     const int64_t* cj = myC + (int64_t)j * (P + 1);
     const int64_t lo0 = cj[r], hi0 = cj[r + 1];
     int64_t res;
     if (k == 0) {
         res = lo0;
-    } else if (k == kLBins) {
+    } else if (k == kMergeBins) {
         res = hi0;
     } else {
         // first position of the window with key > t_r[k-1]
-        const uint64_t t = tables[(int64_t)r * kLBounds + (k - 1)];
+        const uint64_t t = tables[(int64_t)r * kMergeBounds + (k - 1)];
         const int64_t sb = seg_begin[j];
         int64_t a = lo0, e = hi0;
         while (a < e) {
@@ -858,109 +910,108 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
         }
         res = a;
     }
-    bnd[((int64_t)r * (kLBins + 1) + k) * P + j] = (int32_t)res;
+    bnd[((int64_t)r * (kMergeBins + 1) + k) * P + j] = (int32_t)res;
 }
 
 __global__ __launch_bounds__(256) void merge_count_kernel(const int32_t* __restrict__ bnd, int P,
                                                           uint32_t* __restrict__ hist) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, k)
-    if (g >= (int64_t)P * kLBins) return;
-    const int k = (int)(g % kLBins), r = (int)(g / kLBins);
-    const int32_t* bk = bnd + ((int64_t)r * (kLBins + 1) + k) * P;
+    if (g >= (int64_t)P * kMergeBins) return;
+    const int k = (int)(g % kMergeBins), r = (int)(g / kMergeBins);
+    const int32_t* bk = bnd + ((int64_t)r * (kMergeBins + 1) + k) * P;
     uint32_t c = 0;
     for (int j = 0; j < P; j++) c += (uint32_t)(bk[P + j] - bk[j]);
     hist[g] = c;
 }
 
 template <int RW>
-__global__ __launch_bounds__(kCBlock) void merge_bucket_kernel(
+__global__ __launch_bounds__(kMergeBlock) void merge_bucket_kernel(
     const uint32_t* __restrict__ recs, uint32_t* __restrict__ dst, LKey lk,
     const int64_t* __restrict__ seg_begin, const int32_t* __restrict__ bnd, int P,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
     const uint64_t* __restrict__ tables, int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow,
-    int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
-    long long* __restrict__ checksum_out) {
+    int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard, uint32_t* __restrict__ medium,
+    uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out) {
+    using C = BSort<kMergeBlock>;
     __shared__ int64_t s_beg[kMergeMaxP];
     __shared__ int s_pref[kMergeMaxP + 1];
-    __shared__ long long s_sum2[kCWaves];
-    __shared__ uint32_t s_bins[2 * (kFine + 1)];  // the gather index first, then the bins
+    __shared__ long long s_sum2[C::W];
+    __shared__ uint32_t s_bins[2 * C::NB];  // the gather index first, then the bins
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    {
-        const int64_t b = blockIdx.x;
-        const uint32_t cnt32 = hist[b];
-        const int r = (int)(b / kLBins), k = (int)(b % kLBins);
-        const int64_t base = bucket_base[b];
-        const uint64_t* t = tables + (int64_t)r * kLBounds;
-        const uint64_t tlo = k > 0 ? t[k - 1] : 0, thi = k < kLBounds ? t[k] : ~0ull;
-        if (cnt32 == 0) return;
-        if (wave == 0) {
-            // the sub-runs of this bucket, sender by sender
-            int len = 0;
-            int64_t beg = 0;
-            if (lane < P) {
-                const int32_t* bk = bnd + ((int64_t)r * (kLBins + 1) + k) * P;
-                beg = seg_begin[lane] + bk[lane];
-                len = bk[P + lane] - bk[lane];
-            }
-            int x = len;
+    const int64_t b = blockIdx.x;
+    const uint32_t cnt32 = hist[b];
+    const int r = (int)(b / kMergeBins), k = (int)(b % kMergeBins);
+    const int64_t base = bucket_base[b];
+    const uint64_t* t = tables + (int64_t)r * kMergeBounds;
+    const uint64_t tlo = k > 0 ? t[k - 1] : 0, thi = k < kMergeBounds ? t[k] : ~0ull;
+    if (cnt32 == 0) return;
+    if (wave == 0) {
+        // the sub-runs of this bucket, sender by sender
+        int len = 0;
+        int64_t beg = 0;
+        if (lane < P) {
+            const int32_t* bk = bnd + ((int64_t)r * (kMergeBins + 1) + k) * P;
+            beg = seg_begin[lane] + bk[lane];
+            len = bk[P + lane] - bk[lane];
+        }
+        int x = len;
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int y = __shfl_up(x, off, 64);
-                if (lane >= off) x += y;
-            }
-            if (lane < P) {
-                s_beg[lane] = beg;
-                s_pref[lane] = x - len;
-            }
-            if (lane == P - 1) s_pref[P] = x;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane < P) {
+            s_beg[lane] = beg;
+            s_pref[lane] = x - len;
+        }
+        if (lane == P - 1) s_pref[P] = x;
+    }
+    __syncthreads();
+    const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
+    const int count = (int)cnt32;
+    // bucket k holds one key value (see single_bucket)
+    const bool single = k == 0 ? thi == 0 : (k < kMergeBounds && tlo + 1 == thi);
+    if (!single && count <= C::CAP) {
+        // index of every record of the bucket, sub-run by sub-run (one wave
+        // per sub-run at a time)
+        for (int j = wave; j < P; j += C::W) {
+            const int a = s_pref[j], len = s_pref[j + 1] - a;
+            for (int l = lane; l < len; l += 64) s_bins[a + l] = (uint32_t)(s_beg[j] + l);
         }
         __syncthreads();
-        const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
-        const int count = (int)cnt32;
-        // bucket k holds one key value (see single_bucket)
-        const bool single = k == 0 ? thi == 0 : (k < kLBounds && tlo + 1 == thi);
-        bool done = false;
-        if (!single && count <= kChunkCap) {
-            // index of every record of the bucket, sub-run by sub-run (one
-            // wave per sub-run at a time)
-            for (int j = wave; j < P; j += kCWaves) {
-                const int a = s_pref[j], len = s_pref[j + 1] - a;
-                for (int l = lane; l < len; l += 64) s_bins[a + l] = (uint32_t)(s_beg[j] + l);
-            }
-            __syncthreads();
-            const IndexSrc<RW> is{recs, s_bins};
-            // an interior bucket's keys lie in (t[k-1], t[k]]
-            const bool bounded = k > 0 && k < kLBins - 1 && tlo < thi;
-            done = sort_bucket_lds<RW>(is, dst, base, count, lk, s_bins, checksum_out, bounded,
-                                       bounded ? tlo + 1 : 0, bounded ? thi : ~0ull);
-        }
-        if (!done) {
-            // copy in sender order: final for a single-value bucket,
-            // otherwise the input of the radix path
-            long long csum = 0;
-            for (int i = tid; i < count; i += kCBlock) {
-                uint32_t x[RW];
-                gs.load(i, x);
-                RecIO<RW>::store(dst, base + i, x);
-                if (checksum_out) {
+        const IndexSrc<RW> is{recs, s_bins};
+        // an interior bucket's keys lie in (t[k-1], t[k]]
+        const bool bounded = k > 0 && k < kMergeBins - 1 && tlo < thi;
+        if (sort_bucket_lds<RW, kMergeBlock>(is, dst, base, count, lk, s_bins, checksum_out, bounded,
+                                             bounded ? tlo + 1 : 0, bounded ? thi : ~0ull))
+            return;
+    }
+    // copy in sender order: final for a single-value bucket, otherwise the
+    // input of the radix path
+    long long csum = 0;
+    for (int i = tid; i < count; i += C::B) {
+        uint32_t x[RW];
+        gs.load(i, x);
+        RecIO<RW>::store(dst, base + i, x);
+        if (checksum_out) {
 #pragma unroll
-                    for (int w = 0; w < RW; w++) csum += sbyte_sum(x[w]);
-                }
-            }
-            if (!single && tid == 0) {
-                if (count <= kChunkCap) {
-                    hard[atomicAdd(nhard, 1u)] = (uint32_t)b;  // a crowded bin: the LDS radix sort
-                } else {
-                    const uint32_t slot = atomicAdd(noverflow, 1u);
-                    if ((int)slot < max_overflow) {
-                        overflow[2 * slot] = base;
-                        overflow[2 * slot + 1] = (int64_t)cnt32;
-                    }
-                }
-            }
-            if (checksum_out) block_checksum_add(csum, s_sum2, checksum_out);
+            for (int w = 0; w < RW; w++) csum += sbyte_sum(x[w]);
         }
     }
+    if (!single && tid == 0) {
+        if (count <= C::CAP) {
+            hard[atomicAdd(nhard, 1u)] = (uint32_t)b;  // a crowded bin: the LDS radix sort
+        } else if (count <= kChunkCap) {
+            medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread in-LDS sort
+        } else {
+            const uint32_t slot = atomicAdd(noverflow, 1u);
+            if ((int)slot < max_overflow) {
+                overflow[2 * slot] = base;
+                overflow[2 * slot + 1] = (int64_t)cnt32;
+            }
+        }
+    }
+    if (checksum_out) block_checksum_add<C::W>(csum, s_sum2, checksum_out);
 }
 
 // The listed buckets: stable LSD radix sort in LDS over the bytes of
@@ -1042,8 +1093,9 @@ int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int 
     return 0;
 }
 
-int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st) {
-    hipLaunchKernelGGL(lscan_kernel, dim3(nseg), dim3(1024), 0, st, hist, seg_begin, base);
+int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st, int nb) {
+    if (nb != 1024 && nb != 2048) return -1;
+    hipLaunchKernelGGL(lscan_kernel, dim3(nseg), dim3(1024), 0, st, hist, seg_begin, base, nb);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
@@ -1113,7 +1165,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
 
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* myC, int P,
                         const uint64_t* tables, int32_t* bnd, hipStream_t st) {
-    const int64_t n = (int64_t)P * P * (kLBins + 1);
+    const int64_t n = (int64_t)P * P * (kMergeBins + 1);
     const unsigned g = (unsigned)((n + 255) / 256);
     const uint32_t* r = (const uint32_t*)recs;
     switch (rw) {
@@ -1126,7 +1178,7 @@ int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t*
 }
 
 int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st) {
-    const int64_t n = (int64_t)P * kLBins;
+    const int64_t n = (int64_t)P * kMergeBins;
     hipLaunchKernelGGL(merge_count_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bnd, P, hist);
     MPSG_CHECK(hipGetLastError());
     return 0;
@@ -1135,14 +1187,29 @@ int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st
 int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
                         const uint64_t* tables, int64_t* overflow, uint32_t* noverflow, int max_overflow,
-                        uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st) {
+                        uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum,
+                        hipStream_t st) {
     if (P > kMergeMaxP) return -1;
-    const unsigned g = (unsigned)((int64_t)P * kLBins);
+    const unsigned g = (unsigned)((int64_t)P * kMergeBins);
     const uint32_t* r = (const uint32_t*)recs;
     uint32_t* d = (uint32_t*)dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(g), dim3(kCBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum); break;
-        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(g), dim3(kCBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, checksum); break;
+        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(g), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(g), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_listed_sort(int rw, void* recs, const LKey& lk, const uint32_t* list, uint32_t nlist,
+                       const uint32_t* hist, const int64_t* bucket_base, const uint64_t* tables, int nbins,
+                       int nbounds, uint32_t* hard, uint32_t* nhard, hipStream_t st) {
+    if (nlist == 0) return 0;
+    uint32_t* r = (uint32_t*)recs;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(listed_sort_kernel<2>, dim3(nlist), dim3(kCBlock), 0, st, r, lk, list, hist, bucket_base, tables, nbins, nbounds, hard, nhard); break;
+        case 4: hipLaunchKernelGGL(listed_sort_kernel<4>, dim3(nlist), dim3(kCBlock), 0, st, r, lk, list, hist, bucket_base, tables, nbins, nbounds, hard, nhard); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
