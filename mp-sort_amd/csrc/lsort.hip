@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void ltile_scan_kernel(const uint16_t* __restr
 // the bucket scatter: a onesweep pass with 1024 bins
 template <int RW, int ITEMS>
 __global__ __launch_bounds__(1024) void lscatter_kernel(
-    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg, LKey lk,
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg, int64_t ntiles, LKey lk,
     const uint64_t* __restrict__ tables, int64_t tstride, const uint32_t* __restrict__ tile_off,
     uint32_t* __restrict__ error_flag, uint32_t* __restrict__ eq_dst) {
     constexpr int BLOCK = 1024, NW = 16;
@@ -217,7 +217,12 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (int i = tid; i < NW * kLBins; i += BLOCK) (&s_whist[0][0])[i] = 0;
-    const int64_t tile = blockIdx.x;
+    // workgroups go to the 8 XCDs round robin; XCD x takes the contiguous
+    // tiles [x * per, (x + 1) * per), so runs of neighbouring tiles meet in
+    // one L2 (the grid is 8 * per; the padding workgroups exit)
+    const int64_t per = gridDim.x / 8;
+    const int64_t tile = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (tile >= ntiles) return;
     const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
     for (int i = tid; i < kLBounds; i += BLOCK) s_tab[i] = tables[(int64_t)s * tstride + i];
     const int64_t t = tile - seg.tile_start[s];
@@ -1115,12 +1120,13 @@ int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegD
                     const LKey& lk, const uint64_t* tables, int64_t tstride, const uint32_t* tile_off, uint32_t* err,
                     hipStream_t st) {
     if (ntiles <= 0) return 0;
+    const unsigned grid = (unsigned)(8 * ((ntiles + 7) / 8));
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
     uint32_t* e = (uint32_t*)eq_dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, tstride, tile_off, err, e); break;
-        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3((unsigned)ntiles), dim3(1024), 0, st, s, d, sd, lk, tables, tstride, tile_off, err, e); break;
+        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, lk, tables, tstride, tile_off, err, e); break;
+        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, lk, tables, tstride, tile_off, err, e); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
