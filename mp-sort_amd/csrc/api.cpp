@@ -797,12 +797,15 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     MPSG_CHECK(hipStreamSynchronize(st));
     // per-tile bucket counts (u16) and per-tile offsets (u32)
     const size_t tc_bytes = ((size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint16_t) + 255) & ~(size_t)255;
-    if ((rc = c.ltile.ensure(tc_bytes + (size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint32_t)))) return rc;
+    const size_t to_bytes = ((size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint32_t) + 255) & ~(size_t)255;
+    if ((rc = c.ltile.ensure(tc_bytes + to_bytes + (size_t)n * sizeof(uint16_t)))) return rc;
     uint16_t* tile_cnt = (uint16_t*)c.ltile.p;
     uint32_t* tile_off = (uint32_t*)((char*)c.ltile.p + tc_bytes);
+    uint16_t* bkt = (uint16_t*)((char*)c.ltile.p + tc_bytes + to_bytes);  // bucket of every record
     SegDev sd{L.begin, L.count, L.ltiles, P, 0};
     if ((rc = mark(0))) return rc;
-    if ((rc = launch_lhist(rw, src, sd, ntiles, (int)trecs, lk, L.tables, L.tstride, L.lhist, tile_cnt, cs_in, st)))
+    if ((rc = launch_lhist(rw, src, sd, ntiles, (int)trecs, lk, L.tables, L.tstride, L.lhist, tile_cnt, cs_in, bkt,
+                           st)))
         return rc;
     if ((rc = launch_lscan(P, L.lhist, L.begin, L.bbase, st))) return rc;
     if ((rc = launch_ltile_scan(tile_cnt, sd, L.bbase, tile_off, st))) return rc;
@@ -810,7 +813,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     // single-value buckets are written to their final slots by the scatter
     // unless that would overwrite records the scatter still reads
     const bool eq_direct = dst != src;
-    if ((rc = launch_lscatter(rw, src, tmp, eq_direct ? dst : tmp, sd, ntiles, lk, L.tables, L.tstride, tile_off,
+    if ((rc = launch_lscatter(rw, src, tmp, eq_direct ? dst : tmp, sd, ntiles, bkt, L.tables, L.tstride, tile_off,
                               L.err, st)))
         return rc;
     if ((rc = mark(2))) return rc;

@@ -33,10 +33,11 @@ inline LKey make_lkey(const KeySpec& ks) {
 }
 
 // tables: segment s uses tables + s * tstride (tstride 0: one table shared);
-// tile_cnt receives every tile's 1024 bucket counts
+// tile_cnt receives every tile's 1024 bucket counts, bkt[i] the bucket of
+// record i (for the scatter)
 int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
                  const uint64_t* tables, int64_t tstride, uint32_t* hist, uint16_t* tile_cnt, long long* checksum,
-                 hipStream_t st);
+                 uint16_t* bkt, hipStream_t st);
 // nb buckets per segment: 1024 or 2048
 int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st,
                  int nb = kLBins);
@@ -44,10 +45,11 @@ int lscatter_tile(int rw);
 // tile_off[tile][b]: offset of the tile's bucket-b run inside its segment
 int launch_ltile_scan(const uint16_t* tile_cnt, const SegDev& sd, const int64_t* bucket_base, uint32_t* tile_off,
                       hipStream_t st);
-// records of single-value buckets go to eq_dst, the rest to dst
+// records of single-value buckets go to eq_dst, the rest to dst; bkt: the
+// buckets written by launch_lhist
 int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegDev& sd, int64_t ntiles,
-                    const LKey& lk, const uint64_t* tables, int64_t tstride, const uint32_t* tile_off, uint32_t* err,
-                    hipStream_t st);
+                    const uint16_t* bkt, const uint64_t* tables, int64_t tstride, const uint32_t* tile_off,
+                    uint32_t* err, hipStream_t st);
 // Sorts every bucket (src -> dst, same slots).  eq_done: the scatter already
 // wrote the single-value buckets to dst.  Buckets above kChunkCap go to the
 // overflow list (*noverflow entries of {base, count}); hard (one uint32 per

@@ -60,7 +60,8 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
                                                        const uint64_t* __restrict__ tables, int64_t tstride,
                                                        uint32_t* __restrict__ hist,
                                                        uint16_t* __restrict__ tile_cnt,
-                                                       long long* __restrict__ checksum) {
+                                                       long long* __restrict__ checksum,
+                                                       uint16_t* __restrict__ bkt) {
     __shared__ uint32_t s_hist[kLBins];  // the block's counts for its current segment
     __shared__ uint32_t s_th[kLBins];    // counts of the current tile
     __shared__ uint64_t s_tab[kLBounds];
@@ -101,7 +102,9 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
 #pragma unroll
                 for (int w = 0; w < RW; w++) csum += sbyte_sum(r[w]);
             }
-            atomicAdd(&s_th[bucket_of(lkey<RW>(r, lk), s_tab)], 1u);
+            const uint32_t b = bucket_of(lkey<RW>(r, lk), s_tab);
+            bkt[i] = (uint16_t)b;  // for the scatter
+            atomicAdd(&s_th[b], 1u);
         }
         __syncthreads();
         // the tile's counts (for the scatter's offsets), added to the segment
@@ -200,9 +203,9 @@ __global__ __launch_bounds__(256) void ltile_scan_kernel(const uint16_t* __restr
 // the bucket scatter: a onesweep pass with 1024 bins
 template <int RW, int ITEMS>
 __global__ __launch_bounds__(1024) void lscatter_kernel(
-    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg, int64_t ntiles, LKey lk,
-    const uint64_t* __restrict__ tables, int64_t tstride, const uint32_t* __restrict__ tile_off,
-    uint32_t* __restrict__ error_flag, uint32_t* __restrict__ eq_dst) {
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg, int64_t ntiles,
+    const uint16_t* __restrict__ bkt, const uint64_t* __restrict__ tables, int64_t tstride,
+    const uint32_t* __restrict__ tile_off, uint32_t* __restrict__ error_flag, uint32_t* __restrict__ eq_dst) {
     constexpr int BLOCK = 1024, NW = 16;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int WAVE_TILE = 64 * ITEMS;
@@ -211,7 +214,6 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
     __shared__ uint16_t s_whist[NW][kLBins];
     __shared__ uint32_t s_excl[kLBins];
     __shared__ int64_t s_gdelta[kLBins];
-    __shared__ uint64_t s_tab[kLBounds];
     __shared__ uint32_t s_single[kLBins / 32];
     __shared__ uint32_t s_tmp[16];
 
@@ -224,7 +226,6 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
     const int64_t tile = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (tile >= ntiles) return;
     const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
-    for (int i = tid; i < kLBounds; i += BLOCK) s_tab[i] = tables[(int64_t)s * tstride + i];
     const int64_t t = tile - seg.tile_start[s];
     const int64_t sbeg = seg.begin[s];
     const int64_t send = sbeg + seg.count[s];
@@ -232,15 +233,20 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
     const int64_t end = min(send, begin + (int64_t)TILE);
     const int count = (int)(end - begin);
 
+    // records and their buckets (computed by the histogram pass)
     uint32_t r[ITEMS][RW];
+    uint32_t bk[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const int64_t i = begin + wave * WAVE_TILE + k * 64 + lane;
-        if (i < end) RecIO<RW>::load(src, i, r[k]);
+        bk[k] = kLBins;
+        if (i < end) {
+            RecIO<RW>::load(src, i, r[k]);
+            bk[k] = bkt[i];
+        }
     }
-    __syncthreads();  // table in LDS
     {
-        const uint64_t m = __ballot(eq_dst != dst && single_bucket(s_tab, tid));
+        const uint64_t m = __ballot(eq_dst != dst && single_bucket(tables + (int64_t)s * tstride, tid));
         if (lane == 0) {
             s_single[2 * wave] = (uint32_t)m;
             s_single[2 * wave + 1] = (uint32_t)(m >> 32);
@@ -254,7 +260,7 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
     for (int k = 0; k < ITEMS; k++) {
         const int64_t i = begin + wave * WAVE_TILE + k * 64 + lane;
         const bool valid = i < end;
-        const uint32_t d = valid ? bucket_of(lkey<RW>(r[k], lk), s_tab) : (uint32_t)kLBins;
+        const uint32_t d = valid ? bk[k] : (uint32_t)kLBins;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 10; b++) {
@@ -1084,14 +1090,14 @@ __global__ __launch_bounds__(kCBlock) void lsd_sort_kernel(
 // host launchers
 int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
                  const uint64_t* tables, int64_t tstride, uint32_t* hist, uint16_t* tile_cnt, long long* checksum,
-                 hipStream_t st) {
+                 uint16_t* bkt, hipStream_t st) {
     if (ntiles <= 0) return 0;
     const int64_t tpb = (ntiles + 2047) / 2048;
     const int64_t blocks = (ntiles + tpb - 1) / tpb;
     const uint32_t* s = (const uint32_t*)src;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(lhist_kernel<2>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, tile_cnt, checksum); break;
-        case 4: hipLaunchKernelGGL(lhist_kernel<4>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, tile_cnt, checksum); break;
+        case 2: hipLaunchKernelGGL(lhist_kernel<2>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, tile_cnt, checksum, bkt); break;
+        case 4: hipLaunchKernelGGL(lhist_kernel<4>, dim3((unsigned)blocks), dim3(kBlock), 0, st, s, sd, ntiles, tpb, tile_recs, lk, tables, tstride, hist, tile_cnt, checksum, bkt); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
@@ -1117,16 +1123,16 @@ int launch_ltile_scan(const uint16_t* tile_cnt, const SegDev& sd, const int64_t*
 }
 
 int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegDev& sd, int64_t ntiles,
-                    const LKey& lk, const uint64_t* tables, int64_t tstride, const uint32_t* tile_off, uint32_t* err,
-                    hipStream_t st) {
+                    const uint16_t* bkt, const uint64_t* tables, int64_t tstride, const uint32_t* tile_off,
+                    uint32_t* err, hipStream_t st) {
     if (ntiles <= 0) return 0;
     const unsigned grid = (unsigned)(8 * ((ntiles + 7) / 8));
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
     uint32_t* e = (uint32_t*)eq_dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, lk, tables, tstride, tile_off, err, e); break;
-        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, lk, tables, tstride, tile_off, err, e); break;
+        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, bkt, tables, tstride, tile_off, err, e); break;
+        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, bkt, tables, tstride, tile_off, err, e); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
