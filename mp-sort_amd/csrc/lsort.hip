@@ -95,16 +95,27 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
         const int64_t sbeg = seg.begin[s];
         const int64_t begin = sbeg + t * tile_recs;
         const int64_t end = min(sbeg + seg.count[s], begin + (int64_t)tile_recs);
-        for (int64_t i = begin + tid; i < end; i += kBlock) {
-            uint32_t r[RW];
-            RecIO<RW>::load(src, i, r);
-            if (checksum) {
+        // U records per thread at a time, all U loads in flight before the
+        // first is used
+        constexpr int U = 4;
+        for (int64_t i0 = begin + tid; i0 < end; i0 += U * kBlock) {
+            uint32_t r[U][RW];
 #pragma unroll
-                for (int w = 0; w < RW; w++) csum += sbyte_sum(r[w]);
+            for (int u = 0; u < U; u++)
+                if (i0 + (int64_t)u * kBlock < end) RecIO<RW>::load(src, i0 + (int64_t)u * kBlock, r[u]);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t i = i0 + (int64_t)u * kBlock;
+                if (i < end) {
+                    if (checksum) {
+#pragma unroll
+                        for (int w = 0; w < RW; w++) csum += sbyte_sum(r[u][w]);
+                    }
+                    const uint32_t b = bucket_of(lkey<RW>(r[u], lk), s_tab);
+                    bkt[i] = (uint16_t)b;  // for the scatter
+                    atomicAdd(&s_th[b], 1u);
+                }
             }
-            const uint32_t b = bucket_of(lkey<RW>(r, lk), s_tab);
-            bkt[i] = (uint16_t)b;  // for the scatter
-            atomicAdd(&s_th[b], 1u);
         }
         __syncthreads();
         // the tile's counts (for the scatter's offsets), added to the segment
