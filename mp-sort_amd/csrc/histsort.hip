@@ -338,22 +338,36 @@ __global__ __launch_bounds__(kBlock) void exchange_kernel(const uint32_t* __rest
         }
     }
     __syncthreads();
-    for (int64_t o = o0 + tid; o < o1; o += kBlock) {
-        int64_t sidx;
-        if (local) {
-            const int q = upper_index<int64_t>(s_rd, n, o);
-            sidx = s_rs[q] + (o - s_rd[q]);
-        } else {
-            const int q = upper_index<int64_t>(run_dst, nruns, o);
-            sidx = run_src[q] + (o - run_dst[q]);
+    // U records per thread at a time: sources found, then all U loads in
+    // flight before the first store
+    constexpr int U = 4;
+    for (int64_t oo = o0 + tid; oo < o1; oo += U * kBlock) {
+        int64_t sidx[U];
+        uint32_t r[U][RW];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t o = oo + (int64_t)u * kBlock;
+            sidx[u] = -1;
+            if (o < o1) {
+                if (local) {
+                    const int q = upper_index<int64_t>(s_rd, n, o);
+                    sidx[u] = s_rs[q] + (o - s_rd[q]);
+                } else {
+                    const int q = upper_index<int64_t>(run_dst, nruns, o);
+                    sidx[u] = run_src[q] + (o - run_dst[q]);
+                }
+                if (sidx[u] < 0 || sidx[u] >= ntot) {  // only reachable with an inconsistent layout
+                    atomicOr(error_flag, 32u);
+                    sidx[u] = -1;
+                }
+            }
         }
-        if (sidx < 0 || sidx >= ntot) {  // only reachable with an inconsistent layout
-            atomicOr(error_flag, 32u);
-            continue;
-        }
-        uint32_t r[RW];
-        RecIO<RW>::load(src, sidx, r);
-        RecIO<RW>::store(dst, o, r);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (sidx[u] >= 0) RecIO<RW>::load(src, sidx[u], r[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (sidx[u] >= 0) RecIO<RW>::store(dst, oo + (int64_t)u * kBlock, r[u]);
     }
 }
 
@@ -394,28 +408,45 @@ __global__ __launch_bounds__(kBlock) void exchange_hist_kernel(
         }
     }
     __syncthreads();
-    for (int64_t o = o0 + tid; o < o1; o += kBlock) {
-        int64_t sidx;
-        if (local) {
-            const int q = upper_index<int64_t>(s_rd, n, o);
-            sidx = s_rs[q] + (o - s_rd[q]);
-        } else {
-            const int q = upper_index<int64_t>(run_dst, nruns, o);
-            sidx = run_src[q] + (o - run_dst[q]);
+    // U records per thread at a time: sources found, then all U loads in
+    // flight before the first store
+    constexpr int U = 4;
+    for (int64_t oo = o0 + tid; oo < o1; oo += U * kBlock) {
+        int64_t sidx[U];
+        uint32_t r[U][RW];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t o = oo + (int64_t)u * kBlock;
+            sidx[u] = -1;
+            if (o < o1) {
+                if (local) {
+                    const int q = upper_index<int64_t>(s_rd, n, o);
+                    sidx[u] = s_rs[q] + (o - s_rd[q]);
+                } else {
+                    const int q = upper_index<int64_t>(run_dst, nruns, o);
+                    sidx[u] = run_src[q] + (o - run_dst[q]);
+                }
+                if (sidx[u] < 0 || sidx[u] >= ntot) {
+                    atomicOr(error_flag, 32u);
+                    sidx[u] = -1;
+                }
+            }
         }
-        if (sidx < 0 || sidx >= ntot) {
-            atomicOr(error_flag, 32u);
-            continue;
-        }
-        uint32_t r[RW];
-        RecIO<RW>::load(src, sidx, r);
-        RecIO<RW>::store(dst, o, r);
-        if (one_receiver) {
-            for (int p = 0; p < npass; p++) atomicAdd(&s_hist[p * kBins + digit_of<RW>(r, pt.p[p])], 1u);
-        } else {
-            const int rr = upper_index<int64_t>(C, P, o);
-            uint32_t* h = hist + (int64_t)rr * nb;
-            for (int p = 0; p < npass; p++) atomicAdd(&h[p * kBins + digit_of<RW>(r, pt.p[p])], 1u);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (sidx[u] >= 0) RecIO<RW>::load(src, sidx[u], r[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (sidx[u] < 0) continue;
+            const int64_t o = oo + (int64_t)u * kBlock;
+            RecIO<RW>::store(dst, o, r[u]);
+            if (one_receiver) {
+                for (int p = 0; p < npass; p++) atomicAdd(&s_hist[p * kBins + digit_of<RW>(r[u], pt.p[p])], 1u);
+            } else {
+                const int rr = upper_index<int64_t>(C, P, o);
+                uint32_t* h = hist + (int64_t)rr * nb;
+                for (int p = 0; p < npass; p++) atomicAdd(&h[p * kBins + digit_of<RW>(r[u], pt.p[p])], 1u);
+            }
         }
     }
     if (one_receiver) {

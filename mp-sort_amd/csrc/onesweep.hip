@@ -69,16 +69,26 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict
         const int64_t sbeg = seg.begin[s];
         const int64_t begin = sbeg + t * tile_recs;
         const int64_t end = min(sbeg + seg.count[s], begin + (int64_t)tile_recs);
-        for (int64_t i = begin + tid; i < end; i += kBlock) {
-            uint32_t r[RW];
-            RecIO<RW>::load(src, i, r);
-            if (checksum) {
+        // U records per thread at a time, all U loads in flight before the
+        // first is used
+        constexpr int U = 4;
+        for (int64_t i0 = begin + tid; i0 < end; i0 += U * kBlock) {
+            uint32_t r[U][RW];
 #pragma unroll
-                for (int w = 0; w < RW; w++) csum += sbyte_sum(r[w]);
-            }
-            for (int p = 0; p < npass; p++) {
-                const uint32_t d = digit_of<RW>(r, pt.p[p]);
-                atomicAdd(&s_hist[p * kBins + d], 1u);
+            for (int u = 0; u < U; u++)
+                if (i0 + (int64_t)u * kBlock < end) RecIO<RW>::load(src, i0 + (int64_t)u * kBlock, r[u]);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (i0 + (int64_t)u * kBlock < end) {
+                    if (checksum) {
+#pragma unroll
+                        for (int w = 0; w < RW; w++) csum += sbyte_sum(r[u][w]);
+                    }
+                    for (int p = 0; p < npass; p++) {
+                        const uint32_t d = digit_of<RW>(r[u], pt.p[p]);
+                        atomicAdd(&s_hist[p * kBins + d], 1u);
+                    }
+                }
             }
         }
     }
