@@ -186,20 +186,20 @@ __global__ __launch_bounds__(BLOCK) void onesweep_kernel(
     const int64_t end = min(send, begin + (int64_t)TILE);
     const int count = (int)(end - begin);
 
+    // every load issued before any is used (all ITEMS in flight at once);
+    // slots past the tile read its last record again and are not records
     uint32_t r[ITEMS][RW];
     uint32_t dg[ITEMS];
     uint32_t loc[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const int64_t i = begin + wave * WAVE_TILE + k * 64 + lane;
-        if (i < end) {
-            RecIO<RW>::load(src, i, r[k]);
-            dg[k] = digit_of<RW>(r[k], pd);
-        } else {
+        if (end > begin) RecIO<RW>::load(src, i < end ? i : end - 1, r[k]);
+    }
 #pragma unroll
-            for (int w = 0; w < RW; w++) r[k][w] = 0;
-            dg[k] = kBins;  // not a record
-        }
+    for (int k = 0; k < ITEMS; k++) {
+        const int64_t i = begin + wave * WAVE_TILE + k * 64 + lane;
+        dg[k] = i < end ? digit_of<RW>(r[k], pd) : (uint32_t)kBins;  // kBins: not a record
     }
 
     // rank inside the wave: items in (k, lane) order are in input order
