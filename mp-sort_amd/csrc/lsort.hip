@@ -352,16 +352,17 @@ __device__ __forceinline__ void load_bucket(const uint32_t* __restrict__ src, in
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     mn = ~0ull;
     mx = 0;
+    // every load issued before any is used (indices past the bucket read its
+    // last record again and are ignored)
+#pragma unroll
+    for (int k = 0; k < kCItems; k++) RecIO<RW>::load(src, base + min(wave * kCWspan + k * 64 + lane, count - 1), r[k]);
 #pragma unroll
     for (int k = 0; k < kCItems; k++) {
-        const int i = wave * kCWspan + k * 64 + lane;
-        kk[k] = 0;
-        if (i < count) {
-            RecIO<RW>::load(src, base + i, r[k]);
-            kk[k] = lkey<RW>(r[k], lk);
-            mn = kk[k] < mn ? kk[k] : mn;
-            mx = kk[k] > mx ? kk[k] : mx;
-        }
+        const bool valid = wave * kCWspan + k * 64 + lane < count;
+        const uint64_t key = lkey<RW>(r[k], lk);
+        kk[k] = valid ? key : 0;
+        mn = valid && key < mn ? key : mn;
+        mx = valid && key > mx ? key : mx;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -1058,6 +1059,7 @@ __global__ __launch_bounds__(kCBlock) void lsd_sort_kernel(
         const int64_t b = hard[q];
         const int64_t base = bucket_base[b];
         const int count = (int)hist[b];
+        if (count <= 0 || count > kChunkCap) continue;  // uniform over the block
         uint32_t r[kCItems][RW];
         uint64_t kk[kCItems];
         uint64_t mn, mx;
