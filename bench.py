@@ -32,9 +32,9 @@ PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_
 # HBM bytes per launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # each scaled by the counter's reading for a plain copy kernel of known size
 # (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
-TRAFFIC_ONESWEEP = 9069400576
-TRAFFIC_BUCKET_SORT = 10241614131
-TRAFFIC_MERGE_BUCKET = 9206444442
+TRAFFIC_ONESWEEP = 9117529088
+TRAFFIC_BUCKET_SORT = 9488592282
+TRAFFIC_MERGE_BUCKET = 9649530982
 # per-side stages of the learned local sort, as timed by the library
 LEARNED_KERNELS = ["hist_A", "scatter_A", "bucket_sort_A", "lds_radix_A", "fallback_A",
                    "hist_B", "scatter_B", "bucket_sort_B", "lds_radix_B", "fallback_B"]
