@@ -817,16 +817,26 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
                               L.err, st)))
         return rc;
     if ((rc = mark(2))) return rc;
-    if ((rc = launch_chunk_sort(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, L.tstride, (int64_t)P * kLBins, L.overflow,
-                                L.noverflow, kMaxOverflowList, L.hard, L.noverflow + 1, cs_out, eq_direct, st,
-                                ev ? ev[3] : nullptr)))
+    // buckets up to 4096 records: 512-thread workgroups (two per CU); larger
+    // ones (up to kChunkCap) by the 1024-thread form from a list, then the
+    // crowded ones by the LDS radix sort
+    if ((rc = launch_chunk_sort_small(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, L.tstride, (int64_t)P * kLBins,
+                                      L.overflow, L.noverflow, kMaxOverflowList, L.hard, L.noverflow + 1, L.medium,
+                                      L.noverflow + 2, cs_out, eq_direct, st)))
         return rc;
-    if ((rc = mark(4))) return rc;
-    uint32_t novh[2] = {0, 0}, errs = 0;
+    if ((rc = mark(3))) return rc;
+    uint32_t novh[3] = {0, 0, 0}, errs = 0;
     MPSG_CHECK(hipMemcpyAsync(novh, L.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(&errs, L.err, sizeof errs, hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipStreamSynchronize(st));
     if (errs) return fail(MPSORT_ERR_TIMEOUT, "learned bucket scatter: device flag 0x%x", errs);
+    if ((rc = launch_listed_sort(rw, tmp, dst, lk, L.medium, novh[2], L.lhist, L.bbase, L.tables, kLBins, L.tstride,
+                                 L.hard, L.noverflow + 1, cs_out, st)))
+        return rc;
+    if ((rc = launch_lsd_sort(rw, tmp, dst, lk, L.lhist, L.bbase, L.hard, L.noverflow + 1, (int64_t)P * kLBins, cs_out,
+                              st)))
+        return rc;
+    if ((rc = mark(4))) return rc;
     const uint32_t nov = novh[0];
     lc->hard_buckets += (int)novh[1];
     lc->overflow_buckets += (int)nov;
@@ -1045,8 +1055,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             uint32_t novh[3] = {0, 0, 0};
             MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
             MPSG_CHECK(hipStreamSynchronize(st));
-            if ((rc = launch_listed_sort(rw, final_dst, lk, LB.medium, novh[2], LB.lhist, LB.bbase, LB.tables,
-                                         kMergeBins, kMergeBounds, LB.hard, LB.noverflow + 1, st)))
+            if ((rc = launch_listed_sort(rw, final_dst, final_dst, lk, LB.medium, novh[2], LB.lhist, LB.bbase,
+                                         LB.tables, kMergeBins, kMergeBounds, LB.hard, LB.noverflow + 1, nullptr, st)))
                 return rc;
             if ((rc = launch_lsd_sort(rw, final_dst, final_dst, lk, LB.lhist, LB.bbase, LB.hard, LB.noverflow + 1,
                                       (int64_t)P * kMergeBins, nullptr, st)))

@@ -73,11 +73,19 @@ int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, con
                         const uint64_t* tables, int64_t* overflow, uint32_t* noverflow, int max_overflow,
                         uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum,
                         hipStream_t st);
-// in-place in-LDS sort of the nlist listed buckets (nbins buckets and nbounds
-// table boundaries per segment); crowded ones are appended to hard
-int launch_listed_sort(int rw, void* recs, const LKey& lk, const uint32_t* list, uint32_t nlist,
+// in-LDS sort (1024-thread form) of the nlist listed buckets, src -> dst at
+// the same slots (src may equal dst); nbins buckets per segment, segment s
+// uses tables + s * tstride; crowded ones are appended to hard
+int launch_listed_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* list, uint32_t nlist,
                        const uint32_t* hist, const int64_t* bucket_base, const uint64_t* tables, int nbins,
-                       int nbounds, uint32_t* hard, uint32_t* nhard, hipStream_t st);
+                       int64_t tstride, uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st);
+// launch_chunk_sort with 512-thread workgroups (two per CU): buckets above
+// 4096 records (up to kChunkCap) are listed in medium (*nmedium entries) for
+// launch_listed_sort; no LDS radix pass is launched
+int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
+                            const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
+                            int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
+                            uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st);
 // LDS radix sort of the listed buckets (src may equal dst)
 int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                     const int64_t* bucket_base, const uint32_t* hard, const uint32_t* nhard, int64_t max_buckets,

@@ -788,15 +788,19 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
 
 // One workgroup per bucket; the bucket's count, base and table bounds are
 // read together.
-template <int RW>
-__global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
+// BLOCK = 512: buckets above BSort<512>::CAP (up to kChunkCap) go to the
+// medium list for the 1024-thread form (medium must then be given).
+template <int RW, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void interp_sort_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
     const uint64_t* __restrict__ tables, int64_t tstride, int64_t nbuckets, int64_t* __restrict__ overflow,
     uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
-    long long* __restrict__ checksum_out, int eq_done) {
-    __shared__ long long s_sum[kCWaves];
-    __shared__ uint32_t s_bins[2 * BSort<kCBlock>::NB];
+    uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out,
+    int eq_done) {
+    using C = BSort<BLOCK>;
+    __shared__ long long s_sum[C::W];
+    __shared__ uint32_t s_bins[2 * C::NB];
     const int tid = threadIdx.x;
     // count, base and the table entries t[k-1], t[k] around bucket k
     struct Meta {
@@ -827,14 +831,16 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
         } else if (eq_done && single) {
             if (checksum_out) {
                 long long csum = 0;
-                for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += kCBlock) {
+                for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += BLOCK) {
                     uint32_t rr[RW];
                     RecIO<RW>::load(dst, i, rr);
 #pragma unroll
                     for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
                 }
-                block_checksum_add(csum, s_sum, checksum_out);
+                block_checksum_add<C::W>(csum, s_sum, checksum_out);
             }
+        } else if (cnt32 > (uint32_t)C::CAP && cnt32 <= (uint32_t)kChunkCap) {
+            if (tid == 0) medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread form
         } else if (cnt32 > (uint32_t)kChunkCap) {
             if (tid == 0) {
                 const uint32_t slot = atomicAdd(noverflow, 1u);
@@ -847,7 +853,7 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
             const RunSrc<RW> rs{src, base};
             // an interior bucket's keys lie in (t[k-1], t[k]]
             const bool bounded = tables && kb > 0 && kb < kLBins - 1 && cur.tlo < cur.thi;
-            if (!sort_bucket_lds<RW, kCBlock>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
+            if (!sort_bucket_lds<RW, BLOCK>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
                                      bounded ? cur.tlo + 1 : 0, bounded ? cur.thi : ~0ull))
                 if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
         }
@@ -861,21 +867,22 @@ __global__ __launch_bounds__(kCBlock) void interp_sort_kernel(
 // sort's list.
 template <int RW>
 __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
-    uint32_t* __restrict__ recs, LKey lk, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hist,
-    const int64_t* __restrict__ bucket_base, const uint64_t* __restrict__ tables, int nbins, int nbounds,
-    uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard) {
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base, const uint64_t* __restrict__ tables,
+    int nbins, int64_t tstride, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
+    long long* __restrict__ checksum_out) {
     __shared__ uint32_t s_bins[2 * BSort<kCBlock>::NB];
     const uint32_t b = list[blockIdx.x];
     const uint32_t cnt32 = hist[b];
     const int64_t base = bucket_base[b];
     const int k = (int)(b % (uint32_t)nbins);
-    const uint64_t* t = tables + (int64_t)(b / (uint32_t)nbins) * nbounds;
-    const uint64_t tlo = k > 0 ? t[k - 1] : 0, thi = k < nbounds ? t[k] : ~0ull;
+    const uint64_t* t = tables + (int64_t)(b / (uint32_t)nbins) * tstride;
+    const uint64_t tlo = k > 0 ? t[k - 1] : 0, thi = k < nbins - 1 ? t[k] : ~0ull;
     if (cnt32 == 0 || cnt32 > (uint32_t)kChunkCap) return;
-    // every record is in registers before the first store: in place is safe
-    const RunSrc<RW> rs{recs, base};
+    // every record is in registers before the first store: src == dst is safe
+    const RunSrc<RW> rs{src, base};
     const bool bounded = k > 0 && k < nbins - 1 && tlo < thi;
-    if (!sort_bucket_lds<RW, kCBlock>(rs, recs, base, (int)cnt32, lk, s_bins, nullptr, bounded,
+    if (!sort_bucket_lds<RW, kCBlock>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
                                       bounded ? tlo + 1 : 0, bounded ? thi : ~0ull))
         if (threadIdx.x == 0) hard[atomicAdd(nhard, 1u)] = b;
 }
@@ -1167,20 +1174,48 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
     const unsigned grid = (unsigned)nbuckets;
     switch (rw) {
         case 2:
-            hipLaunchKernelGGL(interp_sort_kernel<2>, dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
-                               tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
+            hipLaunchKernelGGL((interp_sort_kernel<2, kCBlock>), dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
+                               tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, nullptr, nullptr,
+                               checksum, ed);
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                (const uint32_t*)hard, (const uint32_t*)nhard, checksum);
             break;
         case 4:
-            hipLaunchKernelGGL(interp_sort_kernel<4>, dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
-                               tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, checksum, ed);
+            hipLaunchKernelGGL((interp_sort_kernel<4, kCBlock>), dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
+                               tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, nullptr, nullptr,
+                               checksum, ed);
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                (const uint32_t*)hard, (const uint32_t*)nhard, checksum);
+            break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
+                            const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
+                            int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
+                            uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st) {
+    if (nbuckets <= 0) return 0;
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    const int ed = eq_done ? 1 : 0;
+    const unsigned grid = (unsigned)nbuckets;
+    switch (rw) {
+        case 2:
+            hipLaunchKernelGGL((interp_sort_kernel<2, kMergeBlock>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
+                               bucket_base, tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard,
+                               medium, nmedium, checksum, ed);
+            break;
+        case 4:
+            hipLaunchKernelGGL((interp_sort_kernel<4, kMergeBlock>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
+                               bucket_base, tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard,
+                               medium, nmedium, checksum, ed);
             break;
         default: return -1;
     }
@@ -1227,14 +1262,15 @@ int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, con
     return 0;
 }
 
-int launch_listed_sort(int rw, void* recs, const LKey& lk, const uint32_t* list, uint32_t nlist,
+int launch_listed_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* list, uint32_t nlist,
                        const uint32_t* hist, const int64_t* bucket_base, const uint64_t* tables, int nbins,
-                       int nbounds, uint32_t* hard, uint32_t* nhard, hipStream_t st) {
+                       int64_t tstride, uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st) {
     if (nlist == 0) return 0;
-    uint32_t* r = (uint32_t*)recs;
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(listed_sort_kernel<2>, dim3(nlist), dim3(kCBlock), 0, st, r, lk, list, hist, bucket_base, tables, nbins, nbounds, hard, nhard); break;
-        case 4: hipLaunchKernelGGL(listed_sort_kernel<4>, dim3(nlist), dim3(kCBlock), 0, st, r, lk, list, hist, bucket_base, tables, nbins, nbounds, hard, nhard); break;
+        case 2: hipLaunchKernelGGL(listed_sort_kernel<2>, dim3(nlist), dim3(kCBlock), 0, st, s, d, lk, list, hist, bucket_base, tables, nbins, tstride, hard, nhard, checksum); break;
+        case 4: hipLaunchKernelGGL(listed_sort_kernel<4>, dim3(nlist), dim3(kCBlock), 0, st, s, d, lk, list, hist, bucket_base, tables, nbins, tstride, hard, nhard, checksum); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
