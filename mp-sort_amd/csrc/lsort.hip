@@ -485,14 +485,17 @@ constexpr int kInterpMax = 32;
 // Workgroup shapes of the in-LDS sort: 1024 threads sort up to 8192 records
 // in 4095 bins (65 x 63), 512 threads up to 4096 records in 2015 bins
 // (65 x 31) with half the LDS, so that two workgroups share a CU.
-template <int BLOCK>
+// ITEMS records per thread (CAP = BLOCK * ITEMS).  The bin arrays take BW
+// words of LDS: NB u16 bin starts, then NB u32 bin counters.
+template <int BLOCK, int ITEMS = kCItems>
 struct BSort {
-    static constexpr int B = BLOCK, W = BLOCK / 64, CAP = BLOCK * kCItems;
-    static constexpr int SUB = BLOCK == 1024 ? 63 : 31;  // bins per sample interval
-    static constexpr int FINE = (kSamp + 1) * SUB;       // bins
-    static constexpr int NB = CAP / 2;                   // bin counters (FINE + 1 rounded up)
-    static constexpr int TPS = BLOCK / kSamp;            // threads per sample in the sample sort
-    static_assert(NB >= FINE + 1 && NB % BLOCK == 0 && kSamp % TPS == 0, "in-LDS sort shape");
+    static constexpr int B = BLOCK, W = BLOCK / 64, I = ITEMS, CAP = BLOCK * ITEMS;
+    static constexpr int SUB = BLOCK == 1024 ? 63 : 31;          // bins per sample interval
+    static constexpr int FINE = (kSamp + 1) * SUB;               // bins
+    static constexpr int NB = (FINE + 1 + BLOCK - 1) / BLOCK * BLOCK;  // bin counters
+    static constexpr int BW = NB / 2 + NB;                       // words of the bin arrays
+    static constexpr int TPS = BLOCK / kSamp;                    // threads per sample in the sample sort
+    static_assert(NB % 2 == 0 && kSamp % TPS == 0 && CAP < 65536, "in-LDS sort shape");
 };
 
 // the records of one bucket, contiguous in src from base
@@ -528,15 +531,15 @@ struct IndexSrc {
 
 // Sorts one bucket of count (1 .. BSort<BLOCK>::CAP) records, given by src,
 // into dst[base .. base + count) with a BLOCK-thread workgroup.  Returns false (and writes nothing) when a bin
-// is crowded beyond kRunMax.  s_bins: 2 * BSort<BLOCK>::NB words of LDS owned by
+// is crowded beyond kRunMax.  s_bins: BSort<BLOCK, ITEMS>::BW words of LDS owned by
 // the caller, who may keep data there that src reads while loading.  When
 // bounded, every key lies in [lo, hi] (the bucket's boundaries) and the
 // bucket's minimum and maximum are not computed.
-template <int RW, int BLOCK, class Src>
+template <int RW, int BLOCK, class Src, int ITEMS = kCItems>
 __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
                                                 const LKey& lk, uint32_t* s_bins, long long* __restrict__ checksum_out,
                                                 bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull) {
-    using C = BSort<BLOCK>;
+    using C = BSort<BLOCK, ITEMS>;
     __shared__ uint64_t s_gkey[C::CAP + 4];  // key of each bin-order slot (+4: unrolled reads)
     __shared__ uint16_t s_sbin[C::CAP];      // bin of each bin-order slot
     __shared__ uint16_t s_ord[C::CAP];       // bucket position of each bin-order slot
@@ -552,8 +555,8 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     __shared__ uint32_t s_max[C::W];
     __shared__ long long s_sum[C::W];
     __shared__ int s_long;
-    uint32_t* s_start = s_bins;
-    uint32_t* s_fill = s_bins + C::NB;
+    uint16_t* s_start = reinterpret_cast<uint16_t*>(s_bins);  // NB u16
+    uint32_t* s_fill = s_bins + C::NB / 2;                    // NB u32
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
@@ -563,13 +566,13 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     // q < 64 also reads the sample key of record floor(q * count / 64).
     uint32_t rs[RW];
     src.load(((tid & (kSamp - 1)) * count) >> 6, rs);
-    uint32_t r[kCItems][RW];
+    uint32_t r[C::I][RW];
 #pragma unroll
-    for (int k = 0; k < kCItems; k++) src.load(min(k * C::B + tid, count - 1), r[k]);
-    uint64_t kk[kCItems];
+    for (int k = 0; k < C::I; k++) src.load(min(k * C::B + tid, count - 1), r[k]);
+    uint64_t kk[C::I];
     uint64_t mn = ~0ull, mx = 0;
 #pragma unroll
-    for (int k = 0; k < kCItems; k++) {
+    for (int k = 0; k < C::I; k++) {
         const bool valid = k * C::B + tid < count;
         const uint64_t key = lkey<RW>(r[k], lk);
         kk[k] = valid ? key : 0;
@@ -606,7 +609,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     if (mn == mx) {
         // one key value: the stable order is the current order
 #pragma unroll
-        for (int k = 0; k < kCItems; k++) {
+        for (int k = 0; k < C::I; k++) {
             const int i = k * C::B + tid;
             if (i < count) {
                 RecIO<RW>::store(dst, base + i, r[k]);
@@ -679,10 +682,10 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         sub = sub > (uint32_t)(C::SUB - 1) ? (uint32_t)(C::SUB - 1) : sub;
         return (uint32_t)j * C::SUB + sub;
     };
-    uint32_t dbin[kCItems];
+    uint32_t dbin[C::I];
     for (;;) {
 #pragma unroll
-        for (int k = 0; k < kCItems; k++) {
+        for (int k = 0; k < C::I; k++) {
             const int i = k * C::B + tid;
             dbin[k] = 0;
             if (i < count) {
@@ -721,7 +724,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     }
     // group by bin: key, bin and bucket position of every bin-order slot
 #pragma unroll
-    for (int k = 0; k < kCItems; k++) {
+    for (int k = 0; k < C::I; k++) {
         const int i = k * C::B + tid;
         if (i < count) {
             const uint32_t o = s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u);
@@ -772,7 +775,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     __syncthreads();
     if (s_long) return false;
 #pragma unroll
-    for (int k = 0; k < kCItems; k++) {
+    for (int k = 0; k < C::I; k++) {
         const int i = k * C::B + tid;
         if (i < count) {
             RecIO<RW>::store(dst, base + s_pos[i], r[k]);
@@ -790,17 +793,17 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
 // read together.
 // BLOCK = 512: buckets above BSort<512>::CAP (up to kChunkCap) go to the
 // medium list for the 1024-thread form (medium must then be given).
-template <int RW, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void interp_sort_kernel(
+template <int RW, int BLOCK, int ITEMS = kCItems>
+__global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
     const uint64_t* __restrict__ tables, int64_t tstride, int64_t nbuckets, int64_t* __restrict__ overflow,
     uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
     uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out,
     int eq_done) {
-    using C = BSort<BLOCK>;
+    using C = BSort<BLOCK, ITEMS>;
     __shared__ long long s_sum[C::W];
-    __shared__ uint32_t s_bins[2 * C::NB];
+    __shared__ uint32_t s_bins[C::BW];
     const int tid = threadIdx.x;
     // count, base and the table entries t[k-1], t[k] around bucket k
     struct Meta {
@@ -853,7 +856,7 @@ __global__ __launch_bounds__(BLOCK) void interp_sort_kernel(
             const RunSrc<RW> rs{src, base};
             // an interior bucket's keys lie in (t[k-1], t[k]]
             const bool bounded = tables && kb > 0 && kb < kLBins - 1 && cur.tlo < cur.thi;
-            if (!sort_bucket_lds<RW, BLOCK>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
+            if (!sort_bucket_lds<RW, BLOCK, RunSrc<RW>, ITEMS>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
                                      bounded ? cur.tlo + 1 : 0, bounded ? cur.thi : ~0ull))
                 if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
         }
@@ -871,7 +874,7 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base, const uint64_t* __restrict__ tables,
     int nbins, int64_t tstride, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
     long long* __restrict__ checksum_out) {
-    __shared__ uint32_t s_bins[2 * BSort<kCBlock>::NB];
+    __shared__ uint32_t s_bins[BSort<kCBlock>::BW];
     const uint32_t b = list[blockIdx.x];
     const uint32_t cnt32 = hist[b];
     const int64_t base = bucket_base[b];
@@ -907,6 +910,7 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
 //                        radix path
 constexpr int kMergeMaxP = 64;
 constexpr int kMergeBlock = 512;
+constexpr int kSmallItems = 9;  // FirstSort buckets up to 4608 records in 512-thread workgroups
 static_assert(BSort<kMergeBlock>::CAP == kMergeCap, "merged SecondSort capacity");
 
 template <int RW>
@@ -966,7 +970,7 @@ __global__ __launch_bounds__(kMergeBlock) void merge_bucket_kernel(
     __shared__ int64_t s_beg[kMergeMaxP];
     __shared__ int s_pref[kMergeMaxP + 1];
     __shared__ long long s_sum2[C::W];
-    __shared__ uint32_t s_bins[2 * C::NB];  // the gather index first, then the bins
+    __shared__ uint32_t s_bins[C::CAP > C::BW ? C::CAP : C::BW];  // the gather index first, then the bins
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t b = blockIdx.x;
     const uint32_t cnt32 = hist[b];
@@ -1208,12 +1212,12 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
     const unsigned grid = (unsigned)nbuckets;
     switch (rw) {
         case 2:
-            hipLaunchKernelGGL((interp_sort_kernel<2, kMergeBlock>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
+            hipLaunchKernelGGL((interp_sort_kernel<2, kMergeBlock, kSmallItems>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
                                bucket_base, tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard,
                                medium, nmedium, checksum, ed);
             break;
         case 4:
-            hipLaunchKernelGGL((interp_sort_kernel<4, kMergeBlock>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
+            hipLaunchKernelGGL((interp_sort_kernel<4, kMergeBlock, kSmallItems>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
                                bucket_base, tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard,
                                medium, nmedium, checksum, ed);
             break;
