@@ -646,9 +646,11 @@ struct LearnedSide {
     uint32_t* tickets;                // 4
     uint32_t* err;                    // 4
     int64_t* overflow;                // 2 * kMaxOverflowList
-    uint32_t* noverflow;              // 4: [0] oversized buckets, [1] buckets for the LDS radix sort
+    uint32_t* noverflow;              // 4: [0] oversized buckets, [1] buckets for the LDS radix sort,
+                                      //    [2] medium, [3] second
     uint32_t* hard;                   // nseg * buckets per segment
     uint32_t* medium;                 // nseg * buckets per segment (merged SecondSort)
+    uint32_t* second;                 // nseg * buckets per segment / 2 (merged SecondSort)
     int64_t tstride = kLBounds;       // segment s uses tables + s * tstride (0: one shared table)
 };
 
@@ -670,6 +672,7 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
         ls->noverflow = a.template take<uint32_t>(4);
         ls->hard = a.template take<uint32_t>((size_t)P * kMergeBins);
         ls->medium = a.template take<uint32_t>((size_t)P * kMergeBins);
+        ls->second = a.template take<uint32_t>((size_t)P * kMergeBins / 2);
     } else {
         a.template take<int64_t>(P);
         a.template take<int64_t>(P);
@@ -685,6 +688,7 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
         a.template take<uint32_t>(4);
         a.template take<uint32_t>((size_t)P * kMergeBins);
         a.template take<uint32_t>((size_t)P * kMergeBins);
+        a.template take<uint32_t>((size_t)P * kMergeBins / 2);
     }
 }
 
@@ -1044,17 +1048,23 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = launch_fill_u32(LB.noverflow, 0u, 4, st))) return rc;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
             if ((rc = markB(1)) || (rc = markB(2))) return rc;
-            if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase, LB.tables,
-                                          LB.overflow, LB.noverflow, kMaxOverflowList, LB.hard, LB.noverflow + 1,
-                                          LB.medium, LB.noverflow + 2, cs_direct ? h.checksum + 1 : nullptr, st)))
-                return rc;
+            // bucket pairs, then the second buckets of the pairs that did
+            // not fit together
+            uint32_t novh[4] = {0, 0, 0, 0};
+            for (int round = 0; round < 2; round++) {
+                if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase,
+                                              LB.tables, round ? LB.second : nullptr, novh[3], LB.second,
+                                              LB.noverflow + 3, LB.overflow, LB.noverflow, kMaxOverflowList, LB.hard,
+                                              LB.noverflow + 1, LB.medium, LB.noverflow + 2,
+                                              cs_direct ? h.checksum + 1 : nullptr, st)))
+                    return rc;
+                MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
+                MPSG_CHECK(hipStreamSynchronize(st));
+            }
             if ((rc = markB(3))) return rc;
             // buckets above the workgroup's capacity and crowded ones were
             // copied in place in sender order: the 1024-thread in-LDS sort,
             // then the LDS radix sort
-            uint32_t novh[3] = {0, 0, 0};
-            MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
-            MPSG_CHECK(hipStreamSynchronize(st));
             if ((rc = launch_listed_sort(rw, final_dst, final_dst, lk, LB.medium, novh[2], LB.lhist, LB.bbase,
                                          LB.tables, kMergeBins, kMergeBounds, LB.hard, LB.noverflow + 1, nullptr, st)))
                 return rc;

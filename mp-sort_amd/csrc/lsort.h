@@ -10,7 +10,7 @@ constexpr int kChunkCap = 8192; // largest bucket sorted in LDS
 // merged SecondSort: buckets per receiver, table boundaries, in-LDS capacity
 constexpr int kMergeBins = 2048;
 constexpr int kMergeBounds = 2047;
-constexpr int kMergeCap = 4096;
+constexpr int kMergeCap = 4608;
 
 // key of a sort record as one uint64 (keys of at most 8 bytes): bytes
 // [sh/8, sh/8 + key_bytes) of words w0..w0+2, masked, XOR-ed with the sign
@@ -66,13 +66,17 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* myC, int P,
                         const uint64_t* tables, int32_t* bnd, hipStream_t st);
 int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st);
-// buckets above kMergeCap (up to kChunkCap) are copied in sender order and
-// listed in medium (*nmedium entries) for launch_listed_sort
+// tasks == nullptr: one workgroup per pair of buckets (2m, 2m + 1) of a
+// receiver; the second buckets of pairs that are not sorted together are
+// listed in second (*nsecond entries, up to P * kMergeBins / 2), for a
+// launch with tasks = that list and ntasks = its length.  Buckets above
+// kMergeCap (up to kChunkCap) are copied in sender order and listed in medium
+// (*nmedium entries) for launch_listed_sort.
 int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
-                        const uint64_t* tables, int64_t* overflow, uint32_t* noverflow, int max_overflow,
-                        uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum,
-                        hipStream_t st);
+                        const uint64_t* tables, const uint32_t* tasks, uint32_t ntasks, uint32_t* second,
+                        uint32_t* nsecond, int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard,
+                        uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum, hipStream_t st);
 // in-LDS sort (1024-thread form) of the nlist listed buckets, src -> dst at
 // the same slots (src may equal dst); nbins buckets per segment, segment s
 // uses tables + s * tstride; crowded ones are appended to hard

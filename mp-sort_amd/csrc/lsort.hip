@@ -513,7 +513,7 @@ template <int RW>
 struct GatherSrc {
     const uint32_t* recs;
     const int64_t* begin;
-    const int* pref;
+    const int* pref;  // 64 entries: pref[nrun .. 63] lie above every position
     int nrun;
     __device__ __forceinline__ void load(int i, uint32_t (&r)[RW]) const {
         const int j = upper_index<int>(pref, nrun, i);
@@ -521,13 +521,31 @@ struct GatherSrc {
     }
 };
 
-// the same records through a table of their indices (LDS)
-template <int RW>
-struct IndexSrc {
-    const uint32_t* recs;
-    const uint32_t* index;
-    __device__ __forceinline__ void load(int i, uint32_t (&r)[RW]) const { RecIO<RW>::load(recs, index[i], r); }
-};
+// records idx[0..N) of src into r[0..N)
+template <int RW, int N, class Src>
+__device__ __forceinline__ void load_items(const Src& src, const int (&idx)[N], uint32_t (&r)[N][RW]) {
+#pragma unroll
+    for (int k = 0; k < N; k++) src.load(idx[k], r[k]);
+}
+
+// the gathered form: the N sub-run searches advance level by level, so each
+// level's N table reads are in flight together
+template <int RW, int N>
+__device__ __forceinline__ void load_items(const GatherSrc<RW>& src, const int (&idx)[N], uint32_t (&r)[N][RW]) {
+    int j[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) j[k] = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        int v[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = src.pref[j[k] + step];
+#pragma unroll
+        for (int k = 0; k < N; k++) j[k] = v[k] <= idx[k] ? j[k] + step : j[k];
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) RecIO<RW>::load(src.recs, src.begin[j[k]] + (idx[k] - src.pref[j[k]]), r[k]);
+}
 
 // Sorts one bucket of count (1 .. BSort<BLOCK>::CAP) records, given by src,
 // into dst[base .. base + count) with a BLOCK-thread workgroup.  Returns false (and writes nothing) when a bin
@@ -564,11 +582,15 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     // flight at once: item k of thread t is record k * BLOCK + t (indices
     // past the bucket read its last record again and are ignored); thread
     // q < 64 also reads the sample key of record floor(q * count / 64).
-    uint32_t rs[RW];
-    src.load(((tid & (kSamp - 1)) * count) >> 6, rs);
-    uint32_t r[C::I][RW];
+    // r[C::I]: the sample record
+    uint32_t r[C::I + 1][RW];
+    {
+        int idx[C::I + 1];
 #pragma unroll
-    for (int k = 0; k < C::I; k++) src.load(min(k * C::B + tid, count - 1), r[k]);
+        for (int k = 0; k < C::I; k++) idx[k] = min(k * C::B + tid, count - 1);
+        idx[C::I] = ((tid & (kSamp - 1)) * count) >> 6;
+        load_items<RW>(src, idx, r);
+    }
     uint64_t kk[C::I];
     uint64_t mn = ~0ull, mx = 0;
 #pragma unroll
@@ -579,7 +601,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         mn = valid && key < mn ? key : mn;
         mx = valid && key > mx ? key : mx;
     }
-    if (tid < kSamp) s_samp[tid] = lkey<RW>(rs, lk);
+    if (tid < kSamp) s_samp[tid] = lkey<RW>(r[C::I], lk);
     if (tid == 0) {
         s_crowd = 0;
         s_long = 0;
@@ -894,14 +916,17 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
 // SecondSort without the exchange.  Receiver r's records from sender j are
 // positions [myC[j][r], myC[j][r+1]) of sender j's sorted segment; bucket k of
 // r holds the keys in (t_r[k-1], t_r[k]], i.e. one sub-run per sender.  Every
-// receiver has kMergeBins buckets (tables of kMergeBounds boundaries), so a
-// bucket averages half the records of a FirstSort bucket and is sorted by a
-// kMergeBlock-thread workgroup with half the LDS: two share a CU.
+// receiver has kMergeBins buckets (tables of kMergeBounds boundaries), twice
+// as many as a FirstSort segment, because the model's boundaries at this
+// resolution leave bucket sizes spread widely.  A kMergeBlock-thread
+// workgroup (kSmallItems records per thread, two share a CU) takes buckets
+// 2m and 2m + 1 of a receiver and sorts them in one pass when together they
+// fit (their sub-runs are adjacent in every sender), else one by one.
 //   merge_bounds_kernel  bnd[r][k][j] = start of bucket k's sub-run in sender
 //                        j (relative to j's segment), k = 0..kMergeBins (the
 //                        P senders of one bucket side by side)
 //   merge_count_kernel   bucket sizes (sum over senders)
-//   merge_bucket_kernel  one workgroup per (receiver, bucket): gathers the
+//   merge_bucket_kernel  one workgroup per (receiver, bucket pair): gathers the
 //                        sub-runs in sender order (the order the exchange
 //                        would have left them in) and sorts them in LDS;
 //                        single-value buckets are copied; buckets above the
@@ -911,7 +936,7 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
 constexpr int kMergeMaxP = 64;
 constexpr int kMergeBlock = 512;
 constexpr int kSmallItems = 9;  // FirstSort buckets up to 4608 records in 512-thread workgroups
-static_assert(BSort<kMergeBlock>::CAP == kMergeCap, "merged SecondSort capacity");
+static_assert(BSort<kMergeBlock, kSmallItems>::CAP == kMergeCap, "merged SecondSort capacity");
 
 template <int RW>
 __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __restrict__ recs, LKey lk,
@@ -958,67 +983,94 @@ __global__ __launch_bounds__(256) void merge_count_kernel(const int32_t* __restr
     hist[g] = c;
 }
 
+// sub-runs of receiver r's buckets [ka, kb) in sender order: s_beg[j] =
+// first record (absolute), s_pref[j] = position of sender j's records in the
+// bucket, s_pref[P] = the bucket's size (wave 0 writes, the caller syncs)
+__device__ __forceinline__ void merge_subruns(const int32_t* __restrict__ bnd, const int64_t* __restrict__ seg_begin,
+                                              int P, int r, int ka, int kb, int64_t* s_beg, int* s_pref) {
+    const int lane = threadIdx.x & 63;
+    int len = 0;
+    int64_t beg = 0;
+    if (lane < P) {
+        const int32_t* ba = bnd + ((int64_t)r * (kMergeBins + 1) + ka) * P;
+        const int32_t* bb = bnd + ((int64_t)r * (kMergeBins + 1) + kb) * P;
+        beg = seg_begin[lane] + ba[lane];
+        len = bb[lane] - ba[lane];
+    }
+    int x = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane < P) {
+        s_beg[lane] = beg;
+        s_pref[lane] = x - len;
+    }
+    if (lane == P - 1) s_pref[P] = x;
+    if (lane > P) s_pref[lane] = 0x7fffffff;  // above every position
+}
+
+// tasks == nullptr: workgroup m of receiver r takes buckets 2m and 2m + 1,
+// both in one pass when they fit together, else bucket 2m alone, listing
+// bucket 2m + 1 in second (*nsecond entries) for the launch over that list
+// (tasks = second).  Either way a workgroup sorts at most once.
 template <int RW>
-__global__ __launch_bounds__(kMergeBlock) void merge_bucket_kernel(
+__global__ __launch_bounds__(kMergeBlock, 2048 / kMergeBlock) void merge_bucket_kernel(
     const uint32_t* __restrict__ recs, uint32_t* __restrict__ dst, LKey lk,
     const int64_t* __restrict__ seg_begin, const int32_t* __restrict__ bnd, int P,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
-    const uint64_t* __restrict__ tables, int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow,
+    const uint64_t* __restrict__ tables, const uint32_t* __restrict__ tasks, uint32_t* __restrict__ second,
+    uint32_t* __restrict__ nsecond, int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow,
     int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard, uint32_t* __restrict__ medium,
     uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out) {
-    using C = BSort<kMergeBlock>;
+    using C = BSort<kMergeBlock, kSmallItems>;
     __shared__ int64_t s_beg[kMergeMaxP];
     __shared__ int s_pref[kMergeMaxP + 1];
     __shared__ long long s_sum2[C::W];
-    __shared__ uint32_t s_bins[C::CAP > C::BW ? C::CAP : C::BW];  // the gather index first, then the bins
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int64_t b = blockIdx.x;
-    const uint32_t cnt32 = hist[b];
-    const int r = (int)(b / kMergeBins), k = (int)(b % kMergeBins);
+    __shared__ uint32_t s_bins[C::BW];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    int64_t b;  // the first bucket of this workgroup
+    uint32_t c0, c1 = 0;
+    if (tasks) {
+        b = tasks[blockIdx.x];
+        c0 = hist[b];
+    } else {
+        b = 2 * (int64_t)blockIdx.x;
+        c0 = hist[b];
+        c1 = hist[b + 1];
+    }
+    const int r = (int)(b / kMergeBins), ka = (int)(b % kMergeBins);
+    const bool joint = c0 > 0 && c1 > 0 && c0 + c1 <= (uint32_t)C::CAP;
+    if (!tasks && !joint && c1 > 0 && tid == 0) second[atomicAdd(nsecond, 1u)] = (uint32_t)(b + 1);
+    if (c0 == 0 && !joint) return;
+    int kb = joint ? ka + 2 : ka + 1;  // buckets [ka, kb)
+    int count = joint ? (int)(c0 + c1) : (int)c0;
     const int64_t base = bucket_base[b];
     const uint64_t* t = tables + (int64_t)r * kMergeBounds;
-    const uint64_t tlo = k > 0 ? t[k - 1] : 0, thi = k < kMergeBounds ? t[k] : ~0ull;
-    if (cnt32 == 0) return;
-    if (wave == 0) {
-        // the sub-runs of this bucket, sender by sender
-        int len = 0;
-        int64_t beg = 0;
-        if (lane < P) {
-            const int32_t* bk = bnd + ((int64_t)r * (kMergeBins + 1) + k) * P;
-            beg = seg_begin[lane] + bk[lane];
-            len = bk[P + lane] - bk[lane];
-        }
-        int x = len;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
-        }
-        if (lane < P) {
-            s_beg[lane] = beg;
-            s_pref[lane] = x - len;
-        }
-        if (lane == P - 1) s_pref[P] = x;
-    }
+    const uint64_t tlo = ka > 0 ? t[ka - 1] : 0;
+    const uint64_t thi = kb - 1 < kMergeBounds ? t[kb - 1] : ~0ull;
+    if (wave == 0) merge_subruns(bnd, seg_begin, P, r, ka, kb, s_beg, s_pref);
     __syncthreads();
     const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
-    const int count = (int)cnt32;
-    // bucket k holds one key value (see single_bucket)
-    const bool single = k == 0 ? thi == 0 : (k < kMergeBounds && tlo + 1 == thi);
-    if (!single && count <= C::CAP) {
-        // index of every record of the bucket, sub-run by sub-run (one wave
-        // per sub-run at a time)
-        for (int j = wave; j < P; j += C::W) {
-            const int a = s_pref[j], len = s_pref[j + 1] - a;
-            for (int l = lane; l < len; l += 64) s_bins[a + l] = (uint32_t)(s_beg[j] + l);
-        }
+    // bucket ka alone holds one key value (see single_bucket)
+    const bool single = !joint && (ka == 0 ? thi == 0 : (ka < kMergeBounds && tlo + 1 == thi));
+    // the keys of buckets [ka, kb) lie in (t[ka-1], t[kb-1]]
+    const bool bounded = ka > 0 && kb < kMergeBins && tlo < thi;
+    if (!single && count <= C::CAP &&
+        sort_bucket_lds<RW, kMergeBlock, GatherSrc<RW>, kSmallItems>(gs, dst, base, count, lk, s_bins, checksum_out,
+                                                                     bounded, bounded ? tlo + 1 : 0,
+                                                                     bounded ? thi : ~0ull))
+        return;
+    if (joint) {
+        // a crowded bin: bucket ka is copied for the LDS radix sort here,
+        // bucket ka + 1 goes to the launch over the list
+        if (tid == 0) second[atomicAdd(nsecond, 1u)] = (uint32_t)(b + 1);
+        kb = ka + 1;
+        count = (int)c0;
         __syncthreads();
-        const IndexSrc<RW> is{recs, s_bins};
-        // an interior bucket's keys lie in (t[k-1], t[k]]
-        const bool bounded = k > 0 && k < kMergeBins - 1 && tlo < thi;
-        if (sort_bucket_lds<RW, kMergeBlock>(is, dst, base, count, lk, s_bins, checksum_out, bounded,
-                                             bounded ? tlo + 1 : 0, bounded ? thi : ~0ull))
-            return;
+        if (wave == 0) merge_subruns(bnd, seg_begin, P, r, ka, kb, s_beg, s_pref);
+        __syncthreads();
     }
     // copy in sender order: final for a single-value bucket, otherwise the
     // input of the radix path
@@ -1041,7 +1093,7 @@ __global__ __launch_bounds__(kMergeBlock) void merge_bucket_kernel(
             const uint32_t slot = atomicAdd(noverflow, 1u);
             if ((int)slot < max_overflow) {
                 overflow[2 * slot] = base;
-                overflow[2 * slot + 1] = (int64_t)cnt32;
+                overflow[2 * slot + 1] = (int64_t)count;
             }
         }
     }
@@ -1077,7 +1129,9 @@ __global__ __launch_bounds__(kCBlock) void lsd_sort_kernel(
         __syncthreads();  // LDS of the previous bucket is free
         load_bucket<RW>(src, base, count, lk, r, kk, s_red, mn, mx);
         const uint64_t span = mx - mn;
-        const int nbytes = (64 - __clzll((long long)span) + 7) >> 3;
+        // at least one pass: it writes the order the rank step reads, also
+        // when every key is equal
+        const int nbytes = max(1, (64 - __clzll((long long)span) + 7) >> 3);
 #pragma unroll
         for (int k = 0; k < kCItems; k++) {
             const int i = wave * kCWspan + k * 64 + lane;
@@ -1250,16 +1304,17 @@ int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st
 
 int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
-                        const uint64_t* tables, int64_t* overflow, uint32_t* noverflow, int max_overflow,
-                        uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum,
-                        hipStream_t st) {
+                        const uint64_t* tables, const uint32_t* tasks, uint32_t ntasks, uint32_t* second,
+                        uint32_t* nsecond, int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard,
+                        uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum, hipStream_t st) {
     if (P > kMergeMaxP) return -1;
-    const unsigned g = (unsigned)((int64_t)P * kMergeBins);
+    const unsigned g = tasks ? ntasks : (unsigned)((int64_t)P * kMergeBins / 2);  // listed buckets or bucket pairs
+    if (g == 0) return 0;
     const uint32_t* r = (const uint32_t*)recs;
     uint32_t* d = (uint32_t*)dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(g), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
-        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(g), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(g), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, tasks, second, nsecond, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(g), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, tasks, second, nsecond, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());

@@ -70,6 +70,32 @@ def test_learned_equals_plain_and_counts(dist, mode):
     model.close()
 
 
+@pytest.mark.parametrize("mode", ["merged", "exchange"])
+@pytest.mark.parametrize("nvalues", [40, 400])
+def test_one_value_buckets_of_a_uniform_model(nvalues, mode):
+    """A model of uniform keys and data of few values spread over the key
+    range: a bucket then holds one value hundreds or thousands of times
+    without being a single-value bucket of the tables, so the LDS sort finds a
+    crowded bin and lists it for the LDS radix sort, whose keys are then all
+    equal (40 values: FirstSort buckets; 400: SecondSort buckets)."""
+    n, P = 1 << 20, 16
+    note("one-value buckets nvalues=%d mode=%s" % (nvalues, mode))
+    model = trained("uniform", n)
+    g = numpy.random.default_rng(nvalues)
+    values = (numpy.arange(nvalues, dtype=numpy.uint64) * numpy.uint64((1 << 64) // nvalues - 12345)
+              + numpy.uint64(777))
+    a = workloads.records16(values[g.integers(0, nvalues, n)])
+    ref, _ = gpu_sort(a, even(n, P), even(n, P), 'key')
+    flags = _lib.LEARNED_EXCHANGE if mode == "exchange" else 0
+    out = learned.sort_learned(a, model, 'key', nin=even(n, P), flags=flags)
+    from mpsort import last_stats
+    s = last_stats()
+    note("  overflow=%d lds_radix=%d" % (s['buckets_overflow'], s['buckets_lds_radix']))
+    model.close()
+    assert numpy.array_equal(out, ref), "learned output differs from the plain path"
+    assert s['buckets_lds_radix'] > 0
+
+
 @pytest.mark.parametrize("P", [1, 3, 8, 64])
 def test_learned_against_oracle(P):
     n = 1 << 15
