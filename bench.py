@@ -31,13 +31,17 @@ HBM_PEAK_GBS = 8000.0
 PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_second_sort_ms", "t_total_ms")
 # HBM bytes per launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # each scaled by the counter's reading for a plain copy kernel of known size
-# (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
-TRAFFIC_ONESWEEP = 9117529088
-TRAFFIC_BUCKET_SORT = 9488592282
-TRAFFIC_MERGE_BUCKET = 9649530982
-# per-side stages of the learned local sort, as timed by the library
+# (profiles/histsort_clustered_2e28x16B_hbm_traffic.md); the merge kernel's
+# figure is the mean of its two launches
+TRAFFIC_ONESWEEP = 9117778747
+TRAFFIC_BUCKET_SORT = 9330422144
+TRAFFIC_MERGE_BUCKET = 4669927920
+# per-side stages of the learned local sort, as timed by the library; with
+# the merged SecondSort also its two gather-and-sort launches (bucket pairs,
+# then the listed second buckets), which bucket_sort_B contains
 LEARNED_KERNELS = ["hist_A", "scatter_A", "bucket_sort_A", "lds_radix_A", "fallback_A",
-                   "hist_B", "scatter_B", "bucket_sort_B", "lds_radix_B", "fallback_B"]
+                   "hist_B", "scatter_B", "bucket_sort_B", "lds_radix_B", "fallback_B",
+                   "merge_pairs_B", "merge_second_B"]
 
 
 def log(*a):
@@ -420,9 +424,9 @@ def main():
     last = st_plain[-1]
     alg_bytes = 2.0 * n * 16  # one read and one write of every record
 
-    def roof(launch_ms, kernel, traffic):
+    def roof(launch_ms, kernel, traffic, launch_bytes=alg_bytes):
         avg = sum(launch_ms) / len(launch_ms)
-        achieved = alg_bytes / (avg * 1e-3) / 1e9
+        achieved = launch_bytes / (avg * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
                 "avg_launch_ms": round(avg, 4), "launches": len(launch_ms)}
@@ -434,12 +438,20 @@ def main():
         # all n records: the FirstSort's in-LDS bucket sort or the merged
         # SecondSort's gather-and-sort, whichever takes longer
         ib = LEARNED_KERNELS.index
-        sides = {"bucket_sort_A": ("interp_sort_kernel<4> (FirstSort in-LDS bucket sort, 2*n*16 B per launch)",
-                                   TRAFFIC_BUCKET_SORT),
-                 "bucket_sort_B": ("merge_bucket_kernel<4> (SecondSort gather of the senders' sub-runs and "
-                                   "in-LDS bucket sort, 2*n*16 B per launch)", TRAFFIC_MERGE_BUCKET)}
-        dom = max(sides, key=lambda k: sum(st["pass_ms"][ib(k)] for st in st_learned))
-        roofline = roof([st["pass_ms"][ib(dom)] for st in st_learned], *sides[dom])
+        merged = len(st_learned[0]["pass_ms"]) >= len(LEARNED_KERNELS)
+        # launches of each candidate per step, algorithmic bytes per launch
+        cand = {"interp": (["bucket_sort_A"], "interp_sort_kernel<4,512,9> (FirstSort in-LDS bucket sort, one "
+                                               "launch per step over all n records: 2*n*16 B per launch)",
+                           TRAFFIC_BUCKET_SORT, alg_bytes)}
+        if merged:
+            cand["merge"] = (["merge_pairs_B", "merge_second_B"],
+                             "merge_bucket_kernel<4> (SecondSort gather of the senders' sub-runs and in-LDS "
+                             "bucket sort; two launches per step, bucket pairs then the second buckets of the "
+                             "pairs that did not fit together, reading and writing every record once between "
+                             "them: n*16 B per launch on average)", TRAFFIC_MERGE_BUCKET, alg_bytes / 2)
+        dom = max(cand, key=lambda c: sum(st["pass_ms"][ib(k)] for st in st_learned for k in cand[c][0]))
+        names, kname, traffic, lbytes = cand[dom]
+        roofline = roof([st["pass_ms"][ib(k)] for st in st_learned for k in names], kname, traffic, lbytes)
         result_learned["roofline_by_stage"] = {
             k: {"avg_ms": round(sum(st["pass_ms"][ib(k)] for st in st_learned) / len(st_learned), 4),
                 "GB/s": round((alg_bytes / 2 if k.startswith("hist") else alg_bytes) /

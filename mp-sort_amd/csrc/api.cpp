@@ -994,6 +994,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     LearnedCounters lc;
     hipEvent_t* evA = cs.timing ? c.pass_ev[0] : nullptr;
     hipEvent_t* evB = cs.timing && P > 1 ? c.pass_ev[1] : nullptr;
+    bool merged_side = false;
 
     // FirstSort: every rank's keys span the whole distribution, so all
     // segments share one table
@@ -1027,6 +1028,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n;
         // the merged form uses kMergeBins buckets per receiver
         const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 32);
+        merged_side = merged;
         const int nbB = merged ? kMergeBounds : kLBounds;
         LB.tstride = nbB;
         if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, st)))
@@ -1052,12 +1054,14 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             // not fit together
             uint32_t novh[4] = {0, 0, 0, 0};
             for (int round = 0; round < 2; round++) {
+                if ((rc = markB(6 + 2 * round))) return rc;
                 if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase,
                                               LB.tables, round ? LB.second : nullptr, novh[3], LB.second,
                                               LB.noverflow + 3, LB.overflow, LB.noverflow, kMaxOverflowList, LB.hard,
                                               LB.noverflow + 1, LB.medium, LB.noverflow + 2,
                                               cs_direct ? h.checksum + 1 : nullptr, st)))
                     return rc;
+                if ((rc = markB(7 + 2 * round))) return rc;
                 MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
                 MPSG_CHECK(hipStreamSynchronize(st));
             }
@@ -1138,6 +1142,12 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         hipEvent_t* e = side == 0 ? evA : evB;
         if (!e) continue;
         for (int k = 0; k < 5; k++) stats->pass_ms[np++] = ev_ms(e[k], e[k + 1]);
+    }
+    // merged SecondSort: its two gather-and-sort launches (bucket pairs, then
+    // the listed second buckets), inside the in-LDS bucket sort entry
+    if (evB && merged_side) {
+        stats->pass_ms[np++] = ev_ms(evB[6], evB[7]);
+        stats->pass_ms[np++] = ev_ms(evB[8], evB[9]);
     }
     stats->npass_ms = np;
 

@@ -690,10 +690,10 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     }
     __syncthreads();
     int mode = count >= kInterpMax * kSamp && s_crowd ? 1 : 0;  // 0: plain bins, 1: sampled bins
+    // sampled: j = #{samples <= key}, then the interval's offset shifted and
+    // scaled into SUB bins
     auto bin_of = [&](uint64_t key) -> uint32_t {
         if (mode == 0) return plain_bin(key);
-        // sampled: j = #{samples <= key}, then the interval's offset shifted
-        // and scaled into SUB bins
         int j = 0;
 #pragma unroll
         for (int step = kSamp / 2; step > 0; step >>= 1)
@@ -763,16 +763,33 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     // key.  Equal keys keep their bucket order.
     if (tid < 4) s_gkey[count + tid] = ~0ull;
     __syncthreads();
-    for (int o = tid; o < count; o += C::B) {
-        const uint32_t d = s_sbin[o];
-        const uint32_t i = s_ord[o];
-        const uint32_t s0 = s_start[d], e = s_start[d + 1];
+    // the slots' bins, positions, keys and bin bounds are read for all items
+    // before the first is ranked, so the reads are in flight together
+    uint32_t rb[C::I], ri[C::I], rs0[C::I], re[C::I];
+    uint64_t rk[C::I];
+#pragma unroll
+    for (int k = 0; k < C::I; k++) {
+        const int o = min(k * C::B + tid, count - 1);
+        rb[k] = s_sbin[o];
+        ri[k] = s_ord[o];
+        rk[k] = s_gkey[o];
+    }
+#pragma unroll
+    for (int k = 0; k < C::I; k++) {
+        rs0[k] = s_start[rb[k]];
+        re[k] = s_start[rb[k] + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < C::I; k++) {
+        if (k * C::B + tid >= count) continue;
+        const uint32_t i = ri[k];
+        const uint32_t s0 = rs0[k], e = re[k];
         if (e - s0 == 1) {
             s_pos[i] = (uint16_t)s0;
         } else if (e - s0 > (uint32_t)kRunMax) {
             s_long = 1;
         } else {
-            const uint64_t key = s_gkey[o];
+            const uint64_t key = rk[k];
             uint32_t lt = 0, eq = 0;
             for (uint32_t j = s0; j < e; j += 4) {
                 uint64_t k4[4];
