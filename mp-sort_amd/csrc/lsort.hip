@@ -482,18 +482,18 @@ __device__ __forceinline__ void lds_count_pass(const uint16_t* s_in, uint16_t* s
 // i % BLOCK.
 constexpr int kSamp = 64;
 constexpr int kInterpMax = 32;
-// Workgroup shapes of the in-LDS sort: 1024 threads sort up to 8192 records
-// in 4095 bins (65 x 63), 512 threads up to 4096 records in 2015 bins
-// (65 x 31) with half the LDS, so that two workgroups share a CU.
-// ITEMS records per thread (CAP = BLOCK * ITEMS).  The bin arrays take BW
-// words of LDS: NB u16 bin starts, then NB u32 bin counters.
+// Workgroup shapes of the in-LDS sort: 1024 threads x 8 records (one
+// workgroup per CU) or 512 threads x 9 records (two share a CU), in 4095
+// bins (65 sample intervals x 63).  ITEMS records per thread (CAP = BLOCK *
+// ITEMS).  The bin arrays take BW words of LDS: NB u16 bin starts, then NB
+// u16 bin counters, two to a word.
 template <int BLOCK, int ITEMS = kCItems>
 struct BSort {
     static constexpr int B = BLOCK, W = BLOCK / 64, I = ITEMS, CAP = BLOCK * ITEMS;
-    static constexpr int SUB = BLOCK == 1024 ? 63 : 31;          // bins per sample interval
+    static constexpr int SUB = 63;                               // bins per sample interval
     static constexpr int FINE = (kSamp + 1) * SUB;               // bins
     static constexpr int NB = (FINE + 1 + BLOCK - 1) / BLOCK * BLOCK;  // bin counters
-    static constexpr int BW = NB / 2 + NB;                       // words of the bin arrays
+    static constexpr int BW = NB;                                // words of the bin arrays
     static constexpr int TPS = BLOCK / kSamp;                    // threads per sample in the sample sort
     static_assert(NB % 2 == 0 && kSamp % TPS == 0 && CAP < 65536, "in-LDS sort shape");
 };
@@ -559,9 +559,9 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
                                                 bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull) {
     using C = BSort<BLOCK, ITEMS>;
     __shared__ uint64_t s_gkey[C::CAP + 4];  // key of each bin-order slot (+4: unrolled reads)
-    __shared__ uint16_t s_sbin[C::CAP];      // bin of each bin-order slot
+    __shared__ uint16_t s_sbin[C::CAP];      // bin of each bin-order slot, then the final
+                                             // slot of each bucket position (s_pos)
     __shared__ uint16_t s_ord[C::CAP];       // bucket position of each bin-order slot
-    __shared__ uint16_t s_pos[C::CAP];       // final slot of each bucket position
     __shared__ uint64_t s_samp[kSamp];    // the sample, in bucket order
     __shared__ uint64_t s_sorted[kSamp];  // the sample, sorted
     __shared__ int s_crowd;
@@ -574,7 +574,13 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     __shared__ long long s_sum[C::W];
     __shared__ int s_long;
     uint16_t* s_start = reinterpret_cast<uint16_t*>(s_bins);  // NB u16
-    uint32_t* s_fill = s_bins + C::NB / 2;                    // NB u32
+    uint32_t* s_fill = s_bins + C::NB / 2;                    // NB u16 counters, two per word
+    uint16_t* s_pos = s_sbin;
+    // one more record in bin d: the count before it
+    auto bump = [&](uint32_t d) -> uint32_t {
+        const uint32_t sh = (d & 1u) << 4;
+        return (atomicAdd(&s_fill[d >> 1], 1u << sh) >> sh) & 0xffffu;
+    };
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
@@ -644,7 +650,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         if (checksum_out) block_checksum_add<C::W>(csum, s_sum, checksum_out);
         return true;
     }
-    for (int i = tid; i < C::NB; i += C::B) s_fill[i] = 0;
+    for (int i = tid; i < C::NB / 2; i += C::B) s_fill[i] = 0;
     // plain bins: offsets from the minimum shifted to 32 bits, then scaled
     // (shift, conversion and scaling are all monotone)
     const int pbits = 64 - __clzll((long long)((mx - mn) | 1));
@@ -712,18 +718,24 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             dbin[k] = 0;
             if (i < count) {
                 dbin[k] = bin_of(kk[k]);
-                atomicAdd(&s_fill[dbin[k]], 1u);
+                bump(dbin[k]);
             }
         }
         __syncthreads();
         // exclusive scan of the NB counts (those past FINE zero), NB / BLOCK
         // per thread; the counters are reset for the grouping below
         constexpr int PER = C::NB / C::B;
+        static_assert(PER % 2 == 0, "bin counters in pairs");
         uint32_t c[PER], sum = 0, big = 0;
 #pragma unroll
+        for (int q = 0; q < PER / 2; q++) {
+            const uint32_t w = s_fill[tid * (PER / 2) + q];
+            s_fill[tid * (PER / 2) + q] = 0;
+            c[2 * q] = w & 0xffffu;
+            c[2 * q + 1] = w >> 16;
+        }
+#pragma unroll
         for (int j = 0; j < PER; j++) {
-            c[j] = s_fill[tid * PER + j];
-            s_fill[tid * PER + j] = 0;
             sum += c[j];
             big = c[j] > big ? c[j] : big;
         }
@@ -749,7 +761,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     for (int k = 0; k < C::I; k++) {
         const int i = k * C::B + tid;
         if (i < count) {
-            const uint32_t o = s_start[dbin[k]] + atomicAdd(&s_fill[dbin[k]], 1u);
+            const uint32_t o = s_start[dbin[k]] + bump(dbin[k]);
             s_gkey[o] = kk[k];
             s_sbin[o] = (uint16_t)dbin[k];
             s_ord[o] = (uint16_t)i;
@@ -779,6 +791,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         rs0[k] = s_start[rb[k]];
         re[k] = s_start[rb[k] + 1];
     }
+    __syncthreads();  // every slot's bin is read: s_sbin becomes s_pos
 #pragma unroll
     for (int k = 0; k < C::I; k++) {
         if (k * C::B + tid >= count) continue;
