@@ -31,17 +31,16 @@ HBM_PEAK_GBS = 8000.0
 PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_second_sort_ms", "t_total_ms")
 # HBM bytes per launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # each scaled by the counter's reading for a plain copy kernel of known size
-# (profiles/histsort_clustered_2e28x16B_hbm_traffic.md); the merge kernel's
-# figure is the mean of its two launches
+# (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
 TRAFFIC_ONESWEEP = 9117778747
 TRAFFIC_BUCKET_SORT = 9330422144
 TRAFFIC_MERGE_BUCKET = 4669927920
 # per-side stages of the learned local sort, as timed by the library; with
-# the merged SecondSort also its two gather-and-sort launches (bucket pairs,
-# then the listed second buckets), which bucket_sort_B contains
+# the merged SecondSort also its gather-and-sort launch alone (merge_B),
+# which bucket_sort_B contains together with the grouping of the buckets
 LEARNED_KERNELS = ["hist_A", "scatter_A", "bucket_sort_A", "lds_radix_A", "fallback_A",
                    "hist_B", "scatter_B", "bucket_sort_B", "lds_radix_B", "fallback_B",
-                   "merge_pairs_B", "merge_second_B"]
+                   "merge_B"]
 
 
 def log(*a):
@@ -444,11 +443,10 @@ def main():
                                                "launch per step over all n records: 2*n*16 B per launch)",
                            TRAFFIC_BUCKET_SORT, alg_bytes)}
         if merged:
-            cand["merge"] = (["merge_pairs_B", "merge_second_B"],
+            cand["merge"] = (["merge_B"],
                              "merge_bucket_kernel<4> (SecondSort gather of the senders' sub-runs and in-LDS "
-                             "bucket sort; two launches per step, bucket pairs then the second buckets of the "
-                             "pairs that did not fit together, reading and writing every record once between "
-                             "them: n*16 B per launch on average)", TRAFFIC_MERGE_BUCKET, alg_bytes / 2)
+                             "sort of groups of neighbouring buckets, one launch per step over all n records: "
+                             "2*n*16 B per launch)", TRAFFIC_MERGE_BUCKET, alg_bytes)
         dom = max(cand, key=lambda c: sum(st["pass_ms"][ib(k)] for st in st_learned for k in cand[c][0]))
         names, kname, traffic, lbytes = cand[dom]
         roofline = roof([st["pass_ms"][ib(k)] for st in st_learned for k in names], kname, traffic, lbytes)

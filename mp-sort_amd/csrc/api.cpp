@@ -647,10 +647,10 @@ struct LearnedSide {
     uint32_t* err;                    // 4
     int64_t* overflow;                // 2 * kMaxOverflowList
     uint32_t* noverflow;              // 4: [0] oversized buckets, [1] buckets for the LDS radix sort,
-                                      //    [2] medium, [3] second
+                                      //    [2] medium, [3] groups (merged SecondSort)
     uint32_t* hard;                   // nseg * buckets per segment
     uint32_t* medium;                 // nseg * buckets per segment (merged SecondSort)
-    uint32_t* second;                 // nseg * buckets per segment / 2 (merged SecondSort)
+    uint32_t* tasks;                  // nseg * buckets per segment (merged SecondSort groups)
     int64_t tstride = kLBounds;       // segment s uses tables + s * tstride (0: one shared table)
 };
 
@@ -672,7 +672,7 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
         ls->noverflow = a.template take<uint32_t>(4);
         ls->hard = a.template take<uint32_t>((size_t)P * kMergeBins);
         ls->medium = a.template take<uint32_t>((size_t)P * kMergeBins);
-        ls->second = a.template take<uint32_t>((size_t)P * kMergeBins / 2);
+        ls->tasks = a.template take<uint32_t>((size_t)P * kMergeBins);
     } else {
         a.template take<int64_t>(P);
         a.template take<int64_t>(P);
@@ -688,7 +688,7 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
         a.template take<uint32_t>(4);
         a.template take<uint32_t>((size_t)P * kMergeBins);
         a.template take<uint32_t>((size_t)P * kMergeBins);
-        a.template take<uint32_t>((size_t)P * kMergeBins / 2);
+        a.template take<uint32_t>((size_t)P * kMergeBins);
     }
 }
 
@@ -1050,21 +1050,20 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = launch_fill_u32(LB.noverflow, 0u, 4, st))) return rc;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
             if ((rc = markB(1)) || (rc = markB(2))) return rc;
-            // bucket pairs, then the second buckets of the pairs that did
-            // not fit together
+            // groups of neighbouring buckets, one workgroup each
             uint32_t novh[4] = {0, 0, 0, 0};
-            for (int round = 0; round < 2; round++) {
-                if ((rc = markB(6 + 2 * round))) return rc;
-                if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase,
-                                              LB.tables, round ? LB.second : nullptr, novh[3], LB.second,
-                                              LB.noverflow + 3, LB.overflow, LB.noverflow, kMaxOverflowList, LB.hard,
-                                              LB.noverflow + 1, LB.medium, LB.noverflow + 2,
-                                              cs_direct ? h.checksum + 1 : nullptr, st)))
-                    return rc;
-                if ((rc = markB(7 + 2 * round))) return rc;
-                MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
-                MPSG_CHECK(hipStreamSynchronize(st));
-            }
+            if ((rc = launch_merge_group(LB.lhist, P, LB.tasks, LB.noverflow + 3, st))) return rc;
+            MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipStreamSynchronize(st));
+            if ((rc = markB(6))) return rc;
+            if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase,
+                                          LB.tables, LB.tasks, novh[3], LB.overflow, LB.noverflow, kMaxOverflowList,
+                                          LB.hard, LB.noverflow + 1, LB.medium, LB.noverflow + 2,
+                                          cs_direct ? h.checksum + 1 : nullptr, st)))
+                return rc;
+            if ((rc = markB(7))) return rc;
+            MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipStreamSynchronize(st));
             if ((rc = markB(3))) return rc;
             // buckets above the workgroup's capacity and crowded ones were
             // copied in place in sender order: the 1024-thread in-LDS sort,
@@ -1143,12 +1142,9 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         if (!e) continue;
         for (int k = 0; k < 5; k++) stats->pass_ms[np++] = ev_ms(e[k], e[k + 1]);
     }
-    // merged SecondSort: its two gather-and-sort launches (bucket pairs, then
-    // the listed second buckets), inside the in-LDS bucket sort entry
-    if (evB && merged_side) {
-        stats->pass_ms[np++] = ev_ms(evB[6], evB[7]);
-        stats->pass_ms[np++] = ev_ms(evB[8], evB[9]);
-    }
+    // merged SecondSort: its gather-and-sort launch, inside the in-LDS
+    // bucket sort entry
+    if (evB && merged_side) stats->pass_ms[np++] = ev_ms(evB[6], evB[7]);
     stats->npass_ms = np;
 
     c.last_P = P;

@@ -66,17 +66,17 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* myC, int P,
                         const uint64_t* tables, int32_t* bnd, hipStream_t st);
 int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st);
-// tasks == nullptr: one workgroup per pair of buckets (2m, 2m + 1) of a
-// receiver; the second buckets of pairs that are not sorted together are
-// listed in second (*nsecond entries, up to P * kMergeBins / 2), for a
-// launch with tasks = that list and ntasks = its length.  Buckets above
-// kMergeCap (up to kChunkCap) are copied in sender order and listed in medium
-// (*nmedium entries) for launch_listed_sort.
+// groups of neighbouring receiver buckets (together at most kMergeCap
+// records): tasks (up to P * kMergeBins entries), *ntasks (zeroed before)
+int launch_merge_group(const uint32_t* hist, int P, uint32_t* tasks, uint32_t* ntasks, hipStream_t st);
+// one workgroup per group (ntasks = the host's copy of *ntasks).  Buckets
+// above kMergeCap (up to kChunkCap) are copied in sender order and listed in
+// medium (*nmedium entries) for launch_listed_sort.
 int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
-                        const uint64_t* tables, const uint32_t* tasks, uint32_t ntasks, uint32_t* second,
-                        uint32_t* nsecond, int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard,
-                        uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum, hipStream_t st);
+                        const uint64_t* tables, const uint32_t* tasks, uint32_t ntasks, int64_t* overflow,
+                        uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium,
+                        uint32_t* nmedium, long long* checksum, hipStream_t st);
 // in-LDS sort (1024-thread form) of the nlist listed buckets, src -> dst at
 // the same slots (src may equal dst); nbins buckets per segment, segment s
 // uses tables + s * tstride; crowded ones are appended to hard

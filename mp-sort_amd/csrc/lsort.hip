@@ -948,22 +948,27 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
 // r holds the keys in (t_r[k-1], t_r[k]], i.e. one sub-run per sender.  Every
 // receiver has kMergeBins buckets (tables of kMergeBounds boundaries), twice
 // as many as a FirstSort segment, because the model's boundaries at this
-// resolution leave bucket sizes spread widely.  A kMergeBlock-thread
-// workgroup (kSmallItems records per thread, two share a CU) takes buckets
-// 2m and 2m + 1 of a receiver and sorts them in one pass when together they
-// fit (their sub-runs are adjacent in every sender), else one by one.
+// resolution leave bucket sizes spread widely.  Runs of neighbouring buckets
+// of a receiver are grouped greedily (up to kGroupMax buckets, together at
+// most kMergeCap records: their sub-runs are adjacent in every sender) and a
+// kMergeBlock-thread workgroup (kSmallItems records per thread, two share a
+// CU) sorts each group in one pass.
 //   merge_bounds_kernel  bnd[r][k][j] = start of bucket k's sub-run in sender
 //                        j (relative to j's segment), k = 0..kMergeBins (the
 //                        P senders of one bucket side by side)
 //   merge_count_kernel   bucket sizes (sum over senders)
-//   merge_bucket_kernel  one workgroup per (receiver, bucket pair): gathers the
-//                        sub-runs in sender order (the order the exchange
-//                        would have left them in) and sorts them in LDS;
-//                        single-value buckets are copied; buckets above the
-//                        LDS capacity or with a crowded bin are copied in
-//                        that order and listed for the LDS radix sort or the
-//                        radix path
+//   merge_group_kernel   the groups, one workgroup per receiver: greedy over
+//                        chunks of kGroupChunk buckets, one thread per chunk
+//   merge_bucket_kernel  one workgroup per group: gathers the sub-runs in
+//                        sender order (the order the exchange would have
+//                        left them in) and sorts them in LDS; single-value
+//                        buckets are copied; a group with a crowded bin and
+//                        buckets above the LDS capacity are copied bucket by
+//                        bucket in that order and listed for the LDS radix
+//                        sort, the 1024-thread in-LDS sort or the radix path
 constexpr int kMergeMaxP = 64;
+constexpr int kGroupMax = 8;     // buckets per group (task code: bucket << 3 | count - 1)
+constexpr int kGroupChunk = 32;  // groups never cross a chunk of this many buckets
 constexpr int kMergeBlock = 512;
 constexpr int kSmallItems = 9;  // FirstSort buckets up to 4608 records in 512-thread workgroups
 static_assert(BSort<kMergeBlock, kSmallItems>::CAP == kMergeCap, "merged SecondSort capacity");
@@ -1013,6 +1018,48 @@ __global__ __launch_bounds__(256) void merge_count_kernel(const int32_t* __restr
     hist[g] = c;
 }
 
+// groups of the merged SecondSort: tasks[] (receiver by receiver, in bucket
+// order within a receiver; receivers in any order), *ntasks entries.  One
+// 64-thread workgroup per receiver, one thread per chunk.
+__global__ __launch_bounds__(64) void merge_group_kernel(const uint32_t* __restrict__ hist, uint32_t* __restrict__ tasks,
+                                                         uint32_t* __restrict__ ntasks) {
+    constexpr int NCH = kMergeBins / kGroupChunk;  // chunks per receiver
+    static_assert(NCH <= 64 && kMergeBins % kGroupChunk == 0, "one wave walks the chunks");
+    __shared__ uint32_t s_cnt[kMergeBins];
+    const int r = blockIdx.x, lane = threadIdx.x;
+    const uint32_t* h = hist + (int64_t)r * kMergeBins;
+    for (int k = lane; k < kMergeBins; k += 64) s_cnt[k] = h[k];
+    __syncthreads();
+    // the greedy groups of chunk `lane`, counted (out == nullptr) or written
+    auto walk = [&](uint32_t* out) -> uint32_t {
+        uint32_t n = 0;
+        if (lane < NCH) {
+            const int end = (lane + 1) * kGroupChunk;
+            for (int k = lane * kGroupChunk; k < end;) {
+                uint32_t sum = s_cnt[k];
+                int len = 1;
+                while (k + len < end && len < kGroupMax && sum + s_cnt[k + len] <= (uint32_t)kMergeCap)
+                    sum += s_cnt[k + len++];
+                if (out) out[n] = (uint32_t)((((int64_t)r * kMergeBins + k) << 3) | (len - 1));
+                n++;
+                k += len;
+            }
+        }
+        return n;
+    };
+    const uint32_t n = walk(nullptr);
+    uint32_t x = n;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    uint32_t base = 0;
+    if (lane == 63) base = atomicAdd(ntasks, x);
+    base = __shfl(base, 63, 64);
+    walk(tasks + base + (x - n));
+}
+
 // sub-runs of receiver r's buckets [ka, kb) in sender order: s_beg[j] =
 // first record (absolute), s_pref[j] = position of sender j's records in the
 // bucket, s_pref[P] = the bucket's size (wave 0 writes, the caller syncs)
@@ -1041,41 +1088,29 @@ __device__ __forceinline__ void merge_subruns(const int32_t* __restrict__ bnd, c
     if (lane > P) s_pref[lane] = 0x7fffffff;  // above every position
 }
 
-// tasks == nullptr: workgroup m of receiver r takes buckets 2m and 2m + 1,
-// both in one pass when they fit together, else bucket 2m alone, listing
-// bucket 2m + 1 in second (*nsecond entries) for the launch over that list
-// (tasks = second).  Either way a workgroup sorts at most once.
+// One workgroup per group: task = first bucket << 3 | (buckets - 1).
 template <int RW>
 __global__ __launch_bounds__(kMergeBlock, 2048 / kMergeBlock) void merge_bucket_kernel(
     const uint32_t* __restrict__ recs, uint32_t* __restrict__ dst, LKey lk,
     const int64_t* __restrict__ seg_begin, const int32_t* __restrict__ bnd, int P,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
-    const uint64_t* __restrict__ tables, const uint32_t* __restrict__ tasks, uint32_t* __restrict__ second,
-    uint32_t* __restrict__ nsecond, int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow,
-    int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard, uint32_t* __restrict__ medium,
-    uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out) {
+    const uint64_t* __restrict__ tables, const uint32_t* __restrict__ tasks, int64_t* __restrict__ overflow,
+    uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
+    uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out) {
     using C = BSort<kMergeBlock, kSmallItems>;
     __shared__ int64_t s_beg[kMergeMaxP];
     __shared__ int s_pref[kMergeMaxP + 1];
     __shared__ long long s_sum2[C::W];
     __shared__ uint32_t s_bins[C::BW];
     const int tid = threadIdx.x, wave = tid >> 6;
-    int64_t b;  // the first bucket of this workgroup
-    uint32_t c0, c1 = 0;
-    if (tasks) {
-        b = tasks[blockIdx.x];
-        c0 = hist[b];
-    } else {
-        b = 2 * (int64_t)blockIdx.x;
-        c0 = hist[b];
-        c1 = hist[b + 1];
-    }
-    const int r = (int)(b / kMergeBins), ka = (int)(b % kMergeBins);
-    const bool joint = c0 > 0 && c1 > 0 && c0 + c1 <= (uint32_t)C::CAP;
-    if (!tasks && !joint && c1 > 0 && tid == 0) second[atomicAdd(nsecond, 1u)] = (uint32_t)(b + 1);
-    if (c0 == 0 && !joint) return;
-    int kb = joint ? ka + 2 : ka + 1;  // buckets [ka, kb)
-    int count = joint ? (int)(c0 + c1) : (int)c0;
+    const uint32_t task = tasks[blockIdx.x];
+    const int64_t b = task >> 3;  // the group's first bucket
+    const int len = (int)(task & 7u) + 1;
+    const int r = (int)(b / kMergeBins), ka = (int)(b % kMergeBins), kb = ka + len;  // buckets [ka, kb)
+    uint32_t count32 = 0;
+    for (int q = 0; q < len; q++) count32 += hist[b + q];
+    if (count32 == 0) return;
+    const int count = (int)count32;
     const int64_t base = bucket_base[b];
     const uint64_t* t = tables + (int64_t)r * kMergeBounds;
     const uint64_t tlo = ka > 0 ? t[ka - 1] : 0;
@@ -1083,8 +1118,8 @@ __global__ __launch_bounds__(kMergeBlock, 2048 / kMergeBlock) void merge_bucket_
     if (wave == 0) merge_subruns(bnd, seg_begin, P, r, ka, kb, s_beg, s_pref);
     __syncthreads();
     const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
-    // bucket ka alone holds one key value (see single_bucket)
-    const bool single = !joint && (ka == 0 ? thi == 0 : (ka < kMergeBounds && tlo + 1 == thi));
+    // a single bucket that holds one key value (see single_bucket)
+    const bool single = len == 1 && (ka == 0 ? thi == 0 : (ka < kMergeBounds && tlo + 1 == thi));
     // the keys of buckets [ka, kb) lie in (t[ka-1], t[kb-1]]
     const bool bounded = ka > 0 && kb < kMergeBins && tlo < thi;
     if (!single && count <= C::CAP &&
@@ -1092,38 +1127,39 @@ __global__ __launch_bounds__(kMergeBlock, 2048 / kMergeBlock) void merge_bucket_
                                                                      bounded, bounded ? tlo + 1 : 0,
                                                                      bounded ? thi : ~0ull))
         return;
-    if (joint) {
-        // a crowded bin: bucket ka is copied for the LDS radix sort here,
-        // bucket ka + 1 goes to the launch over the list
-        if (tid == 0) second[atomicAdd(nsecond, 1u)] = (uint32_t)(b + 1);
-        kb = ka + 1;
-        count = (int)c0;
-        __syncthreads();
-        if (wave == 0) merge_subruns(bnd, seg_begin, P, r, ka, kb, s_beg, s_pref);
-        __syncthreads();
-    }
-    // copy in sender order: final for a single-value bucket, otherwise the
-    // input of the radix path
+    // copy in sender order, bucket by bucket: final for a single-value
+    // bucket, otherwise the input of the radix path
     long long csum = 0;
-    for (int i = tid; i < count; i += C::B) {
-        uint32_t x[RW];
-        gs.load(i, x);
-        RecIO<RW>::store(dst, base + i, x);
-        if (checksum_out) {
-#pragma unroll
-            for (int w = 0; w < RW; w++) csum += sbyte_sum(x[w]);
+    for (int q = 0; q < len; q++) {
+        const int64_t bq = b + q;
+        const int cq = (int)hist[bq];
+        if (cq == 0) continue;
+        const int64_t baseq = bucket_base[bq];
+        if (len > 1) {
+            __syncthreads();  // the previous bucket's gathers are done
+            if (wave == 0) merge_subruns(bnd, seg_begin, P, r, ka + q, ka + q + 1, s_beg, s_pref);
+            __syncthreads();
         }
-    }
-    if (!single && tid == 0) {
-        if (count <= C::CAP) {
-            hard[atomicAdd(nhard, 1u)] = (uint32_t)b;  // a crowded bin: the LDS radix sort
-        } else if (count <= kChunkCap) {
-            medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread in-LDS sort
-        } else {
-            const uint32_t slot = atomicAdd(noverflow, 1u);
-            if ((int)slot < max_overflow) {
-                overflow[2 * slot] = base;
-                overflow[2 * slot + 1] = (int64_t)count;
+        for (int i = tid; i < cq; i += C::B) {
+            uint32_t x[RW];
+            gs.load(i, x);
+            RecIO<RW>::store(dst, baseq + i, x);
+            if (checksum_out) {
+#pragma unroll
+                for (int w = 0; w < RW; w++) csum += sbyte_sum(x[w]);
+            }
+        }
+        if (!single && tid == 0) {
+            if (cq <= C::CAP) {
+                hard[atomicAdd(nhard, 1u)] = (uint32_t)bq;  // a crowded bin: the LDS radix sort
+            } else if (cq <= kChunkCap) {
+                medium[atomicAdd(nmedium, 1u)] = (uint32_t)bq;  // the 1024-thread in-LDS sort
+            } else {
+                const uint32_t slot = atomicAdd(noverflow, 1u);
+                if ((int)slot < max_overflow) {
+                    overflow[2 * slot] = baseq;
+                    overflow[2 * slot + 1] = (int64_t)cq;
+                }
             }
         }
     }
@@ -1332,19 +1368,25 @@ int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st
     return 0;
 }
 
+int launch_merge_group(const uint32_t* hist, int P, uint32_t* tasks, uint32_t* ntasks, hipStream_t st) {
+    if (P <= 0) return 0;
+    hipLaunchKernelGGL(merge_group_kernel, dim3((unsigned)P), dim3(64), 0, st, hist, tasks, ntasks);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
-                        const uint64_t* tables, const uint32_t* tasks, uint32_t ntasks, uint32_t* second,
-                        uint32_t* nsecond, int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard,
-                        uint32_t* nhard, uint32_t* medium, uint32_t* nmedium, long long* checksum, hipStream_t st) {
+                        const uint64_t* tables, const uint32_t* tasks, uint32_t ntasks, int64_t* overflow,
+                        uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium,
+                        uint32_t* nmedium, long long* checksum, hipStream_t st) {
     if (P > kMergeMaxP) return -1;
-    const unsigned g = tasks ? ntasks : (unsigned)((int64_t)P * kMergeBins / 2);  // listed buckets or bucket pairs
-    if (g == 0) return 0;
+    if (ntasks == 0) return 0;
     const uint32_t* r = (const uint32_t*)recs;
     uint32_t* d = (uint32_t*)dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(g), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, tasks, second, nsecond, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
-        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(g), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, tasks, second, nsecond, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, tasks, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, tasks, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
