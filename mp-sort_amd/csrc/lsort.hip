@@ -101,8 +101,32 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
         for (int64_t i0 = begin + tid; i0 < end; i0 += U * kBlock) {
             uint32_t r[U][RW];
 #pragma unroll
-            for (int u = 0; u < U; u++)
-                if (i0 + (int64_t)u * kBlock < end) RecIO<RW>::load(src, i0 + (int64_t)u * kBlock, r[u]);
+            for (int u = 0; u < U; u++) {
+                if (i0 + (int64_t)u * kBlock < end) {
+                    RecIO<RW>::load(src, i0 + (int64_t)u * kBlock, r[u]);
+                } else {
+#pragma unroll
+                    for (int w = 0; w < RW; w++) r[u][w] = 0;
+                }
+            }
+            // the U table searches advance level by level (each level's U
+            // reads in flight together); positions past the tile search for
+            // key 0 and are ignored
+            uint64_t kq[U];
+            int lo[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                kq[u] = lkey<RW>(r[u], lk);
+                lo[u] = 0;
+            }
+#pragma unroll
+            for (int step = 512; step > 0; step >>= 1) {
+                uint64_t v[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) v[u] = s_tab[lo[u] + step - 1];
+#pragma unroll
+                for (int u = 0; u < U; u++) lo[u] += v[u] < kq[u] ? step : 0;
+            }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int64_t i = i0 + (int64_t)u * kBlock;
@@ -111,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
 #pragma unroll
                         for (int w = 0; w < RW; w++) csum += sbyte_sum(r[u][w]);
                     }
-                    const uint32_t b = bucket_of(lkey<RW>(r[u], lk), s_tab);
+                    const uint32_t b = (uint32_t)lo[u];  // bucket_of(key, s_tab)
                     bkt[i] = (uint16_t)b;  // for the scatter
                     atomicAdd(&s_th[b], 1u);
                 }
@@ -211,22 +235,25 @@ __global__ __launch_bounds__(256) void ltile_scan_kernel(const uint16_t* __restr
 }
 
 // ---------------------------------------------------------------------------
-// the bucket scatter: a onesweep pass with 1024 bins
-template <int RW, int ITEMS>
-__global__ __launch_bounds__(1024) void lscatter_kernel(
+// the bucket scatter: a onesweep pass with 1024 bins.  BLOCK = 512: a tile
+// fits 80 KiB of LDS, so two workgroups share a CU.
+template <int RW, int ITEMS, int BLOCK>
+__global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void lscatter_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, SegDev seg, int64_t ntiles,
     const uint16_t* __restrict__ bkt, const uint64_t* __restrict__ tables, int64_t tstride,
     const uint32_t* __restrict__ tile_off, uint32_t* __restrict__ error_flag, uint32_t* __restrict__ eq_dst) {
-    constexpr int BLOCK = 1024, NW = 16;
+    constexpr int NW = BLOCK / 64, BPT = kLBins / BLOCK;  // waves, buckets per thread
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int WAVE_TILE = 64 * ITEMS;
+    static_assert(kLBins % BLOCK == 0, "buckets per thread");
     __shared__ __attribute__((aligned(16))) uint32_t s_rec[TILE * RW];
     __shared__ uint16_t s_dig[TILE];
     __shared__ uint16_t s_whist[NW][kLBins];
     __shared__ uint32_t s_excl[kLBins];
-    __shared__ int64_t s_gdelta[kLBins];
+    __shared__ int32_t s_gdelta[kLBins];  // output position of the tile's run of a bucket, less its
+                                          // staged position, from the segment's start
     __shared__ uint32_t s_single[kLBins / 32];
-    __shared__ uint32_t s_tmp[16];
+    __shared__ uint32_t s_tmp[NW];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (int i = tid; i < NW * kLBins; i += BLOCK) (&s_whist[0][0])[i] = 0;
@@ -256,11 +283,13 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
             bk[k] = bkt[i];
         }
     }
-    {
-        const uint64_t m = __ballot(eq_dst != dst && single_bucket(tables + (int64_t)s * tstride, tid));
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+        const int d = q * BLOCK + tid;
+        const uint64_t m = __ballot(eq_dst != dst && single_bucket(tables + (int64_t)s * tstride, d));
         if (lane == 0) {
-            s_single[2 * wave] = (uint32_t)m;
-            s_single[2 * wave + 1] = (uint32_t)(m >> 32);
+            s_single[d / 32] = (uint32_t)m;
+            s_single[d / 32 + 1] = (uint32_t)(m >> 32);
         }
     }
 
@@ -288,9 +317,14 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
     }
     __syncthreads();
 
-    const int d = tid;  // one bucket per thread
-    uint32_t total = 0;
-    {
+    // buckets BPT * tid .. BPT * tid + BPT - 1 per thread
+    uint32_t total[BPT], sum = 0;
+    uint32_t toff[BPT];
+#pragma unroll
+    for (int q = 0; q < BPT; q++) toff[q] = tile_off[tile * kLBins + BPT * tid + q];
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+        const int d = BPT * tid + q;
         uint32_t acc = 0;
 #pragma unroll
         for (int w = 0; w < NW; w++) {
@@ -298,11 +332,17 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
             s_whist[w][d] = (uint16_t)acc;
             acc += c;
         }
-        total = acc;
+        total[q] = acc;
+        sum += acc;
     }
-    const uint32_t excl = scan1024(total, s_tmp);
-    s_excl[d] = excl;
-    s_gdelta[d] = sbeg + (int64_t)tile_off[tile * kLBins + d] - (int64_t)excl;
+    uint32_t excl = scan_block<BLOCK>(sum, s_tmp);
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+        const int d = BPT * tid + q;
+        s_excl[d] = excl;
+        s_gdelta[d] = (int32_t)toff[q] - (int32_t)excl;
+        excl += total[q];
+    }
     __syncthreads();
 
 #pragma unroll
@@ -325,7 +365,7 @@ __global__ __launch_bounds__(1024) void lscatter_kernel(
 #pragma unroll
             for (int w = 0; w < RW; w++) rr[w] = s_rec[i * RW + w];
             const uint32_t dd = s_dig[i];
-            const int64_t o = s_gdelta[dd] + i;
+            const int64_t o = sbeg + (int64_t)s_gdelta[dd] + i;
             if (o < sbeg || o >= send) {
                 atomicOr(error_flag, 4u);
                 continue;
@@ -1255,7 +1295,7 @@ int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64
     return 0;
 }
 
-int lscatter_tile(int rw) { return rw == 2 ? 1024 * 8 : 1024 * 4; }
+int lscatter_tile(int rw) { return rw == 2 ? 1024 * 8 : 512 * 6; }
 
 int launch_ltile_scan(const uint16_t* tile_cnt, const SegDev& sd, const int64_t* bucket_base, uint32_t* tile_off,
                       hipStream_t st) {
@@ -1275,8 +1315,8 @@ int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegD
     uint32_t* d = (uint32_t*)dst;
     uint32_t* e = (uint32_t*)eq_dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, bkt, tables, tstride, tile_off, err, e); break;
-        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 4>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, bkt, tables, tstride, tile_off, err, e); break;
+        case 2: hipLaunchKernelGGL((lscatter_kernel<2, 8, 1024>), dim3(grid), dim3(1024), 0, st, s, d, sd, ntiles, bkt, tables, tstride, tile_off, err, e); break;
+        case 4: hipLaunchKernelGGL((lscatter_kernel<4, 6, 512>), dim3(grid), dim3(512), 0, st, s, d, sd, ntiles, bkt, tables, tstride, tile_off, err, e); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
