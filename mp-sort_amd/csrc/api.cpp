@@ -115,6 +115,7 @@ struct DevCtx {
     Buf lookback;       // look-back flags (only ever holds flag words)
     Buf ltile;          // learned scatter: per-tile bucket counts and offsets
     Buf mbnd;           // merged SecondSort: sub-run bounds per (receiver, sender, bucket)
+    Buf mdesc;          // merged SecondSort: group descriptors and their sub-run tables
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     hipEvent_t ev[kEvPhase];
@@ -1055,10 +1056,22 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = launch_merge_group(LB.lhist, P, LB.tasks, LB.noverflow + 3, st))) return rc;
             MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
             MPSG_CHECK(hipStreamSynchronize(st));
+            // descriptors: group, then per sender sub-run start and position
+            const uint32_t ntasks = novh[3];
+            const size_t d_bytes = ((size_t)(ntasks ? ntasks : 1) * sizeof(MergeTask) + 255) & ~(size_t)255;
+            const size_t b_bytes = (size_t)(ntasks ? ntasks : 1) * 64 * sizeof(int64_t);
+            if ((rc = c.mdesc.ensure(d_bytes + b_bytes + (size_t)(ntasks ? ntasks : 1) * 64 * sizeof(int32_t))))
+                return rc;
+            MergeTask* desc = (MergeTask*)c.mdesc.p;
+            int64_t* sub_beg = (int64_t*)((char*)c.mdesc.p + d_bytes);
+            int32_t* sub_pref = (int32_t*)((char*)c.mdesc.p + d_bytes + b_bytes);
+            if ((rc = launch_merge_desc(LB.tasks, ntasks, bnd, pa.begin, P, LB.bbase, LB.tables, desc, sub_beg,
+                                        sub_pref, st)))
+                return rc;
             if ((rc = markB(6))) return rc;
             if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase,
-                                          LB.tables, LB.tasks, novh[3], LB.overflow, LB.noverflow, kMaxOverflowList,
-                                          LB.hard, LB.noverflow + 1, LB.medium, LB.noverflow + 2,
+                                          LB.tables, desc, sub_beg, sub_pref, ntasks, LB.overflow, LB.noverflow,
+                                          kMaxOverflowList, LB.hard, LB.noverflow + 1, LB.medium, LB.noverflow + 2,
                                           cs_direct ? h.checksum + 1 : nullptr, st)))
                 return rc;
             if ((rc = markB(7))) return rc;
@@ -1653,6 +1666,7 @@ int mpsort_gpu_release(void) {
     c.lookback.release();
     c.ltile.release();
     c.mbnd.release();
+    c.mdesc.release();
     c.last_P = 0;
     return 0;
 }

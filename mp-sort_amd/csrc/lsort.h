@@ -69,14 +69,29 @@ int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st
 // groups of neighbouring receiver buckets (together at most kMergeCap
 // records): tasks (up to P * kMergeBins entries), *ntasks (zeroed before)
 int launch_merge_group(const uint32_t* hist, int P, uint32_t* tasks, uint32_t* ntasks, hipStream_t st);
-// one workgroup per group (ntasks = the host's copy of *ntasks).  Buckets
-// above kMergeCap (up to kChunkCap) are copied in sender order and listed in
-// medium (*nmedium entries) for launch_listed_sort.
+// a group's first bucket, its size and the table bounds of its keys
+struct MergeTask {
+    int64_t base;   // output position of the group
+    uint32_t count; // records
+    uint32_t task;  // first bucket << 3 | (buckets - 1)
+    uint64_t tlo;   // keys lie in (tlo, thi]
+    uint64_t thi;
+};
+// descriptors of the ntasks groups (ntasks = the host's copy of *ntasks):
+// desc[q], and the sub-run of sender j at sub_beg[q * 64 + j] (absolute
+// first record) and sub_pref[q * 64 + j] (position in the group)
+int launch_merge_desc(const uint32_t* tasks, uint32_t ntasks, const int32_t* bnd, const int64_t* seg_begin, int P,
+                      const int64_t* bucket_base, const uint64_t* tables, MergeTask* desc, int64_t* sub_beg,
+                      int32_t* sub_pref, hipStream_t st);
+// one workgroup per group.  Buckets above kMergeCap (up to kChunkCap) are
+// copied in sender order and listed in medium (*nmedium entries) for
+// launch_listed_sort.
 int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
-                        const uint64_t* tables, const uint32_t* tasks, uint32_t ntasks, int64_t* overflow,
-                        uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium,
-                        uint32_t* nmedium, long long* checksum, hipStream_t st);
+                        const uint64_t* tables, const MergeTask* desc, const int64_t* sub_beg,
+                        const int32_t* sub_pref, uint32_t ntasks, int64_t* overflow, uint32_t* noverflow,
+                        int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium,
+                        long long* checksum, hipStream_t st);
 // in-LDS sort (1024-thread form) of the nlist listed buckets, src -> dst at
 // the same slots (src may equal dst); nbins buckets per segment, segment s
 // uses tables + s * tstride; crowded ones are appended to hard

@@ -1128,35 +1128,81 @@ __device__ __forceinline__ void merge_subruns(const int32_t* __restrict__ bnd, c
     if (lane > P) s_pref[lane] = 0x7fffffff;  // above every position
 }
 
+// The groups' descriptors, one wave per group: what a sorting workgroup
+// needs before its first record load, at addresses given by its index
+__global__ __launch_bounds__(256) void merge_desc_kernel(const uint32_t* __restrict__ tasks, uint32_t ntasks,
+                                                         const int32_t* __restrict__ bnd,
+                                                         const int64_t* __restrict__ seg_begin, int P,
+                                                         const int64_t* __restrict__ bucket_base,
+                                                         const uint64_t* __restrict__ tables,
+                                                         MergeTask* __restrict__ desc, int64_t* __restrict__ sub_beg,
+                                                         int32_t* __restrict__ sub_pref) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= (int64_t)ntasks) return;
+    const uint32_t task = tasks[q];
+    const int64_t b = task >> 3;
+    const int r = (int)(b / kMergeBins), ka = (int)(b % kMergeBins), kb = ka + (int)(task & 7u) + 1;
+    int len = 0;
+    int64_t beg = 0;
+    if (lane < P) {
+        const int32_t* ba = bnd + ((int64_t)r * (kMergeBins + 1) + ka) * P;
+        const int32_t* bb = bnd + ((int64_t)r * (kMergeBins + 1) + kb) * P;
+        beg = seg_begin[lane] + ba[lane];
+        len = bb[lane] - ba[lane];
+    }
+    int x = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    const int total = __shfl(x, 63, 64);
+    sub_beg[q * 64 + lane] = beg;
+    // positions of the senders' records in the group; entries from P on lie
+    // above every position
+    sub_pref[q * 64 + lane] = lane < P ? x - len : (lane == P ? total : 0x7fffffff);
+    if (lane == 0) {
+        const uint64_t* t = tables + (int64_t)r * kMergeBounds;
+        MergeTask m;
+        m.base = bucket_base[b];
+        m.count = (uint32_t)total;
+        m.task = task;
+        m.tlo = ka > 0 ? t[ka - 1] : 0;
+        m.thi = kb - 1 < kMergeBounds ? t[kb - 1] : ~0ull;
+        desc[q] = m;
+    }
+}
+
 // One workgroup per group: task = first bucket << 3 | (buckets - 1).
 template <int RW>
 __global__ __launch_bounds__(kMergeBlock, 2048 / kMergeBlock) void merge_bucket_kernel(
     const uint32_t* __restrict__ recs, uint32_t* __restrict__ dst, LKey lk,
     const int64_t* __restrict__ seg_begin, const int32_t* __restrict__ bnd, int P,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
-    const uint64_t* __restrict__ tables, const uint32_t* __restrict__ tasks, int64_t* __restrict__ overflow,
-    uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
-    uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out) {
+    const MergeTask* __restrict__ desc, const int64_t* __restrict__ sub_beg, const int32_t* __restrict__ sub_pref,
+    int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard,
+    uint32_t* __restrict__ nhard, uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium,
+    long long* __restrict__ checksum_out) {
     using C = BSort<kMergeBlock, kSmallItems>;
     __shared__ int64_t s_beg[kMergeMaxP];
     __shared__ int s_pref[kMergeMaxP + 1];
     __shared__ long long s_sum2[C::W];
     __shared__ uint32_t s_bins[C::BW];
     const int tid = threadIdx.x, wave = tid >> 6;
-    const uint32_t task = tasks[blockIdx.x];
-    const int64_t b = task >> 3;  // the group's first bucket
-    const int len = (int)(task & 7u) + 1;
+    const MergeTask m = desc[blockIdx.x];
+    if (wave == 0) {
+        s_beg[tid] = sub_beg[(int64_t)blockIdx.x * 64 + tid];
+        s_pref[tid] = sub_pref[(int64_t)blockIdx.x * 64 + tid];
+    }
+    const int64_t b = m.task >> 3;  // the group's first bucket
+    const int len = (int)(m.task & 7u) + 1;
     const int r = (int)(b / kMergeBins), ka = (int)(b % kMergeBins), kb = ka + len;  // buckets [ka, kb)
-    uint32_t count32 = 0;
-    for (int q = 0; q < len; q++) count32 += hist[b + q];
-    if (count32 == 0) return;
-    const int count = (int)count32;
-    const int64_t base = bucket_base[b];
-    const uint64_t* t = tables + (int64_t)r * kMergeBounds;
-    const uint64_t tlo = ka > 0 ? t[ka - 1] : 0;
-    const uint64_t thi = kb - 1 < kMergeBounds ? t[kb - 1] : ~0ull;
-    if (wave == 0) merge_subruns(bnd, seg_begin, P, r, ka, kb, s_beg, s_pref);
+    const int count = (int)m.count;
+    const int64_t base = m.base;
+    const uint64_t tlo = m.tlo, thi = m.thi;
     __syncthreads();
+    if (count == 0) return;
     const GatherSrc<RW> gs{recs, s_beg, s_pref, P};
     // a single bucket that holds one key value (see single_bucket)
     const bool single = len == 1 && (ka == 0 ? thi == 0 : (ka < kMergeBounds && tlo + 1 == thi));
@@ -1415,18 +1461,30 @@ int launch_merge_group(const uint32_t* hist, int P, uint32_t* tasks, uint32_t* n
     return 0;
 }
 
+int launch_merge_desc(const uint32_t* tasks, uint32_t ntasks, const int32_t* bnd, const int64_t* seg_begin, int P,
+                      const int64_t* bucket_base, const uint64_t* tables, MergeTask* desc, int64_t* sub_beg,
+                      int32_t* sub_pref, hipStream_t st) {
+    if (ntasks == 0) return 0;
+    if (P > kMergeMaxP) return -1;
+    hipLaunchKernelGGL(merge_desc_kernel, dim3((ntasks + 3) / 4), dim3(256), 0, st, tasks, ntasks, bnd, seg_begin, P,
+                       bucket_base, tables, desc, sub_beg, sub_pref);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
-                        const uint64_t* tables, const uint32_t* tasks, uint32_t ntasks, int64_t* overflow,
-                        uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium,
-                        uint32_t* nmedium, long long* checksum, hipStream_t st) {
+                        const uint64_t* tables, const MergeTask* desc, const int64_t* sub_beg,
+                        const int32_t* sub_pref, uint32_t ntasks, int64_t* overflow, uint32_t* noverflow,
+                        int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium,
+                        long long* checksum, hipStream_t st) {
     if (P > kMergeMaxP) return -1;
     if (ntasks == 0) return 0;
     const uint32_t* r = (const uint32_t*)recs;
     uint32_t* d = (uint32_t*)dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, tasks, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
-        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, tables, tasks, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, desc, sub_beg, sub_pref, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, desc, sub_beg, sub_pref, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
