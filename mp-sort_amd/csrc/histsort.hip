@@ -89,6 +89,13 @@ __device__ __forceinline__ Key shfl_key(const Key& k, int off) {
     return o;
 }
 
+__device__ __forceinline__ Key bcast_key(const Key& k, int src) {
+    Key o;
+#pragma unroll
+    for (int w = 0; w < 4; w++) o.w[w] = __shfl(k.w[w], src, 64);
+    return o;
+}
+
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void bisect_kernel(
     const uint32_t* __restrict__ recs, int rw, KeySpec ks, const int64_t* __restrict__ sbeg,
@@ -96,6 +103,7 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
     int64_t* __restrict__ win, int64_t* __restrict__ myclt, int64_t* __restrict__ mycle,
     uint64_t* __restrict__ split_out, uint64_t* __restrict__ pminmax, int* __restrict__ rounds_out,
     uint32_t* __restrict__ error_flag, const uint64_t* __restrict__ init, int* __restrict__ exact_out) {
+    __shared__ Key s_cand[kWaves][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * kWaves + wave;  // splitter index
     if (i >= P - 1) return;                   // uniform per wave
@@ -153,23 +161,25 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
         if (key_cmp(hi, mx, nw) > 0) hi = mx;
         long long clt_lo = 0, cle_hi = 0;
         for (int j = lane; j < P; j += 64) {
-            int64_t a = 0, b = scnt[j];
             const int64_t base = sbeg[j];
-            while (a < b) {
-                const int64_t m = a + ((b - a) >> 1);
-                if (key_cmp(load_key(recs, rw, base + m, ks), lo, nw) < 0) a = m + 1; else b = m;
+            // #{< lo} and #{<= hi}: the two searches advance together
+            int64_t a1 = 0, b1 = scnt[j], a2 = 0, b2 = scnt[j];
+            while (a1 < b1 || a2 < b2) {
+                const int64_t m1 = a1 + ((b1 - a1) >> 1), m2 = a2 + ((b2 - a2) >> 1);
+                const bool s1 = a1 < b1, s2 = a2 < b2;
+                const Key k1 = load_key(recs, rw, base + (s1 ? m1 : m2), ks);
+                const Key k2 = load_key(recs, rw, base + (s2 ? m2 : m1), ks);
+                if (s1) {
+                    if (key_cmp(k1, lo, nw) < 0) a1 = m1 + 1; else b1 = m1;
+                }
+                if (s2) {
+                    if (key_cmp(k2, hi, nw) <= 0) a2 = m2 + 1; else b2 = m2;
+                }
             }
-            const int64_t lt = a;
-            a = 0;
-            b = scnt[j];
-            while (a < b) {
-                const int64_t m = a + ((b - a) >> 1);
-                if (key_cmp(load_key(recs, rw, base + m, ks), hi, nw) <= 0) a = m + 1; else b = m;
-            }
-            wi[4 * j + 2] = lt;
-            wi[4 * j + 3] = a;
-            clt_lo += lt;
-            cle_hi += a;
+            wi[4 * j + 2] = a1;
+            wi[4 * j + 3] = a2;
+            clt_lo += a1;
+            cle_hi += a2;
         }
         clt_lo = wave_sum_ll(clt_lo);
         cle_hi = wave_sum_ll(cle_hi);
@@ -184,10 +194,61 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
         test_proposal = key_cmp(prop, L, nw) >= 0 && key_cmp(prop, R, nw) <= 0;
     }
     int exact = 0;
+    bool use_found = false;
+    Key found = mn;
     while (!stable) {
         rounds++;
+        if (!test_proposal && !narrow) {
+            // at most 64 keys left in the windows (the keys in [L, R]): the
+            // splitter is the one of rank Ci - 1 - #{< L} among them, found
+            // directly; the round below then counts it like any other value
+            long long wsum = 0, below = 0;
+            for (int j = lane; j < P; j += 64) {
+                wsum += wi[4 * j + 1] - wi[4 * j + 0];
+                below += wi[4 * j + 0];
+            }
+            const long long mine = wsum;
+            wsum = wave_sum_ll(wsum);
+            below = wave_sum_ll(below);
+            if (wsum > 0 && wsum <= 64) {
+                long long off = mine;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const long long y = __shfl_up(off, o, 64);
+                    if (lane >= o) off += y;
+                }
+                off -= mine;
+                for (int j = lane; j < P; j += 64)
+                    for (int64_t q = wi[4 * j + 0]; q < wi[4 * j + 1]; q++)
+                        s_cand[wave][off++] = load_key(recs, rw, sbeg[j] + q, ks);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const long long r = (long long)Ci - 1 - below;
+                bool hit = false;
+                Key c = mn;
+                if (lane < wsum) {
+                    c = s_cand[wave][lane];
+                    long long lt = 0, le = 0;
+                    for (int t = 0; t < (int)wsum; t++) {
+                        const int cmp = key_cmp(s_cand[wave][t], c, nw);
+                        lt += cmp < 0 ? 1 : 0;
+                        le += cmp <= 0 ? 1 : 0;
+                    }
+                    hit = lt <= r && r < le;
+                }
+                const uint64_t hits = __ballot(hit);
+                if (hits) {
+                    found = bcast_key(c, (int)__ffsll((unsigned long long)hits) - 1);
+                    use_found = true;
+                }
+            }
+        }
         // piter_bisect (internal-parallel.h:168-198)
-        if (test_proposal) {
+        if (use_found) {
+            Pv = found;
+            use_found = false;
+        } else if (test_proposal) {
             Pv = prop;
             test_proposal = false;
         } else if (narrow) {
@@ -201,19 +262,22 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
         long long slt = 0, sle = 0;
         for (int j = lane; j < P; j += 64) {
             int64_t* wj = wi + 4 * j;
-            int64_t a = wj[0], b = wj[1];
             const int64_t base = sbeg[j];
-            while (a < b) {
-                const int64_t m = a + ((b - a) >> 1);
-                if (key_cmp(load_key(recs, rw, base + m, ks), Pv, nw) < 0) a = m + 1; else b = m;
+            // the two searches advance together, so their loads overlap
+            int64_t a1 = wj[0], b1 = wj[1], a2 = a1, b2 = b1;
+            while (a1 < b1 || a2 < b2) {
+                const int64_t m1 = a1 + ((b1 - a1) >> 1), m2 = a2 + ((b2 - a2) >> 1);
+                const bool s1 = a1 < b1, s2 = a2 < b2;
+                const Key k1 = load_key(recs, rw, base + (s1 ? m1 : m2), ks);
+                const Key k2 = load_key(recs, rw, base + (s2 ? m2 : m1), ks);
+                if (s1) {
+                    if (key_cmp(k1, Pv, nw) < 0) a1 = m1 + 1; else b1 = m1;
+                }
+                if (s2) {
+                    if (key_cmp(k2, Pv, nw) <= 0) a2 = m2 + 1; else b2 = m2;
+                }
             }
-            const int64_t lt = a;
-            b = wj[1];
-            while (a < b) {
-                const int64_t m = a + ((b - a) >> 1);
-                if (key_cmp(load_key(recs, rw, base + m, ks), Pv, nw) <= 0) a = m + 1; else b = m;
-            }
-            const int64_t le = a;
+            const int64_t lt = a1, le = a2;
             wj[2] = lt;
             wj[3] = le;
             slt += lt;
