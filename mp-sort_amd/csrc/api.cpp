@@ -105,6 +105,8 @@ struct DevCtx {
     bool init = false;
     int device = 0;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;   // second stream (default priority): work that overlaps the splitter search
+    hipEvent_t fork = nullptr, join = nullptr;
     uint32_t epoch = 0;
     Buf din, dout;      // host staging
     Buf s0, s1, rbuf;   // sort-record buffers
@@ -153,6 +155,9 @@ static int get_ctx(DevCtx** out) {
     if (!c.init) {
         c.device = dev;
         MPSG_CHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+        MPSG_CHECK(hipStreamCreateWithFlags(&c.st2, hipStreamNonBlocking));
+        MPSG_CHECK(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
+        MPSG_CHECK(hipEventCreateWithFlags(&c.join, hipEventDisableTiming));
         for (int i = 0; i < kEvPhase; i++) MPSG_CHECK(hipEventCreate(&c.ev[i]));
         for (int a = 0; a < 2; a++)
             for (int i = 0; i < 2 * kMaxPasses; i++) MPSG_CHECK(hipEventCreate(&c.pass_ev[a][i]));
@@ -1015,6 +1020,28 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                                      cs_direct ? h.checksum : nullptr, nullptr, &lc, st, evA)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
+        // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n;
+        // the merged form uses kMergeBins buckets per receiver
+        const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 32);
+        merged_side = merged;
+        const int nbB = merged ? kMergeBounds : kLBounds;
+        LB.tstride = nbB;
+        const LKey lkB = make_lkey(cs.ks);
+        int32_t* bnd = nullptr;
+        if (merged) {
+            // the receivers' tables and the position of every table entry in
+            // every sender's whole sorted segment need only the FirstSort:
+            // they run on the second stream while the splitters are found
+            if ((rc = c.mbnd.ensure((size_t)P * P * (kMergeBins + 1) * sizeof(int32_t)))) return rc;
+            bnd = (int32_t*)c.mbnd.p;
+            MPSG_CHECK(hipEventRecord(c.fork, st));
+            MPSG_CHECK(hipStreamWaitEvent(c.st2, c.fork, 0));
+            if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, c.st2)))
+                return rc;
+            if ((rc = launch_merge_bounds(rw, c.s0.p, lkB, pa.begin, pa.count, P, LB.tables, bnd, c.st2)))
+                return rc;
+            MPSG_CHECK(hipEventRecord(c.join, c.st2));
+        }
         // splitters: model proposals, checked and repaired by bisection
         const float delta = std::max(2.0f * M.err_max, 8.0f / (float)M.nsample);
         if ((rc = launch_model_propose(md, M.kmax, h.C, P, n, delta, prop, st))) return rc;
@@ -1026,13 +1053,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                                 h.run_src, h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
-        // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n;
-        // the merged form uses kMergeBins buckets per receiver
-        const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 32);
-        merged_side = merged;
-        const int nbB = merged ? kMergeBounds : kLBounds;
-        LB.tstride = nbB;
-        if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, st)))
+        if (!merged &&
+            (rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, st)))
             return rc;
         if (merged) {
             // no exchange: every receiver bucket is gathered from the senders'
@@ -1041,12 +1063,11 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                 if (evB) MPSG_CHECK(hipEventRecord(evB[i], st));
                 return 0;
             };
-            const LKey lk = make_lkey(cs.ks);
-            if ((rc = c.mbnd.ensure((size_t)P * P * (kMergeBins + 1) * sizeof(int32_t)))) return rc;
-            int32_t* bnd = (int32_t*)c.mbnd.p;
+            const LKey lk = lkB;
             if ((rc = markB(0))) return rc;
-            if ((rc = launch_merge_bounds(rw, c.s0.p, lk, pa.begin, h.myC, P, LB.tables, bnd, st))) return rc;
-            if ((rc = launch_merge_count(bnd, P, LB.lhist, st))) return rc;
+            // the bounds, clamped to each receiver's window in each sender
+            MPSG_CHECK(hipStreamWaitEvent(st, c.join, 0));
+            if ((rc = launch_merge_count(bnd, P, h.myC, LB.lhist, st))) return rc;
             if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st, kMergeBins))) return rc;
             if ((rc = launch_fill_u32(LB.noverflow, 0u, 4, st))) return rc;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));

@@ -63,9 +63,11 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
 // SecondSort straight from the senders' sorted segments (no exchange), for
 // P <= 64 receivers with kMergeBounds-boundary tables each:
 //   bnd: P * P * (kMergeBins + 1) int32; hist / bucket_base: P * kMergeBins
-int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* myC, int P,
-                        const uint64_t* tables, int32_t* bnd, hipStream_t st);
-int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st);
+// positions of the table entries in the senders' whole segments (needs no
+// splitters), then clamped to the receivers' windows myC with the sizes
+int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
+                        int P, const uint64_t* tables, int32_t* bnd, hipStream_t st);
+int launch_merge_count(int32_t* bnd, int P, const int64_t* myC, uint32_t* hist, hipStream_t st);
 // groups of neighbouring receiver buckets (together at most kMergeCap
 // records): tasks (up to P * kMergeBins entries), *ntasks (zeroed before)
 int launch_merge_group(const uint32_t* hist, int P, uint32_t* tasks, uint32_t* ntasks, hipStream_t st);

@@ -1014,9 +1014,13 @@ constexpr int kSmallItems = 9;  // FirstSort buckets up to 4608 records in 512-t
 static_assert(BSort<kMergeBlock, kSmallItems>::CAP == kMergeCap, "merged SecondSort capacity");
 
 template <int RW>
+// Positions in the whole sorted segment of every sender (the receivers'
+// windows are applied by merge_count_kernel, so this runs before the
+// splitters are known): bnd[r][k][j] = #{keys of sender j <= t_r[k-1]} for
+// k = 1 .. kMergeBins - 1, 0 for k = 0, the segment's size for kMergeBins.
 __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __restrict__ recs, LKey lk,
                                                            const int64_t* __restrict__ seg_begin,
-                                                           const int64_t* __restrict__ myC, int P,
+                                                           const int64_t* __restrict__ seg_count, int P,
                                                            const uint64_t* __restrict__ tables,
                                                            int32_t* __restrict__ bnd) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, j, k), k = 0..kMergeBins
@@ -1024,18 +1028,17 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
     const int k = (int)(g % (kMergeBins + 1));
     const int64_t rj = g / (kMergeBins + 1);
     const int j = (int)(rj % P), r = (int)(rj / P);
-    const int64_t* cj = myC + (int64_t)j * (P + 1);
-    const int64_t lo0 = cj[r], hi0 = cj[r + 1];
+    const int64_t cnt = seg_count[j];
     int64_t res;
     if (k == 0) {
-        res = lo0;
+        res = 0;
     } else if (k == kMergeBins) {
-        res = hi0;
+        res = cnt;
     } else {
-        // first position of the window with key > t_r[k-1]
+        // first position with key > t_r[k-1]
         const uint64_t t = tables[(int64_t)r * kMergeBounds + (k - 1)];
         const int64_t sb = seg_begin[j];
-        int64_t a = lo0, e = hi0;
+        int64_t a = 0, e = cnt;
         while (a < e) {
             const int64_t m = a + ((e - a) >> 1);
             uint32_t rr[RW];
@@ -1047,14 +1050,26 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
     bnd[((int64_t)r * (kMergeBins + 1) + k) * P + j] = (int32_t)res;
 }
 
-__global__ __launch_bounds__(256) void merge_count_kernel(const int32_t* __restrict__ bnd, int P,
+// The bounds clamped to receiver r's window [myC[j][r], myC[j][r+1]) in
+// every sender j (written back: row k by thread (r, k), the last row also by
+// thread (r, kMergeBins - 1); a row read while its owner writes it clamps to
+// the same value), and the bucket sizes
+__global__ __launch_bounds__(256) void merge_count_kernel(int32_t* __restrict__ bnd, int P,
+                                                          const int64_t* __restrict__ myC,
                                                           uint32_t* __restrict__ hist) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, k)
     if (g >= (int64_t)P * kMergeBins) return;
     const int k = (int)(g % kMergeBins), r = (int)(g / kMergeBins);
-    const int32_t* bk = bnd + ((int64_t)r * (kMergeBins + 1) + k) * P;
+    int32_t* bk = bnd + ((int64_t)r * (kMergeBins + 1) + k) * P;
     uint32_t c = 0;
-    for (int j = 0; j < P; j++) c += (uint32_t)(bk[P + j] - bk[j]);
+    for (int j = 0; j < P; j++) {
+        const int64_t* cj = myC + (int64_t)j * (P + 1);
+        const int32_t lo0 = (int32_t)cj[r], hi0 = (int32_t)cj[r + 1];
+        const int32_t a = min(max(bk[j], lo0), hi0), b = min(max(bk[P + j], lo0), hi0);
+        c += (uint32_t)(b - a);
+        bk[j] = a;
+        if (k == kMergeBins - 1) bk[P + j] = b;
+    }
     hist[g] = c;
 }
 
@@ -1433,23 +1448,23 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
     return 0;
 }
 
-int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* myC, int P,
-                        const uint64_t* tables, int32_t* bnd, hipStream_t st) {
+int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
+                        int P, const uint64_t* tables, int32_t* bnd, hipStream_t st) {
     const int64_t n = (int64_t)P * P * (kMergeBins + 1);
     const unsigned g = (unsigned)((n + 255) / 256);
     const uint32_t* r = (const uint32_t*)recs;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, myC, P, tables, bnd); break;
-        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, myC, P, tables, bnd); break;
+        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, bnd); break;
+        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, bnd); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_merge_count(const int32_t* bnd, int P, uint32_t* hist, hipStream_t st) {
+int launch_merge_count(int32_t* bnd, int P, const int64_t* myC, uint32_t* hist, hipStream_t st) {
     const int64_t n = (int64_t)P * kMergeBins;
-    hipLaunchKernelGGL(merge_count_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bnd, P, hist);
+    hipLaunchKernelGGL(merge_count_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bnd, P, myC, hist);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
