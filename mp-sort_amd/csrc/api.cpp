@@ -106,7 +106,7 @@ struct DevCtx {
     int device = 0;
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;   // second stream (default priority): work that overlaps the splitter search
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, fork0 = nullptr, proposed = nullptr;
     uint32_t epoch = 0;
     Buf din, dout;      // host staging
     Buf s0, s1, rbuf;   // sort-record buffers
@@ -158,6 +158,8 @@ static int get_ctx(DevCtx** out) {
         MPSG_CHECK(hipStreamCreateWithFlags(&c.st2, hipStreamNonBlocking));
         MPSG_CHECK(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
         MPSG_CHECK(hipEventCreateWithFlags(&c.join, hipEventDisableTiming));
+        MPSG_CHECK(hipEventCreateWithFlags(&c.fork0, hipEventDisableTiming));
+        MPSG_CHECK(hipEventCreateWithFlags(&c.proposed, hipEventDisableTiming));
         for (int i = 0; i < kEvPhase; i++) MPSG_CHECK(hipEventCreate(&c.ev[i]));
         for (int a = 0; a < 2; a++)
             for (int i = 0; i < 2 * kMaxPasses; i++) MPSG_CHECK(hipEventCreate(&c.pass_ev[a][i]));
@@ -1016,6 +1018,13 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         for (int e = kEvFindP; e <= kEvSecond; e++) MPSG_CHECK(hipEventRecord(c.ev[e], st));
     } else {
+        // the model's splitter proposals need only the counts: on the second
+        // stream, during the FirstSort
+        const float delta = std::max(2.0f * M.err_max, 8.0f / (float)M.nsample);
+        MPSG_CHECK(hipEventRecord(c.fork0, st));
+        MPSG_CHECK(hipStreamWaitEvent(c.st2, c.fork0, 0));
+        if ((rc = launch_model_propose(md, M.kmax, h.C, P, n, delta, prop, c.st2))) return rc;
+        MPSG_CHECK(hipEventRecord(c.proposed, c.st2));
         if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, c.s0.p, c.s1.p,
                                      cs_direct ? h.checksum : nullptr, nullptr, &lc, st, evA)))
             return rc;
@@ -1042,9 +1051,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                 return rc;
             MPSG_CHECK(hipEventRecord(c.join, c.st2));
         }
-        // splitters: model proposals, checked and repaired by bisection
-        const float delta = std::max(2.0f * M.err_max, 8.0f / (float)M.nsample);
-        if ((rc = launch_model_propose(md, M.kmax, h.C, P, n, delta, prop, st))) return rc;
+        // splitters: the model's proposals, checked and repaired by bisection
+        MPSG_CHECK(hipStreamWaitEvent(st, c.proposed, 0));
         if ((rc = launch_bisect(c.s0.p, rw, cs.ks, pa.begin, pa.count, P, h.C, h.win, h.myclt, h.mycle, h.split,
                                 h.pminmax, h.rounds, h.err, st, prop, exact)))
             return rc;
