@@ -196,13 +196,21 @@ __global__ __launch_bounds__(1024) void lscan_kernel(const uint32_t* __restrict_
                                                      const int64_t* __restrict__ seg_begin,
                                                      int64_t* __restrict__ base, int nb) {
     __shared__ uint32_t s_tmp[16];
-    const int s = blockIdx.x, tid = threadIdx.x, per = nb / 1024;
+    const int s = blockIdx.x, tid = threadIdx.x, per = nb / 1024;  // 1 .. 4 buckets per thread
     const uint32_t* h = hist + (int64_t)s * nb + (int64_t)tid * per;
-    uint32_t c[2] = {h[0], per > 1 ? h[1] : 0u};
-    uint32_t ex = scan1024(c[0] + c[1], s_tmp);
+    uint32_t c[4], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        c[q] = q < per ? h[q] : 0u;
+        sum += c[q];
+    }
+    uint32_t ex = scan1024(sum, s_tmp);
     int64_t* bs = base + (int64_t)s * nb + (int64_t)tid * per;
-    bs[0] = seg_begin[s] + (int64_t)ex;
-    if (per > 1) bs[1] = seg_begin[s] + (int64_t)(ex + c[0]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q < per) bs[q] = seg_begin[s] + (int64_t)ex;
+        ex += c[q];
+    }
 }
 
 // offset of every (tile, bucket) run inside its segment: the bucket's start
@@ -1350,7 +1358,7 @@ int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int 
 }
 
 int launch_lscan(int nseg, const uint32_t* hist, const int64_t* seg_begin, int64_t* base, hipStream_t st, int nb) {
-    if (nb != 1024 && nb != 2048) return -1;
+    if (nb % 1024 != 0 || nb > 4096) return -1;
     hipLaunchKernelGGL(lscan_kernel, dim3(nseg), dim3(1024), 0, st, hist, seg_begin, base, nb);
     MPSG_CHECK(hipGetLastError());
     return 0;

@@ -352,7 +352,7 @@ __global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, co
     tables[g] = model_inverse_leaves(m, q);
 }
 
-// per table (one block each, nb <= 2048): running max so the table is sorted
+// per table (one block each, nb <= 4096): running max so the table is sorted
 // even if FP32 rounding made F locally non-monotone; then every heavy key v
 // (ascending) gets the boundary pair {v - 1, v}, i.e. a bucket holding v
 // alone.  Overwriting the two boundaries around v's position keeps the table
@@ -360,13 +360,19 @@ __global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, co
 // just after it, or is dropped when the table has no room left.
 __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__ tables, int ntables, int nb,
                                                           const uint64_t* __restrict__ heavy, int nheavy) {
-    __shared__ uint64_t s_t[2048];
+    constexpr int E = 4;  // entries per thread
+    __shared__ uint64_t s_t[1024 * E];
     __shared__ uint64_t s_w[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint64_t* tb = tables + (int64_t)blockIdx.x * nb;
-    // entries 2 tid and 2 tid + 1: their max, then a running max over threads
-    const uint64_t e0 = 2 * tid < nb ? tb[2 * tid] : 0, e1 = 2 * tid + 1 < nb ? tb[2 * tid + 1] : 0;
-    uint64_t v = e1 > e0 ? e1 : e0;
+    // entries E tid .. E tid + E - 1: their max, then a running max over
+    // threads
+    uint64_t e[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) e[q] = E * tid + q < nb ? tb[E * tid + q] : 0;
+    uint64_t v = 0;
+#pragma unroll
+    for (int q = 0; q < E; q++) v = e[q] > v ? e[q] : v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         const uint64_t u = __shfl_up(v, off, 64);
@@ -376,14 +382,15 @@ __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__
     __syncthreads();
     uint64_t pre = 0;
     for (int w = 0; w < wave; w++) pre = s_w[w] > pre ? s_w[w] : pre;
-    // running max before this thread's pair
+    // running max before this thread's entries
     uint64_t before = __shfl_up(v, 1, 64);
     if (lane == 0) before = 0;
     before = pre > before ? pre : before;
-    const uint64_t m0 = before > e0 ? before : e0;
-    const uint64_t m1 = m0 > e1 ? m0 : e1;
-    if (2 * tid < nb) s_t[2 * tid] = m0;
-    if (2 * tid + 1 < nb) s_t[2 * tid + 1] = m1;
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        before = before > e[q] ? before : e[q];
+        if (E * tid + q < nb) s_t[E * tid + q] = before;
+    }
     __syncthreads();
     if (tid == 0 && nb >= 2) {
         int last = -1;  // slot of the previous heavy key
@@ -415,8 +422,9 @@ __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__
         }
     }
     __syncthreads();
-    if (2 * tid < nb) tb[2 * tid] = s_t[2 * tid];
-    if (2 * tid + 1 < nb) tb[2 * tid + 1] = s_t[2 * tid + 1];
+#pragma unroll
+    for (int q = 0; q < E; q++)
+        if (E * tid + q < nb) tb[E * tid + q] = s_t[E * tid + q];
 }
 
 // smallest key in [kmin, kmax] with F(key) >= q for a monotone F, found by
@@ -549,7 +557,7 @@ int launch_model_tables(const ModelDev& m, uint64_t kmax, const float* q0, const
     (void)kmax;
     hipLaunchKernelGGL(model_tables_kernel, dim3(nblk(tot, 128)), dim3(128), 0, st, m, q0, q1, ntables, nb, tables);
     MPSG_CHECK(hipGetLastError());
-    if (nb > 2048) return -1;
+    if (nb > 4096) return -1;
     hipLaunchKernelGGL(tables_fix_kernel, dim3(ntables), dim3(1024), 0, st, tables, ntables, nb, heavy, nheavy);
     MPSG_CHECK(hipGetLastError());
     return 0;
