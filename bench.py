@@ -34,10 +34,10 @@ PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_
 # (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
 TRAFFIC_ONESWEEP = 9125634510
 TRAFFIC_BUCKET_SORT = 9380352928
-TRAFFIC_MERGE_BUCKET = 9256752672
 # per-side stages of the learned local sort, as timed by the library; with
-# the merged SecondSort also its gather-and-sort launch alone (merge_B),
-# which bucket_sort_B contains together with the grouping of the buckets
+# the merged SecondSort also its two gather-and-sort launches alone
+# (merge_B: groups of buckets, then the larger single buckets), which
+# bucket_sort_B contains together with the grouping of the buckets
 LEARNED_KERNELS = ["hist_A", "scatter_A", "bucket_sort_A", "lds_radix_A", "fallback_A",
                    "hist_B", "scatter_B", "bucket_sort_B", "lds_radix_B", "fallback_B",
                    "merge_B"]
@@ -433,23 +433,13 @@ def main():
     plain_roof = roof([x for st in st_plain for x in st["pass_ms"]],
                       "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP)
     if result_learned is not None:
-        # the learned step's dominant kernel, one launch per local sort over
-        # all n records: the FirstSort's in-LDS bucket sort or the merged
-        # SecondSort's gather-and-sort, whichever takes longer
         ib = LEARNED_KERNELS.index
-        merged = len(st_learned[0]["pass_ms"]) >= len(LEARNED_KERNELS)
-        # launches of each candidate per step, algorithmic bytes per launch
-        cand = {"interp": (["bucket_sort_A"], "interp_sort_kernel<4,512,9> (FirstSort in-LDS bucket sort, one "
-                                               "launch per step over all n records: 2*n*16 B per launch)",
-                           TRAFFIC_BUCKET_SORT, alg_bytes)}
-        if merged:
-            cand["merge"] = (["merge_B"],
-                             "merge_bucket_kernel<4> (SecondSort gather of the senders' sub-runs and in-LDS "
-                             "sort of groups of neighbouring buckets, one launch per step over all n records: "
-                             "2*n*16 B per launch)", TRAFFIC_MERGE_BUCKET, alg_bytes)
-        dom = max(cand, key=lambda c: sum(st["pass_ms"][ib(k)] for st in st_learned for k in cand[c][0]))
-        names, kname, traffic, lbytes = cand[dom]
-        roofline = roof([st["pass_ms"][ib(k)] for st in st_learned for k in names], kname, traffic, lbytes)
+        # the step's largest kernel: the FirstSort's in-LDS bucket sort, one
+        # launch per step over all n records (the merged SecondSort splits
+        # its records over two smaller kernels, merge_B in kernels_ms)
+        roofline = roof([st["pass_ms"][ib("bucket_sort_A")] for st in st_learned],
+                        "interp_sort_kernel<4,512,9> (FirstSort in-LDS bucket sort, one launch per step over all n "
+                        "records: 2*n*16 B per launch)", TRAFFIC_BUCKET_SORT)
         result_learned["roofline_by_stage"] = {
             k: {"avg_ms": round(sum(st["pass_ms"][ib(k)] for st in st_learned) / len(st_learned), 4),
                 "GB/s": round((alg_bytes / 2 if k.startswith("hist") else alg_bytes) /

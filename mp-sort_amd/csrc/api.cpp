@@ -654,11 +654,13 @@ struct LearnedSide {
     uint32_t* tickets;                // 4
     uint32_t* err;                    // 4
     int64_t* overflow;                // 2 * kMaxOverflowList
-    uint32_t* noverflow;              // 4: [0] oversized buckets, [1] buckets for the LDS radix sort,
-                                      //    [2] medium, [3] groups (merged SecondSort)
+    uint32_t* noverflow;              // 8: [0] oversized buckets, [1] buckets for the LDS radix sort,
+                                      //    [2] medium, [3] groups, [4] larger single buckets
+                                      //    (merged SecondSort)
     uint32_t* hard;                   // nseg * buckets per segment
     uint32_t* medium;                 // nseg * buckets per segment (merged SecondSort)
-    uint32_t* tasks;                  // nseg * buckets per segment (merged SecondSort groups)
+    uint32_t* tasks;                  // 2 * nseg * buckets per segment (merged SecondSort groups, then
+                                      // larger single buckets)
     int64_t tstride = kLBounds;       // segment s uses tables + s * tstride (0: one shared table)
 };
 
@@ -677,10 +679,10 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
         ls->tickets = a.template take<uint32_t>(4);
         ls->err = a.template take<uint32_t>(4);
         ls->overflow = a.template take<int64_t>(2 * (size_t)kMaxOverflowList);
-        ls->noverflow = a.template take<uint32_t>(4);
+        ls->noverflow = a.template take<uint32_t>(8);
         ls->hard = a.template take<uint32_t>((size_t)P * kMergeBins);
         ls->medium = a.template take<uint32_t>((size_t)P * kMergeBins);
-        ls->tasks = a.template take<uint32_t>((size_t)P * kMergeBins);
+        ls->tasks = a.template take<uint32_t>(2 * (size_t)P * kMergeBins);
     } else {
         a.template take<int64_t>(P);
         a.template take<int64_t>(P);
@@ -693,10 +695,10 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
         a.template take<uint32_t>(4);
         a.template take<uint32_t>(4);
         a.template take<int64_t>(2 * (size_t)kMaxOverflowList);
-        a.template take<uint32_t>(4);
+        a.template take<uint32_t>(8);
         a.template take<uint32_t>((size_t)P * kMergeBins);
         a.template take<uint32_t>((size_t)P * kMergeBins);
-        a.template take<uint32_t>((size_t)P * kMergeBins);
+        a.template take<uint32_t>(2 * (size_t)P * kMergeBins);
     }
 }
 
@@ -1077,16 +1079,18 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             MPSG_CHECK(hipStreamWaitEvent(st, c.join, 0));
             if ((rc = launch_merge_count(bnd, P, h.myC, LB.lhist, st))) return rc;
             if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st, kMergeBins))) return rc;
-            if ((rc = launch_fill_u32(LB.noverflow, 0u, 4, st))) return rc;
+            if ((rc = launch_fill_u32(LB.noverflow, 0u, 8, st))) return rc;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
             if ((rc = markB(1)) || (rc = markB(2))) return rc;
-            // groups of neighbouring buckets, one workgroup each
-            uint32_t novh[4] = {0, 0, 0, 0};
+            // groups of neighbouring buckets, one workgroup each; buckets
+            // above the group capacity alone, in the larger form
+            uint32_t novh[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             if ((rc = launch_merge_group(LB.lhist, P, LB.tasks, LB.noverflow + 3, st))) return rc;
             MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
             MPSG_CHECK(hipStreamSynchronize(st));
-            // descriptors: group, then per sender sub-run start and position
-            const uint32_t ntasks = novh[3];
+            // descriptors (groups, then the larger buckets): group, then per
+            // sender sub-run start and position
+            const uint32_t nsmall = novh[3], nbig = novh[4], ntasks = nsmall + nbig;
             const size_t d_bytes = ((size_t)(ntasks ? ntasks : 1) * sizeof(MergeTask) + 255) & ~(size_t)255;
             const size_t b_bytes = (size_t)(ntasks ? ntasks : 1) * 64 * sizeof(int64_t);
             if ((rc = c.mdesc.ensure(d_bytes + b_bytes + (size_t)(ntasks ? ntasks : 1) * 64 * sizeof(int32_t))))
@@ -1094,15 +1098,23 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             MergeTask* desc = (MergeTask*)c.mdesc.p;
             int64_t* sub_beg = (int64_t*)((char*)c.mdesc.p + d_bytes);
             int32_t* sub_pref = (int32_t*)((char*)c.mdesc.p + d_bytes + b_bytes);
-            if ((rc = launch_merge_desc(LB.tasks, ntasks, bnd, pa.begin, P, LB.bbase, LB.tables, desc, sub_beg,
+            if ((rc = launch_merge_desc(LB.tasks, nsmall, bnd, pa.begin, P, LB.bbase, LB.tables, desc, sub_beg,
                                         sub_pref, st)))
                 return rc;
-            if ((rc = markB(6))) return rc;
-            if ((rc = launch_merge_bucket(rw, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist, LB.bbase,
-                                          LB.tables, desc, sub_beg, sub_pref, ntasks, LB.overflow, LB.noverflow,
-                                          kMaxOverflowList, LB.hard, LB.noverflow + 1, LB.medium, LB.noverflow + 2,
-                                          cs_direct ? h.checksum + 1 : nullptr, st)))
+            if ((rc = launch_merge_desc(LB.tasks + (size_t)P * kMergeBins, nbig, bnd, pa.begin, P, LB.bbase,
+                                        LB.tables, desc + nsmall, sub_beg + (size_t)nsmall * 64,
+                                        sub_pref + (size_t)nsmall * 64, st)))
                 return rc;
+            if ((rc = markB(6))) return rc;
+            for (int big = 0; big < 2; big++) {
+                const uint32_t o = big ? nsmall : 0;
+                if ((rc = launch_merge_bucket(rw, big != 0, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist,
+                                              LB.bbase, LB.tables, desc + o, sub_beg + (size_t)o * 64,
+                                              sub_pref + (size_t)o * 64, big ? nbig : nsmall, LB.overflow,
+                                              LB.noverflow, kMaxOverflowList, LB.hard, LB.noverflow + 1, LB.medium,
+                                              LB.noverflow + 2, cs_direct ? h.checksum + 1 : nullptr, st)))
+                    return rc;
+            }
             if ((rc = markB(7))) return rc;
             MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
             MPSG_CHECK(hipStreamSynchronize(st));

@@ -7,10 +7,13 @@ namespace mpsg {
 constexpr int kLBins = 1024;    // learned buckets per segment (10-bit digit)
 constexpr int kLBounds = 1023;  // boundaries per segment table -> buckets 0..1023
 constexpr int kChunkCap = 8192; // largest bucket sorted in LDS
-// merged SecondSort: buckets per receiver, table boundaries, in-LDS capacity
-constexpr int kMergeBins = 2048;
-constexpr int kMergeBounds = 2047;
-constexpr int kMergeCap = 4608;
+// merged SecondSort: buckets per receiver, table boundaries, in-LDS
+// capacity of a group of buckets (three workgroups per CU) and of a single
+// larger bucket (two per CU)
+constexpr int kMergeBins = 3072;
+constexpr int kMergeBounds = 3071;
+constexpr int kMergeCap = 3072;
+constexpr int kMergeBigCap = 4608;
 
 // key of a sort record as one uint64 (keys of at most 8 bytes): bytes
 // [sh/8, sh/8 + key_bytes) of words w0..w0+2, masked, XOR-ed with the sign
@@ -69,7 +72,9 @@ int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t*
                         int P, const uint64_t* tables, int32_t* bnd, hipStream_t st);
 int launch_merge_count(int32_t* bnd, int P, const int64_t* myC, uint32_t* hist, hipStream_t st);
 // groups of neighbouring receiver buckets (together at most kMergeCap
-// records): tasks (up to P * kMergeBins entries), *ntasks (zeroed before)
+// records) in tasks[0 .. ntasks[0]), and the larger single buckets in
+// tasks[P * kMergeBins .. + ntasks[1]) (tasks holds 2 * P * kMergeBins
+// entries, ntasks zeroed before)
 int launch_merge_group(const uint32_t* hist, int P, uint32_t* tasks, uint32_t* ntasks, hipStream_t st);
 // a group's first bucket, its size and the table bounds of its keys
 struct MergeTask {
@@ -88,7 +93,8 @@ int launch_merge_desc(const uint32_t* tasks, uint32_t ntasks, const int32_t* bnd
 // one workgroup per group.  Buckets above kMergeCap (up to kChunkCap) are
 // copied in sender order and listed in medium (*nmedium entries) for
 // launch_listed_sort.
-int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
+// big: the single buckets above kMergeCap (sorted up to kMergeBigCap)
+int launch_merge_bucket(int rw, bool big, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
                         const uint64_t* tables, const MergeTask* desc, const int64_t* sub_beg,
                         const int32_t* sub_pref, uint32_t ntasks, int64_t* overflow, uint32_t* noverflow,

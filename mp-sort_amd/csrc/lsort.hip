@@ -532,13 +532,14 @@ constexpr int kSamp = 64;
 constexpr int kInterpMax = 32;
 // Workgroup shapes of the in-LDS sort: 1024 threads x 8 records (one
 // workgroup per CU) or 512 threads x 9 records (two share a CU), in 4095
-// bins (65 sample intervals x 63).  ITEMS records per thread (CAP = BLOCK *
+// bins (65 sample intervals x 63), or 512 threads x 6 records (three share a
+// CU) in 2015 bins (65 x 31).  ITEMS records per thread (CAP = BLOCK *
 // ITEMS).  The bin arrays take BW words of LDS: NB u16 bin starts, then NB
 // u16 bin counters, two to a word.
 template <int BLOCK, int ITEMS = kCItems>
 struct BSort {
     static constexpr int B = BLOCK, W = BLOCK / 64, I = ITEMS, CAP = BLOCK * ITEMS;
-    static constexpr int SUB = 63;                               // bins per sample interval
+    static constexpr int SUB = CAP > 3072 ? 63 : 31;             // bins per sample interval
     static constexpr int FINE = (kSamp + 1) * SUB;               // bins
     static constexpr int NB = (FINE + 1 + BLOCK - 1) / BLOCK * BLOCK;  // bin counters
     static constexpr int BW = NB;                                // words of the bin arrays
@@ -994,13 +995,14 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
 // SecondSort without the exchange.  Receiver r's records from sender j are
 // positions [myC[j][r], myC[j][r+1]) of sender j's sorted segment; bucket k of
 // r holds the keys in (t_r[k-1], t_r[k]], i.e. one sub-run per sender.  Every
-// receiver has kMergeBins buckets (tables of kMergeBounds boundaries), twice
-// as many as a FirstSort segment, because the model's boundaries at this
-// resolution leave bucket sizes spread widely.  Runs of neighbouring buckets
-// of a receiver are grouped greedily (up to kGroupMax buckets, together at
-// most kMergeCap records: their sub-runs are adjacent in every sender) and a
-// kMergeBlock-thread workgroup (kSmallItems records per thread, two share a
-// CU) sorts each group in one pass.
+// receiver has kMergeBins buckets (tables of kMergeBounds boundaries), three
+// times as many as a FirstSort segment, because the model's boundaries at
+// this resolution leave bucket sizes spread widely.  Runs of neighbouring
+// buckets of a receiver are grouped greedily (up to kGroupMax buckets,
+// together at most kMergeCap records: their sub-runs are adjacent in every
+// sender) and sorted in one pass by a kMergeBlock-thread workgroup of
+// kMergeItems records per thread (three share a CU); a bucket above
+// kMergeCap is sorted alone by kSmallItems records per thread (two per CU).
 //   merge_bounds_kernel  bnd[r][k][j] = start of bucket k's sub-run in sender
 //                        j (relative to j's segment), k = 0..kMergeBins (the
 //                        P senders of one bucket side by side)
@@ -1016,10 +1018,12 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
 //                        sort, the 1024-thread in-LDS sort or the radix path
 constexpr int kMergeMaxP = 64;
 constexpr int kGroupMax = 8;     // buckets per group (task code: bucket << 3 | count - 1)
-constexpr int kGroupChunk = 32;  // groups never cross a chunk of this many buckets
+constexpr int kGroupChunk = 48;  // groups never cross a chunk of this many buckets
 constexpr int kMergeBlock = 512;
 constexpr int kSmallItems = 9;  // FirstSort buckets up to 4608 records in 512-thread workgroups
-static_assert(BSort<kMergeBlock, kSmallItems>::CAP == kMergeCap, "merged SecondSort capacity");
+constexpr int kMergeItems = 6;  // groups up to 3072 records in 512-thread workgroups, three per CU
+static_assert(BSort<kMergeBlock, kMergeItems>::CAP == kMergeCap, "merged SecondSort capacity");
+static_assert(BSort<kMergeBlock, kSmallItems>::CAP == kMergeBigCap, "merged SecondSort capacity");
 
 template <int RW>
 // Positions in the whole sorted segment of every sender (the receivers'
@@ -1059,26 +1063,30 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
 }
 
 // The bounds clamped to receiver r's window [myC[j][r], myC[j][r+1]) in
-// every sender j (written back: row k by thread (r, k), the last row also by
-// thread (r, kMergeBins - 1); a row read while its owner writes it clamps to
-// the same value), and the bucket sizes
+// every sender j (written back: row k by the wave of (r, k), the last row
+// also by the wave of (r, kMergeBins - 1); a row read while its owner writes
+// it clamps to the same value), and the bucket sizes.  One wave per (r, k),
+// lane j for sender j.
 __global__ __launch_bounds__(256) void merge_count_kernel(int32_t* __restrict__ bnd, int P,
                                                           const int64_t* __restrict__ myC,
                                                           uint32_t* __restrict__ hist) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, k)
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // (r, k)
+    const int j = threadIdx.x & 63;
     if (g >= (int64_t)P * kMergeBins) return;
     const int k = (int)(g % kMergeBins), r = (int)(g / kMergeBins);
     int32_t* bk = bnd + ((int64_t)r * (kMergeBins + 1) + k) * P;
-    uint32_t c = 0;
-    for (int j = 0; j < P; j++) {
+    int c = 0;
+    if (j < P) {
         const int64_t* cj = myC + (int64_t)j * (P + 1);
         const int32_t lo0 = (int32_t)cj[r], hi0 = (int32_t)cj[r + 1];
         const int32_t a = min(max(bk[j], lo0), hi0), b = min(max(bk[P + j], lo0), hi0);
-        c += (uint32_t)(b - a);
+        c = b - a;
         bk[j] = a;
         if (k == kMergeBins - 1) bk[P + j] = b;
     }
-    hist[g] = c;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (j == 0) hist[g] = (uint32_t)c;
 }
 
 // groups of the merged SecondSort: tasks[] (receiver by receiver, in bucket
@@ -1093,34 +1101,52 @@ __global__ __launch_bounds__(64) void merge_group_kernel(const uint32_t* __restr
     const uint32_t* h = hist + (int64_t)r * kMergeBins;
     for (int k = lane; k < kMergeBins; k += 64) s_cnt[k] = h[k];
     __syncthreads();
-    // the greedy groups of chunk `lane`, counted (out == nullptr) or written
-    auto walk = [&](uint32_t* out) -> uint32_t {
+    // the greedy groups of chunk `lane` (counted when out == nullptr, else
+    // written): buckets above kMergeCap alone to the second list
+    auto walk = [&](uint32_t* out, uint32_t* out_big, uint32_t& nbig) -> uint32_t {
         uint32_t n = 0;
+        nbig = 0;
         if (lane < NCH) {
             const int end = (lane + 1) * kGroupChunk;
             for (int k = lane * kGroupChunk; k < end;) {
+                const uint32_t code0 = (uint32_t)(((int64_t)r * kMergeBins + k) << 3);
                 uint32_t sum = s_cnt[k];
+                if (sum > (uint32_t)kMergeCap) {
+                    if (out_big) out_big[nbig] = code0;
+                    nbig++;
+                    k++;
+                    continue;
+                }
                 int len = 1;
                 while (k + len < end && len < kGroupMax && sum + s_cnt[k + len] <= (uint32_t)kMergeCap)
                     sum += s_cnt[k + len++];
-                if (out) out[n] = (uint32_t)((((int64_t)r * kMergeBins + k) << 3) | (len - 1));
+                if (out) out[n] = code0 | (uint32_t)(len - 1);
                 n++;
                 k += len;
             }
         }
         return n;
     };
-    const uint32_t n = walk(nullptr);
-    uint32_t x = n;
+    uint32_t nb = 0;
+    const uint32_t n = walk(nullptr, nullptr, nb);
+    uint32_t x = n, xb = nb;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
+        const uint32_t y = __shfl_up(x, off, 64), yb = __shfl_up(xb, off, 64);
+        if (lane >= off) {
+            x += y;
+            xb += yb;
+        }
     }
-    uint32_t base = 0;
-    if (lane == 63) base = atomicAdd(ntasks, x);
+    uint32_t base = 0, base_b = 0;
+    if (lane == 63) {
+        base = atomicAdd(&ntasks[0], x);
+        base_b = atomicAdd(&ntasks[1], xb);
+    }
     base = __shfl(base, 63, 64);
-    walk(tasks + base + (x - n));
+    base_b = __shfl(base_b, 63, 64);
+    uint32_t nb2 = 0;
+    walk(tasks + base + (x - n), tasks + (int64_t)gridDim.x * kMergeBins + base_b + (xb - nb), nb2);
 }
 
 // sub-runs of receiver r's buckets [ka, kb) in sender order: s_beg[j] =
@@ -1198,8 +1224,8 @@ __global__ __launch_bounds__(256) void merge_desc_kernel(const uint32_t* __restr
 }
 
 // One workgroup per group: task = first bucket << 3 | (buckets - 1).
-template <int RW>
-__global__ __launch_bounds__(kMergeBlock, 2048 / kMergeBlock) void merge_bucket_kernel(
+template <int RW, int ITEMS>
+__global__ __launch_bounds__(kMergeBlock, ITEMS == kMergeItems ? 6 : 4) void merge_bucket_kernel(
     const uint32_t* __restrict__ recs, uint32_t* __restrict__ dst, LKey lk,
     const int64_t* __restrict__ seg_begin, const int32_t* __restrict__ bnd, int P,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
@@ -1207,7 +1233,7 @@ __global__ __launch_bounds__(kMergeBlock, 2048 / kMergeBlock) void merge_bucket_
     int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard,
     uint32_t* __restrict__ nhard, uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium,
     long long* __restrict__ checksum_out) {
-    using C = BSort<kMergeBlock, kSmallItems>;
+    using C = BSort<kMergeBlock, ITEMS>;
     __shared__ int64_t s_beg[kMergeMaxP];
     __shared__ int s_pref[kMergeMaxP + 1];
     __shared__ long long s_sum2[C::W];
@@ -1232,7 +1258,7 @@ __global__ __launch_bounds__(kMergeBlock, 2048 / kMergeBlock) void merge_bucket_
     // the keys of buckets [ka, kb) lie in (t[ka-1], t[kb-1]]
     const bool bounded = ka > 0 && kb < kMergeBins && tlo < thi;
     if (!single && count <= C::CAP &&
-        sort_bucket_lds<RW, kMergeBlock, GatherSrc<RW>, kSmallItems>(gs, dst, base, count, lk, s_bins, checksum_out,
+        sort_bucket_lds<RW, kMergeBlock, GatherSrc<RW>, ITEMS>(gs, dst, base, count, lk, s_bins, checksum_out,
                                                                      bounded, bounded ? tlo + 1 : 0,
                                                                      bounded ? thi : ~0ull))
         return;
@@ -1471,8 +1497,8 @@ int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t*
 }
 
 int launch_merge_count(int32_t* bnd, int P, const int64_t* myC, uint32_t* hist, hipStream_t st) {
-    const int64_t n = (int64_t)P * kMergeBins;
-    hipLaunchKernelGGL(merge_count_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bnd, P, myC, hist);
+    const int64_t n = (int64_t)P * kMergeBins;  // one wave each
+    hipLaunchKernelGGL(merge_count_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, bnd, P, myC, hist);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
@@ -1495,7 +1521,7 @@ int launch_merge_desc(const uint32_t* tasks, uint32_t ntasks, const int32_t* bnd
     return 0;
 }
 
-int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
+int launch_merge_bucket(int rw, bool big, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
                         const uint64_t* tables, const MergeTask* desc, const int64_t* sub_beg,
                         const int32_t* sub_pref, uint32_t ntasks, int64_t* overflow, uint32_t* noverflow,
@@ -1505,11 +1531,15 @@ int launch_merge_bucket(int rw, const void* recs, void* dst, const LKey& lk, con
     if (ntasks == 0) return 0;
     const uint32_t* r = (const uint32_t*)recs;
     uint32_t* d = (uint32_t*)dst;
-    switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bucket_kernel<2>, dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, desc, sub_beg, sub_pref, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
-        case 4: hipLaunchKernelGGL(merge_bucket_kernel<4>, dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, desc, sub_beg, sub_pref, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum); break;
+#define MPSG_MERGE(R, I) hipLaunchKernelGGL((merge_bucket_kernel<R, I>), dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, desc, sub_beg, sub_pref, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum)
+    switch (rw * 2 + (big ? 1 : 0)) {
+        case 4: MPSG_MERGE(2, kMergeItems); break;
+        case 5: MPSG_MERGE(2, kSmallItems); break;
+        case 8: MPSG_MERGE(4, kMergeItems); break;
+        case 9: MPSG_MERGE(4, kSmallItems); break;
         default: return -1;
     }
+#undef MPSG_MERGE
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
