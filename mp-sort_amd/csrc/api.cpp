@@ -706,6 +706,7 @@ struct LearnedCounters {
     int hard_buckets = 0;
     int overflow_buckets = 0;
     int global_fallback = 0;
+    int unbucketed = 0;  // sides sorted without the bucket histogram (no bucket bases)
 };
 
 // Radix sort of the listed segments (device list of nov {begin, count}
@@ -797,6 +798,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         const int64_t big = *std::max_element(cnt.begin(), cnt.end());
         if (big > (int64_t)kLBins * kChunkCap / 2) {
             lc->global_fallback++;
+            lc->unbucketed++;
             if ((rc = mark(0))) return rc;
             int passes = 0;
             SortBuffers b{const_cast<void*>(src), dst, tmp};
@@ -1049,7 +1051,10 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             MPSG_CHECK(hipStreamWaitEvent(c.st2, c.fork, 0));
             if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, c.st2)))
                 return rc;
-            if ((rc = launch_merge_bounds(rw, c.s0.p, lkB, pa.begin, pa.count, P, LB.tables, bnd, c.st2)))
+            // the FirstSort's bucket bases narrow every search to one bucket
+            const bool narrow = lc.unbucketed == 0;
+            if ((rc = launch_merge_bounds(rw, c.s0.p, lkB, pa.begin, pa.count, P, LB.tables,
+                                          narrow ? LA.tables : nullptr, narrow ? LA.bbase : nullptr, bnd, c.st2)))
                 return rc;
             MPSG_CHECK(hipEventRecord(c.join, c.st2));
         }

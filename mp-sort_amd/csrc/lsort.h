@@ -67,9 +67,12 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
 // P <= 64 receivers with kMergeBounds-boundary tables each:
 //   bnd: P * P * (kMergeBins + 1) int32; hist / bucket_base: P * kMergeBins
 // positions of the table entries in the senders' whole segments (needs no
-// splitters), then clamped to the receivers' windows myC with the sizes
+// splitters), then clamped to the receivers' windows myC with the sizes.
+// With the FirstSort's shared table ftab (kLBounds entries) and its bucket
+// bases fbase (absolute, kLBins per segment), each search covers one bucket.
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
-                        int P, const uint64_t* tables, int32_t* bnd, hipStream_t st);
+                        int P, const uint64_t* tables, const uint64_t* ftab, const int64_t* fbase, int32_t* bnd,
+                        hipStream_t st);
 int launch_merge_count(int32_t* bnd, int P, const int64_t* myC, uint32_t* hist, hipStream_t st);
 // groups of neighbouring receiver buckets (together at most kMergeCap
 // records) in tasks[0 .. ntasks[0]), and the larger single buckets in

@@ -1034,6 +1034,8 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
                                                            const int64_t* __restrict__ seg_begin,
                                                            const int64_t* __restrict__ seg_count, int P,
                                                            const uint64_t* __restrict__ tables,
+                                                           const uint64_t* __restrict__ ftab,
+                                                           const int64_t* __restrict__ fbase,
                                                            int32_t* __restrict__ bnd) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, j, k), k = 0..kMergeBins
     if (g >= (int64_t)P * P * (kMergeBins + 1)) return;
@@ -1051,6 +1053,18 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
         const uint64_t t = tables[(int64_t)r * kMergeBounds + (k - 1)];
         const int64_t sb = seg_begin[j];
         int64_t a = 0, e = cnt;
+        if (ftab) {
+            // it lies in FirstSort bucket m = #{ftab entries < t}, the keys in
+            // (ftab[m-1], ftab[m]]: every key of earlier buckets is <= t,
+            // every key of later ones > t
+            int m = 0;
+#pragma unroll
+            for (int step = 512; step > 0; step >>= 1)
+                if (m + step <= kLBounds && ftab[m + step - 1] < t) m += step;
+            const int64_t* fb = fbase + (int64_t)j * kLBins;
+            a = fb[m] - sb;
+            e = m + 1 < kLBins ? fb[m + 1] - sb : cnt;
+        }
         while (a < e) {
             const int64_t m = a + ((e - a) >> 1);
             uint32_t rr[RW];
@@ -1483,13 +1497,14 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
 }
 
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
-                        int P, const uint64_t* tables, int32_t* bnd, hipStream_t st) {
+                        int P, const uint64_t* tables, const uint64_t* ftab, const int64_t* fbase, int32_t* bnd,
+                        hipStream_t st) {
     const int64_t n = (int64_t)P * P * (kMergeBins + 1);
     const unsigned g = (unsigned)((n + 255) / 256);
     const uint32_t* r = (const uint32_t*)recs;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, bnd); break;
-        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, bnd); break;
+        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, fbase, bnd); break;
+        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, fbase, bnd); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
