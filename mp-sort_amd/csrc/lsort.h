@@ -93,10 +93,11 @@ struct MergeTask {
 int launch_merge_desc(const uint32_t* tasks, uint32_t ntasks, const int32_t* bnd, const int64_t* seg_begin, int P,
                       const int64_t* bucket_base, const uint64_t* tables, MergeTask* desc, int64_t* sub_beg,
                       int32_t* sub_pref, hipStream_t st);
-// one workgroup per group.  Buckets above kMergeCap (up to kChunkCap) are
+// one workgroup per task of the list: big = false for the groups (up to
+// kMergeCap records), true for the single larger buckets, which are sorted
+// up to kMergeBigCap records; buckets above that (up to kChunkCap) are
 // copied in sender order and listed in medium (*nmedium entries) for
 // launch_listed_sort.
-// big: the single buckets above kMergeCap (sorted up to kMergeBigCap)
 int launch_merge_bucket(int rw, bool big, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
                         const uint64_t* tables, const MergeTask* desc, const int64_t* sub_beg,
