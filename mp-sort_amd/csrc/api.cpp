@@ -1035,7 +1035,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n;
         // the merged form uses kMergeBins buckets per receiver
-        const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 32);
+        // (positions inside a sender's segment are kept as int32)
+        const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 31);
         merged_side = merged;
         const int nbB = merged ? kMergeBounds : kLBounds;
         LB.tstride = nbB;
