@@ -1115,7 +1115,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             for (int big = 0; big < 2; big++) {
                 const uint32_t o = big ? nsmall : 0;
                 if ((rc = launch_merge_bucket(rw, big != 0, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist,
-                                              LB.bbase, LB.tables, desc + o, sub_beg + (size_t)o * 64,
+                                              LB.bbase, desc + o, sub_beg + (size_t)o * 64,
                                               sub_pref + (size_t)o * 64, big ? nbig : nsmall, LB.overflow,
                                               LB.noverflow, kMaxOverflowList, LB.hard, LB.noverflow + 1, LB.medium,
                                               LB.noverflow + 2, cs_direct ? h.checksum + 1 : nullptr, st)))

@@ -1538,7 +1538,7 @@ int launch_merge_desc(const uint32_t* tasks, uint32_t ntasks, const int32_t* bnd
 
 int launch_merge_bucket(int rw, bool big, const void* recs, void* dst, const LKey& lk, const int64_t* seg_begin,
                         const int32_t* bnd, int P, const uint32_t* hist, const int64_t* bucket_base,
-                        const uint64_t* tables, const MergeTask* desc, const int64_t* sub_beg,
+                        const MergeTask* desc, const int64_t* sub_beg,
                         const int32_t* sub_pref, uint32_t ntasks, int64_t* overflow, uint32_t* noverflow,
                         int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium,
                         long long* checksum, hipStream_t st) {

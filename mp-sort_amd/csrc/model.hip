@@ -360,6 +360,7 @@ __global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, co
 // just after it, or is dropped when the table has no room left.
 __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__ tables, int ntables, int nb,
                                                           const uint64_t* __restrict__ heavy, int nheavy) {
+    (void)ntables;  // one block per table
     constexpr int E = 4;  // entries per thread
     __shared__ uint64_t s_t[1024 * E];
     __shared__ uint64_t s_w[16];
