@@ -152,4 +152,16 @@ int launch_add_u32(uint32_t* p, uint32_t v, hipStream_t st);  // *p += v on the 
 // epoch counter for the look-back flags (process wide, per device)
 uint32_t next_epoch(int device);
 
+// workgroups of kernel f (threads each) resident at once on the current
+// device; 2048 when the runtime cannot say
+template <typename F>
+inline int64_t resident_blocks(F f, int threads) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, threads, 0) != hipSuccess || cus <= 0 || per <= 0)
+        return 2048;
+    return (int64_t)cus * per;
+}
+
 }  // namespace mpsg

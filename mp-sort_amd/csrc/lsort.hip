@@ -65,12 +65,17 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
     __shared__ uint32_t s_hist[kLBins];  // the block's counts for its current segment
     __shared__ uint32_t s_th[kLBins];    // counts of the current tile
     __shared__ uint64_t s_tab[kLBounds];
+    // guide: cell c of the keys kmin + [c << gsh, (c + 1) << gsh) holds
+    // buckets s_guide[c] .. s_guide[c + 1] only (kmin, kmax: the table's ends)
+    __shared__ uint16_t s_guide[kGuideCells + 1];
     __shared__ long long s_sum[kWaves];
     const int tid = threadIdx.x;
     for (int i = tid; i < kLBins; i += kBlock) {
         s_hist[i] = 0;
         s_th[i] = 0;
     }
+    uint64_t kmin = 0, krange = 0;
+    int gsh = 0;
     const int64_t t0 = (int64_t)blockIdx.x * tpb;
     const int64_t t1 = min(ntiles, t0 + tpb);
     long long csum = 0;
@@ -87,8 +92,18 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
                     s_hist[i] = 0;
                 }
             }
-            for (int i = tid; i < kLBounds; i += kBlock) s_tab[i] = tables[(int64_t)s * tstride + i];
-            __syncthreads();
+            if (cur < 0 || tstride != 0) {
+                for (int i = tid; i < kLBounds; i += kBlock) s_tab[i] = tables[(int64_t)s * tstride + i];
+                __syncthreads();
+                kmin = s_tab[0];
+                krange = s_tab[kLBounds - 1] - kmin;
+                gsh = krange ? max(0, 64 - __clzll((long long)krange) - kGuideBits) : 0;
+                for (int c = tid; c <= kGuideCells; c += kBlock) {
+                    const uint64_t off = (uint64_t)c << gsh;  // c << gsh < 2^64: gsh <= 64 - kGuideBits
+                    s_guide[c] = (uint16_t)(c == kGuideCells || off > krange ? kLBounds : bucket_of(kmin + off, s_tab));
+                }
+                __syncthreads();
+            }
             cur = s;
         }
         const int64_t t = tile - seg.tile_start[s];
@@ -109,23 +124,42 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
                     for (int w = 0; w < RW; w++) r[u][w] = 0;
                 }
             }
-            // the U table searches advance level by level (each level's U
-            // reads in flight together); positions past the tile search for
-            // key 0 and are ignored
+            // the guide narrows each search to the buckets of the key's cell
+            // (lo .. lo + cnt); the U lower-bound searches then advance
+            // together, their reads in flight at once.  Positions past the
+            // tile search for key 0 and are ignored.
             uint64_t kq[U];
-            int lo[U];
+            int lo[U], cnt[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 kq[u] = lkey<RW>(r[u], lk);
-                lo[u] = 0;
+                const uint64_t d = kq[u] - kmin;
+                const bool inside = kq[u] >= kmin && d <= krange;
+                const int c = inside ? (int)(d >> gsh) : 0;  // < kGuideCells
+                const int g0 = s_guide[c], g1 = s_guide[c + 1];
+                lo[u] = inside ? g0 : (kq[u] < kmin ? 0 : kLBounds);
+                cnt[u] = inside ? g1 - g0 : 0;
             }
+            for (;;) {
+                bool more = false;
 #pragma unroll
-            for (int step = 512; step > 0; step >>= 1) {
+                for (int u = 0; u < U; u++) more |= cnt[u] > 0;
+                if (!more) break;
                 uint64_t v[U];
 #pragma unroll
-                for (int u = 0; u < U; u++) v[u] = s_tab[lo[u] + step - 1];
+                for (int u = 0; u < U; u++) v[u] = s_tab[min(lo[u] + (cnt[u] >> 1), kLBounds - 1)];
 #pragma unroll
-                for (int u = 0; u < U; u++) lo[u] += v[u] < kq[u] ? step : 0;
+                for (int u = 0; u < U; u++) {
+                    const int half = cnt[u] >> 1;
+                    if (cnt[u] > 0) {
+                        if (v[u] < kq[u]) {
+                            lo[u] += half + 1;
+                            cnt[u] -= half + 1;
+                        } else {
+                            cnt[u] = half;
+                        }
+                    }
+                }
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -1385,7 +1419,9 @@ int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int 
                  const uint64_t* tables, int64_t tstride, uint32_t* hist, uint16_t* tile_cnt, long long* checksum,
                  uint16_t* bkt, hipStream_t st) {
     if (ntiles <= 0) return 0;
-    const int64_t tpb = (ntiles + 2047) / 2048;
+    // one round of resident workgroups: each takes an equal run of tiles
+    const int64_t slots = rw == 2 ? resident_blocks(lhist_kernel<2>, kBlock) : resident_blocks(lhist_kernel<4>, kBlock);
+    const int64_t tpb = (ntiles + slots - 1) / slots;
     const int64_t blocks = (ntiles + tpb - 1) / tpb;
     const uint32_t* s = (const uint32_t*)src;
     switch (rw) {
