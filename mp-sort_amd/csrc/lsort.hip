@@ -578,6 +578,7 @@ struct BSort {
     static constexpr int NB = (FINE + 1 + BLOCK - 1) / BLOCK * BLOCK;  // bin counters
     static constexpr int BW = NB;                                // words of the bin arrays
     static constexpr int TPS = BLOCK / kSamp;                    // threads per sample in the sample sort
+    static constexpr int RU = ITEMS >= 8 ? 8 : 4;                // keys per LDS read group in the ranking
     static_assert(NB % 2 == 0 && kSamp % TPS == 0 && CAP < 65536, "in-LDS sort shape");
 };
 
@@ -641,7 +642,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
                                                 const LKey& lk, uint32_t* s_bins, long long* __restrict__ checksum_out,
                                                 bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull) {
     using C = BSort<BLOCK, ITEMS>;
-    __shared__ uint64_t s_gkey[C::CAP + 4];  // key of each bin-order slot (+4: unrolled reads)
+    __shared__ uint64_t s_gkey[C::CAP + 8];  // key of each bin-order slot (+8: unrolled reads)
     __shared__ uint16_t s_sbin[C::CAP];      // bin of each bin-order slot, then the final
                                              // slot of each bucket position (s_pos)
     __shared__ uint16_t s_ord[C::CAP];       // bucket position of each bin-order slot
@@ -853,10 +854,10 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     __syncthreads();
     // final slots, one bin-order slot per thread (threads of a wave mostly
     // share a bin, so the loops below stay coherent).  The keys of a bin are
-    // read four at a time; the up to three read past its end are keys of
+    // read RU at a time; the up to RU - 1 read past its end are keys of
     // later bins (larger) or the padding (all ones), which add no smaller
     // key.  Equal keys keep their bucket order.
-    if (tid < 4) s_gkey[count + tid] = ~0ull;
+    if (tid < 8) s_gkey[count + tid] = ~0ull;
     __syncthreads();
     // the slots' bins, positions, keys and bin bounds are read for all items
     // before the first is ranked, so the reads are in flight together
@@ -887,12 +888,12 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         } else {
             const uint64_t key = rk[k];
             uint32_t lt = 0, eq = 0;
-            for (uint32_t j = s0; j < e; j += 4) {
-                uint64_t k4[4];
+            for (uint32_t j = s0; j < e; j += C::RU) {
+                uint64_t k4[C::RU];
 #pragma unroll
-                for (int u = 0; u < 4; u++) k4[u] = s_gkey[j + u];
+                for (int u = 0; u < C::RU; u++) k4[u] = s_gkey[j + u];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < C::RU; u++) {
                     lt += k4[u] < key ? 1u : 0u;
                     eq += k4[u] == key ? 1u : 0u;
                 }
