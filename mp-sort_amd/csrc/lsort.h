@@ -8,7 +8,7 @@ constexpr int kLBins = 1024;    // learned buckets per segment (10-bit digit)
 constexpr int kLBounds = 1023;  // boundaries per segment table -> buckets 0..1023
 constexpr int kGuideBits = 12;  // bucket-search guide of the histogram pass: 4096 key cells
 constexpr int kGuideCells = 1 << kGuideBits;
-constexpr int kChunkCap = 8192; // largest bucket sorted in LDS
+constexpr int kChunkCap = 9216; // largest bucket sorted in LDS (1024 threads x 9)
 // merged SecondSort: buckets per receiver, table boundaries, in-LDS
 // capacity of a group of buckets (three workgroups per CU) and of a single
 // larger bucket (two per CU)

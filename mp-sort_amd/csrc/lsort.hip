@@ -578,7 +578,7 @@ struct BSort {
     static constexpr int NB = (FINE + 1 + BLOCK - 1) / BLOCK * BLOCK;  // bin counters
     static constexpr int BW = NB;                                // words of the bin arrays
     static constexpr int TPS = BLOCK / kSamp;                    // threads per sample in the sample sort
-    static constexpr int RU = ITEMS >= 8 ? 8 : 4;                // keys per LDS read group in the ranking
+    static constexpr int RU = ITEMS >= 8 && BLOCK <= 512 ? 8 : 4;  // keys per LDS read group in the ranking
     static_assert(NB % 2 == 0 && kSamp % TPS == 0 && CAP < 65536, "in-LDS sort shape");
 };
 

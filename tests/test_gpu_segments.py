@@ -120,7 +120,7 @@ def test_segments_tuned_plan(tmp_path, monkeypatch):
         plan.run()
         info = plan.info()
         note("tuned: %s" % info)
-        assert info['block'] in (256, 512, 1024) and 0 <= info['threshold'] <= 8192
+        assert info["block"] in (256, 512, 1024) and 0 <= info["threshold"] <= 9216
         g = numpy.empty_like(a)
         _lib.check(lib.mpsort_gpu_memcpy_d2h(g.ctypes.data, d, a.nbytes))
         plan.close()
