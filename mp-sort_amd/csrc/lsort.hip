@@ -564,7 +564,7 @@ __device__ __forceinline__ void lds_count_pass(const uint16_t* s_in, uint16_t* s
 // i % BLOCK.
 constexpr int kSamp = 64;
 constexpr int kInterpMax = 32;
-// Workgroup shapes of the in-LDS sort: 1024 threads x 8 records (one
+// Workgroup shapes of the in-LDS sort: 1024 threads x 9 records (one
 // workgroup per CU) or 512 threads x 9 records (two share a CU), in 4095
 // bins (65 sample intervals x 63), or 512 threads x 6 records (three share a
 // CU) in 2015 bins (65 x 31).  ITEMS records per thread (CAP = BLOCK *
