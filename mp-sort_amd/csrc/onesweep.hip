@@ -497,10 +497,8 @@ template <int RW>
 static void launch_hist(const SegSortPlan& pl, const void* src, int npass, const PassTable& pt,
                         long long* cs, hipStream_t st) {
     SegDev sd{pl.d_begin, pl.d_count, pl.d_tile_start, pl.nseg, 0};
-    // one round of resident workgroups, each over an equal contiguous run of
-    // tiles (a second, partial round would leave most CUs idle)
-    const int64_t slots = resident_blocks(hist_kernel<RW>, kBlock);
-    const int64_t tpb = (pl.ntiles + slots - 1) / slots;
+    // about 2048 blocks (8 per CU), each over a contiguous run of tiles
+    const int64_t tpb = (pl.ntiles + 2047) / 2048;
     const int64_t blocks = (pl.ntiles + tpb - 1) / tpb;
     hipLaunchKernelGGL(hist_kernel<RW>, dim3((unsigned)blocks), dim3(kBlock), 0, st,
                        (const uint32_t*)src, sd, pl.ntiles, tpb, pl.tile.block * pl.tile.items, npass,
