@@ -313,34 +313,57 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
     }
 }
 
-// one thread per receiving rank r (mpsort-mpi.c:663-727); also fills the
-// fixed columns 0 and P of the count matrices
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ long long wave_incl_scan_ll(long long v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// One wave per receiving rank r (mpsort-mpi.c:663-727), lane j for senders
+// j, j + 64, ...: the serial fill from sender 0 takes from sender j
+// min(supply_j, max(0, deficit - sum of the earlier supplies)) (a negative
+// supply takes nothing and, while some deficit remains, raises the error
+// flag, as in the serial loop).  Wave 0 also fills the fixed columns 0 and P
+// of the count matrices.
 __global__ void layout_kernel(int P, const int64_t* __restrict__ C, const int64_t* __restrict__ scnt,
                               int64_t* __restrict__ myclt, int64_t* __restrict__ mycle,
                               int64_t* __restrict__ myC, uint32_t* __restrict__ error_flag) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= P) return;
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= P) return;  // uniform per wave
     const int P1 = P + 1;
     const bool last = (r == P - 1);
     long long sure = 0;
-    for (int j = 0; j < P; j++) sure += last ? scnt[j] : myclt[(int64_t)j * P1 + r + 1];
-    long long deficit = C[r + 1] - sure;
-    if (deficit < 0) atomicOr(error_flag, 4u);
-    for (int j = 0; j < P; j++) {
-        const int64_t lt = last ? scnt[j] : myclt[(int64_t)j * P1 + r + 1];
-        const int64_t le = last ? scnt[j] : mycle[(int64_t)j * P1 + r + 1];
-        const int64_t supply = le - lt;
-        int64_t take = 0;
-        if (deficit > 0) {
-            if (supply < 0) atomicOr(error_flag, 4u);
-            take = supply <= deficit ? supply : deficit;
-            if (take < 0) take = 0;
-            deficit -= take;
+    for (int j = lane; j < P; j += 64) sure += last ? scnt[j] : myclt[(int64_t)j * P1 + r + 1];
+    sure = wave_sum_ll(sure);
+    const long long deficit = C[r + 1] - sure;
+    if (deficit < 0 && lane == 0) atomicOr(error_flag, 4u);
+    long long before = 0;  // supplies of the senders before this chunk
+    for (int j0 = 0; j0 < P; j0 += 64) {
+        const int j = j0 + lane;
+        int64_t lt = 0, le = 0;
+        if (j < P) {
+            lt = last ? scnt[j] : myclt[(int64_t)j * P1 + r + 1];
+            le = last ? scnt[j] : mycle[(int64_t)j * P1 + r + 1];
         }
-        myC[(int64_t)j * P1 + r + 1] = lt + take;
+        const long long supply = le - lt;
+        const long long sp = supply > 0 ? supply : 0;
+        const long long incl = wave_incl_scan_ll(sp);
+        long long rem = deficit - (before + incl - sp);
+        rem = rem > 0 ? rem : 0;
+        if (j < P) {
+            if (rem > 0 && supply < 0) atomicOr(error_flag, 4u);
+            myC[(int64_t)j * P1 + r + 1] = lt + (sp < rem ? sp : rem);
+        }
+        before += __shfl(incl, 63, 64);
     }
     if (r == 0) {
-        for (int j = 0; j < P; j++) {
+        for (int j = lane; j < P; j += 64) {
             myC[(int64_t)j * P1] = 0;
             myclt[(int64_t)j * P1] = 0;
             mycle[(int64_t)j * P1] = 0;
@@ -350,26 +373,37 @@ __global__ void layout_kernel(int P, const int64_t* __restrict__ C, const int64_
     }
 }
 
-// one thread per receiving rank: send counts, receive displacements, and the
-// run table (receiver-major, i.e. in output order)
+// One wave per receiving rank r, lane j for senders j, j + 64, ...: send
+// counts, receive displacements (the prefix of the counts over the senders)
+// and the run table (receiver-major, i.e. in output order)
 __global__ void runs_kernel(int P, const int64_t* __restrict__ C, const int64_t* __restrict__ sbeg,
                             const int64_t* __restrict__ myC, int64_t* __restrict__ send,
                             int64_t* __restrict__ run_dst, int64_t* __restrict__ run_src,
                             uint32_t* __restrict__ error_flag) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= P) return;
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= P) return;  // uniform per wave
     const int P1 = P + 1;
-    int64_t displ = 0;
-    for (int j = 0; j < P; j++) {
-        const int64_t a = myC[(int64_t)j * P1 + r], b = myC[(int64_t)j * P1 + r + 1];
+    long long displ = 0;  // counts of the senders before this chunk
+    for (int j0 = 0; j0 < P; j0 += 64) {
+        const int j = j0 + lane;
+        int64_t a = 0, b = 0;
+        if (j < P) {
+            a = myC[(int64_t)j * P1 + r];
+            b = myC[(int64_t)j * P1 + r + 1];
+        }
         const int64_t cnt = b - a;
-        if (cnt < 0) atomicOr(error_flag, 8u);
-        send[(int64_t)j * P + r] = cnt;
-        run_dst[(int64_t)r * P + j] = C[r] + displ;
-        run_src[(int64_t)r * P + j] = sbeg[j] + a;
-        displ += cnt > 0 ? cnt : 0;
+        const long long cp = cnt > 0 ? cnt : 0;
+        const long long incl = wave_incl_scan_ll(cp);
+        if (j < P) {
+            if (cnt < 0) atomicOr(error_flag, 8u);
+            send[(int64_t)j * P + r] = cnt;
+            run_dst[(int64_t)r * P + j] = C[r] + displ + (incl - cp);
+            run_src[(int64_t)r * P + j] = sbeg[j] + a;
+        }
+        displ += __shfl(incl, 63, 64);
     }
-    if (displ != C[r + 1] - C[r]) atomicOr(error_flag, 16u);
+    if (lane == 0 && displ != C[r + 1] - C[r]) atomicOr(error_flag, 16u);
 }
 
 constexpr int kExchChunk = 4096;
@@ -647,7 +681,7 @@ int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sb
 int launch_layout(int P, const int64_t* C, const int64_t* scnt, const int64_t* sbeg, int64_t* myclt,
                   int64_t* mycle, int64_t* myC, int64_t* send, int64_t* run_dst, int64_t* run_src,
                   uint32_t* err, hipStream_t st) {
-    const int blocks = (P + kBlock - 1) / kBlock;
+    const int blocks = (P + kWaves - 1) / kWaves;  // a wave per receiving rank
     hipLaunchKernelGGL(layout_kernel, dim3(blocks), dim3(kBlock), 0, st, P, C, scnt, myclt, mycle,
                        myC, err);
     MPSG_CHECK(hipGetLastError());

@@ -65,7 +65,7 @@ def test_config1_shape_small():
     assert st.first_passes == 8
 
 
-@pytest.mark.parametrize("P", [1, 2, 3, 4, 7, 16, 64])
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 7, 16, 64, 65, 130, 300])
 def test_rank_counts(P):
     n = 5000 + 37 * P
     a = rec16(n, P)
@@ -82,6 +82,18 @@ def test_duplicates(nkeys):
     n = 20000
     a = rec16(n, 7 + nkeys, nkeys)
     check_parity(a, even(n, 8), even(n, 8), 'key')
+
+
+@pytest.mark.parametrize("P", [70, 200])
+def test_duplicates_many_ranks(P):
+    """Few key values over more ranks than one wave has lanes: the layout's
+    fill from the equal-key supplies crosses the 64-sender chunks."""
+    n = 30000
+    a = rec16(n, 11 + P, 3)
+    g = numpy.random.default_rng(7 * P)
+    nin = numpy.diff(numpy.concatenate([[0], numpy.sort(g.integers(0, n + 1, P - 1)), [n]]))
+    nout = numpy.diff(numpy.concatenate([[0], numpy.sort(g.integers(0, n + 1, P - 1)), [n]]))
+    check_parity(a, nin, nout, 'key')
 
 
 def test_empty_ranks_and_outputs():
