@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void lhist_kernel(const uint32_t* __restric
                 krange = s_tab[kLBounds - 1] - kmin;
                 gsh = krange ? max(0, 64 - __clzll((long long)krange) - kGuideBits) : 0;
                 for (int c = tid; c <= kGuideCells; c += kBlock) {
-                    const uint64_t off = (uint64_t)c << gsh;  // c << gsh < 2^64: gsh <= 64 - kGuideBits
+                    const uint64_t off = (uint64_t)c << gsh;  // exact for c < kGuideCells (gsh <= 64 - kGuideBits)
                     s_guide[c] = (uint16_t)(c == kGuideCells || off > krange ? kLBounds : bucket_of(kmin + off, s_tab));
                 }
                 __syncthreads();
