@@ -259,12 +259,14 @@ __global__ __launch_bounds__(256) void ltile_scan_kernel(const uint16_t* __restr
     const int64_t t1 = seg.tile_start[s + 1];
     uint32_t run = (uint32_t)(bucket_base[(int64_t)s * kLBins + b] - seg.begin[s]);
     int64_t t = seg.tile_start[s];
-    for (; t + 8 <= t1; t += 8) {
-        uint32_t c[8];
+    // 16 tiles' counts in flight at once per thread (one workgroup per CU:
+    // the loads' latency, not bandwidth, bounds this pass)
+    for (; t + 16 <= t1; t += 16) {
+        uint32_t c[16];
 #pragma unroll
-        for (int k = 0; k < 8; k++) c[k] = tile_cnt[(t + k) * kLBins + b];
+        for (int k = 0; k < 16; k++) c[k] = tile_cnt[(t + k) * kLBins + b];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < 16; k++) {
             tile_off[(t + k) * kLBins + b] = run;
             run += c[k];
         }
