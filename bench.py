@@ -32,8 +32,8 @@ PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_
 # HBM bytes per launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # each scaled by the counter's reading for a plain copy kernel of known size
 # (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
-TRAFFIC_ONESWEEP = 9124084577
-TRAFFIC_BUCKET_SORT = 9539309920
+TRAFFIC_ONESWEEP = 9117953756
+TRAFFIC_BUCKET_SORT = 9445766080
 # per-side stages of the learned local sort, as timed by the library; with
 # the merged SecondSort also its two gather-and-sort launches alone
 # (merge_B: groups of buckets, then the larger single buckets), which
