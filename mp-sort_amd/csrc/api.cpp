@@ -18,6 +18,8 @@
 #include "lsort.h"
 #include "model.h"
 #include "../../include/mpsort_gpu.h"
+#include "../../include/mp-mpiu.h"
+#include <thread>
 
 namespace mpsg {
 
@@ -35,8 +37,8 @@ int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, int n
                          const int64_t* run_dst, const int64_t* run_src, uint32_t* err,
                          const int64_t* C, int P, const KeySpec& ks, uint32_t* hist, hipStream_t st);
 int launch_pack(const void* in, int64_t n, int64_t elsize, int64_t key_off, int key_bytes,
-                void* out, int rw, long long* checksum, hipStream_t st);
-int launch_gather(const void* in, int64_t n, int64_t elsize, const void* srec, int rw, void* out,
+                void* out, int rw, int iwords, long long* checksum, hipStream_t st);
+int launch_gather(const void* in, int64_t n, int64_t elsize, const void* srec, int rw, int iwords, void* out,
                   long long* checksum, uint32_t* err, hipStream_t st);
 
 static char g_err[1024];
@@ -46,6 +48,40 @@ static mpsort_gpu_malloc_fn g_malloc = nullptr;
 static mpsort_gpu_free_fn g_free = nullptr;
 static void* g_alloc_ud = nullptr;
 static mpsort_gpu_stats g_last;
+
+// ---- host allocation hook (mp-mpiu.h:4-16): the library's large host
+// temporaries go through it
+static void* host_malloc_default(const char*, size_t size, const char*, const int, void*) { return malloc(size); }
+static void host_free_default(void* p, const char*, const int, void*) { free(p); }
+static mpiu_malloc_func g_hmalloc = host_malloc_default;
+static mpiu_free_func g_hfree = host_free_default;
+static void* g_hud = nullptr;
+
+// host threads for per-record loops over n records (at least 2^16 records
+// each): MPSORT_GPU_HOST_THREADS, else the hardware threads up to 16
+static int host_threads(int64_t n) {
+    static int T = 0;
+    if (!T) {
+        const char* e = getenv("MPSORT_GPU_HOST_THREADS");
+        int v = e ? atoi(e) : (int)std::thread::hardware_concurrency();
+        if (!e && v > 16) v = 16;
+        T = v < 1 ? 1 : v;
+    }
+    const int64_t want = (n + (1 << 16) - 1) >> 16;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(T, want));
+}
+
+// f(t, lo, hi) on T threads over equal ranges of [0, n)
+template <class F>
+static void parallel_ranges(int64_t n, int T, F f) {
+    if (T <= 1) {
+        f(0, (int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++) th.emplace_back([&f, n, T, t]() { f(t, n / T * t + std::min<int64_t>(t, n % T), n / T * (t + 1) + std::min<int64_t>(t + 1, n % T)); });
+    for (auto& x : th) x.join();
+}
 
 static int fail(int code, const char* fmt, ...) {
     va_list ap;
@@ -302,6 +338,7 @@ struct CallShape {
     int64_t elsize;
     int64_t key_off_user;  // general path: key bytes at this offset of the caller's record
     int key_bytes_user;
+    int iwords;            // general path: words of the source index (1: u32, 2: u64)
     bool checksum;
     bool timing;
     bool exchange;  // learned path: keep the explicit exchange before the SecondSort
@@ -407,7 +444,7 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
             // the radix-callback form: sort records were built on the host
             MPSG_CHECK(hipMemcpyAsync(c.rbuf.p, packed_host, n * recbytes, hipMemcpyHostToDevice, st));
         } else {
-            if ((rc = launch_pack(d_in, n, cs.elsize, cs.key_off_user, cs.key_bytes_user, c.rbuf.p, rw,
+            if ((rc = launch_pack(d_in, n, cs.elsize, cs.key_off_user, cs.key_bytes_user, c.rbuf.p, rw, cs.iwords,
                                   cs.checksum ? h.checksum : nullptr, st)))
                 return rc;
         }
@@ -450,7 +487,7 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         MPSG_CHECK(hipEventRecord(c.ev[kEvSecond], st));
     }
     if (cs.general) {
-        if ((rc = launch_gather(d_in, n, cs.elsize, c.rbuf.p, rw, d_out,
+        if ((rc = launch_gather(d_in, n, cs.elsize, c.rbuf.p, rw, cs.iwords, d_out,
                                 cs.checksum ? h.checksum + 1 : nullptr, h.err, st)))
             return rc;
     }
@@ -512,7 +549,8 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         return fail(MPSORT_ERR_LAYOUT, "layout check failed (device flags 0x%x): the splitter search "
                                        "or layout solve is inconsistent",
                     errs[0]);
-    if (cs.checksum && sums[0] != sums[1])
+    // with host-built sort records the caller compares the host's input sum
+    if (cs.checksum && !packed_host && sums[0] != sums[1])
         return fail(MPSORT_ERR_CHECKSUM, "MPSort: Data changed after sorting; checksum mismatch (%lld != %lld)",
                     sums[0], sums[1]);
     return 0;
@@ -535,8 +573,15 @@ static int validate_counts(const int64_t* nin, int P, const int64_t* nout, int64
     return 0;
 }
 
+// packed sort records of key_bytes key bytes and the source index: words
+// (2, 4 or 8), or 0 when they do not fit 32 bytes
+static int packed_words(int64_t key_bytes, int iwords) {
+    const int64_t b = key_bytes + 4 * iwords;
+    return b <= 8 ? 2 : (b <= 16 ? 4 : (b <= 32 ? 8 : 0));
+}
+
 static int shape_for(const mpsort_key_desc* key, size_t elsize, unsigned flags, const void* base, const void* out,
-                     CallShape* cs) {
+                     int64_t ntot, CallShape* cs) {
     if (!key) return fail(MPSORT_ERR_ARG, "key descriptor is NULL");
     if (elsize == 0) return fail(MPSORT_ERR_ARG, "elsize is 0");
     if (key->width != 4 && key->width != 8) return fail(MPSORT_ERR_UNSUPPORTED, "key word width %d not 4 or 8", key->width);
@@ -555,9 +600,12 @@ static int shape_for(const mpsort_key_desc* key, size_t elsize, unsigned flags, 
         cs->rw = (int)(elsize / 4);
         fill_keyspec(&cs->ks, cs->rw, (int)key->offset, (int)kb, key->width, key->is_signed);
     } else {
-        if (kb + 4 > 32) return fail(MPSORT_ERR_UNSUPPORTED, "key of %lld bytes too long for the packed path", (long long)kb);
+        cs->iwords = ntot >= ((int64_t)1 << 32) ? 2 : 1;
+        cs->rw = packed_words(kb, cs->iwords);
+        if (!cs->rw)
+            return fail(MPSORT_ERR_UNSUPPORTED, "key of %lld bytes and a %d-byte index exceed the 32-byte packed record",
+                        (long long)kb, 4 * cs->iwords);
         cs->general = true;
-        cs->rw = kb + 4 <= 8 ? 2 : (kb + 4 <= 16 ? 4 : 8);
         fill_keyspec(&cs->ks, cs->rw, 0, (int)kb, key->width, key->is_signed);
         cs->key_off_user = key->offset;
         cs->key_bytes_user = (int)kb;
@@ -996,7 +1044,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     void* recs_in = const_cast<void*>(d_in);
     void* final_dst = d_out;
     if (cs.general) {
-        if ((rc = launch_pack(d_in, n, cs.elsize, cs.key_off_user, cs.key_bytes_user, c.rbuf.p, rw,
+        if ((rc = launch_pack(d_in, n, cs.elsize, cs.key_off_user, cs.key_bytes_user, c.rbuf.p, rw, cs.iwords,
                               cs.checksum ? h.checksum : nullptr, st)))
             return rc;
         recs_in = c.rbuf.p;
@@ -1138,9 +1186,24 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             const uint32_t nov = novh[0];
             lc.overflow_buckets += (int)nov;
             lc.hard_buckets += (int)novh[1];
-            if (nov && (rc = sort_listed_segments(c, planB, rw, LB.overflow, nov, final_dst, final_dst, c.s1.p,
-                                                  nullptr, st)))
+            if (nov > (uint32_t)kMaxFallbackSegs) {
+                // more oversized buckets than the bucket fallback takes (the
+                // list may also have stopped at its capacity): every one of
+                // them was copied to its slots in sender order, so a radix
+                // sort of each receiver's whole segment finishes the sort
+                lc.global_fallback++;
+                if (cs_direct && (rc = launch_fill_u32((uint32_t*)(h.checksum + 1), 0u, 2, st))) return rc;
+                int passes = 0;
+                SortBuffers b{final_dst, final_dst, c.s1.p};
+                if ((rc = segmented_sort(planB, b, st, false, cs_direct, &passes, nullptr))) return rc;
+                // no digit pass ran (one key value): the output checksum is
+                // taken by a copy
+                if (passes == 0 && cs_direct && (rc = launch_copy(final_dst, c.s1.p, n, rw, h.checksum + 1, st)))
+                    return rc;
+            } else if (nov && (rc = sort_listed_segments(c, planB, rw, LB.overflow, nov, final_dst, final_dst,
+                                                         c.s1.p, nullptr, st))) {
                 return rc;
+            }
             if ((rc = markB(5))) return rc;
         } else {
             if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, P * P, h.run_dst, h.run_src, h.err, st))) return rc;
@@ -1152,8 +1215,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         MPSG_CHECK(hipEventRecord(c.ev[kEvSecond], st));
     }
     if (cs.general) {
-        if ((rc = launch_gather(d_in, n, cs.elsize, c.rbuf.p, rw, d_out, cs.checksum ? h.checksum + 1 : nullptr,
-                                h.err, st)))
+        if ((rc = launch_gather(d_in, n, cs.elsize, c.rbuf.p, rw, cs.iwords, d_out,
+                                cs.checksum ? h.checksum + 1 : nullptr, h.err, st)))
             return rc;
     }
     MPSG_CHECK(hipEventRecord(c.ev[kEvEnd], st));
@@ -1426,10 +1489,14 @@ static int seg_group_enqueue(SegGroup& G, hipStream_t st) {
     }
     if (G.nlarge) {
         if ((rc = launch_add_u32(G.d_counters + 2, (uint32_t)kMaxPasses, st))) return rc;
-        if ((rc = seg_count(G.plan, G.in, nullptr, st))) return rc;
+        if ((rc = seg_count(G.plan, G.in, nullptr, st)))
+            return rc < 0 && !g_err[0] ? fail(MPSORT_ERR_UNSUPPORTED, "segmented sort: tile %d x %d not supported for "
+                                              "%d-byte records", G.plan.tile.block, G.plan.tile.items, 4 * G.rw) : rc;
         SortBuffers b{const_cast<void*>(G.in), G.out, G.tmp.p};
         int passes = 0;
-        if ((rc = seg_passes(G.plan, b, st, nullptr, &passes, nullptr))) return rc;
+        if ((rc = seg_passes(G.plan, b, st, nullptr, &passes, nullptr)))
+            return rc < 0 && !g_err[0] ? fail(MPSORT_ERR_UNSUPPORTED, "segmented sort: tile %d x %d not supported for "
+                                              "%d-byte records", G.plan.tile.block, G.plan.tile.items, 4 * G.rw) : rc;
     }
     return 0;
 }
@@ -1505,7 +1572,66 @@ static int seg_group_tune(SegGroup& G, DevCtx& c) {
     return seg_group_prepare(G, c.st);
 }
 
+// Host staging and the in-place guard around one device sort (run).  With
+// device pointers, a packed-record call whose output overlaps its input
+// gathers the caller's records from a copy: the gather reads records by
+// index while it writes the output.
+template <class Run>
+static int staged_call(DevCtx* c, const CallShape& cs, const void* base, void* out, bool devptr, int P,
+                       mpsort_gpu_stats* st, Run run) {
+    int rc = 0;
+    const int64_t ntot = cs.ntot;
+    const void* d_in = base;
+    void* d_out = out;
+    const size_t bytes = (size_t)ntot * (size_t)cs.elsize;
+    hipEvent_t e0 = c->ev[kEvStart], e1 = c->ev[kEvEnd];
+    if (!devptr && ntot > 0) {
+        if ((rc = c->din.ensure(bytes))) return rc;
+        if ((rc = c->dout.ensure(bytes))) return rc;
+        MPSG_CHECK(hipEventRecord(e0, c->st));
+        MPSG_CHECK(hipMemcpyAsync(c->din.p, base, bytes, hipMemcpyHostToDevice, c->st));
+        MPSG_CHECK(hipEventRecord(e1, c->st));
+        MPSG_CHECK(hipStreamSynchronize(c->st));
+        st->t_h2d_ms = ev_ms(e0, e1);
+        d_in = c->din.p;
+        d_out = c->dout.p;
+    } else if (devptr && cs.general && ntot > 0) {
+        const char *a = (const char*)base, *b = (const char*)out;
+        if (a < b + bytes && b < a + bytes) {
+            if ((rc = c->din.ensure(bytes))) return rc;
+            MPSG_CHECK(hipMemcpyAsync(c->din.p, base, bytes, hipMemcpyDeviceToDevice, c->st));
+            d_in = c->din.p;
+        }
+    }
+    if (ntot > 0) {
+        rc = run(d_in, d_out);
+    } else {
+        st->nvranks = P;
+        c->last_P = 0;
+    }
+    if (!rc && !devptr && ntot > 0) {
+        MPSG_CHECK(hipEventRecord(e0, c->st));
+        MPSG_CHECK(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, c->st));
+        MPSG_CHECK(hipEventRecord(e1, c->st));
+        MPSG_CHECK(hipStreamSynchronize(c->st));
+        st->t_d2h_ms = ev_ms(e0, e1);
+    }
+    return rc;
+}
+
 extern "C" {
+
+void* mpiu_malloc(const char* name, size_t size, const char* file, const int line) {
+    return g_hmalloc(name, size, file, line, g_hud);
+}
+void mpiu_free(void* ptr, const char* file, const int line) {
+    if (ptr) g_hfree(ptr, file, line, g_hud);
+}
+void mpiu_set_malloc(mpiu_malloc_func m, mpiu_free_func f, void* userdata) {
+    g_hmalloc = m ? m : host_malloc_default;
+    g_hfree = f ? f : host_free_default;
+    g_hud = userdata;
+}
 
 const char* mpsort_gpu_last_error(void) { return g_err; }
 const char* mpsort_gpu_version(void) { return "mpsort-mi355x 0.1.0 (gfx950)"; }
@@ -1525,47 +1651,19 @@ int mpsort_gpu_newarray(const void* base, const int64_t* nin, int P, void* out, 
     CallShape cs;
     memset(&cs, 0, sizeof cs);
     const bool devptr = (flags & MPSORT_GPU_DEVICE_PTRS) != 0;
-    rc = shape_for(key, elsize, flags, devptr ? base : nullptr, devptr ? out : nullptr, &cs);
+    rc = shape_for(key, elsize, flags, devptr ? base : nullptr, devptr ? out : nullptr, ntot, &cs);
     if (rc) return rc;
     cs.P = P;
     cs.ntot = ntot;
-    if (cs.general && ntot >= (int64_t)1 << 32)
-        return fail(MPSORT_ERR_UNSUPPORTED, "packed path holds at most 2^32-1 records");
+    if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");
     DevCtx* c = nullptr;
     if ((rc = get_ctx(&c))) return rc;
     mpsort_gpu_stats st;
     memset(&st, 0, sizeof st);
-    if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");
-
-    const void* d_in = base;
-    void* d_out = out;
-    const size_t bytes = (size_t)ntot * elsize;
-    hipEvent_t e0 = c->ev[kEvStart], e1 = c->ev[kEvEnd];
-    if (!devptr && ntot > 0) {
-        if ((rc = c->din.ensure(bytes))) return rc;
-        if ((rc = c->dout.ensure(bytes))) return rc;
-        MPSG_CHECK(hipEventRecord(e0, c->st));
-        MPSG_CHECK(hipMemcpyAsync(c->din.p, base, bytes, hipMemcpyHostToDevice, c->st));
-        MPSG_CHECK(hipEventRecord(e1, c->st));
-        MPSG_CHECK(hipStreamSynchronize(c->st));
-        st.t_h2d_ms = ev_ms(e0, e1);
-        d_in = c->din.p;
-        d_out = c->dout.p;
-    }
     remember_key(*c, key, 0, 0);
-    if (ntot > 0) {
-        rc = run_histsort(*c, cs, d_in, d_out, nin, nout, nullptr, &st);
-    } else {
-        st.nvranks = P;
-        c->last_P = 0;
-    }
-    if (!rc && !devptr && ntot > 0) {
-        MPSG_CHECK(hipEventRecord(e0, c->st));
-        MPSG_CHECK(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, c->st));
-        MPSG_CHECK(hipEventRecord(e1, c->st));
-        MPSG_CHECK(hipStreamSynchronize(c->st));
-        st.t_d2h_ms = ev_ms(e0, e1);
-    }
+    rc = staged_call(c, cs, base, out, devptr, P, &st, [&](const void* d_in, void* d_out) {
+        return run_histsort(*c, cs, d_in, d_out, nin, nout, nullptr, &st);
+    });
     g_last = st;
     return rc;
 }
@@ -1578,10 +1676,14 @@ int mpsort_gpu_newarray_radix(const void* base, const int64_t* nin, int P, void*
     int rc = validate_counts(nin, P, nout, &ntot);
     if (rc) return rc;
     if (!radix) return fail(MPSORT_ERR_ARG, "radix callback is NULL");
-    if (rsize == 0 || rsize + 4 > 32) return fail(MPSORT_ERR_UNSUPPORTED, "rsize %zu not in [1, 28]", rsize);
     if (elsize == 0) return fail(MPSORT_ERR_ARG, "elsize is 0");
     if (flags & MPSORT_GPU_DEVICE_PTRS) return fail(MPSORT_ERR_ARG, "the radix-callback form takes host pointers");
-    if (ntot >= (int64_t)1 << 32) return fail(MPSORT_ERR_UNSUPPORTED, "at most 2^32-1 records");
+    const int iwords = ntot >= ((int64_t)1 << 32) ? 2 : 1;
+    const int rw = rsize > 0 && rsize <= 32 ? packed_words((int64_t)rsize, iwords) : 0;
+    if (!rw)
+        return fail(MPSORT_ERR_UNSUPPORTED, "rsize %zu and a %d-byte index exceed the 32-byte packed record", rsize,
+                    4 * iwords);
+    if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");
     DevCtx* c = nullptr;
     if ((rc = get_ctx(&c))) return rc;
     mpsort_gpu_stats st;
@@ -1592,9 +1694,9 @@ int mpsort_gpu_newarray_radix(const void* base, const int64_t* nin, int P, void*
     cs.ntot = ntot;
     cs.general = true;
     cs.elsize = (int64_t)elsize;
-    cs.rw = rsize + 4 <= 8 ? 2 : (rsize + 4 <= 16 ? 4 : 8);
+    cs.rw = rw;
+    cs.iwords = iwords;
     fill_keyspec(&cs.ks, cs.rw, 0, (int)rsize, 0, 0);
-    cs.checksum = !(flags & MPSORT_GPU_NO_CHECKSUM);
     cs.timing = (flags & MPSORT_GPU_TIMING) != 0;
     remember_key(*c, nullptr, 1, (int64_t)rsize);
     if (ntot == 0) {
@@ -1602,33 +1704,54 @@ int mpsort_gpu_newarray_radix(const void* base, const int64_t* nin, int P, void*
         c->last_P = 0;
         return 0;
     }
-    // materialise the radix once per record, with the source index
-    const size_t rb = (size_t)cs.rw * 4;
-    std::vector<uint32_t> packed((size_t)ntot * cs.rw, 0u);
-    std::vector<unsigned char> tmp(rsize + 8);
-    for (int64_t i = 0; i < ntot; i++) {
-        radix((const char*)base + i * elsize, tmp.data(), arg);
-        memcpy(&packed[(size_t)i * cs.rw], tmp.data(), rsize);
-        packed[(size_t)i * cs.rw + cs.rw - 1] = (uint32_t)i;
-    }
-    (void)rb;
-    const size_t bytes = (size_t)ntot * elsize;
-    if ((rc = c->din.ensure(bytes))) return rc;
-    if ((rc = c->dout.ensure(bytes))) return rc;
-    MPSG_CHECK(hipMemcpy(c->din.p, base, bytes, hipMemcpyHostToDevice));
-    // checksum of the input on the host (the device only sees packed records)
+    // the radix of every record, materialised once, with its position; and
+    // the input checksum -- both by host threads over record ranges
+    const size_t rb = (size_t)rw * 4;
+    uint32_t* packed = (uint32_t*)mpiu_malloc("mpsort_gpu:radix", (size_t)ntot * rb, __FILE__, __LINE__);
+    if (!packed) return fail(MPSORT_ERR_NOMEM, "host allocation of %zu bytes failed", (size_t)ntot * rb);
+    const int T = host_threads(ntot);
+    std::vector<long long> part(T, 0);
+    parallel_ranges(ntot, T, [&](int t, int64_t lo, int64_t hi) {
+        unsigned char tmp[64];
+        long long sum = 0;
+        for (int64_t i = lo; i < hi; i++) {
+            const unsigned char* rec = (const unsigned char*)base + (size_t)i * elsize;
+            memset(tmp, 0, sizeof tmp);
+            radix(rec, tmp, arg);
+            uint32_t* w = packed + (size_t)i * rw;
+            memcpy(w, tmp, rb - 4 * (size_t)iwords);
+            if (iwords == 2) {
+                w[rw - 2] = (uint32_t)i;
+                w[rw - 1] = (uint32_t)((uint64_t)i >> 32);
+            } else {
+                w[rw - 1] = (uint32_t)i;
+            }
+            for (size_t b = 0; b < elsize; b++) sum += (signed char)rec[b];
+        }
+        part[t] = sum;
+    });
     long long sum_in = 0;
-    for (size_t b = 0; b < bytes; b++) sum_in += ((const signed char*)base)[b];
-    cs.checksum = false;
-    rc = run_histsort(*c, cs, c->din.p, c->dout.p, nin, nout, packed.data(), &st);
+    for (long long v : part) sum_in += v;
+    const size_t bytes = (size_t)ntot * elsize;
+    if ((rc = c->din.ensure(bytes)) || (rc = c->dout.ensure(bytes))) {
+        mpiu_free(packed, __FILE__, __LINE__);
+        return rc;
+    }
+    if (hipMemcpy(c->din.p, base, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        mpiu_free(packed, __FILE__, __LINE__);
+        return fail(MPSORT_ERR_HIP, "copy of the records to the device failed");
+    }
+    // the gather's device checksum covers the output; the input sum is the
+    // host's (the device sees only the packed records)
+    cs.checksum = !(flags & MPSORT_GPU_NO_CHECKSUM);
+    rc = run_histsort(*c, cs, c->din.p, c->dout.p, nin, nout, packed, &st);
+    mpiu_free(packed, __FILE__, __LINE__);
     if (!rc) {
         MPSG_CHECK(hipMemcpy(out, c->dout.p, bytes, hipMemcpyDeviceToHost));
-        long long sum_out = 0;
-        for (size_t b = 0; b < bytes; b++) sum_out += ((const signed char*)out)[b];
         st.checksum_in = sum_in;
-        st.checksum_out = sum_out;
-        if (!(flags & MPSORT_GPU_NO_CHECKSUM) && sum_in != sum_out)
-            rc = fail(MPSORT_ERR_CHECKSUM, "MPSort: Data changed after sorting; checksum mismatch");
+        if (cs.checksum && sum_in != st.checksum_out)
+            rc = fail(MPSORT_ERR_CHECKSUM, "MPSort: Data changed after sorting; checksum mismatch (%lld != %lld)",
+                      sum_in, st.checksum_out);
     }
     g_last = st;
     return rc;
@@ -1640,10 +1763,18 @@ int mpsort_gpu_last_stats(mpsort_gpu_stats* s) {
     return 0;
 }
 
+// The reference's timer lines in its order (mpsort-mpi.c:366-602): seconds
+// since the previous line.  Stages that one kernel does together with the
+// next stage print 0: PmaxPmin runs inside the bisection kernel (its time is
+// under findP, which covers every bisection round; bisectNNNN names the
+// round count), LayDistr inside the layout solve.
 void mpsort_gpu_report_last_run(void) {
     const mpsort_gpu_stats& s = g_last;
     printf("FirstSort: %g\n", s.t_first_sort_ms * 1e-3);
+    printf("PmaxPmin: %g\n", 0.0);
+    printf("bisect%04d: %g\n", s.bisect_rounds, 0.0);
     printf("findP: %g\n", s.t_findp_ms * 1e-3);
+    printf("LayDistr: %g\n", 0.0);
     printf("LaySolve: %g\n", s.t_laysolve_ms * 1e-3);
     printf("Exchange: %g\n", s.t_exchange_ms * 1e-3);
     printf("SecondSort: %g\n", s.t_second_sort_ms * 1e-3);
@@ -2149,7 +2280,7 @@ int mpsort_gpu_newarray_learned(const void* base, const int64_t* nin, int P, voi
     CallShape cs;
     memset(&cs, 0, sizeof cs);
     const bool devptr = (flags & MPSORT_GPU_DEVICE_PTRS) != 0;
-    rc = shape_for(key, elsize, flags, devptr ? base : nullptr, devptr ? out : nullptr, &cs);
+    rc = shape_for(key, elsize, flags, devptr ? base : nullptr, devptr ? out : nullptr, ntot, &cs);
     if (rc) return rc;
     const int64_t kb = (int64_t)key->width * key->nwords;
     if (kb > 8 || kb != model->m.key_bytes)
@@ -2157,46 +2288,19 @@ int mpsort_gpu_newarray_learned(const void* base, const int64_t* nin, int P, voi
                     model->m.key_bytes, (long long)kb);
     if (cs.rw != 2 && cs.rw != 4) {
         // the bucket kernels take 8- and 16-byte sort records: pack {key, index}
-        shape_for(key, elsize, flags | MPSORT_GPU_FORCE_GENERAL, nullptr, nullptr, &cs);
+        if ((rc = shape_for(key, elsize, flags | MPSORT_GPU_FORCE_GENERAL, nullptr, nullptr, ntot, &cs))) return rc;
     }
     cs.P = P;
     cs.ntot = ntot;
-    if (cs.general && ntot >= (int64_t)1 << 32)
-        return fail(MPSORT_ERR_UNSUPPORTED, "packed path holds at most 2^32-1 records");
+    if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");
     DevCtx* c = nullptr;
     if ((rc = get_ctx(&c))) return rc;
     mpsort_gpu_stats st;
     memset(&st, 0, sizeof st);
-    if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");
-    const void* d_in = base;
-    void* d_out = out;
-    const size_t bytes = (size_t)ntot * elsize;
-    hipEvent_t e0 = c->ev[kEvStart], e1 = c->ev[kEvEnd];
-    if (!devptr && ntot > 0) {
-        if ((rc = c->din.ensure(bytes))) return rc;
-        if ((rc = c->dout.ensure(bytes))) return rc;
-        MPSG_CHECK(hipEventRecord(e0, c->st));
-        MPSG_CHECK(hipMemcpyAsync(c->din.p, base, bytes, hipMemcpyHostToDevice, c->st));
-        MPSG_CHECK(hipEventRecord(e1, c->st));
-        MPSG_CHECK(hipStreamSynchronize(c->st));
-        st.t_h2d_ms = ev_ms(e0, e1);
-        d_in = c->din.p;
-        d_out = c->dout.p;
-    }
     remember_key(*c, key, 0, 0);
-    if (ntot > 0) {
-        rc = run_histsort_learned(*c, cs, d_in, d_out, nin, nout, model->m, &st);
-    } else {
-        st.nvranks = P;
-        c->last_P = 0;
-    }
-    if (!rc && !devptr && ntot > 0) {
-        MPSG_CHECK(hipEventRecord(e0, c->st));
-        MPSG_CHECK(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, c->st));
-        MPSG_CHECK(hipEventRecord(e1, c->st));
-        MPSG_CHECK(hipStreamSynchronize(c->st));
-        st.t_d2h_ms = ev_ms(e0, e1);
-    }
+    rc = staged_call(c, cs, base, out, devptr, P, &st, [&](const void* d_in, void* d_out) {
+        return run_histsort_learned(*c, cs, d_in, d_out, nin, nout, model->m, &st);
+    });
     g_last = st;
     return rc;
 }
@@ -2244,6 +2348,7 @@ int mpsort_gpu_segplan_add(mpsort_gpu_segplan* p, const void* d_in, void* d_out,
     G->rw = (int)(elsize / 4);
     fill_keyspec(&G->ks, G->rw, (int)key->offset, (int)kb, key->width, key->is_signed);
     G->lk = make_lkey(G->ks);
+    G->tune.tile = default_tile(G->rw);
     G->nseg = nseg;
     G->count.assign(seg_count, seg_count + nseg);
     G->begin.resize(nseg);

@@ -112,15 +112,18 @@ static inline uint32_t quant21(double x)
  * 21 bits per axis.  Writes {u64 morton, i64 index} records when rec != NULL,
  * and the keys alone when keys != NULL.
  */
-void mpsort_gen_clustered(uint64_t *keys, void *rec, size_t n, uint64_t seed,
-                          int nclump, double median_r, double sigma_ln, double frac_uniform)
+/* The field (clump centres and radii) comes from field_seed, the points
+ * from seed: two seeds with one field_seed are two realisations of one
+ * clustered field. */
+void mpsort_gen_clustered_field(uint64_t *keys, void *rec, size_t n, uint64_t field_seed, uint64_t seed,
+                                int nclump, double median_r, double sigma_ln, double frac_uniform)
 {
     double *cx = malloc(sizeof(double) * 4 * (size_t)nclump);
     for (int c = 0; c < nclump; c++) {
-        cx[4 * c + 0] = stream_unit(seed, 10, (uint64_t)c, 0);
-        cx[4 * c + 1] = stream_unit(seed, 10, (uint64_t)c, 1);
-        cx[4 * c + 2] = stream_unit(seed, 10, (uint64_t)c, 2);
-        cx[4 * c + 3] = median_r * exp(sigma_ln * stream_normal(seed, 11, (uint64_t)c, 0));
+        cx[4 * c + 0] = stream_unit(field_seed, 10, (uint64_t)c, 0);
+        cx[4 * c + 1] = stream_unit(field_seed, 10, (uint64_t)c, 1);
+        cx[4 * c + 2] = stream_unit(field_seed, 10, (uint64_t)c, 2);
+        cx[4 * c + 3] = median_r * exp(sigma_ln * stream_normal(field_seed, 11, (uint64_t)c, 0));
     }
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; i++) {
@@ -142,6 +145,22 @@ void mpsort_gen_clustered(uint64_t *keys, void *rec, size_t n, uint64_t seed,
         if (rec) { ((uint64_t *)rec)[2 * i] = m; ((uint64_t *)rec)[2 * i + 1] = i; }
     }
     free(cx);
+}
+
+/* the field and the points from one seed */
+void mpsort_gen_clustered(uint64_t *keys, void *rec, size_t n, uint64_t seed,
+                          int nclump, double median_r, double sigma_ln, double frac_uniform)
+{
+    mpsort_gen_clustered_field(keys, rec, n, seed, seed, nclump, median_r, sigma_ln, frac_uniform);
+}
+
+/* A radix callback of the reference's form (mpsort.h:1-4) for tests and
+ * benchmarks of the callback path: arg points to {int64 offset, int64
+ * nwords}; the radix is the nwords little-endian u64 words at offset. */
+void mpsort_radix_u64_words(const void *ptr, void *radix, void *arg)
+{
+    const int64_t *a = (const int64_t *)arg;
+    memcpy(radix, (const char *)ptr + a[0], 8 * (size_t)a[1]);
 }
 
 /* log-space mixture of 8 Gaussians cast to uint64 (clipped to the range) */

@@ -558,15 +558,17 @@ __global__ __launch_bounds__(kBlock) void exchange_hist_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// general record path: pack {key bytes, pad, u32 index} sort records from the
-// caller's records, and gather the caller's records by index at the end
-// {key bytes, u32 index} sort records of rw words from the caller's records.
+// general record path: pack {key bytes, pad, index} sort records from the
+// caller's records, and gather the caller's records by index at the end.
+// The index is the record's position: one word (u32) in the last word of the
+// sort record, or two words (u64, little-endian) in the last two when there
+// are 2^32 records or more (iwords = 1 or 2).
 // vec: records of a multiple of 16 bytes, 16-byte aligned (read as 16-byte
 // vectors for the checksum; the key words come from the same lines again);
 // kwords: key offset and length are whole words.
 __global__ __launch_bounds__(kBlock) void pack_kernel(const uint8_t* __restrict__ in, int64_t n,
                                                       int64_t elsize, int64_t key_off, int key_bytes,
-                                                      uint32_t* __restrict__ out, int rw,
+                                                      uint32_t* __restrict__ out, int rw, int iwords,
                                                       long long* __restrict__ checksum, int vec, int kwords) {
     __shared__ long long s_sum[kWaves];
     long long csum = 0;
@@ -591,7 +593,12 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(const uint8_t* __restrict_
         } else {
             for (int b = 0; b < key_bytes; b++) w[b >> 2] |= (uint32_t)rec[key_off + b] << ((b & 3) * 8);
         }
-        w[rw - 1] = (uint32_t)i;
+        if (iwords == 2) {
+            w[rw - 2] = (uint32_t)i;
+            w[rw - 1] = (uint32_t)((uint64_t)i >> 32);
+        } else {
+            w[rw - 1] = (uint32_t)i;
+        }
         if (rw == 4) {
             reinterpret_cast<uint4*>(out)[i] = make_uint4(w[0], w[1], w[2], w[3]);
         } else if (rw == 2) {
@@ -614,7 +621,7 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(const uint8_t* __restrict_
 
 __global__ __launch_bounds__(kBlock) void gather_kernel(const uint8_t* __restrict__ in, int64_t n,
                                                         int64_t elsize, const uint32_t* __restrict__ srec,
-                                                        int rw, uint8_t* __restrict__ out,
+                                                        int rw, int iwords, uint8_t* __restrict__ out,
                                                         long long* __restrict__ checksum,
                                                         uint32_t* __restrict__ error_flag) {
     __shared__ long long s_sum[kWaves];
@@ -623,8 +630,9 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const uint8_t* __restric
     const bool vec = (elsize & 15) == 0 && ((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0;
     for (int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x; o < n;
          o += (int64_t)gridDim.x * kBlock) {
-        const int64_t i = (int64_t)srec[o * rw + rw - 1];
-        if (i >= n) {  // only reachable with corrupted sort records
+        const int64_t i = iwords == 2 ? (int64_t)(((uint64_t)srec[o * rw + rw - 1] << 32) | srec[o * rw + rw - 2])
+                                      : (int64_t)srec[o * rw + rw - 1];
+        if (i < 0 || i >= n) {  // only reachable with corrupted sort records
             if (error_flag) atomicOr(error_flag, 64u);
             continue;
         }
@@ -731,25 +739,25 @@ int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, int n
 }
 
 int launch_pack(const void* in, int64_t n, int64_t elsize, int64_t key_off, int key_bytes,
-                void* out, int rw, long long* checksum, hipStream_t st) {
+                void* out, int rw, int iwords, long long* checksum, hipStream_t st) {
     if (n <= 0) return 0;
     int64_t blocks = (n + kBlock - 1) / kBlock;
     if (blocks > 8192) blocks = 8192;
     const int vec = (elsize % 16 == 0 && (uintptr_t)in % 16 == 0) ? 1 : 0;
     const int kwords = (key_off % 4 == 0 && key_bytes % 4 == 0 && elsize % 4 == 0 && (uintptr_t)in % 4 == 0) ? 1 : 0;
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, (const uint8_t*)in,
-                       n, elsize, key_off, key_bytes, (uint32_t*)out, rw, checksum, vec, kwords);
+                       n, elsize, key_off, key_bytes, (uint32_t*)out, rw, iwords, checksum, vec, kwords);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_gather(const void* in, int64_t n, int64_t elsize, const void* srec, int rw, void* out,
+int launch_gather(const void* in, int64_t n, int64_t elsize, const void* srec, int rw, int iwords, void* out,
                   long long* checksum, uint32_t* err, hipStream_t st) {
     if (n <= 0) return 0;
     int64_t blocks = (n + kBlock - 1) / kBlock;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(gather_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, (const uint8_t*)in,
-                       n, elsize, (const uint32_t*)srec, rw, (uint8_t*)out, checksum, err);
+                       n, elsize, (const uint32_t*)srec, rw, iwords, (uint8_t*)out, checksum, err);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }

@@ -22,6 +22,31 @@
 
 #include "../../include/mpsort.h"
 #include "../../include/mpsort_gpu.h"
+#include "../../include/mp-mpiu.h"
+
+/* MPIU_Set_verbose_malloc (mp-mpiu.h:11): the host allocation hook prints
+ * every allocation and free with the caller's rank */
+static void *verbose_malloc(const char *name, size_t size, const char *file, const int line, void *userdata)
+{
+    int rank = 0;
+    MPI_Comm_rank((MPI_Comm)(intptr_t)userdata, &rank);
+    void *p = malloc(size);
+    fprintf(stderr, "MPIU_Malloc: T%04d %16p : %s size = %zu, %s:%d\n", rank, p, name, size, file, line);
+    return p;
+}
+
+static void verbose_free(void *ptr, const char *file, const int line, void *userdata)
+{
+    int rank = 0;
+    MPI_Comm_rank((MPI_Comm)(intptr_t)userdata, &rank);
+    fprintf(stderr, "MPIU_Free: T%04d %16p : %s:%d\n", rank, ptr, file, line);
+    free(ptr);
+}
+
+void MPIU_Set_verbose_malloc(MPI_Comm comm)
+{
+    mpiu_set_malloc(verbose_malloc, verbose_free, (void *)(intptr_t)comm);
+}
 
 static void fatal(MPI_Comm comm, int line, const char *file, const char *what, const char *detail)
 {
@@ -90,8 +115,8 @@ void mpsort_mpi_newarray_impl(void *base, size_t nmemb, void *out, size_t outnme
             a += nin[r];
             b += nout[r];
         }
-        gin = malloc(elsize * (size_t)(tin > 0 ? tin : 1));
-        gout = malloc(elsize * (size_t)(tin > 0 ? tin : 1));
+        gin = mpiu_malloc("mpsort_mpi:gather", elsize * (size_t)(tin > 0 ? tin : 1), __FILE__, __LINE__);
+        gout = mpiu_malloc("mpsort_mpi:scatter", elsize * (size_t)(tin > 0 ? tin : 1), __FILE__, __LINE__);
         if (!gin || !gout) fatal(comm, line, file, "mpsort_mpi_newarray", "host allocation failed");
     }
     MPI_Gatherv(base, (int)nmemb, rec, gin, cin, din, rec, 0, comm);
@@ -107,8 +132,8 @@ void mpsort_mpi_newarray_impl(void *base, size_t nmemb, void *out, size_t outnme
     free(all);
     free(cin);
     free(nin);
-    free(gin);
-    free(gout);
+    mpiu_free(gin, __FILE__, __LINE__);
+    mpiu_free(gout, __FILE__, __LINE__);
 }
 
 void mpsort_mpi_impl(void *base, size_t nmemb, size_t elsize,

@@ -50,6 +50,7 @@ EXPORTS = [
     "mpsort_gpu_newarray_learned", "mpsort_gpu_segplan_create", "mpsort_gpu_segplan_add",
     "mpsort_gpu_segplan_run", "mpsort_gpu_segplan_info", "mpsort_gpu_segplan_free",
     "mpsort_gpu_sort_segments",
+    "mpiu_malloc", "mpiu_free", "mpiu_set_malloc",
 ]
 
 SEGPLAN_GRAPH = 1 << 0

@@ -68,17 +68,22 @@ def records16(keys, base=0):
     return out
 
 
-def clustered_records(n, seed=2, nclump=None, median_r=None, sigma_ln=None, frac_uniform=None):
-    """{u64 morton, i64 index} records of the clustered Morton field."""
+def clustered_records(n, seed=2, nclump=None, median_r=None, sigma_ln=None, frac_uniform=None,
+                      field_seed=None):
+    """{u64 morton, i64 index} records of the clustered Morton field.
+
+    The clumps come from field_seed (default: seed) and the points from
+    seed, so records of one field_seed and two seeds are two realisations of
+    one field."""
     c = dict(CLUSTER)
     for k, v in (("nclump", nclump), ("median_r", median_r), ("sigma_ln", sigma_ln),
                  ("frac_uniform", frac_uniform)):
         if v is not None:
             c[k] = v
     out = numpy.empty(n, dtype=REC16_SIGNED)
-    _gen().mpsort_gen_clustered(None, _p(out), SZ(n), U64(seed), ctypes.c_int(int(c["nclump"])),
-                                ctypes.c_double(c["median_r"]), ctypes.c_double(c["sigma_ln"]),
-                                ctypes.c_double(c["frac_uniform"]))
+    _gen().mpsort_gen_clustered_field(None, _p(out), SZ(n), U64(seed if field_seed is None else field_seed),
+                                      U64(seed), ctypes.c_int(int(c["nclump"])), ctypes.c_double(c["median_r"]),
+                                      ctypes.c_double(c["sigma_ln"]), ctypes.c_double(c["frac_uniform"]))
     return out
 
 

@@ -258,3 +258,22 @@ def test_model_is_monotone():
     k = numpy.sort(workloads.uniform(200000, 3) >> numpy.uint64(1))
     f = m.eval(k)
     assert numpy.all(numpy.diff(f) >= 0)
+
+
+def test_merged_secondsort_with_thousands_of_oversized_buckets():
+    """2^26 records on 2 virtual ranks: every receiver bucket of the merged
+    SecondSort holds about 10,900 records, above the largest in-LDS bucket,
+    so more buckets overflow than the bucket fallback lists; the receivers
+    are then radix-sorted whole.  Output equals the plain path's."""
+    n, P = 1 << 26, 2
+    note("merged overflow n=%d P=%d" % (n, P))
+    model = trained("uniform", 1 << 22)
+    a = workloads.records16(keys_of("uniform", n, 9))
+    ref, _ = gpu_sort(a, even(n, P), even(n, P), 'key')
+    out = learned.sort_learned(a, model, 'key', nin=even(n, P))
+    st = _lib.last_stats()
+    note("overflow buckets %d, plain sides %d" % (st.buckets_overflow, st.plain_sides))
+    assert st.buckets_overflow > 4096
+    assert st.plain_sides >= 1
+    assert numpy.array_equal(out, ref)
+    assert st.checksum_in == st.checksum_out

@@ -127,3 +127,20 @@ def test_segments_tuned_plan(tmp_path, monkeypatch):
     finally:
         lib.mpsort_gpu_dfree(d)
     assert numpy.array_equal(g, expected(a, sizes))
+
+
+def test_32_byte_records_default_and_tuned():
+    """32-byte records: the default tile must be one the digit passes take
+    for 8-word records (plain run and the tuned plan)"""
+    dt = numpy.dtype([('key', 'i8'), ('value', 'i8'), ('pad', 'u8', 2)])
+    sizes = numpy.array([0, 5, 3000, 70000, 1, 250000, 12345], dtype=numpy.int64)
+    n = int(sizes.sum())
+    g = numpy.random.default_rng(71)
+    a = numpy.zeros(n, dt)
+    a['key'] = g.integers(-2 ** 63, 2 ** 63 - 1, n, dtype=numpy.int64)
+    a['value'] = numpy.arange(n)
+    note("segments 32B n=%d" % n)
+    want = expected(a, sizes)
+    for tune in (False, True):
+        out = segments.sort_segments(a, sizes, orderby='key', tune=tune)
+        assert numpy.array_equal(out, want), tune

@@ -244,3 +244,44 @@ def test_mlp_oracle_is_monotone():
     xs = numpy.linspace(0, 1, 2001)
     ys = [oracle.mlp_forward(raw, float(v)) for v in xs]
     assert all(b >= a - 1e-12 for a, b in zip(ys, ys[1:]))
+
+
+# ---- the reference's Issue7 data (test_mpsort.py:354-556) -----------------
+def _issue7():
+    import os
+    z = numpy.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "issue7.npz"))
+    return [int(c) for c in z['counts']], z['records'], z['expected'], z['words']
+
+
+def test_issue7_fixture_is_the_reference_test_data():
+    """The fixture holds the 12 rank chunks of Issue7B64 and the records
+    test_issue7 builds from them: radix word 1 = column 0, radix word 0 =
+    column 5 (column 4 << 32 is 0 in u4 arithmetic)."""
+    counts, recs, expected, words = _issue7()
+    assert counts == [0, 0, 2, 7, 1, 0, 4, 0, 0, 6, 14, 5]
+    assert words.shape == (39, 10) and recs.dtype.itemsize == 40
+    assert numpy.array_equal(recs['radix'][:, 1], words[:, 0])
+    assert numpy.array_equal(recs['radix'][:, 0], words[:, 5])
+    assert not recs['ext'].any()
+    # the answer is the stable order of (word 1, word 0)
+    order = sorted(range(len(recs)), key=lambda i: (int(recs['radix'][i, 1]), int(recs['radix'][i, 0])))
+    assert numpy.array_equal(expected, recs[order])
+
+
+def test_issue7_oracle_matches_fixture():
+    counts, recs, expected, _ = _issue7()
+    key = oracle.key_for(recs.dtype, 'radix')
+    out, info = oracle.histogram_sort(recs, counts, counts, key)
+    assert info['rc'] == 0
+    assert numpy.array_equal(out, expected)
+
+
+def test_issue7_reference_matches_fixture():
+    if not oracle.ref_available():
+        pytest.skip("compiled reference not built (oracle/_ref)")
+    counts, recs, expected, _ = _issue7()
+    key = oracle.key_for(recs.dtype, 'radix')
+    assert numpy.array_equal(oracle.ref_radix_sort(recs, key), expected)
+    if oracle.mpi_available():
+        out, _ = oracle.ref_mpi_sort(recs, counts, counts, key)
+        assert numpy.array_equal(out, expected)
