@@ -82,6 +82,27 @@ int mpsort_gpu_newarray(const void *base, const int64_t *nin, int P,
                         void *out, const int64_t *nout, size_t elsize,
                         const mpsort_key_desc *key, unsigned flags);
 
+/* Size of the array behind mpsort_gpu_newarray_noalloc (each GPU the library
+ * runs on holds one until the process ends). */
+#define MPSORT_GPU_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code between HIP graph captures, tight serving loops, harnesses
+ * that forbid hipMalloc: mpsort_gpu_newarray on DEVICE pointers (the flag is
+ * implied), plain path, with every device workspace carved from a fixed
+ * array of MPSORT_GPU_NOALLOC_SCRATCH_BYTES per GPU instead of allocated.  It
+ * calls no hipMalloc, hipFree or other allocating routine.  A sort whose
+ * workspace does not fit (about 2 x n sort records plus tables: roughly 4
+ * million 16-byte records over 64 ranks) fails with MPSORT_ERR_NOMEM and
+ * names its need.  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU race.  Output, counts and statistics equal
+ * mpsort_gpu_newarray's.
+ */
+int mpsort_gpu_newarray_noalloc(const void *base, const int64_t *nin, int P,
+                                void *out, const int64_t *nout, size_t elsize,
+                                const mpsort_key_desc *key, unsigned flags);
+
 /* The same with the reference's opaque radix callback (mpsort.h:1-4): the
  * radix of every record is materialised once on the host (rsize <= 28 bytes,
  * compared as a little-endian unsigned integer, radixsort.c:146-195), the

@@ -41,6 +41,13 @@ int launch_pack(const void* in, int64_t n, int64_t elsize, int64_t key_off, int 
 int launch_gather(const void* in, int64_t n, int64_t elsize, const void* srec, int rw, int iwords, void* out,
                   long long* checksum, uint32_t* err, hipStream_t st);
 
+// Workspace of mpsort_gpu_newarray_noalloc: a fixed array in each GPU's
+// memory, held for the life of the process.  Host code never uses this name
+// as an address: the device address comes from hipGetSymbolAddress (per
+// device, at context set-up) and is kept in DevCtx::scratch.
+constexpr size_t kScratchBytes = (size_t)MPSORT_GPU_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 static char g_err[1024];
 static int g_options = 0;
 static int g_vranks = 0;  // 0 = not set yet (environment decides)
@@ -156,6 +163,8 @@ struct DevCtx {
     Buf mdesc;          // merged SecondSort: group descriptors and their sub-run tables
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    void* scratch = nullptr;      // this device's copy of the scratch array
+    hipError_t scratch_err = hipSuccess;
     hipEvent_t ev[kEvPhase];
     hipEvent_t pass_ev[2][2 * kMaxPasses];
     // last call's device arrays for mpsort_gpu_last_counts
@@ -200,6 +209,10 @@ static int get_ctx(DevCtx** out) {
         for (int a = 0; a < 2; a++)
             for (int i = 0; i < 2 * kMaxPasses; i++) MPSG_CHECK(hipEventCreate(&c.pass_ev[a][i]));
         c.lookback.zero_on_grow = true;
+        // the device address of this GPU's scratch array (the device is
+        // current); only the allocation-free entry point uses it
+        c.scratch_err = hipGetSymbolAddress(&c.scratch, HIP_SYMBOL(scratch));
+        if (c.scratch_err != hipSuccess) c.scratch = nullptr;
         c.init = true;
     }
     *out = &c;
@@ -350,42 +363,86 @@ static double ev_ms(hipEvent_t a, hipEvent_t b) {
     return (double)ms;
 }
 
+// Device workspaces of one plain histogram sort and their sizes.  The
+// sort-record buffers hold n sort records each; s0 is needed with more than
+// one rank or packed records, rbuf with packed records only.
+struct HistNeed {
+    size_t small = 0, s0 = 0, s1 = 0, rbuf = 0, lookback = 0;
+};
+struct HistWork {
+    char* small = nullptr;
+    void *s0 = nullptr, *s1 = nullptr, *rbuf = nullptr;
+    uint64_t* lookback = nullptr;
+    int64_t* host_tab = nullptr;   // 7 P + 4 int64 (tables staged to the device)
+    bool clear_lookback = false;   // the flag words may hold other data: clear them
+};
+
+static int64_t tiles_of(const int64_t* counts, int P, int64_t T) {
+    int64_t t = 0;
+    for (int i = 0; i < P; i++) t += (counts[i] + T - 1) / T;
+    return t;
+}
+
+static HistNeed histsort_need(const CallShape& cs, const int64_t* nin, const int64_t* nout) {
+    HistNeed w;
+    Sizer sz;
+    lay_plan(sz, cs.P, cs.ks.key_bytes, nullptr);
+    lay_plan(sz, cs.P, cs.ks.key_bytes, nullptr);
+    lay_hist(sz, cs.P, nullptr);
+    w.small = sz.off + 4096;
+    const size_t rec = (size_t)cs.ntot * (size_t)cs.rw * 4;
+    w.s0 = (cs.P > 1 || cs.general) ? rec : 0;
+    w.s1 = rec;
+    w.rbuf = cs.general ? rec : 0;
+    const TileCfg tile = default_tile(cs.rw);
+    const int64_t T = (int64_t)tile.block * tile.items;
+    const int64_t nt = std::max<int64_t>(1, std::max(tiles_of(nin, cs.P, T), tiles_of(nout, cs.P, T)));
+    w.lookback = (size_t)nt * kBins * sizeof(uint64_t);
+    return w;
+}
+
+// the per-device cache, grown to the need
+static int histwork_cached(DevCtx& c, const CallShape& cs, const HistNeed& need, HistWork* w) {
+    int rc = 0;
+    if ((rc = c.small.ensure(need.small))) return rc;
+    if (need.s0 && (rc = c.s0.ensure(need.s0))) return rc;
+    if ((rc = c.s1.ensure(need.s1))) return rc;
+    if (need.rbuf && (rc = c.rbuf.ensure(need.rbuf))) return rc;
+    if ((rc = c.lookback.ensure(need.lookback))) return rc;
+    if ((rc = ensure_pinned(c, (size_t)(7 * cs.P + 4) * sizeof(int64_t) + 4096))) return rc;
+    w->small = (char*)c.small.p;
+    w->s0 = c.s0.p;
+    w->s1 = c.s1.p;
+    w->rbuf = c.rbuf.p;
+    w->lookback = (uint64_t*)c.lookback.p;
+    w->host_tab = (int64_t*)c.pinned;
+    w->clear_lookback = false;
+    return 0;
+}
+
 // The histogram sort on device-resident data.  d_in / d_out are the caller's
 // records (device); for the general path they are packed into sort records.
 static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* d_out,
                         const int64_t* nin, const int64_t* nout, const void* packed_host,
-                        mpsort_gpu_stats* stats) {
+                        mpsort_gpu_stats* stats, const HistWork& w) {
     hipStream_t st = c.st;
     const int P = cs.P;
     const int rw = cs.rw;
     const int64_t n = cs.ntot;
     const size_t recbytes = (size_t)rw * 4;
     const int npass = cs.ks.key_bytes;
+    int rc = 0;
 
     // ---- workspace
-    Sizer sz;
-    lay_plan(sz, P, npass, nullptr);
-    lay_plan(sz, P, npass, nullptr);
-    lay_hist(sz, P, nullptr);
-    int rc = c.small.ensure(sz.off + 4096);
-    if (rc) return rc;
-    Arena ar{(char*)c.small.p};
+    Arena ar{w.small};
     PlanArrays pa, pb;
     HistArrays h;
     lay_plan(ar, P, npass, &pa);
     lay_plan(ar, P, npass, &pb);
     lay_hist(ar, P, &h);
 
-    if (P > 1 || cs.general) {
-        if ((rc = c.s0.ensure(n * recbytes))) return rc;
-    }
-    if ((rc = c.s1.ensure(n * recbytes))) return rc;
-    if (cs.general && (rc = c.rbuf.ensure(n * recbytes))) return rc;
-
     // host tables: [beginA, countA, tsA(P+1), beginB, countB, tsB(P+1), C(P+1)]
-    const size_t tab = (size_t)(7 * P + 4);
-    if ((rc = ensure_pinned(c, tab * sizeof(int64_t) + 4096))) return rc;
-    int64_t* ht = (int64_t*)c.pinned;
+    int64_t* ht = w.host_tab;
     int64_t *hbA = ht, *hcA = ht + P, *htA = ht + 2 * P;
     int64_t *hbB = ht + 3 * P + 1, *hcB = ht + 4 * P + 1, *htB = ht + 5 * P + 1;
     int64_t* hC = ht + 6 * P + 2;
@@ -397,7 +454,9 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     hC[0] = 0;
     for (int t = 0; t < P; t++) hC[t + 1] = hC[t] + nout[t];
     const int64_t ntiles = ntA > ntB ? ntA : ntB;
-    if ((rc = c.lookback.ensure((size_t)(ntiles > 0 ? ntiles : 1) * kBins * sizeof(uint64_t)))) return rc;
+    if (w.clear_lookback &&
+        (rc = launch_fill_u32((uint32_t*)w.lookback, 0u, (int64_t)(ntiles > 0 ? ntiles : 1) * kBins * 2, st)))
+        return rc;
 
     MPSG_CHECK(hipMemcpyAsync(pa.begin, hbA, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
     MPSG_CHECK(hipMemcpyAsync(pa.count, hcA, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
@@ -426,7 +485,7 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         p.d_hist = a.hist;
         p.d_digit_base = a.digit_base;
         p.d_pass_flags = a.pass_flags;
-        p.d_lookback = (uint64_t*)c.lookback.p;
+        p.d_lookback = w.lookback;
         p.d_tickets = a.tickets;
         p.d_checksum = h.checksum;
         return p;
@@ -442,14 +501,14 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     if (cs.general) {
         if (packed_host) {
             // the radix-callback form: sort records were built on the host
-            MPSG_CHECK(hipMemcpyAsync(c.rbuf.p, packed_host, n * recbytes, hipMemcpyHostToDevice, st));
+            MPSG_CHECK(hipMemcpyAsync(w.rbuf, packed_host, n * recbytes, hipMemcpyHostToDevice, st));
         } else {
-            if ((rc = launch_pack(d_in, n, cs.elsize, cs.key_off_user, cs.key_bytes_user, c.rbuf.p, rw, cs.iwords,
+            if ((rc = launch_pack(d_in, n, cs.elsize, cs.key_off_user, cs.key_bytes_user, w.rbuf, rw, cs.iwords,
                                   cs.checksum ? h.checksum : nullptr, st)))
                 return rc;
         }
-        recs_in = c.rbuf.p;
-        final_dst = c.rbuf.p;
+        recs_in = w.rbuf;
+        final_dst = w.rbuf;
     }
     const bool cs_direct = cs.checksum && !cs.general;
 
@@ -459,15 +518,15 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     if (P == 1) {
         // one rank: FirstSort is the whole sort (the exchange is the identity
         // and SecondSort re-sorts a sorted run)
-        SortBuffers b{recs_in, final_dst, c.s1.p};
+        SortBuffers b{recs_in, final_dst, w.s1};
         if ((rc = segmented_sort(planA, b, st, cs_direct, cs_direct, &first_passes, pevA))) return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         for (int e = kEvFindP; e <= kEvSecond; e++) MPSG_CHECK(hipEventRecord(c.ev[e], st));
     } else {
-        SortBuffers b1{recs_in, c.s0.p, c.s1.p};
+        SortBuffers b1{recs_in, w.s0, w.s1};
         if ((rc = segmented_sort(planA, b1, st, cs_direct, false, &first_passes, pevA))) return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
-        if ((rc = launch_bisect(c.s0.p, rw, cs.ks, pa.begin, pa.count, P, h.C, h.win, h.myclt, h.mycle,
+        if ((rc = launch_bisect(w.s0, rw, cs.ks, pa.begin, pa.count, P, h.C, h.win, h.myclt, h.mycle,
                                 h.split, h.pminmax, h.rounds, h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFindP], st));
@@ -477,17 +536,17 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
         // the exchange also builds the SecondSort's digit histograms
         if ((rc = seg_clear(planB, st))) return rc;
-        if ((rc = launch_exchange_hist(c.s0.p, c.s1.p, n, rw, P * P, h.run_dst, h.run_src, h.err, h.C, P,
+        if ((rc = launch_exchange_hist(w.s0, w.s1, n, rw, P * P, h.run_dst, h.run_src, h.err, h.C, P,
                                        cs.ks, planB.d_hist, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
-        SortBuffers b2{c.s1.p, final_dst, c.s0.p};
+        SortBuffers b2{w.s1, final_dst, w.s0};
         if ((rc = seg_passes(planB, b2, st, cs_direct ? h.checksum + 1 : nullptr, &second_passes, pevB)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvSecond], st));
     }
     if (cs.general) {
-        if ((rc = launch_gather(d_in, n, cs.elsize, c.rbuf.p, rw, cs.iwords, d_out,
+        if ((rc = launch_gather(d_in, n, cs.elsize, w.rbuf, rw, cs.iwords, d_out,
                                 cs.checksum ? h.checksum + 1 : nullptr, h.err, st)))
             return rc;
     }
@@ -501,9 +560,9 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     MPSG_CHECK(hipMemcpyAsync(&errs[1], pa.pass_flags + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(&errs[2], pb.pass_flags + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(sums, h.checksum, sizeof sums, hipMemcpyDeviceToHost, st));
-    std::vector<int> rounds(P > 1 ? P - 1 : 1, 0);
+    static int rounds[4096];
     if (P > 1)
-        MPSG_CHECK(hipMemcpyAsync(rounds.data(), h.rounds, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipMemcpyAsync(rounds, h.rounds, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipStreamSynchronize(st));
     for (int i = 0; i + 1 < P; i++) rounds_host = rounds[i] > rounds_host ? rounds[i] : rounds_host;
 
@@ -1662,8 +1721,76 @@ int mpsort_gpu_newarray(const void* base, const int64_t* nin, int P, void* out, 
     memset(&st, 0, sizeof st);
     remember_key(*c, key, 0, 0);
     rc = staged_call(c, cs, base, out, devptr, P, &st, [&](const void* d_in, void* d_out) {
-        return run_histsort(*c, cs, d_in, d_out, nin, nout, nullptr, &st);
+        HistWork w;
+        int r2 = histwork_cached(*c, cs, histsort_need(cs, nin, nout), &w);
+        return r2 ? r2 : run_histsort(*c, cs, d_in, d_out, nin, nout, nullptr, &st, w);
     });
+    g_last = st;
+    return rc;
+}
+
+int mpsort_gpu_newarray_noalloc(const void* base, const int64_t* nin, int P, void* out, const int64_t* nout,
+                                size_t elsize, const mpsort_key_desc* key, unsigned flags) {
+    g_err[0] = 0;
+    int64_t ntot = 0;
+    int rc = validate_counts(nin, P, nout, &ntot);
+    if (rc) return rc;
+    if (flags & (MPSORT_GPU_LEARNED | MPSORT_GPU_LEARNED_EXCHANGE))
+        return fail(MPSORT_ERR_UNSUPPORTED, "the allocation-free entry point runs the plain path only");
+    flags |= MPSORT_GPU_DEVICE_PTRS;
+    CallShape cs;
+    memset(&cs, 0, sizeof cs);
+    if ((rc = shape_for(key, elsize, flags, base, out, ntot, &cs))) return rc;
+    cs.P = P;
+    cs.ntot = ntot;
+    if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");
+    DevCtx* c = nullptr;
+    if ((rc = get_ctx(&c))) return rc;
+    if (!c->scratch)
+        return fail(MPSORT_ERR_HIP, "the scratch array of device %d has no address (HIP error %d)", c->device,
+                    (int)c->scratch_err);
+    mpsort_gpu_stats st;
+    memset(&st, 0, sizeof st);
+    remember_key(*c, key, 0, 0);
+    if (ntot == 0) {
+        st.nvranks = P;
+        c->last_P = 0;
+        g_last = st;
+        return 0;
+    }
+    // the workspaces, carved from the scratch array: tables, look-back
+    // flags, sort-record buffers, and (in-place packed records) a copy of
+    // the input for the final gather
+    const HistNeed need = histsort_need(cs, nin, nout);
+    const size_t bytes = (size_t)ntot * elsize;
+    const char *a = (const char*)base, *b = (const char*)out;
+    const size_t incopy = (cs.general && a < b + bytes && b < a + bytes) ? bytes : 0;
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t total = up(need.small) + up(need.lookback) + up(need.s0) + up(need.s1) + up(need.rbuf) + up(incopy);
+    if (total > kScratchBytes)
+        return fail(MPSORT_ERR_NOMEM, "this sort needs %zu bytes of workspace; the allocation-free entry point has "
+                                      "%zu (MPSORT_GPU_NOALLOC_SCRATCH_BYTES)", total, kScratchBytes);
+    static int64_t host_tab[7 * 4096 + 4];
+    char* p = (char*)c->scratch;
+    HistWork w;
+    w.small = p;
+    p += up(need.small);
+    w.lookback = (uint64_t*)p;
+    p += up(need.lookback);
+    w.s0 = need.s0 ? (void*)p : nullptr;
+    p += up(need.s0);
+    w.s1 = p;
+    p += up(need.s1);
+    w.rbuf = need.rbuf ? (void*)p : nullptr;
+    p += up(need.rbuf);
+    w.host_tab = host_tab;
+    w.clear_lookback = true;
+    const void* d_in = base;
+    if (incopy) {
+        MPSG_CHECK(hipMemcpyAsync(p, base, bytes, hipMemcpyDeviceToDevice, c->st));
+        d_in = p;
+    }
+    rc = run_histsort(*c, cs, d_in, out, nin, nout, nullptr, &st, w);
     g_last = st;
     return rc;
 }
@@ -1744,7 +1871,9 @@ int mpsort_gpu_newarray_radix(const void* base, const int64_t* nin, int P, void*
     // the gather's device checksum covers the output; the input sum is the
     // host's (the device sees only the packed records)
     cs.checksum = !(flags & MPSORT_GPU_NO_CHECKSUM);
-    rc = run_histsort(*c, cs, c->din.p, c->dout.p, nin, nout, packed, &st);
+    HistWork w;
+    rc = histwork_cached(*c, cs, histsort_need(cs, nin, nout), &w);
+    if (!rc) rc = run_histsort(*c, cs, c->din.p, c->dout.p, nin, nout, packed, &st, w);
     mpiu_free(packed, __FILE__, __LINE__);
     if (!rc) {
         MPSG_CHECK(hipMemcpy(out, c->dout.p, bytes, hipMemcpyDeviceToHost));

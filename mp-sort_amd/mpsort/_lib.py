@@ -50,7 +50,7 @@ EXPORTS = [
     "mpsort_gpu_newarray_learned", "mpsort_gpu_segplan_create", "mpsort_gpu_segplan_add",
     "mpsort_gpu_segplan_run", "mpsort_gpu_segplan_info", "mpsort_gpu_segplan_free",
     "mpsort_gpu_sort_segments",
-    "mpiu_malloc", "mpiu_free", "mpiu_set_malloc",
+    "mpiu_malloc", "mpiu_free", "mpiu_set_malloc", "mpsort_gpu_newarray_noalloc",
 ]
 
 SEGPLAN_GRAPH = 1 << 0
@@ -118,6 +118,9 @@ def _load():
     lib.mpsort_gpu_newarray.argtypes = [ctypes.c_void_p, i64p, ctypes.c_int, ctypes.c_void_p, i64p,
                                         ctypes.c_size_t, P(KeyDesc), ctypes.c_uint]
     lib.mpsort_gpu_newarray.restype = ctypes.c_int
+    lib.mpsort_gpu_newarray_noalloc.argtypes = [ctypes.c_void_p, i64p, ctypes.c_int, ctypes.c_void_p, i64p,
+                                                ctypes.c_size_t, P(KeyDesc), ctypes.c_uint]
+    lib.mpsort_gpu_newarray_noalloc.restype = ctypes.c_int
     lib.mpsort_gpu_newarray_radix.argtypes = [ctypes.c_void_p, i64p, ctypes.c_int, ctypes.c_void_p,
                                               i64p, ctypes.c_size_t, RADIX_FN, ctypes.c_size_t,
                                               ctypes.c_void_p, ctypes.c_uint]
