@@ -34,6 +34,11 @@ PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_
 # (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
 TRAFFIC_ONESWEEP = 9117953756
 TRAFFIC_BUCKET_SORT = 9445766080
+# average launch time of the same kernels over the same launches (the timed
+# steps of `python bench.py --steps 20 --warmup 5`) from its rocprofv3 kernel
+# trace; filled from the committed trace summary
+ROCPROF_SOURCE = "profiles/histsort_clustered_2e28x16B_kernel_trace.md"
+ROCPROF_MS = {}
 # per-side stages of the learned local sort, as timed by the library; with
 # the merged SecondSort also its two gather-and-sort launches alone
 # (merge_B: groups of buckets, then the larger single buckets), which
@@ -47,27 +52,76 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(recs, nsample):
-    """Reference mpsort_omp (oracle/_ref, -O3) on the first nsample records."""
+def host_cores():
+    """(threads used, logical CPUs of the host, CPUs this process may use)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    # the GPU machine gives one GPU's job a 16-thread share of its host
+    threads = env if env > 0 else min(16, avail)
+    return threads, os.cpu_count() or 1, avail
+
+
+def cpu_baseline(recs, full=True):
+    """The reference on this host's cores, on the same records.
+
+    value: mpsort_omp (radixsort.c + mpsort-omp.c compiled with -O3 -fopenmp
+    from the reference sources, oracle/_ref) on all records, OMP threads =
+    the job's host share, median of 3.  Beside it: the reference Makefile's
+    default build (-g -Wall -fopenmp, no optimisation), one run on all
+    records; and mpirun of mpsort_mpi_newarray (mpsort-mpi.c + mp-mpiu.c)
+    with one rank per thread on a 2^26-record sample, median of 3 (barrier +
+    MPI_Wtime around the call, as bench-mpi.c:205-214)."""
     import numpy
     import oracle
     if not oracle.ref_available():
         return None
-    threads = min(16, os.cpu_count() or 1)
-    sample = numpy.ascontiguousarray(recs[:nsample])
-    key = oracle.key_for(sample.dtype, 'key')
+    threads, ncpu, avail = host_cores()
+    n = len(recs)
+    key = oracle.key_for(recs.dtype, 'key')
     times = []
     for _ in range(3):
-        t0 = time.perf_counter()
-        oracle.ref_mpsort_omp(sample, key, nthreads=threads)
-        times.append(time.perf_counter() - t0)
+        times.append(oracle.ref_mpsort_omp(recs, key, nthreads=threads, timed=True)[1])
+        log("cpu baseline: mpsort_omp -O3 %d threads on %d records: %.3f s" % (threads, n, times[-1]))
     times.sort()
     t = times[1]
-    return {"value": round(nsample / t / 1e6, 3), "unit": "Mrecords/s", "cores": threads,
-            "kind": "reference",
-            "sample": "mpsort_omp (reference radixsort.c + mpsort-omp.c, gcc -O3 -fopenmp) on the "
-                      "first %d records of the same clustered 16 B workload, median of 3, %.3f s"
-                      % (nsample, t)}
+    out = {"value": round(n / t / 1e6, 3), "unit": "Mrecords/s", "cores": threads, "kind": "reference",
+           "sample": "mpsort_omp (reference radixsort.c + mpsort-omp.c, gcc -O3 -fopenmp) on all %d records of "
+                     "the same clustered 16 B workload, %d OpenMP threads (the job's host share; the host has %d "
+                     "logical CPUs, %d available to this process), median of 3: %.3f s" % (n, threads, ncpu, avail, t),
+           "host_logical_cpus": ncpu, "host_cpus_available": avail}
+    if full and oracle.ref_available(default_flags=True):
+        td = oracle.ref_mpsort_omp(recs, key, nthreads=threads, default_flags=True, timed=True)[1]
+        log("cpu baseline: mpsort_omp default build on %d records: %.3f s" % (n, td))
+        out["default_build"] = {"value": round(n / td / 1e6, 3), "s": round(td, 3),
+                                "build": "reference Makefile default CFLAGS -g -Wall -fopenmp (no -O)",
+                                "records": n, "threads": threads}
+    if full and oracle.mpi_available():
+        m = min(n, 1 << 26)
+        sample = numpy.ascontiguousarray(recs[:m])
+        cnt = [m * (t2 + 1) // threads - m * t2 // threads for t2 in range(threads)]
+        try:
+            sorted_m, tm = oracle.ref_mpi_sort(sample, cnt, cnt, key, repeat=3, timeout=300)
+            ok = bool(numpy.all(sorted_m['key'][1:] >= sorted_m['key'][:-1]))
+            log("cpu baseline: mpirun -n %d mpsort_mpi on %d records: %.3f s" % (threads, m, tm))
+            out["mpi"] = {"value": round(m / tm / 1e6, 3), "s": round(tm, 4), "ranks": threads, "records": m,
+                          "sorted": ok,
+                          "build": "mpsort-mpi.c + mp-mpiu.c + radixsort.c, mpicc -O3 (MPICH)"}
+        except Exception as e:  # MPI runtime problems are reported, not fatal
+            out["mpi"] = {"error": str(e)[-300:]}
+    return out
+
+
+def check_stable_sorted(out, n):
+    """sorted by key, ties in input (idx) order, and a permutation of 0..n-1"""
+    import numpy
+    k = out['key']
+    ok = bool(numpy.all(k[1:] >= k[:-1]))
+    ties = k[1:] == k[:-1]
+    ok = ok and bool(numpy.all(out['idx'][1:][ties] > out['idx'][:-1][ties]))
+    del ties
+    seen = numpy.zeros(n, dtype=bool)
+    seen[out['idx']] = True
+    return ok and bool(seen.all())
 
 
 def bench_batched(args):
@@ -277,14 +331,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--vranks", type=int, default=64)
-    ap.add_argument("--cpu-sample-log2", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain path only")
     ap.add_argument("--host-leg", action="store_true",
                     help="also time one call on host buffers (PCIe transfers included)")
-    ap.add_argument("--workload", default="histsort", choices=["histsort", "batched", "snapshot"],
+    ap.add_argument("--workload", default="histsort", choices=["histsort", "batched", "snapshot", "dists"],
                     help="histsort: the 2^28 x 16 B histogram sort (default); batched: 10,000 independent "
-                         "sorts; snapshot: two 2^27 x 32 B particle snapshots sorted by id")
+                         "sorts; snapshot: two 2^27 x 32 B particle snapshots sorted by id; dists: the "
+                         "learned path on config 3's four key distributions")
+    ap.add_argument("--dists", default=",".join(DISTS))
+    ap.add_argument("--cpu-quick", action="store_true",
+                    help="CPU baseline: the -O3 mpsort_omp only (no default build, no mpirun)")
     ap.add_argument("--segments", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain copy kernel of n*16 bytes (byte-counter calibration)")
@@ -294,9 +351,7 @@ def main():
               "on that GPU (use --vranks), and no multi-GPU figure is produced", file=sys.stderr)
         return 2
 
-    import numpy
-    from mpsort import _lib, workloads
-    lib = _lib.lib
+    from mpsort import _lib
     if _lib.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 3
@@ -308,78 +363,130 @@ def main():
         if args.log2n == 28:
             args.log2n = 27
         return bench_snapshot(args)
+    if args.workload == "dists":
+        return bench_dists(args)
 
+    return bench_histsort(args)
+
+
+def _device_copy(recs):
+    from mpsort import _lib
+    lib = _lib.lib
+    d = lib.mpsort_gpu_dmalloc(recs.nbytes)
+    if not d:
+        raise MemoryError("device allocation of %d bytes failed" % recs.nbytes)
+    _lib.check(lib.mpsort_gpu_memcpy_h2d(d, recs.ctypes.data, recs.nbytes))
+    return d
+
+
+def _timed(step, label, steps, warmup):
+    """warmup + timed calls of step; (ms per step, per-call stats of the timed
+    calls, per-call stats of the warmup calls)"""
+    from mpsort import _lib
+    lib = _lib.lib
+    warm = []
+    for i in range(warmup):
+        step()
+        warm.append(_lib.last_stats().as_dict())
+        log("%s warmup %d done" % (label, i))
+    _lib.check(lib.mpsort_gpu_synchronize())
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step()
+        stats.append(_lib.last_stats().as_dict())
+    _lib.check(lib.mpsort_gpu_synchronize())
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, stats, warm
+
+
+def train_model(n, P, key, gen, seed, label):
+    """A model trained by the GPU kernels on one plain GPU sort of a training
+    realisation gen(seed) (2^20 stratified samples with their exact ranks);
+    returns (model, plain sort ms, training s)."""
+    from mpsort import _lib, learned
+    lib = _lib.lib
+    t0 = time.perf_counter()
+    recs = gen(n, seed)
+    log("%s: training realisation (seed %d) generated in %.2f s" % (label, seed, time.perf_counter() - t0))
+    din = _device_copy(recs)
+    del recs
+    dout = lib.mpsort_gpu_dmalloc(n * 16)
+    cnt = [n * (t + 1) // P - n * t // P for t in range(P)]
+    nin = (ctypes.c_int64 * P)(*cnt)
+    _lib.check(lib.mpsort_gpu_newarray(din, nin, P, dout, nin, 16, ctypes.byref(key), _lib.DEVICE_PTRS))
+    _lib.check(lib.mpsort_gpu_synchronize())
+    t_plain = _lib.last_stats().t_total_ms
+    lib.mpsort_gpu_dfree(din)
+    t0 = time.perf_counter()
+    model = learned.train(None, nsample=1 << 20, seed=1, device_ptr=(dout, n, 16, key))
+    t_train = time.perf_counter() - t0
+    lib.mpsort_gpu_dfree(dout)
+    return model, t_plain, t_train
+
+
+def bench_histsort(args):
+    """The headline: 2^28 clustered 16 B records on 64 virtual ranks.
+
+    The model is trained on a plain sort of ANOTHER realisation of the same
+    clustered field (same clumps, independent points: field seed 2, point
+    seed 3); the timed array is the realisation of point seed 2.  Reported
+    beside the reuse rate: the one-shot cost of an unseen field (plain sort
+    of a training realisation + training + the learned sort)."""
+    import numpy
+    from mpsort import _lib, workloads
+    lib = _lib.lib
     n = 1 << args.log2n
     P = args.vranks
-    t0 = time.perf_counter()
-    recs = workloads.clustered_records(n, seed=2)
-    log("generated %d clustered records in %.2f s" % (n, time.perf_counter() - t0))
-
-    nb = recs.nbytes
-    din = lib.mpsort_gpu_dmalloc(nb)
-    dout = lib.mpsort_gpu_dmalloc(nb)
-    if not din or not dout:
-        print("bench.py: device allocation failed", file=sys.stderr)
-        return 4
-    _lib.check(lib.mpsort_gpu_memcpy_h2d(din, recs.ctypes.data, nb))
     cnt = [n * (t + 1) // P - n * t // P for t in range(P)]
     nin = (ctypes.c_int64 * P)(*cnt)
     key = _lib.KeyDesc(offset=0, width=8, is_signed=0, nwords=1, reserved=0)
     flags = _lib.DEVICE_PTRS | _lib.TIMING
+    field = 2
+
+    model = None
+    if not args.no_learned:
+        model, t_plain_train, t_train = train_model(
+            n, P, key, lambda m, sd: workloads.clustered_records(m, seed=sd, field_seed=field), 3, "clustered")
+        minfo = model.info()
+        log("model: trained in %.3f s (%d Adam steps, %d leaves, err_max %.2e, max leaf %d samples)"
+            % (t_train, minfo['steps'], minfo['leaves'], minfo['err_max'], minfo['max_leaf_samples']))
+
+    t0 = time.perf_counter()
+    recs = workloads.clustered_records(n, seed=2, field_seed=field)
+    log("generated %d clustered records (timed realisation) in %.2f s" % (n, time.perf_counter() - t0))
+    nb = recs.nbytes
+    din = _device_copy(recs)
+    dout = lib.mpsort_gpu_dmalloc(nb)
+    if not dout:
+        print("bench.py: device allocation failed", file=sys.stderr)
+        return 4
 
     def plain_step():
         _lib.check(lib.mpsort_gpu_newarray(din, nin, P, dout, nin, 16, ctypes.byref(key), flags))
 
-    def timed(step, label):
-        for i in range(args.warmup):
-            step()
-            log("%s warmup %d done" % (label, i))
-        _lib.check(lib.mpsort_gpu_synchronize())
-        stats = []
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            step()
-            stats.append(_lib.last_stats().as_dict())
-        _lib.check(lib.mpsort_gpu_synchronize())
-        ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        log("%s: %.3f ms per step, %.1f Mrecords/s" % (label, ms, n / (ms * 1e-3) / 1e6))
-        return ms, stats
-
     # ---- plain path ------------------------------------------------------
-    ms_plain, st_plain = timed(plain_step, "plain")
+    ms_plain, st_plain, _ = _timed(plain_step, "plain", args.steps, args.warmup)
+    log("plain: %.3f ms per step, %.1f Mrecords/s" % (ms_plain, n / (ms_plain * 1e-3) / 1e6))
     if args.copy_probe:
         _lib.check(lib.mpsort_gpu_dcopy(dout, din, nb))
         log("copy probe: copy_kernel<4> moved %d bytes in and %d bytes out" % (nb, nb))
         plain_step()
     plain_out = numpy.empty_like(recs)
     _lib.check(lib.mpsort_gpu_memcpy_d2h(plain_out.ctypes.data, dout, nb))
-    k = plain_out['key']
-    ok = bool(numpy.all(k[1:] >= k[:-1]))
-    ties = k[1:] == k[:-1]
-    ok = ok and bool(numpy.all(plain_out['idx'][1:][ties] > plain_out['idx'][:-1][ties]))
-    del k, ties
-    if not ok:
+    if not check_stable_sorted(plain_out, n):
         print("bench.py: plain output is not the stable sort", file=sys.stderr)
         return 5
 
-    # ---- learned path ----------------------------------------------------
-    # model: 2^20 stratified samples with exact ranks from the plain pass
-    # output, trained by the GPU kernels (not part of the timed step)
+    # ---- learned path (model from the other realisation) ----------------
     result_learned = None
-    if not args.no_learned:
-        from mpsort import learned
-        t0 = time.perf_counter()
-        model = learned.train(None, nsample=1 << 20, seed=1, device_ptr=(dout, n, 16, key))
-        t_train = time.perf_counter() - t0
-        minfo = model.info()
-        log("model: trained in %.3f s (%d Adam steps, %d leaves, err_max %.2e, max leaf %d samples)"
-            % (t_train, minfo['steps'], minfo['leaves'], minfo['err_max'], minfo['max_leaf_samples']))
-
+    if model is not None:
         def learned_step():
             _lib.check(lib.mpsort_gpu_newarray_learned(din, nin, P, dout, nin, 16, ctypes.byref(key),
                                                        model._h, flags))
 
-        ms_learned, st_learned = timed(learned_step, "learned")
+        ms_learned, st_learned, warm_learned = _timed(learned_step, "learned", args.steps, args.warmup)
+        log("learned: %.3f ms per step, %.1f Mrecords/s" % (ms_learned, n / (ms_learned * 1e-3) / 1e6))
         lout = numpy.empty_like(recs)
         _lib.check(lib.mpsort_gpu_memcpy_d2h(lout.ctypes.data, dout, nb))
         same = bool(numpy.array_equal(lout, plain_out))
@@ -388,18 +495,28 @@ def main():
             print("bench.py: learned output differs from the plain path", file=sys.stderr)
             return 6
         L = st_learned[-1]
-        result_learned = {"ms_per_step": round(ms_learned, 3),
-                          "value": round(n / (ms_learned * 1e-3) / 1e6, 2),
-                          "bit_identical_to_plain": same,
-                          "train_s": round(t_train, 3), "model": {k2: minfo[k2] for k2 in minfo},
-                          "splitters_exact": L["splitters_exact"],
-                          "splitters_repaired": L["splitters_repaired"],
-                          "repair_rounds": L["repair_rounds"],
-                          "buckets_overflow": L["buckets_overflow"],
-                          "plain_sides": L["plain_sides"],
-                          "buckets_lds_radix": L["buckets_lds_radix"],
-                          "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
-                          "kernels_ms": dict(zip(LEARNED_KERNELS, [round(x, 4) for x in L["pass_ms"]]))}
+        one_shot_ms = t_plain_train + t_train * 1e3 + ms_learned
+        gain = ms_plain - ms_learned
+        result_learned = {
+            "ms_per_step": round(ms_learned, 3),
+            "value": round(n / (ms_learned * 1e-3) / 1e6, 2),
+            "bit_identical_to_plain": same,
+            "model_trained_on": "plain GPU sort of another realisation of the clustered field (same clumps, "
+                                "independent points: field seed %d, point seed 3; timed: point seed 2)" % field,
+            "train_s": round(t_train, 3), "model": {k2: minfo[k2] for k2 in minfo},
+            "one_shot": {"ms": round(one_shot_ms, 2), "plain_sort_of_training_array_ms": round(t_plain_train, 3),
+                         "train_ms": round(t_train * 1e3, 1), "learned_sort_ms": round(ms_learned, 3),
+                         "vs_plain": round(ms_plain / one_shot_ms, 3),
+                         "break_even_sorts": (int(numpy.ceil((t_plain_train + t_train * 1e3) / gain))
+                                              if gain > 0 else None)},
+            "splitters_exact": L["splitters_exact"],
+            "splitters_repaired": L["splitters_repaired"],
+            "repair_rounds": L["repair_rounds"],
+            "buckets_overflow": L["buckets_overflow"],
+            "plain_sides": L["plain_sides"],
+            "buckets_lds_radix": L["buckets_lds_radix"],
+            "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
+            "kernels_ms": dict(zip(LEARNED_KERNELS, [round(x, 4) for x in L["pass_ms"]]))}
         log("learned phases: " + json.dumps(result_learned["phases_ms"]))
         log("learned kernels: " + json.dumps(result_learned["kernels_ms"]))
         model.close()
@@ -423,15 +540,23 @@ def main():
     last = st_plain[-1]
     alg_bytes = 2.0 * n * 16  # one read and one write of every record
 
-    def roof(launch_ms, kernel, traffic, launch_bytes=alg_bytes):
+    def roof(launch_ms, kernel, traffic, rocprof_ms, launch_bytes=alg_bytes):
         avg = sum(launch_ms) / len(launch_ms)
         achieved = launch_bytes / (avg * 1e-3) / 1e9
-        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
-                "avg_launch_ms": round(avg, 4), "launches": len(launch_ms)}
+        r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+             "avg_launch_ms": round(avg, 4), "launches": len(launch_ms),
+             "timing": "HIP events around each launch on the library's stream, the timed steps only "
+                       "(warmup launches, the cold first one included, are left out)"}
+        if rocprof_ms:
+            r["rocprof"] = {"avg_launch_ms": rocprof_ms, "frac": round(launch_bytes / (rocprof_ms * 1e-3) / 1e9 /
+                                                                        HBM_PEAK_GBS, 4),
+                            "source": ROCPROF_SOURCE}
+        return r
 
     plain_roof = roof([x for st in st_plain for x in st["pass_ms"]],
-                      "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP)
+                      "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP,
+                      ROCPROF_MS.get("onesweep"))
     if result_learned is not None:
         ib = LEARNED_KERNELS.index
         # the step's largest kernel: the FirstSort's in-LDS bucket sort, one
@@ -439,12 +564,12 @@ def main():
         # its records over two smaller kernels, merge_B in kernels_ms)
         roofline = roof([st["pass_ms"][ib("bucket_sort_A")] for st in st_learned],
                         "interp_sort_kernel<4,512,9> (FirstSort in-LDS bucket sort, one launch per step over all n "
-                        "records: 2*n*16 B per launch)", TRAFFIC_BUCKET_SORT)
+                        "records: 2*n*16 B per launch)", TRAFFIC_BUCKET_SORT, ROCPROF_MS.get("bucket_sort_A"))
         result_learned["roofline_by_stage"] = {
             k: {"avg_ms": round(sum(st["pass_ms"][ib(k)] for st in st_learned) / len(st_learned), 4),
                 "GB/s": round((alg_bytes / 2 if k.startswith("hist") else alg_bytes) /
                               (sum(st["pass_ms"][ib(k)] for st in st_learned) / len(st_learned) * 1e-3) / 1e9, 1)}
-            for k in ("hist_A", "scatter_A", "bucket_sort_A", "hist_B", "scatter_B", "bucket_sort_B")}
+            for k in ("hist_A", "scatter_A", "bucket_sort_A", "merge_B")}
     else:
         roofline = plain_roof
     log("plain phases (ms): first %.3f findP %.3f lay %.3f exch %.3f second %.3f total %.3f; passes %d+%d, "
@@ -455,7 +580,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(recs, 1 << min(args.cpu_sample_log2, args.log2n))
+        cpu = cpu_baseline(recs, full=not args.cpu_quick)
     value_plain = n / (ms_plain * 1e-3) / 1e6
     if result_learned is not None:
         value, ms, path = result_learned["value"], result_learned["ms_per_step"], "learned"
@@ -474,10 +599,90 @@ def main():
         "learned": result_learned,
         "host_pointer_call": host_leg,
         "learned_vs_plain": round(value / value_plain, 3) if result_learned else None,
+        "sort_equivalent_GBps": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
         "roofline": roofline,
         "plain_roofline": plain_roof,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+DISTS = ("uniform", "lognormal", "zipf", "nearly_sorted")
+
+
+def bench_dists(args):
+    """Config 3's four key distributions at the headline shape (2^28 x 16 B,
+    64 virtual ranks): uniform u64, a log-space mixture of 8 Gaussians, Zipf
+    s=1.1 over 2^20 values, a sorted ramp with 1% of the entries swapped in
+    random pairs.  For each: the plain sort, a model trained on a plain sort
+    of an independent 2^24-record realisation of the distribution, the
+    learned sort (checked byte for byte against the plain output), and the
+    learned path's splitter repair and fallback counts."""
+    import numpy
+    from mpsort import _lib, workloads
+    lib = _lib.lib
+    n = 1 << args.log2n
+    P = args.vranks
+    cnt = [n * (t + 1) // P - n * t // P for t in range(P)]
+    nin = (ctypes.c_int64 * P)(*cnt)
+    key = _lib.KeyDesc(offset=0, width=8, is_signed=0, nwords=1, reserved=0)
+    flags = _lib.DEVICE_PTRS | _lib.TIMING
+    gens = {"uniform": workloads.uniform, "lognormal": workloads.lognormal_mix, "zipf": workloads.zipf,
+            "nearly_sorted": workloads.nearly_sorted}
+    out = {}
+    for dist in args.dists.split(","):
+        g = gens[dist]
+        model, _, t_train = train_model(1 << 24, P, key, lambda m, sd: workloads.records16(g(m, sd)), 101, dist)
+        t0 = time.perf_counter()
+        recs = workloads.records16(g(n, 5))
+        log("%s: %d records generated in %.2f s" % (dist, n, time.perf_counter() - t0))
+        din = _device_copy(recs)
+        dout = lib.mpsort_gpu_dmalloc(recs.nbytes)
+
+        def plain_step():
+            _lib.check(lib.mpsort_gpu_newarray(din, nin, P, dout, nin, 16, ctypes.byref(key), flags))
+
+        def learned_step():
+            _lib.check(lib.mpsort_gpu_newarray_learned(din, nin, P, dout, nin, 16, ctypes.byref(key),
+                                                       model._h, flags))
+
+        ms_p, st_p, _ = _timed(plain_step, dist + " plain", args.steps, args.warmup)
+        ref = numpy.empty_like(recs)
+        _lib.check(lib.mpsort_gpu_memcpy_d2h(ref.ctypes.data, dout, recs.nbytes))
+        ok = check_stable_sorted(ref, n)
+        ms_l, st_l, _ = _timed(learned_step, dist + " learned", args.steps, args.warmup)
+        got = numpy.empty_like(recs)
+        _lib.check(lib.mpsort_gpu_memcpy_d2h(got.ctypes.data, dout, recs.nbytes))
+        same = bool(numpy.array_equal(got, ref))
+        del got, ref, recs
+        lib.mpsort_gpu_dfree(din)
+        lib.mpsort_gpu_dfree(dout)
+        L = st_l[-1]
+        out[dist] = {"plain_ms": round(ms_p, 3), "learned_ms": round(ms_l, 3),
+                     "learned_vs_plain": round(ms_p / ms_l, 3),
+                     "learned_Mrecords_s": round(n / (ms_l * 1e-3) / 1e6, 1),
+                     "plain_stable_sorted": ok, "learned_bit_identical_to_plain": same,
+                     "train_s": round(t_train, 3), "model_err_max": model.info()["err_max"],
+                     "splitters_exact": L["splitters_exact"], "splitters_repaired": L["splitters_repaired"],
+                     "repair_rounds": L["repair_rounds"], "plain_bisect_rounds": st_p[-1]["bisect_rounds"],
+                     "t_findp_ms": {"learned": round(L["t_findp_ms"], 4), "plain": round(st_p[-1]["t_findp_ms"], 4)},
+                     "buckets_overflow": L["buckets_overflow"], "plain_sides": L["plain_sides"],
+                     "buckets_lds_radix": L["buckets_lds_radix"],
+                     "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES}}
+        model.close()
+        log("%s: %s" % (dist, json.dumps(out[dist])))
+        if not (ok and same):
+            print("bench.py: %s output check failed" % dist, file=sys.stderr)
+            return 6
+    worst = min(v["learned_vs_plain"] for v in out.values())
+    result = {"metric": "Mrecords/s, learned path on config-3 key distributions (2^%d x 16 B, %d virtual ranks)"
+                        % (args.log2n, P),
+              "value": round(min(v["learned_Mrecords_s"] for v in out.values()), 1), "unit": "Mrecords/s",
+              "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+              "config": {"workload": "config 3 distributions " + ",".join(out), "records": n, "vranks": P},
+              "worst_learned_vs_plain": worst, "distributions": out}
     print(json.dumps(result), flush=True)
     return 0
 

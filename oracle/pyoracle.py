@@ -5,6 +5,7 @@ import os
 import shutil
 import subprocess
 import tempfile
+import time
 
 import numpy
 
@@ -219,13 +220,18 @@ def ref_radix_sort(records, key, default_flags=False):
     return out
 
 
-def ref_mpsort_omp(records, key, nthreads=0, default_flags=False, inplace=False):
+def ref_mpsort_omp(records, key, nthreads=0, default_flags=False, inplace=False, timed=False):
+    """The compiled reference mpsort_omp; with timed=True also the seconds of
+    the call itself (the copy of the input made before it is not counted)."""
     out = records if inplace else numpy.array(records, copy=True)
+    t = 0.0
     if len(out):
+        t0 = time.perf_counter()
         _reflib(default_flags).ref_mpsort_omp(_ptr(out), ctypes.c_size_t(len(out)),
                                               ctypes.c_size_t(out.dtype.itemsize), ctypes.byref(key),
                                               ctypes.c_int(nthreads))
-    return out
+        t = time.perf_counter() - t0
+    return (out, t) if timed else out
 
 
 def _mpiexec():
