@@ -513,6 +513,7 @@ def bench_histsort(args):
             "splitters_repaired": L["splitters_repaired"],
             "repair_rounds": L["repair_rounds"],
             "buckets_overflow": L["buckets_overflow"],
+            "buckets_overflow_first_sort": L["buckets_overflow_first"],
             "plain_sides": L["plain_sides"],
             "buckets_lds_radix": L["buckets_lds_radix"],
             "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
@@ -667,9 +668,12 @@ def bench_dists(args):
                      "splitters_exact": L["splitters_exact"], "splitters_repaired": L["splitters_repaired"],
                      "repair_rounds": L["repair_rounds"], "plain_bisect_rounds": st_p[-1]["bisect_rounds"],
                      "t_findp_ms": {"learned": round(L["t_findp_ms"], 4), "plain": round(st_p[-1]["t_findp_ms"], 4)},
-                     "buckets_overflow": L["buckets_overflow"], "plain_sides": L["plain_sides"],
+                     "buckets_overflow": L["buckets_overflow"],
+                     "buckets_overflow_first_sort": L["buckets_overflow_first"],
+                     "plain_sides": L["plain_sides"],
                      "buckets_lds_radix": L["buckets_lds_radix"],
-                     "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES}}
+                     "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
+                     "kernels_ms": dict(zip(LEARNED_KERNELS, [round(x, 4) for x in L["pass_ms"]]))}
         model.close()
         log("%s: %s" % (dist, json.dumps(out[dist])))
         if not (ok and same):

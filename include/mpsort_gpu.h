@@ -144,7 +144,7 @@ typedef struct {
     double t_model_ms;
     double t_bucket_ms;
     int32_t buckets_lds_radix; /* buckets sorted by the in-LDS radix sort */
-    int32_t reserved3;
+    int32_t buckets_overflow_first; /* of buckets_overflow, those of the FirstSort */
 } mpsort_gpu_stats;
 
 int mpsort_gpu_last_stats(mpsort_gpu_stats *stats);

@@ -24,10 +24,12 @@
 namespace mpsg {
 
 // histsort.hip
+// model (optional): values tested are the model's interpolated quantiles
+// instead of midpoints; exact[i] = 1 when the first one was the splitter
 int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sbeg,
                   const int64_t* scnt, int P, const int64_t* C, int64_t* win, int64_t* myclt,
                   int64_t* mycle, uint64_t* split, uint64_t* pminmax, int* rounds, uint32_t* err,
-                  hipStream_t st, const uint64_t* init = nullptr, int* exact = nullptr);
+                  hipStream_t st, const ModelDev* model = nullptr, int* exact = nullptr);
 int launch_layout(int P, const int64_t* C, const int64_t* scnt, const int64_t* sbeg, int64_t* myclt,
                   int64_t* mycle, int64_t* myC, int64_t* send, int64_t* run_dst, int64_t* run_src,
                   uint32_t* err, hipStream_t st);
@@ -148,8 +150,6 @@ struct DevCtx {
     bool init = false;
     int device = 0;
     hipStream_t st = nullptr;
-    hipStream_t st2 = nullptr;   // second stream (default priority): work that overlaps the splitter search
-    hipEvent_t fork = nullptr, join = nullptr, fork0 = nullptr, proposed = nullptr;
     uint32_t epoch = 0;
     Buf din, dout;      // host staging
     Buf s0, s1, rbuf;   // sort-record buffers
@@ -200,11 +200,6 @@ static int get_ctx(DevCtx** out) {
     if (!c.init) {
         c.device = dev;
         MPSG_CHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
-        MPSG_CHECK(hipStreamCreateWithFlags(&c.st2, hipStreamNonBlocking));
-        MPSG_CHECK(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming));
-        MPSG_CHECK(hipEventCreateWithFlags(&c.join, hipEventDisableTiming));
-        MPSG_CHECK(hipEventCreateWithFlags(&c.fork0, hipEventDisableTiming));
-        MPSG_CHECK(hipEventCreateWithFlags(&c.proposed, hipEventDisableTiming));
         for (int i = 0; i < kEvPhase; i++) MPSG_CHECK(hipEventCreate(&c.ev[i]));
         for (int a = 0; a < 2; a++)
             for (int i = 0; i < 2 * kMaxPasses; i++) MPSG_CHECK(hipEventCreate(&c.pass_ev[a][i]));
@@ -291,7 +286,7 @@ static void lay_hist(A& a, int P, HistArrays* h) {
     const size_t P1 = (size_t)P + 1, PP = (size_t)P * P, Pm = P > 1 ? (size_t)P - 1 : 1;
     if constexpr (std::is_same<A, Arena>::value) {
         h->C = a.template take<int64_t>(P1);
-        h->win = a.template take<int64_t>(Pm * P * 4);
+        h->win = a.template take<int64_t>(Pm * P * 6);
         h->myclt = a.template take<int64_t>(P * P1);
         h->mycle = a.template take<int64_t>(P * P1);
         h->myC = a.template take<int64_t>(P * P1);
@@ -305,7 +300,7 @@ static void lay_hist(A& a, int P, HistArrays* h) {
         h->checksum = a.template take<long long>(2);
     } else {
         a.template take<int64_t>(P1);
-        a.template take<int64_t>(Pm * P * 4);
+        a.template take<int64_t>(Pm * P * 6);
         a.template take<int64_t>(P * P1);
         a.template take<int64_t>(P * P1);
         a.template take<int64_t>(P * P1);
@@ -758,6 +753,7 @@ struct LearnedSide {
     int64_t* bbase;                   // nseg * buckets per segment
     uint64_t* tables;                 // nseg * boundaries per segment
     float *q0, *q1;                   // nseg
+    int* nedge;                       // nseg: edge boundaries of each table (0: none)
     uint32_t* tickets;                // 4
     uint32_t* err;                    // 4
     int64_t* overflow;                // 2 * kMaxOverflowList
@@ -783,6 +779,7 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
         ls->tables = a.template take<uint64_t>((size_t)P * kMergeBounds);
         ls->q0 = a.template take<float>(P);
         ls->q1 = a.template take<float>(P);
+        ls->nedge = a.template take<int>(P);
         ls->tickets = a.template take<uint32_t>(4);
         ls->err = a.template take<uint32_t>(4);
         ls->overflow = a.template take<int64_t>(2 * (size_t)kMaxOverflowList);
@@ -799,6 +796,7 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
         a.template take<uint64_t>((size_t)P * kMergeBounds);
         a.template take<float>(P);
         a.template take<float>(P);
+        a.template take<int>(P);
         a.template take<uint32_t>(4);
         a.template take<uint32_t>(4);
         a.template take<int64_t>(2 * (size_t)kMaxOverflowList);
@@ -818,18 +816,23 @@ struct LearnedCounters {
 
 // Radix sort of the listed segments (device list of nov {begin, count}
 // pairs) from src into dst at the same positions; tmp is the third buffer
-// of the digit passes.  plain supplies the record shape and key.
-static int sort_listed_segments(DevCtx& c, const SegSortPlan& plain, int rw, const int64_t* d_list, uint32_t nov,
+// of the digit passes.  plain supplies the record shape and key.  Segments
+// of one key value are in stable order already: copied, not sorted.
+static int sort_listed_segments(DevCtx& c, const SegSortPlan& plain, int rw, int64_t* d_list, uint32_t nov,
                                 const void* src, void* dst, void* tmp, long long* cs_out, hipStream_t st) {
     int rc = 0, passes = 0;
+    if ((rc = launch_one_value_filter(rw, src, dst, make_lkey(plain.ks), d_list, nov, cs_out, st))) return rc;
     std::vector<int64_t> ov(2 * (size_t)nov);
     MPSG_CHECK(hipMemcpyAsync(ov.data(), d_list, sizeof(int64_t) * 2 * nov, hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipStreamSynchronize(st));
-    const int ns = (int)nov;
+    // the segments left to sort
+    std::vector<int> ord;
+    for (int i = 0; i < (int)nov; i++)
+        if (ov[2 * i + 1] > 0) ord.push_back(i);
+    const int ns = (int)ord.size();
+    if (ns == 0) return 0;
     std::vector<int64_t> ob(ns), oc(ns), ot(ns + 1), ob2(ns);
     // segments must be in ascending position order for the tile tables
-    std::vector<int> ord(ns);
-    for (int i = 0; i < ns; i++) ord[i] = i;
     std::sort(ord.begin(), ord.end(), [&](int a, int b2) { return ov[2 * a] < ov[2 * b2]; });
     for (int i = 0; i < ns; i++) { ob[i] = ov[2 * ord[i]]; oc[i] = ov[2 * ord[i] + 1]; }
     const TileCfg tile = default_tile(rw);
@@ -877,9 +880,12 @@ static int sort_listed_segments(DevCtx& c, const SegSortPlan& plain, int rw, con
 // model-bucketed local sort of every segment of `side` from src into dst
 // (tmp receives the bucket scatter).  plain is the radix plan of the same
 // segments, used when too many buckets overflow.
+// eo (optional): the buckets sorted in LDS also write the positions of the
+// merged SecondSort's table entries.
 static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64_t n, const LearnedSide& L,
                               const SegSortPlan& plain, const void* src, void* dst, void* tmp, long long* cs_in,
-                              long long* cs_out, LearnedCounters* lc, hipStream_t st, hipEvent_t* ev) {
+                              long long* cs_out, LearnedCounters* lc, hipStream_t st, hipEvent_t* ev,
+                              const EntryOut* eo = nullptr) {
     int rc = 0;
     // ev (optional): 6 events around bucket histogram, scatter, in-LDS
     // bucket sort, in-LDS radix sort of crowded buckets and the overflow
@@ -945,7 +951,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     // crowded ones by the LDS radix sort
     if ((rc = launch_chunk_sort_small(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, L.tstride, (int64_t)P * kLBins,
                                       L.overflow, L.noverflow, kMaxOverflowList, L.hard, L.noverflow + 1, L.medium,
-                                      L.noverflow + 2, cs_out, eq_direct, st)))
+                                      L.noverflow + 2, cs_out, eq_direct, st, eo)))
         return rc;
     if ((rc = mark(3))) return rc;
     uint32_t novh[3] = {0, 0, 0}, errs = 0;
@@ -954,7 +960,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     MPSG_CHECK(hipStreamSynchronize(st));
     if (errs) return fail(MPSORT_ERR_TIMEOUT, "learned bucket scatter: device flag 0x%x", errs);
     if ((rc = launch_listed_sort(rw, tmp, dst, lk, L.medium, novh[2], L.lhist, L.bbase, L.tables, kLBins, L.tstride,
-                                 L.hard, L.noverflow + 1, cs_out, st)))
+                                 L.hard, L.noverflow + 1, cs_out, st, eo)))
         return rc;
     if ((rc = launch_lsd_sort(rw, tmp, dst, lk, L.lhist, L.bbase, L.hard, L.noverflow + 1, (int64_t)P * kLBins, cs_out,
                               st)))
@@ -1012,14 +1018,12 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     Sizer lz;
     lay_lside(lz, P, nullptr);
     lay_lside(lz, P, nullptr);
-    lz.take<uint64_t>(3 * (size_t)P + 3);
     lz.take<int>((size_t)P + 1);
     if ((rc = c.lsmall.ensure(lz.off + 4096))) return rc;
     Arena la{(char*)c.lsmall.p};
     LearnedSide LA, LB;
     lay_lside(la, P, &LA);
     lay_lside(la, P, &LB);
-    uint64_t* prop = la.take<uint64_t>(3 * (size_t)P + 3);
     int* exact = la.take<int>((size_t)P + 1);
 
     if ((rc = c.s0.ensure(n * recbytes))) return rc;
@@ -1114,12 +1118,22 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     hipEvent_t* evA = cs.timing ? c.pass_ev[0] : nullptr;
     hipEvent_t* evB = cs.timing && P > 1 ? c.pass_ev[1] : nullptr;
     bool merged_side = false;
+    int overflow_first = 0;
 
-    // FirstSort: every rank's keys span the whole distribution, so all
-    // segments share one table
-    LA.tstride = 0;
+    // FirstSort: one table per segment over the quantile range its keys
+    // cover (the whole distribution unless a sample of the segment's keys
+    // says they come from one part of it, as in nearly sorted input)
+    LA.tstride = kLBounds;
     LB.tstride = kLBounds;
-    if ((rc = launch_model_tables(md, M.kmax, LA.q0, LA.q1, 1, kLBounds, LA.tables, M.heavy, M.nheavy, st)))
+    {
+        uint64_t xm = 0;
+        for (int b = 0; b < cs.ks.key_bytes && b < 8; b++) xm |= (uint64_t)cs.ks.xmask[b] << (8 * b);
+        if ((rc = launch_seg_range(md, recs_in, (int64_t)recbytes, cs.ks.key_off, cs.ks.key_bytes, xm, LA.begin,
+                                   LA.count, P, LA.q0, LA.q1, LA.nedge, st)))
+            return rc;
+    }
+    if ((rc = launch_model_tables(md, M.kmax, LA.q0, LA.q1, P, kLBounds, LA.tables, M.heavy, M.nheavy, st,
+                                  LA.nedge)))
         return rc;
     if (P == 1) {
         if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, final_dst, c.s1.p,
@@ -1129,47 +1143,57 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         for (int e = kEvFindP; e <= kEvSecond; e++) MPSG_CHECK(hipEventRecord(c.ev[e], st));
     } else {
-        // the model's splitter proposals need only the counts: on the second
-        // stream, during the FirstSort
-        const float delta = std::max(2.0f * M.err_max, 8.0f / (float)M.nsample);
-        MPSG_CHECK(hipEventRecord(c.fork0, st));
-        MPSG_CHECK(hipStreamWaitEvent(c.st2, c.fork0, 0));
-        if ((rc = launch_model_propose(md, M.kmax, h.C, P, n, delta, prop, c.st2))) return rc;
-        MPSG_CHECK(hipEventRecord(c.proposed, c.st2));
-        if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, c.s0.p, c.s1.p,
-                                     cs_direct ? h.checksum : nullptr, nullptr, &lc, st, evA)))
-            return rc;
-        MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n;
-        // the merged form uses kMergeBins buckets per receiver
-        // (positions inside a sender's segment are kept as int32)
+        // the merged form uses kMergeBins buckets per receiver (positions
+        // inside a sender's segment are kept as int32)
         const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 31);
         merged_side = merged;
         const int nbB = merged ? kMergeBounds : kLBounds;
         LB.tstride = nbB;
         const LKey lkB = make_lkey(cs.ks);
         int32_t* bnd = nullptr;
+        EntryOut eo{};
         if (merged) {
-            // the receivers' tables and the position of every table entry in
-            // every sender's whole sorted segment need only the FirstSort:
-            // they run on the second stream while the splitters are found
-            if ((rc = c.mbnd.ensure((size_t)P * P * (kMergeBins + 1) * sizeof(int32_t)))) return rc;
+            // the receivers' tables need only the model and C: made now, so
+            // that the FirstSort's bucket sorts write the position of every
+            // table entry in their sorted buckets (no search afterwards)
+            const size_t bnd_bytes = ((size_t)P * P * (kMergeBins + 1) * sizeof(int32_t) + 255) & ~(size_t)255;
+            const size_t rng_bytes = ((size_t)P * (kLBins + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+            if ((rc = c.mbnd.ensure(bnd_bytes + rng_bytes + (size_t)P * kLBins))) return rc;
             bnd = (int32_t*)c.mbnd.p;
-            MPSG_CHECK(hipEventRecord(c.fork, st));
-            MPSG_CHECK(hipStreamWaitEvent(c.st2, c.fork, 0));
-            if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, c.st2)))
+            if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, st, nullptr,
+                                          true)))
                 return rc;
-            // the FirstSort's bucket bases narrow every search to one bucket
-            const bool narrow = lc.unbucketed == 0;
-            if ((rc = launch_merge_bounds(rw, c.s0.p, lkB, pa.begin, pa.count, P, LB.tables,
-                                          narrow ? LA.tables : nullptr, narrow ? LA.bbase : nullptr, bnd, c.st2)))
+            eo.keys = LB.tables;
+            eo.range = (uint32_t*)((char*)c.mbnd.p + bnd_bytes);
+            eo.seg_begin = pa.begin;
+            eo.bnd = bnd;
+            eo.done = (uint8_t*)((char*)c.mbnd.p + bnd_bytes + rng_bytes);
+            eo.P = P;
+            if ((rc = launch_entry_ranges(LA.tables, LA.tstride, P, LB.tables, const_cast<uint32_t*>(eo.range), st)))
                 return rc;
-            MPSG_CHECK(hipEventRecord(c.join, c.st2));
+            if ((rc = launch_fill_u32((uint32_t*)eo.done, 0u, (int64_t)P * kLBins / 4, st))) return rc;
         }
-        // splitters: the model's proposals, checked and repaired by bisection
-        MPSG_CHECK(hipStreamWaitEvent(st, c.proposed, 0));
+        if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, c.s0.p, c.s1.p,
+                                     cs_direct ? h.checksum : nullptr, nullptr, &lc, st, evA, merged ? &eo : nullptr)))
+            return rc;
+        overflow_first = lc.overflow_buckets;
+        if (merged) {
+            if (lc.unbucketed == 0) {
+                // entries of the buckets sorted outside LDS, and the fixed ends
+                if ((rc = launch_entries_fixup(rw, c.s0.p, lkB, eo, LA.lhist, LA.bbase, pa.count, st))) return rc;
+            } else {
+                // the side was radix-sorted without buckets: search every
+                // entry in every sender's whole segment
+                if ((rc = launch_merge_bounds(rw, c.s0.p, lkB, pa.begin, pa.count, P, LB.tables, nullptr, 0, nullptr,
+                                              bnd, st)))
+                    return rc;
+            }
+        }
+        MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
+        // splitters: values from the model, each checked with exact counts
         if ((rc = launch_bisect(c.s0.p, rw, cs.ks, pa.begin, pa.count, P, h.C, h.win, h.myclt, h.mycle, h.split,
-                                h.pminmax, h.rounds, h.err, st, prop, exact)))
+                                h.pminmax, h.rounds, h.err, st, &md, exact)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFindP], st));
         if ((rc = launch_layout(P, h.C, pa.count, pa.begin, h.myclt, h.mycle, h.myC, h.send, h.run_dst,
@@ -1189,7 +1213,6 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             const LKey lk = lkB;
             if ((rc = markB(0))) return rc;
             // the bounds, clamped to each receiver's window in each sender
-            MPSG_CHECK(hipStreamWaitEvent(st, c.join, 0));
             if ((rc = launch_merge_count(bnd, P, h.myC, LB.lhist, st))) return rc;
             if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st, kMergeBins))) return rc;
             if ((rc = launch_fill_u32(LB.noverflow, 0u, 8, st))) return rc;
@@ -1308,6 +1331,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     stats->splitters_repaired = P > 1 ? P - 1 - nexact : 0;
     stats->buckets = P * kLBins + (P > 1 ? P * (LB.tstride == kMergeBounds ? kMergeBins : kLBins) : 0);
     stats->buckets_overflow = lc.overflow_buckets;
+    stats->buckets_overflow_first = P == 1 ? lc.overflow_buckets : overflow_first;
     stats->t_first_sort_ms = ev_ms(c.ev[kEvStart], c.ev[kEvFirst]);
     stats->t_findp_ms = ev_ms(c.ev[kEvFirst], c.ev[kEvFindP]);
     stats->t_laysolve_ms = ev_ms(c.ev[kEvFindP], c.ev[kEvLayout]);

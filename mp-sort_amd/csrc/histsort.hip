@@ -16,6 +16,7 @@
 //                   copied from its sender run; receiver buffers hold the
 //                   sender runs in sender order.
 #include "device.h"
+#include "model_dev.h"
 
 namespace mpsg {
 
@@ -96,13 +97,85 @@ __device__ __forceinline__ Key bcast_key(const Key& k, int src) {
     return o;
 }
 
+// total records over the segments (every lane gets it)
+__device__ __forceinline__ int64_t scnt_total(const int64_t* __restrict__ scnt, int P) {
+    long long t = 0;
+    for (int j = threadIdx.x & 63; j < P; j += 64) t += scnt[j];
+    return (int64_t)wave_sum_ll(t);
+}
+
 // ---------------------------------------------------------------------------
+// Window slots per (splitter, segment): [lo, hi) of the segment's candidate
+// records, then #{< v} and #{<= v} of the value v tested last (two values in
+// the learned search: slots 2-3 and 4-5).
+constexpr int kWinSlots = 6;
+
+// #{< v} and #{<= v} in records [a, b) of a sorted segment; the two searches
+// advance together, so their loads overlap
+__device__ __forceinline__ void count_in(const uint32_t* __restrict__ recs, int rw, const KeySpec& ks, int64_t base,
+                                         int64_t a, int64_t b, const Key& v, int nw, int64_t* lt, int64_t* le) {
+    int64_t a1 = a, b1 = b, a2 = a, b2 = b;
+    while (a1 < b1 || a2 < b2) {
+        const int64_t m1 = a1 + ((b1 - a1) >> 1), m2 = a2 + ((b2 - a2) >> 1);
+        const bool s1 = a1 < b1, s2 = a2 < b2;
+        const Key k1 = load_key(recs, rw, base + (s1 ? m1 : m2), ks);
+        const Key k2 = load_key(recs, rw, base + (s2 ? m2 : m1), ks);
+        if (s1) {
+            if (key_cmp(k1, v, nw) < 0) a1 = m1 + 1; else b1 = m1;
+        }
+        if (s2) {
+            if (key_cmp(k2, v, nw) <= 0) a2 = m2 + 1; else b2 = m2;
+        }
+    }
+    *lt = a1;
+    *le = a2;
+}
+
+// The value of rank r (0-based) among the candidates of the windows, when
+// there are at most 64 of them (wsum; `mine` this lane's share): gathered
+// into LDS, ranked by the wave.  Returns false when r is out of range.
+__device__ __forceinline__ bool select_direct(const uint32_t* __restrict__ recs, int rw, const KeySpec& ks,
+                                              const int64_t* __restrict__ sbeg, int P, const int64_t* wi,
+                                              long long wsum, long long mine, long long r, int nw, Key* s_cand,
+                                              Key* out) {
+    const int lane = threadIdx.x & 63;
+    long long off = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long y = __shfl_up(off, o, 64);
+        if (lane >= o) off += y;
+    }
+    off -= mine;
+    for (int j = lane; j < P; j += 64)
+        for (int64_t q = wi[kWinSlots * j + 0]; q < wi[kWinSlots * j + 1]; q++)
+            s_cand[off++] = load_key(recs, rw, sbeg[j] + q, ks);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    bool hit = false;
+    Key c = {{0, 0, 0, 0}};
+    if (lane < wsum) {
+        c = s_cand[lane];
+        long long lt = 0, le = 0;
+        for (int t = 0; t < (int)wsum; t++) {
+            const int cmp = key_cmp(s_cand[t], c, nw);
+            lt += cmp < 0 ? 1 : 0;
+            le += cmp <= 0 ? 1 : 0;
+        }
+        hit = lt <= r && r < le;
+    }
+    const uint64_t hits = __ballot(hit);
+    if (!hits) return false;
+    *out = bcast_key(c, (int)__ffsll((unsigned long long)hits) - 1);
+    return true;
+}
+
 __global__ __launch_bounds__(kBlock) void bisect_kernel(
     const uint32_t* __restrict__ recs, int rw, KeySpec ks, const int64_t* __restrict__ sbeg,
     const int64_t* __restrict__ scnt, int P, const int64_t* __restrict__ C,
     int64_t* __restrict__ win, int64_t* __restrict__ myclt, int64_t* __restrict__ mycle,
     uint64_t* __restrict__ split_out, uint64_t* __restrict__ pminmax, int* __restrict__ rounds_out,
-    uint32_t* __restrict__ error_flag, const uint64_t* __restrict__ init, int* __restrict__ exact_out) {
+    uint32_t* __restrict__ error_flag, ModelDev md, int use_model, int* __restrict__ exact_out) {
     __shared__ Key s_cand[kWaves][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * kWaves + wave;  // splitter index
@@ -137,174 +210,249 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
         for (int w = 0; w < 4; w++) { pminmax[w] = mn.w[w]; pminmax[4 + w] = mx.w[w]; }
     }
 
-    // piter_init: every bracket starts at [Pmin, Pmax]
     Key L = mn, R = mx, Pv = mn;
-    bool narrow = false, stable = false;
-    int rounds = 0;
+    int rounds = 0, exact = 0;
     const int64_t Ci = C[i + 1];
-    int64_t* wi = win + (int64_t)i * P * 4;
+    int64_t* wi = win + (int64_t)i * P * kWinSlots;
     for (int j = lane; j < P; j += 64) {
-        wi[4 * j + 0] = 0;
-        wi[4 * j + 1] = scnt[j];
+        wi[kWinSlots * j + 0] = 0;
+        wi[kWinSlots * j + 1] = scnt[j];
     }
-    // Learned proposals (single-word keys): init[3i..3i+2] = {lo, proposal,
-    // hi}.  The bracket side that exact counts confirm replaces Pmin / Pmax
-    // (target k[C-1] >= lo iff #{< lo} < C; <= hi iff #{<= hi} >= C), and the
-    // proposal is the first value tested.  Bisection then continues with the
-    // reference's rules inside the bracket.
-    bool test_proposal = false;
-    Key prop = mn;
-    if (init && nw == 1) {
-        Key lo = {{init[3 * i], 0, 0, 0}}, hi = {{init[3 * i + 2], 0, 0, 0}};
-        prop = Key{{init[3 * i + 1], 0, 0, 0}};
-        if (key_cmp(lo, mn, nw) < 0) lo = mn;
-        if (key_cmp(hi, mx, nw) > 0) hi = mx;
-        long long clt_lo = 0, cle_hi = 0;
-        for (int j = lane; j < P; j += 64) {
-            const int64_t base = sbeg[j];
-            // #{< lo} and #{<= hi}: the two searches advance together
-            int64_t a1 = 0, b1 = scnt[j], a2 = 0, b2 = scnt[j];
-            while (a1 < b1 || a2 < b2) {
-                const int64_t m1 = a1 + ((b1 - a1) >> 1), m2 = a2 + ((b2 - a2) >> 1);
-                const bool s1 = a1 < b1, s2 = a2 < b2;
-                const Key k1 = load_key(recs, rw, base + (s1 ? m1 : m2), ks);
-                const Key k2 = load_key(recs, rw, base + (s2 ? m2 : m1), ks);
-                if (s1) {
-                    if (key_cmp(k1, lo, nw) < 0) a1 = m1 + 1; else b1 = m1;
-                }
-                if (s2) {
-                    if (key_cmp(k2, hi, nw) <= 0) a2 = m2 + 1; else b2 = m2;
-                }
-            }
-            wi[4 * j + 2] = a1;
-            wi[4 * j + 3] = a2;
-            clt_lo += a1;
-            cle_hi += a2;
-        }
-        clt_lo = wave_sum_ll(clt_lo);
-        cle_hi = wave_sum_ll(cle_hi);
-        const bool lo_ok = clt_lo < Ci, hi_ok = cle_hi >= Ci;
-        if (lo_ok) L = lo;
-        if (hi_ok) R = hi;
-        for (int j = lane; j < P; j += 64) {
-            if (lo_ok) wi[4 * j + 0] = wi[4 * j + 2];
-            if (hi_ok) wi[4 * j + 1] = wi[4 * j + 3];
-        }
-        rounds++;
-        test_proposal = key_cmp(prop, L, nw) >= 0 && key_cmp(prop, R, nw) <= 0;
-    }
-    int exact = 0;
-    bool use_found = false;
-    Key found = mn;
-    while (!stable) {
-        rounds++;
-        if (!test_proposal && !narrow) {
-            // at most 64 keys left in the windows (the keys in [L, R]): the
-            // splitter is the one of rank Ci - 1 - #{< L} among them, found
-            // directly; the round below then counts it like any other value
+
+    if (use_model && nw == 1 && Ci > 0) {
+        // Learned search (a rank of at least 1: C = 0 takes the reference's
+        // bisection, which ends next to Pmin).  Every value tested is checked with exact counts,
+        // so the result is the reference's splitter k[C-1]; only the values
+        // tested differ from bisection.  Round 1 tests the model's proposal
+        // F^-1(C / n).  Later rounds test two values together (their four
+        // searches per segment overlap): the window-weighted median of every
+        // segment's candidate at the target's fractional position f in the
+        // windows (a key taken from the data, good for any distribution,
+        // duplicates included), and the key interpolated between the
+        // bracket's ends at f (good for keys spread evenly, such as a nearly
+        // sorted ramp).  When one end has moved twice in a row the target is
+        // taken an eighth of the window past its estimate, towards the other
+        // end.  An end found too large or too small leaves the window
+        // together with every record equal to it.  Once every candidate has
+        // one value, or at most 64 remain, the splitter is read off directly.
+        bool Lt = false, Rt = false;  // the ends have been tested (and left the windows)
+        int moved = 0, repeats = 0;
+        bool done = false;
+        while (!done) {
+            rounds++;
             long long wsum = 0, below = 0;
+            Key fk = {{~0ull, 0, 0, 0}}, lk = {{0, 0, 0, 0}};
             for (int j = lane; j < P; j += 64) {
-                wsum += wi[4 * j + 1] - wi[4 * j + 0];
-                below += wi[4 * j + 0];
+                const int64_t lo = wi[kWinSlots * j + 0], hi = wi[kWinSlots * j + 1];
+                wsum += hi - lo;
+                below += lo;
+                if (hi > lo) {
+                    const Key a = load_key(recs, rw, sbeg[j] + lo, ks), b = load_key(recs, rw, sbeg[j] + hi - 1, ks);
+                    if (key_cmp(a, fk, nw) < 0) fk = a;
+                    if (key_cmp(b, lk, nw) > 0) lk = b;
+                }
             }
             const long long mine = wsum;
             wsum = wave_sum_ll(wsum);
             below = wave_sum_ll(below);
-            if (wsum > 0 && wsum <= 64) {
-                long long off = mine;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const Key a = shfl_key(fk, off), b = shfl_key(lk, off);
+                if (key_cmp(a, fk, nw) < 0) fk = a;
+                if (key_cmp(b, lk, nw) > 0) lk = b;
+            }
+            Key v[2];
+            int nv = 0;
+            if (wsum > 0 && key_cmp(fk, lk, nw) == 0) {
+                v[nv++] = fk;  // every candidate has this value
+            } else if (wsum > 0 && wsum <= 64) {
+                Key c;
+                if (select_direct(recs, rw, ks, sbeg, P, wi, wsum, mine, (long long)Ci - 1 - below, nw, s_cand[wave], &c))
+                    v[nv++] = c;
+            } else if (rounds == 1) {
+                v[nv++] = Key{{model_inverse_leaves(md, (float)(((double)Ci - 0.5) / (double)scnt_total(scnt, P))), 0,
+                               0, 0}};
+            } else if (wsum > 0) {
+                double t = (double)Ci - 0.5;
+                if (repeats >= 2) t += (moved < 0 ? 1.0 : -1.0) * (double)wsum * 0.125;
+                double f = (t - (double)below) / (double)wsum;
+                f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+                // this lane's candidate (its widest window) and weight
+                Key c = {{~0ull, 0, 0, 0}};
+                long long wt = 0, best = 0;
+                for (int j = lane; j < P; j += 64) {
+                    const int64_t lo = wi[kWinSlots * j + 0], hi = wi[kWinSlots * j + 1];
+                    wt += hi - lo;
+                    if (hi - lo > best) {
+                        best = hi - lo;
+                        int64_t q = lo + (int64_t)(f * (double)(hi - lo));
+                        q = q >= hi ? hi - 1 : q;
+                        c = load_key(recs, rw, sbeg[j] + q, ks);
+                    }
+                }
+                // weighted median over the lanes: bitonic sort of (key, weight)
+#pragma unroll
+                for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+                    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                        const Key oc = shfl_key(c, jj);
+                        const long long ow = __shfl_xor(wt, jj, 64);
+                        const bool up = (lane & k) == 0, lower = (lane & jj) == 0;
+                        const bool take_min = (up == lower);
+                        const bool other_less = key_cmp(oc, c, nw) < 0;
+                        if (take_min ? other_less : (key_cmp(oc, c, nw) > 0)) {
+                            c = oc;
+                            wt = ow;
+                        }
+                    }
+                }
+                long long cw = wt;
 #pragma unroll
                 for (int o = 1; o < 64; o <<= 1) {
-                    const long long y = __shfl_up(off, o, 64);
-                    if (lane >= o) off += y;
+                    const long long y = __shfl_up(cw, o, 64);
+                    if (lane >= o) cw += y;
                 }
-                off -= mine;
-                for (int j = lane; j < P; j += 64)
-                    for (int64_t q = wi[4 * j + 0]; q < wi[4 * j + 1]; q++)
-                        s_cand[wave][off++] = load_key(recs, rw, sbeg[j] + q, ks);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const long long r = (long long)Ci - 1 - below;
-                bool hit = false;
-                Key c = mn;
-                if (lane < wsum) {
-                    c = s_cand[wave][lane];
-                    long long lt = 0, le = 0;
-                    for (int t = 0; t < (int)wsum; t++) {
-                        const int cmp = key_cmp(s_cand[wave][t], c, nw);
-                        lt += cmp < 0 ? 1 : 0;
-                        le += cmp <= 0 ? 1 : 0;
-                    }
-                    hit = lt <= r && r < le;
-                }
-                const uint64_t hits = __ballot(hit);
-                if (hits) {
-                    found = bcast_key(c, (int)__ffsll((unsigned long long)hits) - 1);
-                    use_found = true;
-                }
+                const uint64_t med = __ballot(2 * cw >= wsum);
+                v[nv++] = bcast_key(c, (int)__ffsll((unsigned long long)med) - 1);
+                // the interpolated key
+                const double span = (double)(R.w[0] - L.w[0]);
+                uint64_t d = (uint64_t)(span * f);
+                d = d > R.w[0] - L.w[0] ? R.w[0] - L.w[0] : d;
+                v[nv++] = Key{{L.w[0] + d, 0, 0, 0}};
             }
-        }
-        // piter_bisect (internal-parallel.h:168-198)
-        if (use_found) {
-            Pv = found;
-            use_found = false;
-        } else if (test_proposal) {
-            Pv = prop;
-            test_proposal = false;
-        } else if (narrow) {
-            Pv = R;
-            stable = true;
-        } else {
-            Pv = key_mid(L, R, nw);
-            if (key_cmp(Pv, L, nw) <= 0) narrow = true;
-        }
-        // counts of < Pv and <= Pv per segment, inside the window
-        long long slt = 0, sle = 0;
-        for (int j = lane; j < P; j += 64) {
-            int64_t* wj = wi + 4 * j;
-            const int64_t base = sbeg[j];
-            // the two searches advance together, so their loads overlap
-            int64_t a1 = wj[0], b1 = wj[1], a2 = a1, b2 = b1;
-            while (a1 < b1 || a2 < b2) {
-                const int64_t m1 = a1 + ((b1 - a1) >> 1), m2 = a2 + ((b2 - a2) >> 1);
-                const bool s1 = a1 < b1, s2 = a2 < b2;
-                const Key k1 = load_key(recs, rw, base + (s1 ? m1 : m2), ks);
-                const Key k2 = load_key(recs, rw, base + (s2 ? m2 : m1), ks);
-                if (s1) {
-                    if (key_cmp(k1, Pv, nw) < 0) a1 = m1 + 1; else b1 = m1;
-                }
-                if (s2) {
-                    if (key_cmp(k2, Pv, nw) <= 0) a2 = m2 + 1; else b2 = m2;
-                }
+            // keep values that can still be the splitter: inside (L, R), or
+            // an end not tested yet
+            int kept = 0;
+            for (int q = 0; q < nv; q++) {
+                const int cl = key_cmp(v[q], L, nw), cr = key_cmp(v[q], R, nw);
+                if ((cl > 0 || (cl == 0 && !Lt)) && (cr < 0 || (cr == 0 && !Rt)) &&
+                    !(kept == 1 && key_cmp(v[q], v[0], nw) == 0))
+                    v[kept++] = v[q];
             }
-            const int64_t lt = a1, le = a2;
-            wj[2] = lt;
-            wj[3] = le;
-            slt += lt;
-            sle += le;
-        }
-        slt = wave_sum_ll(slt);
-        sle = wave_sum_ll(sle);
-        // piter_accept (internal-parallel.h:224-248)
-        if (slt < Ci && Ci <= sle) {
-            if (init && key_cmp(Pv, prop, nw) == 0) exact = 1;
-            stable = true;
-        } else if (!stable) {
-            const bool too_big = slt >= Ci;
-            if (too_big) R = Pv; else L = Pv;
+            nv = kept;
+            if (nv == 0) v[nv++] = !Lt ? L : (!Rt ? R : key_mid(L, R, nw));
+            // counts of the values in every segment's window
+            long long slt[2] = {0, 0}, sle[2] = {0, 0};
             for (int j = lane; j < P; j += 64) {
-                int64_t* wj = wi + 4 * j;
-                if (too_big) wj[1] = wj[3]; else wj[0] = wj[2];
+                int64_t* wj = wi + kWinSlots * j;
+                const int64_t base = sbeg[j];
+                for (int q = 0; q < nv; q++) {
+                    int64_t lt, le;
+                    count_in(recs, rw, ks, base, wj[0], wj[1], v[q], nw, &lt, &le);
+                    wj[2 + 2 * q] = lt;
+                    wj[3 + 2 * q] = le;
+                    slt[q] += lt;
+                    sle[q] += le;
+                }
+            }
+            int side = 0;
+            for (int q = 0; q < nv && !done; q++) {
+                slt[q] = wave_sum_ll(slt[q]);
+                sle[q] = wave_sum_ll(sle[q]);
+                if (slt[q] < Ci && Ci <= sle[q]) {
+                    Pv = v[q];
+                    done = true;
+                    if (rounds == 1) exact = 1;  // the proposal was the splitter
+                    if (q == 1)
+                        for (int j = lane; j < P; j += 64) {
+                            wi[kWinSlots * j + 2] = wi[kWinSlots * j + 4];
+                            wi[kWinSlots * j + 3] = wi[kWinSlots * j + 5];
+                        }
+                }
+            }
+            if (!done) {
+                for (int q = 0; q < nv; q++) {
+                    if (slt[q] >= Ci) {
+                        // too large: the splitter is below v
+                        if (key_cmp(v[q], R, nw) < 0 || !Rt) {
+                            R = v[q];
+                            Rt = true;
+                            side = 1;
+                            for (int j = lane; j < P; j += 64) wi[kWinSlots * j + 1] = wi[kWinSlots * j + 2 + 2 * q];
+                        }
+                    } else {
+                        // too small: the splitter is above v
+                        if (key_cmp(v[q], L, nw) > 0 || !Lt) {
+                            L = v[q];
+                            Lt = true;
+                            side = -1;
+                            for (int j = lane; j < P; j += 64) wi[kWinSlots * j + 0] = wi[kWinSlots * j + 3 + 2 * q];
+                        }
+                    }
+                }
+                repeats = (side != 0 && side == moved) ? repeats + 1 : 1;
+                moved = side;
+            }
+            if (rounds > 8 * kMaxKeyBytes + 16) {  // cannot happen for a consistent input
+                if (lane == 0) atomicOr(error_flag, 2u);
+                break;
             }
         }
-        if (rounds > 8 * kMaxKeyBytes + 16) {  // cannot happen for a consistent input
-            if (lane == 0) atomicOr(error_flag, 2u);
-            break;
+    } else {
+        // piter_init / piter_bisect / piter_accept (internal-parallel.h:128-248)
+        bool narrow = false, stable = false;
+        while (!stable) {
+            rounds++;
+            bool use_found = false;
+            Key found = mn;
+            if (!narrow) {
+                // at most 64 keys left in the windows (the keys in [L, R]):
+                // the splitter is the one of rank Ci - 1 - #{< L} among them,
+                // found directly; the round below then counts it like any
+                // other value
+                long long wsum = 0, below = 0;
+                for (int j = lane; j < P; j += 64) {
+                    wsum += wi[kWinSlots * j + 1] - wi[kWinSlots * j + 0];
+                    below += wi[kWinSlots * j + 0];
+                }
+                const long long mine = wsum;
+                wsum = wave_sum_ll(wsum);
+                below = wave_sum_ll(below);
+                if (wsum > 0 && wsum <= 64)
+                    use_found = select_direct(recs, rw, ks, sbeg, P, wi, wsum, mine, (long long)Ci - 1 - below, nw,
+                                              s_cand[wave], &found);
+            }
+            if (use_found) {
+                Pv = found;
+            } else if (narrow) {
+                Pv = R;
+                stable = true;
+            } else {
+                Pv = key_mid(L, R, nw);
+                if (key_cmp(Pv, L, nw) <= 0) narrow = true;
+            }
+            // counts of < Pv and <= Pv per segment, inside the window
+            long long slt = 0, sle = 0;
+            for (int j = lane; j < P; j += 64) {
+                int64_t* wj = wi + kWinSlots * j;
+                int64_t lt, le;
+                count_in(recs, rw, ks, sbeg[j], wj[0], wj[1], Pv, nw, &lt, &le);
+                wj[2] = lt;
+                wj[3] = le;
+                slt += lt;
+                sle += le;
+            }
+            slt = wave_sum_ll(slt);
+            sle = wave_sum_ll(sle);
+            if (slt < Ci && Ci <= sle) {
+                stable = true;
+            } else if (!stable) {
+                const bool too_big = slt >= Ci;
+                if (too_big) R = Pv; else L = Pv;
+                for (int j = lane; j < P; j += 64) {
+                    int64_t* wj = wi + kWinSlots * j;
+                    if (too_big) wj[1] = wj[3]; else wj[0] = wj[2];
+                }
+            }
+            if (rounds > 8 * kMaxKeyBytes + 16) {  // cannot happen for a consistent input
+                if (lane == 0) atomicOr(error_flag, 2u);
+                break;
+            }
         }
     }
     for (int j = lane; j < P; j += 64) {
-        myclt[(int64_t)j * P1 + i + 1] = wi[4 * j + 2];
-        mycle[(int64_t)j * P1 + i + 1] = wi[4 * j + 3];
+        myclt[(int64_t)j * P1 + i + 1] = wi[kWinSlots * j + 2];
+        mycle[(int64_t)j * P1 + i + 1] = wi[kWinSlots * j + 3];
     }
     if (lane == 0) {
         for (int w = 0; w < 4; w++) split_out[(int64_t)i * 4 + w] = Pv.w[w];
@@ -677,11 +825,14 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const uint8_t* __restric
 int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sbeg,
                   const int64_t* scnt, int P, const int64_t* C, int64_t* win, int64_t* myclt,
                   int64_t* mycle, uint64_t* split, uint64_t* pminmax, int* rounds, uint32_t* err,
-                  hipStream_t st, const uint64_t* init, int* exact) {
+                  hipStream_t st, const ModelDev* model, int* exact) {
     if (P < 2) return 0;
     const int blocks = (P - 1 + kWaves - 1) / kWaves;
+    ModelDev md{};
+    if (model) md = *model;
     hipLaunchKernelGGL(bisect_kernel, dim3(blocks), dim3(kBlock), 0, st, (const uint32_t*)recs, rw,
-                       ks, sbeg, scnt, P, C, win, myclt, mycle, split, pminmax, rounds, err, init, exact);
+                       ks, sbeg, scnt, P, C, win, myclt, mycle, split, pminmax, rounds, err, md,
+                       model ? 1 : 0, exact);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }

@@ -37,6 +37,30 @@ inline LKey make_lkey(const KeySpec& ks) {
     return k;
 }
 
+// Positions of the merged SecondSort's receiver-table entries in the
+// senders' sorted segments, written by the FirstSort's bucket sorts while a
+// sorted bucket is in LDS (keys == nullptr: not wanted).  Entry e is entry
+// k - 1 of receiver r's table (e = r * kMergeBounds + k - 1); its position in
+// segment j (#{keys <= entry}, relative to the segment's start) goes to
+// bnd[(r * (kMergeBins + 1) + k) * P + j].
+struct EntryOut {
+    const uint64_t* keys;      // P * kMergeBounds entries, non-decreasing
+    const uint32_t* range;     // per segment kLBins + 1: bucket b holds entries [range[b], range[b + 1])
+    const int64_t* seg_begin;  // absolute first record of every segment
+    int32_t* bnd;
+    uint8_t* done;             // per FirstSort bucket: set once its entries are written
+    int P;
+};
+// the entry ranges of every (segment, bucket): ftab the FirstSort tables
+// (segment j's at ftab + j * ftstride), E the entries
+int launch_entry_ranges(const uint64_t* ftab, int64_t ftstride, int P, const uint64_t* E, uint32_t* range,
+                        hipStream_t st);
+// after the FirstSort: the entries of the buckets no LDS sort handled
+// (binary searches in the sorted buckets of recs) and the fixed bounds k = 0
+// and k = kMergeBins
+int launch_entries_fixup(int rw, const void* recs, const LKey& lk, const EntryOut& eo, const uint32_t* hist,
+                         const int64_t* bucket_base, const int64_t* seg_count, hipStream_t st);
+
 // tables: segment s uses tables + s * tstride (tstride 0: one table shared);
 // tile_cnt receives every tile's 1024 bucket counts, bkt[i] the bucket of
 // record i (for the scatter)
@@ -70,11 +94,12 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
 //   bnd: P * P * (kMergeBins + 1) int32; hist / bucket_base: P * kMergeBins
 // positions of the table entries in the senders' whole segments (needs no
 // splitters), then clamped to the receivers' windows myC with the sizes.
-// With the FirstSort's shared table ftab (kLBounds entries) and its bucket
-// bases fbase (absolute, kLBins per segment), each search covers one bucket.
+// With the FirstSort's tables ftab (kLBounds entries, sender j's at
+// ftab + j * ftstride) and its bucket bases fbase (absolute, kLBins per
+// segment), each search covers one bucket.
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
-                        int P, const uint64_t* tables, const uint64_t* ftab, const int64_t* fbase, int32_t* bnd,
-                        hipStream_t st);
+                        int P, const uint64_t* tables, const uint64_t* ftab, int64_t ftstride, const int64_t* fbase,
+                        int32_t* bnd, hipStream_t st);
 int launch_merge_count(int32_t* bnd, int P, const int64_t* myC, uint32_t* hist, hipStream_t st);
 // groups of neighbouring receiver buckets (together at most kMergeCap
 // records) in tasks[0 .. ntasks[0]), and the larger single buckets in
@@ -111,18 +136,24 @@ int launch_merge_bucket(int rw, bool big, const void* recs, void* dst, const LKe
 // uses tables + s * tstride; crowded ones are appended to hard
 int launch_listed_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* list, uint32_t nlist,
                        const uint32_t* hist, const int64_t* bucket_base, const uint64_t* tables, int nbins,
-                       int64_t tstride, uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st);
+                       int64_t tstride, uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st,
+                       const EntryOut* eo = nullptr);
 // launch_chunk_sort with 512-thread workgroups (two per CU): buckets above
 // 4096 records (up to kChunkCap) are listed in medium (*nmedium entries) for
 // launch_listed_sort; no LDS radix pass is launched
 int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                             const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
                             int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
-                            uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st);
+                            uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st,
+                            const EntryOut* eo = nullptr);
 // LDS radix sort of the listed buckets (src may equal dst)
 int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                     const int64_t* bucket_base, const uint32_t* hard, const uint32_t* nhard, int64_t max_buckets,
                     long long* checksum, hipStream_t st);
+// the listed {begin, count} segments that hold one key value: copied src ->
+// dst when those differ (output checksum added) and their count set to 0
+int launch_one_value_filter(int rw, const void* src, void* dst, const LKey& lk, int64_t* list, uint32_t nlist,
+                            long long* checksum, hipStream_t st);
 constexpr int kMergeMaxPH = 64;  // largest P of the merged SecondSort
 
 }  // namespace mpsg

@@ -477,6 +477,46 @@ __device__ __forceinline__ void block_checksum_add(long long csum, long long* s_
     }
 }
 
+// The receiver-table entries that lie in FirstSort bucket eb of segment ej:
+// entry e gets the bucket's position in its segment (rel) plus count_le(key).
+// The caller marks the bucket done.
+template <int BLOCK, class CountLe>
+__device__ __forceinline__ void write_entries(const EntryOut& eo, int ej, int eb, int64_t base, CountLe count_le) {
+    const uint32_t* rg = eo.range + (int64_t)ej * (kLBins + 1);
+    const uint32_t e0 = rg[eb], e1 = rg[eb + 1];
+    const int64_t rel = base - eo.seg_begin[ej];
+    for (uint32_t e = e0 + threadIdx.x; e < e1; e += BLOCK) {
+        const uint32_t r = e / kMergeBounds, k = e % kMergeBounds + 1;
+        eo.bnd[((int64_t)r * (kMergeBins + 1) + k) * eo.P + ej] = (int32_t)(rel + count_le(eo.keys[e]));
+    }
+    if (threadIdx.x == 0) eo.done[(int64_t)ej * kLBins + eb] = 1;
+}
+
+// true when the block's keys (kmn, kmx: each thread's smallest and largest,
+// ~0 and 0 for none) hold one value; s_mm: 2 W words.  Contains one barrier;
+// s_mm may be written again after the next barrier.
+template <int W>
+__device__ __forceinline__ bool block_one_value(uint64_t kmn, uint64_t kmx, uint64_t* s_mm) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t a = __shfl_xor(kmn, off, 64), b = __shfl_xor(kmx, off, 64);
+        kmn = a < kmn ? a : kmn;
+        kmx = b > kmx ? b : kmx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_mm[threadIdx.x >> 6] = kmn;
+        s_mm[W + (threadIdx.x >> 6)] = kmx;
+    }
+    __syncthreads();
+    uint64_t mn = ~0ull, mx = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        mn = s_mm[w] < mn ? s_mm[w] : mn;
+        mx = s_mm[W + w] > mx ? s_mm[W + w] : mx;
+    }
+    return mn == mx;
+}
+
 // One stable counting pass over 256 digits in LDS.  The records are listed
 // in s_in[0..count) (identity when s_in is null); digit(e) < 256 is the digit
 // of record e; the new order goes to s_out.  Positions are taken wave-major
@@ -642,7 +682,8 @@ __device__ __forceinline__ void load_items(const GatherSrc<RW>& src, const int (
 template <int RW, int BLOCK, class Src, int ITEMS = kCItems>
 __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
                                                 const LKey& lk, uint32_t* s_bins, long long* __restrict__ checksum_out,
-                                                bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull) {
+                                                bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull,
+                                                const EntryOut eo = EntryOut{}, int ej = 0, int eb = 0) {
     using C = BSort<BLOCK, ITEMS>;
     __shared__ uint64_t s_gkey[C::CAP + 8];  // key of each bin-order slot (+8: unrolled reads)
     __shared__ uint16_t s_sbin[C::CAP];      // bin of each bin-order slot, then the final
@@ -734,6 +775,10 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             }
         }
         if (checksum_out) block_checksum_add<C::W>(csum, s_sum, checksum_out);
+        if (eo.keys) {
+            const uint64_t v = mn;
+            write_entries<BLOCK>(eo, ej, eb, base, [&](uint64_t t) { return t >= v ? (uint32_t)count : 0u; });
+        }
         return true;
     }
     for (int i = tid; i < C::NB / 2; i += C::B) s_fill[i] = 0;
@@ -924,6 +969,20 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         }
     }
     if (checksum_out) block_checksum_add<C::W>(csum, s_sum, checksum_out);
+    if (eo.keys) {
+        // #{keys <= t}: every key of the bins before t's bin, and those of
+        // its bin that are <= t (bins are non-decreasing in the key; the keys
+        // lie in [mn, mx])
+        write_entries<BLOCK>(eo, ej, eb, base, [&](uint64_t t) -> uint32_t {
+            if (t < mn) return 0u;
+            if (t >= mx) return (uint32_t)count;
+            const uint32_t d = bin_of(t);
+            const uint32_t a = s_start[d], e = s_start[d + 1];
+            uint32_t c = a;
+            for (uint32_t q = a; q < e; q++) c += s_gkey[q] <= t ? 1u : 0u;
+            return c;
+        });
+    }
     return true;
 }
 
@@ -938,7 +997,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
     const uint64_t* __restrict__ tables, int64_t tstride, int64_t nbuckets, int64_t* __restrict__ overflow,
     uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
     uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out,
-    int eq_done) {
+    int eq_done, EntryOut eo) {
     using C = BSort<BLOCK, ITEMS>;
     __shared__ long long s_sum[C::W];
     __shared__ uint32_t s_bins[C::BW];
@@ -968,8 +1027,14 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
         const int kb = (int)(b % kLBins);
         // bucket k holds one key value (see single_bucket)
         const bool single = tables && (kb == 0 ? cur.thi == 0 : (kb < kLBounds && cur.tlo + 1 == cur.thi));
+        const int ej = (int)(b / kLBins);
         if (cnt32 == 0) {
+            if (eo.keys) write_entries<BLOCK>(eo, ej, kb, base, [](uint64_t) { return 0u; });
         } else if (eq_done && single) {
+            if (eo.keys) {
+                const uint64_t v = cur.thi;
+                write_entries<BLOCK>(eo, ej, kb, base, [&](uint64_t t) { return t >= v ? cnt32 : 0u; });
+            }
             if (checksum_out) {
                 long long csum = 0;
                 for (int64_t i = base + tid; i < base + (int64_t)cnt32; i += BLOCK) {
@@ -995,7 +1060,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
             // an interior bucket's keys lie in (t[k-1], t[k]]
             const bool bounded = tables && kb > 0 && kb < kLBins - 1 && cur.tlo < cur.thi;
             if (!sort_bucket_lds<RW, BLOCK, RunSrc<RW>, ITEMS>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
-                                     bounded ? cur.tlo + 1 : 0, bounded ? cur.thi : ~0ull))
+                                     bounded ? cur.tlo + 1 : 0, bounded ? cur.thi : ~0ull, eo, ej, kb))
                 if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
         }
     }
@@ -1011,7 +1076,7 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk, const uint32_t* __restrict__ list,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base, const uint64_t* __restrict__ tables,
     int nbins, int64_t tstride, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
-    long long* __restrict__ checksum_out) {
+    long long* __restrict__ checksum_out, EntryOut eo) {
     __shared__ uint32_t s_bins[BSort<kCBlock>::BW];
     const uint32_t b = list[blockIdx.x];
     const uint32_t cnt32 = hist[b];
@@ -1024,7 +1089,7 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
     const RunSrc<RW> rs{src, base};
     const bool bounded = k > 0 && k < nbins - 1 && tlo < thi;
     if (!sort_bucket_lds<RW, kCBlock>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
-                                      bounded ? tlo + 1 : 0, bounded ? thi : ~0ull))
+                                      bounded ? tlo + 1 : 0, bounded ? thi : ~0ull, eo, (int)(b / (uint32_t)nbins), k))
         if (threadIdx.x == 0) hard[atomicAdd(nhard, 1u)] = b;
 }
 
@@ -1071,7 +1136,7 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
                                                            const int64_t* __restrict__ seg_begin,
                                                            const int64_t* __restrict__ seg_count, int P,
                                                            const uint64_t* __restrict__ tables,
-                                                           const uint64_t* __restrict__ ftab,
+                                                           const uint64_t* __restrict__ ftab, int64_t ftstride,
                                                            const int64_t* __restrict__ fbase,
                                                            int32_t* __restrict__ bnd) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, j, k), k = 0..kMergeBins
@@ -1091,13 +1156,14 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
         const int64_t sb = seg_begin[j];
         int64_t a = 0, e = cnt;
         if (ftab) {
-            // it lies in FirstSort bucket m = #{ftab entries < t}, the keys in
-            // (ftab[m-1], ftab[m]]: every key of earlier buckets is <= t,
-            // every key of later ones > t
+            // it lies in FirstSort bucket m = #{sender j's table entries < t},
+            // the keys in (ft[m-1], ft[m]]: every key of earlier buckets is
+            // <= t, every key of later ones > t
+            const uint64_t* ft = ftab + (int64_t)j * ftstride;
             int m = 0;
 #pragma unroll
             for (int step = 512; step > 0; step >>= 1)
-                if (m + step <= kLBounds && ftab[m + step - 1] < t) m += step;
+                if (m + step <= kLBounds && ft[m + step - 1] < t) m += step;
             const int64_t* fb = fbase + (int64_t)j * kLBins;
             a = fb[m] - sb;
             e = m + 1 < kLBins ? fb[m + 1] - sb : cnt;
@@ -1416,6 +1482,103 @@ __global__ __launch_bounds__(kCBlock) void lsd_sort_kernel(
     if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
 }
 
+// The listed segments {begin, count} (oversized buckets) that hold one key
+// value are in stable order already -- a bucket's records keep their input
+// (or sender) order through the scatter and the gather -- so they need no
+// sort: copied src -> dst when those differ (with the output checksum) and
+// their count set to 0 in the list.  One workgroup per entry.
+template <int RW>
+__global__ __launch_bounds__(kBlock) void one_value_filter_kernel(const uint32_t* __restrict__ src,
+                                                                  uint32_t* __restrict__ dst, LKey lk,
+                                                                  int64_t* __restrict__ list,
+                                                                  long long* __restrict__ checksum_out) {
+    __shared__ uint64_t s_mm[2 * kWaves];
+    __shared__ long long s_sum[kWaves];
+    const int64_t base = list[2 * blockIdx.x], cnt = list[2 * blockIdx.x + 1];
+    if (cnt <= 0) return;
+    uint64_t kmn = ~0ull, kmx = 0;
+    for (int64_t i = base + threadIdx.x; i < base + cnt; i += kBlock) {
+        uint32_t rr[RW];
+        RecIO<RW>::load(src, i, rr);
+        const uint64_t kk = lkey<RW>(rr, lk);
+        kmn = kk < kmn ? kk : kmn;
+        kmx = kk > kmx ? kk : kmx;
+    }
+    if (!block_one_value<kWaves>(kmn, kmx, s_mm)) return;
+    long long csum = 0;
+    for (int64_t i = base + threadIdx.x; i < base + cnt; i += kBlock) {
+        uint32_t rr[RW];
+        RecIO<RW>::load(src, i, rr);
+        if (src != dst) RecIO<RW>::store(dst, i, rr);
+#pragma unroll
+        for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
+    }
+    if (checksum_out) block_checksum_add<kWaves>(csum, s_sum, checksum_out);
+    __syncthreads();
+    if (threadIdx.x == 0) list[2 * blockIdx.x + 1] = 0;
+}
+
+// range[j][b] = first entry above segment j's FirstSort boundary b - 1 (the
+// entries of bucket b are range[j][b] .. range[j][b + 1] - 1).  One thread
+// per (j, b), b = 0 .. kLBins.
+__global__ void entry_ranges_kernel(const uint64_t* __restrict__ ftab, int64_t ftstride, int P,
+                                    const uint64_t* __restrict__ E, uint32_t* __restrict__ range) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)P * (kLBins + 1)) return;
+    const int j = (int)(g / (kLBins + 1)), b = (int)(g % (kLBins + 1));
+    const uint32_t ne = (uint32_t)P * kMergeBounds;
+    uint32_t res;
+    if (b == 0) {
+        res = 0;
+    } else if (b == kLBins) {
+        res = ne;
+    } else {
+        const uint64_t t = ftab[(int64_t)j * ftstride + b - 1];
+        uint32_t a = 0, e = ne;
+        while (a < e) {
+            const uint32_t m = (a + e) >> 1;
+            if (E[m] <= t) a = m + 1; else e = m;
+        }
+        res = a;
+    }
+    range[g] = res;
+}
+
+// The entries of every FirstSort bucket not marked done (sorted by the LDS
+// radix sort, the overflow fallback or a plain sort of the whole side): a
+// binary search per entry in the sorted bucket.  One workgroup per bucket;
+// the last P * P workgroups write the fixed bounds k = 0 and k = kMergeBins.
+template <int RW>
+__global__ __launch_bounds__(kBlock) void entries_fixup_kernel(const uint32_t* __restrict__ recs, LKey lk,
+                                                               EntryOut eo, const uint32_t* __restrict__ hist,
+                                                               const int64_t* __restrict__ bucket_base,
+                                                               const int64_t* __restrict__ seg_count) {
+    const int64_t nb = (int64_t)eo.P * kLBins;
+    const int64_t bb = blockIdx.x;
+    if (bb >= nb) {
+        const int64_t q = bb - nb;  // (r, j)
+        const int r = (int)(q / eo.P), j = (int)(q % eo.P);
+        if (threadIdx.x == 0) {
+            eo.bnd[((int64_t)r * (kMergeBins + 1)) * eo.P + j] = 0;
+            eo.bnd[((int64_t)r * (kMergeBins + 1) + kMergeBins) * eo.P + j] = (int32_t)seg_count[j];
+        }
+        return;
+    }
+    if (eo.done[bb]) return;
+    const int64_t base = bucket_base[bb];
+    const int64_t cnt = hist[bb];
+    write_entries<kBlock>(eo, (int)(bb / kLBins), (int)(bb % kLBins), base, [&](uint64_t t) -> uint32_t {
+        int64_t a = 0, e = cnt;
+        while (a < e) {
+            const int64_t m = (a + e) >> 1;
+            uint32_t rr[RW];
+            RecIO<RW>::load(recs, base + m, rr);
+            if (lkey<RW>(rr, lk) <= t) a = m + 1; else e = m;
+        }
+        return (uint32_t)a;
+    });
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 int launch_lhist(int rw, const void* src, const SegDev& sd, int64_t ntiles, int tile_recs, const LKey& lk,
@@ -1488,7 +1651,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
         case 2:
             hipLaunchKernelGGL((interp_sort_kernel<2, kCBlock>), dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, nullptr, nullptr,
-                               checksum, ed);
+                               checksum, ed, EntryOut{});
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
@@ -1497,7 +1660,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
         case 4:
             hipLaunchKernelGGL((interp_sort_kernel<4, kCBlock>), dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, nullptr, nullptr,
-                               checksum, ed);
+                               checksum, ed, EntryOut{});
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
@@ -1512,8 +1675,10 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
 int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                             const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
                             int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
-                            uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st) {
+                            uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st,
+                            const EntryOut* eop) {
     if (nbuckets <= 0) return 0;
+    const EntryOut eo = eop ? *eop : EntryOut{};
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
     const int ed = eq_done ? 1 : 0;
@@ -1522,12 +1687,12 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
         case 2:
             hipLaunchKernelGGL((interp_sort_kernel<2, kMergeBlock, kSmallItems>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
                                bucket_base, tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard,
-                               medium, nmedium, checksum, ed);
+                               medium, nmedium, checksum, ed, eo);
             break;
         case 4:
             hipLaunchKernelGGL((interp_sort_kernel<4, kMergeBlock, kSmallItems>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
                                bucket_base, tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard,
-                               medium, nmedium, checksum, ed);
+                               medium, nmedium, checksum, ed, eo);
             break;
         default: return -1;
     }
@@ -1536,14 +1701,14 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
 }
 
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
-                        int P, const uint64_t* tables, const uint64_t* ftab, const int64_t* fbase, int32_t* bnd,
-                        hipStream_t st) {
+                        int P, const uint64_t* tables, const uint64_t* ftab, int64_t ftstride, const int64_t* fbase,
+                        int32_t* bnd, hipStream_t st) {
     const int64_t n = (int64_t)P * P * (kMergeBins + 1);
     const unsigned g = (unsigned)((n + 255) / 256);
     const uint32_t* r = (const uint32_t*)recs;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, fbase, bnd); break;
-        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, fbase, bnd); break;
+        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, ftstride, fbase, bnd); break;
+        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, ftstride, fbase, bnd); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
@@ -1600,13 +1765,15 @@ int launch_merge_bucket(int rw, bool big, const void* recs, void* dst, const LKe
 
 int launch_listed_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* list, uint32_t nlist,
                        const uint32_t* hist, const int64_t* bucket_base, const uint64_t* tables, int nbins,
-                       int64_t tstride, uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st) {
+                       int64_t tstride, uint32_t* hard, uint32_t* nhard, long long* checksum, hipStream_t st,
+                       const EntryOut* eop) {
     if (nlist == 0) return 0;
+    const EntryOut eo = eop ? *eop : EntryOut{};
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(listed_sort_kernel<2>, dim3(nlist), dim3(kCBlock), 0, st, s, d, lk, list, hist, bucket_base, tables, nbins, tstride, hard, nhard, checksum); break;
-        case 4: hipLaunchKernelGGL(listed_sort_kernel<4>, dim3(nlist), dim3(kCBlock), 0, st, s, d, lk, list, hist, bucket_base, tables, nbins, tstride, hard, nhard, checksum); break;
+        case 2: hipLaunchKernelGGL(listed_sort_kernel<2>, dim3(nlist), dim3(kCBlock), 0, st, s, d, lk, list, hist, bucket_base, tables, nbins, tstride, hard, nhard, checksum, eo); break;
+        case 4: hipLaunchKernelGGL(listed_sort_kernel<4>, dim3(nlist), dim3(kCBlock), 0, st, s, d, lk, list, hist, bucket_base, tables, nbins, tstride, hard, nhard, checksum, eo); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
@@ -1623,6 +1790,42 @@ int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const ui
     switch (rw) {
         case 2: hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base, hard, nhard, checksum); break;
         case 4: hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base, hard, nhard, checksum); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_one_value_filter(int rw, const void* src, void* dst, const LKey& lk, int64_t* list, uint32_t nlist,
+                            long long* checksum, hipStream_t st) {
+    if (nlist == 0) return 0;
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(one_value_filter_kernel<2>, dim3(nlist), dim3(kBlock), 0, st, s, d, lk, list, checksum); break;
+        case 4: hipLaunchKernelGGL(one_value_filter_kernel<4>, dim3(nlist), dim3(kBlock), 0, st, s, d, lk, list, checksum); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_entry_ranges(const uint64_t* ftab, int64_t ftstride, int P, const uint64_t* E, uint32_t* range,
+                        hipStream_t st) {
+    const int64_t n = (int64_t)P * (kLBins + 1);
+    hipLaunchKernelGGL(entry_ranges_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ftab, ftstride, P, E,
+                       range);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_entries_fixup(int rw, const void* recs, const LKey& lk, const EntryOut& eo, const uint32_t* hist,
+                         const int64_t* bucket_base, const int64_t* seg_count, hipStream_t st) {
+    const unsigned grid = (unsigned)((int64_t)eo.P * kLBins + (int64_t)eo.P * eo.P);
+    const uint32_t* r = (const uint32_t*)recs;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(entries_fixup_kernel<2>, dim3(grid), dim3(kBlock), 0, st, r, lk, eo, hist, bucket_base, seg_count); break;
+        case 4: hipLaunchKernelGGL(entries_fixup_kernel<4>, dim3(grid), dim3(kBlock), 0, st, r, lk, eo, hist, bucket_base, seg_count); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());

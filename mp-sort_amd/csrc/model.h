@@ -41,10 +41,17 @@ int launch_leaves(const float* x, const float* y, int64_t S, const float* eff, i
 // xstart[l] = smallest x the root routes to leaf l or beyond
 int launch_leaf_xstart(const float* eff, int L, int linear_root, float* xstart, hipStream_t st);
 int launch_model_eval(const ModelDev& m, const uint64_t* keys, int64_t n, float* out, hipStream_t st);
+// nedge (optional): per table, boundaries kept on each side of [q0, q1];
+// in_span: heavy keys only inside each table's own span (tables of
+// consecutive quantile ranges then stay in order one after the other)
 int launch_model_tables(const ModelDev& m, uint64_t kmax, const float* q0, const float* q1, int ntables,
-                        int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st);
-int launch_model_propose(const ModelDev& m, uint64_t kmax, const int64_t* C, int P, int64_t n, float delta,
-                         uint64_t* prop, hipStream_t st);
+                        int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st,
+                        const int* nedge = nullptr, bool in_span = false);
+// per segment of unsorted records: the quantile range of its keys and its
+// edge boundaries (see model_tables_kernel)
+int launch_seg_range(const ModelDev& m, const void* recs, int64_t elsize, int64_t key_off, int key_bytes,
+                     uint64_t key_xmask, const int64_t* begin, const int64_t* count, int nseg, float* q0, float* q1,
+                     int* nedge, hipStream_t st);
 int launch_model_sample(const void* recs, int64_t elsize, int64_t key_off, int key_bytes, uint64_t key_xmask,
                         int64_t n, int64_t S, uint64_t seed, const ModelDev& m, uint64_t* skeys, float* x,
                         float* y, hipStream_t st);

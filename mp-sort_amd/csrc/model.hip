@@ -19,6 +19,7 @@
 // counts) and bucket boundaries (integer keys, compared exactly).
 #include "device.h"
 #include "model.h"
+#include "model_dev.h"
 
 namespace mpsg {
 
@@ -35,27 +36,6 @@ __global__ void mlp_effective_kernel(const float* __restrict__ raw, float* __res
     const bool positive = (i >= kW1 && i < kW1 + kH) || (i >= kW2 && i < kW2 + kH * kH) ||
                           (i >= kW3 && i < kW3 + kH);
     eff[i] = positive ? softplus_f(v) : v;
-}
-
-// forward pass with effective weights held in LDS
-__device__ __forceinline__ float mlp_forward(const float* __restrict__ e, float x, float* h1, float* h2) {
-#pragma unroll
-    for (int j = 0; j < kH; j++) h1[j] = tanhf(e[kW1 + j] * x + e[kB1 + j]);
-    float r = e[kB3];
-#pragma unroll 4
-    for (int k = 0; k < kH; k++) {
-        float z = e[kB2 + k];
-#pragma unroll
-        for (int j = 0; j < kH; j++) z += e[kW2 + k * kH + j] * h1[j];
-        h2[k] = tanhf(z);
-        r += e[kW3 + k] * h2[k];
-    }
-    return r;
-}
-
-__device__ __forceinline__ float mlp_forward_only(const float* __restrict__ e, float x) {
-    float h1[kH], h2[kH];
-    return mlp_forward(e, x, h1, h2);
 }
 
 // Loss = mean over the batch of (r(x_i) - y_i)^2.  Block b computes the
@@ -246,24 +226,6 @@ __global__ void leaf_fit_kernel(const float* __restrict__ x, const float* __rest
     (void)ylast_default;
 }
 
-// ---- evaluation ----------------------------------------------------------
-__device__ __forceinline__ float key_to_x(uint64_t key, const ModelDev& m) {
-    const uint64_t d = key > m.kmin ? key - m.kmin : 0ull;
-    float x = m.log_input ? (float)(log2((double)d + 1.0) * m.log_scale) : (float)((double)d * m.inv_range);
-    return x > 1.f ? 1.f : x;
-}
-
-__device__ __forceinline__ float model_eval(const float* __restrict__ e, const ModelDev& m, uint64_t key) {
-    const float x = key_to_x(key, m);
-    const float r = m.linear_root ? x : mlp_forward_only(e, x);
-    int l = (int)floorf(r * (float)m.L);
-    l = l < 0 ? 0 : (l >= m.L ? m.L - 1 : l);
-    const float* lf = m.leaves + 4 * (int64_t)l;
-    float f = lf[0] * x + lf[1];
-    f = f < lf[2] ? lf[2] : (f > lf[3] ? lf[3] : f);
-    return f;
-}
-
 __global__ void model_eval_kernel(ModelDev m, const uint64_t* __restrict__ keys, int64_t n,
                                   float* __restrict__ out) {
     __shared__ float e[kMlpParams];
@@ -271,17 +233,6 @@ __global__ void model_eval_kernel(ModelDev m, const uint64_t* __restrict__ keys,
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = model_eval(e, m, keys[i]);
-}
-
-// smallest key in [kmin, kmax] with F(key) >= q  (bisection on the integer key)
-__device__ uint64_t model_inverse(const float* __restrict__ e, const ModelDev& m, uint64_t kmax, float q) {
-    uint64_t lo = m.kmin, hi = kmax;
-    if (model_eval(e, m, hi) < q) return hi;
-    while (lo < hi) {
-        const uint64_t mid = lo + ((hi - lo) >> 1);
-        if (model_eval(e, m, mid) >= q) hi = mid; else lo = mid + 1;
-    }
-    return lo;
 }
 
 // leaf of x under the root (as model_eval routes it)
@@ -310,46 +261,100 @@ __global__ void leaf_xstart_kernel(ModelDev m, float* __restrict__ xstart) {
     xstart[l] = __uint_as_float(lo);
 }
 
-// key with x(key) ~ x (the inverse of key_to_x), clamped to [kmin, kmax]
-__device__ __forceinline__ uint64_t x_to_key(const ModelDev& m, float x) {
-    const double span = (double)(m.kmax - m.kmin);
-    double d = m.log_input ? exp2((double)x / m.log_scale) - 1.0 : (double)x * span;
-    d = d < 0.0 ? 0.0 : (d > span ? span : d);
-    const uint64_t k = m.kmin + (uint64_t)d;
-    return k > m.kmax ? m.kmax : k;
-}
-
-// F^-1(q) from the leaves alone: the first leaf whose range reaches q, then
-// the inverse of its line, kept inside the leaf's x interval.  Monotone in q;
-// the boundary tables need no more (bucket membership is an exact key
-// comparison whatever the boundaries are).
-__device__ uint64_t model_inverse_leaves(const ModelDev& m, float q) {
-    int lo = 0, hi = m.L;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (m.leaves[4 * (int64_t)mid + 3] < q) lo = mid + 1; else hi = mid;
-    }
-    if (lo >= m.L) return m.kmax;
-    const float* lf = m.leaves + 4 * (int64_t)lo;
-    const float x0 = m.xstart[lo], x1 = fminf(m.xstart[lo + 1], 1.0f);
-    float x = x0;
-    if (lf[2] < q && lf[0] > 0.f) {
-        x = (q - lf[1]) / lf[0];
-        x = x < x0 ? x0 : (x > x1 ? x1 : x);
-    }
-    return x_to_key(m, x);
-}
-
 // Boundary tables: table t has nb boundaries for quantile targets
-// q = q0[t] + (q1[t] - q0[t]) * (i + 1) / (nb + 1); tables are made
-// non-decreasing.  One thread per boundary.
+// q = q0[t] + (q1[t] - q0[t]) * (i + 1) / (nb + 1).  With nedge[t] = E > 0
+// the first E targets split [0, q0) evenly, the last E split (q1, 1], and
+// the nb - 2E between them split [q0, q1]: a table for a segment whose keys
+// mostly lie in one part of the distribution that still spreads the few
+// keys outside it.  Tables are made non-decreasing afterwards.  One thread
+// per boundary.
 __global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, const float* __restrict__ q1,
-                                    int ntables, int nb, uint64_t* __restrict__ tables) {
+                                    const int* __restrict__ nedge, int ntables, int nb,
+                                    uint64_t* __restrict__ tables) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (int64_t)ntables * nb) return;
     const int t = (int)(g / nb), i = (int)(g % nb);
-    const float q = q0[t] + (q1[t] - q0[t]) * (float)(i + 1) / (float)(nb + 1);
+    const int E = nedge ? nedge[t] : 0;
+    const float a = q0[t], b = q1[t];
+    float q;
+    if (E > 0 && i < E) {
+        q = a * (float)(i + 1) / (float)(E + 1);
+    } else if (E > 0 && i >= nb - E) {
+        q = b + (1.f - b) * (float)(i - (nb - E) + 1) / (float)(E + 1);
+    } else {
+        q = a + (b - a) * (float)(i - E + 1) / (float)(nb - 2 * E + 1);
+    }
     tables[g] = model_inverse_leaves(m, q);
+}
+
+// Quantile range of each segment's keys under the model, from 256 keys read
+// at evenly spaced positions of the segment (unsorted records of elsize
+// bytes, key_bytes-byte key at key_off, XOR key_xmask): the narrowest
+// interval of F values that holds 246 of the 256 (so up to 10 keys from
+// elsewhere in the distribution, on either side, do not widen it).  A
+// segment whose interval covers less than half of [0, 1] (its keys come from
+// one part of the distribution, as in nearly sorted input) gets [q0, q1] =
+// that interval widened by 1/64 of its width on each side and nedge =
+// kEdgeBounds boundaries on each side for the keys outside it; the others get
+// [0, 1] and no edges.  One wave per segment.
+constexpr int kEdgeBounds = 32;
+__global__ __launch_bounds__(64) void seg_range_kernel(ModelDev m, const uint8_t* __restrict__ recs, int64_t elsize,
+                                                       int64_t key_off, int key_bytes, uint64_t key_xmask,
+                                                       const int64_t* __restrict__ begin,
+                                                       const int64_t* __restrict__ count, float* __restrict__ q0,
+                                                       float* __restrict__ q1, int* __restrict__ nedge) {
+    __shared__ float s_f[256];
+    __shared__ float s_w[64];
+    const int s = blockIdx.x, lane = threadIdx.x;
+    const int64_t b = begin[s], c = count[s];
+    float f[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        f[j] = 0.f;
+        if (c > 0) {
+            const int i = lane * 4 + j;
+            int64_t pos = b + (int64_t)(((double)i + 0.5) * (double)c / 256.0);
+            pos = pos >= b + c ? b + c - 1 : pos;
+            const uint8_t* p = recs + pos * elsize + key_off;
+            uint64_t k = 0;
+            for (int q = 0; q < key_bytes; q++) k |= (uint64_t)p[q] << (8 * q);
+            f[j] = model_eval(m.eff, m, k ^ key_xmask);
+        }
+    }
+    // rank of each value among the 256 (ties by sample index)
+    int rank[4] = {0, 0, 0, 0};
+    for (int src = 0; src < 64; src++) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            const float v = __shfl(f[jj], src, 64);
+            const int vi = src * 4 + jj;
+#pragma unroll
+            for (int j = 0; j < 4; j++) rank[j] += (v < f[j] || (v == f[j] && vi < lane * 4 + j)) ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) s_f[rank[j]] = f[j];
+    __syncthreads();
+    // windows of 246 consecutive sorted values starting at 0 .. 10
+    constexpr int kKeep = 246;
+    s_w[lane] = lane <= 256 - kKeep ? s_f[lane + kKeep - 1] - s_f[lane] : 2.f;
+    __syncthreads();
+    if (lane == 0) {
+        int best = 0;
+        for (int q = 1; q <= 256 - kKeep; q++)
+            if (s_w[q] < s_w[best]) best = q;
+        const float lo = s_f[best], hi = s_f[best + kKeep - 1], w = hi - lo;
+        if (c < 4096 || !(w <= 0.5f)) {
+            q0[s] = 0.f;
+            q1[s] = 1.f;
+            nedge[s] = 0;
+        } else {
+            const float pad = w / 64.f + 1e-5f;
+            q0[s] = fmaxf(0.f, lo - pad);
+            q1[s] = fminf(1.f, hi + pad);
+            nedge[s] = kEdgeBounds;
+        }
+    }
 }
 
 // per table (one block each, nb <= 4096): running max so the table is sorted
@@ -359,7 +364,7 @@ __global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, co
 // sorted; a pair that would overwrite the previous heavy key's pair moves
 // just after it, or is dropped when the table has no room left.
 __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__ tables, int ntables, int nb,
-                                                          const uint64_t* __restrict__ heavy, int nheavy) {
+                                                          const uint64_t* __restrict__ heavy, int nheavy, int in_span) {
     (void)ntables;  // one block per table
     constexpr int E = 4;  // entries per thread
     __shared__ uint64_t s_t[1024 * E];
@@ -395,8 +400,13 @@ __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__
     __syncthreads();
     if (tid == 0 && nb >= 2) {
         int last = -1;  // slot of the previous heavy key
+        // in_span: heavy keys outside the table's span are left to the other
+        // tables, so that tables made for consecutive quantile ranges stay in
+        // order one after the other
+        const uint64_t tfirst = s_t[0], tlast = s_t[nb - 1];
         for (int h = 0; h < nheavy; h++) {
             const uint64_t hv = heavy[h];
+            if (in_span && (hv < tfirst || hv > tlast)) continue;
             if (hv == 0) {
                 s_t[0] = 0;  // bucket 0 = keys <= 0
                 last = 0;
@@ -426,46 +436,6 @@ __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__
 #pragma unroll
     for (int q = 0; q < E; q++)
         if (E * tid + q < nb) tb[E * tid + q] = s_t[E * tid + q];
-}
-
-// smallest key in [kmin, kmax] with F(key) >= q for a monotone F, found by
-// one wave with 64 probes per step (a 64-ary search, ~11 steps for 64-bit
-// keys).  Every lane returns the result.
-__device__ uint64_t model_inverse_wave(const float* __restrict__ e, const ModelDev& m, uint64_t kmax, float q) {
-    const int lane = threadIdx.x & 63;
-    uint64_t lo = m.kmin, hi = kmax;
-    if (model_eval(e, m, hi) < q) return hi;
-    while (lo < hi) {
-        // probe l at lo + floor(span * (l + 1) / 64); probe 63 is hi
-        const uint64_t span = hi - lo;
-        const uint64_t l1 = (uint64_t)(lane + 1);
-        const uint64_t p = lo + (span >> 6) * l1 + (((span & 63) * l1) >> 6);
-        const bool ok = model_eval(e, m, p) >= q;
-        const uint64_t mask = __ballot(ok) | (1ull << 63);
-        const int first = __builtin_ctzll(mask);
-        const uint64_t pf = __shfl(p, first, 64);
-        const uint64_t pb = __shfl(p, first > 0 ? first - 1 : 0, 64);
-        hi = pf;
-        lo = first > 0 ? pb + 1 : lo;
-    }
-    return lo;
-}
-
-// splitter proposals: for target counts c (fraction q = c / n), the proposal
-// F^-1(q) and a bracket F^-1(q -+ delta).  One block per splitter, one wave
-// per value.
-__global__ __launch_bounds__(192) void model_propose_kernel(ModelDev m, uint64_t kmax, const int64_t* __restrict__ C,
-                                                            int P, int64_t n, float delta,
-                                                            uint64_t* __restrict__ prop) {
-    __shared__ float e[kMlpParams];
-    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) e[i] = m.eff[i];
-    __syncthreads();
-    const int i = blockIdx.x, w = threadIdx.x >> 6;
-    if (i >= P - 1) return;
-    // F approximates the fraction of keys <= key; the splitter is k[C-1]
-    const float q = (float)((double)C[i + 1] / (double)n) + (float)(w - 1) * delta;
-    const uint64_t v = model_inverse_wave(e, m, kmax, q);
-    if ((threadIdx.x & 63) == 0) prop[3 * i + w] = v;
 }
 
 // training samples from a sorted record array: S stratified positions with
@@ -553,21 +523,26 @@ int launch_model_eval(const ModelDev& m, const uint64_t* keys, int64_t n, float*
 }
 
 int launch_model_tables(const ModelDev& m, uint64_t kmax, const float* q0, const float* q1, int ntables,
-                        int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st) {
+                        int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st,
+                        const int* nedge, bool in_span) {
     const int64_t tot = (int64_t)ntables * nb;
     (void)kmax;
-    hipLaunchKernelGGL(model_tables_kernel, dim3(nblk(tot, 128)), dim3(128), 0, st, m, q0, q1, ntables, nb, tables);
+    hipLaunchKernelGGL(model_tables_kernel, dim3(nblk(tot, 128)), dim3(128), 0, st, m, q0, q1, nedge, ntables, nb,
+                       tables);
     MPSG_CHECK(hipGetLastError());
     if (nb > 4096) return -1;
-    hipLaunchKernelGGL(tables_fix_kernel, dim3(ntables), dim3(1024), 0, st, tables, ntables, nb, heavy, nheavy);
+    hipLaunchKernelGGL(tables_fix_kernel, dim3(ntables), dim3(1024), 0, st, tables, ntables, nb, heavy, nheavy,
+                       in_span ? 1 : 0);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_model_propose(const ModelDev& m, uint64_t kmax, const int64_t* C, int P, int64_t n, float delta,
-                         uint64_t* prop, hipStream_t st) {
-    if (P < 2) return 0;
-    hipLaunchKernelGGL(model_propose_kernel, dim3(P - 1), dim3(192), 0, st, m, kmax, C, P, n, delta, prop);
+int launch_seg_range(const ModelDev& m, const void* recs, int64_t elsize, int64_t key_off, int key_bytes,
+                     uint64_t key_xmask, const int64_t* begin, const int64_t* count, int nseg, float* q0, float* q1,
+                     int* nedge, hipStream_t st) {
+    if (nseg <= 0) return 0;
+    hipLaunchKernelGGL(seg_range_kernel, dim3(nseg), dim3(64), 0, st, m, (const uint8_t*)recs, elsize, key_off,
+                       key_bytes, key_xmask, begin, count, q0, q1, nedge);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
