@@ -516,6 +516,7 @@ def bench_histsort(args):
             "buckets_overflow_first_sort": L["buckets_overflow_first"],
             "plain_sides": L["plain_sides"],
             "buckets_lds_radix": L["buckets_lds_radix"],
+            "buckets_listed": L["buckets_listed"],
             "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
             "kernels_ms": dict(zip(LEARNED_KERNELS, [round(x, 4) for x in L["pass_ms"]]))}
         log("learned phases: " + json.dumps(result_learned["phases_ms"]))
@@ -672,6 +673,7 @@ def bench_dists(args):
                      "buckets_overflow_first_sort": L["buckets_overflow_first"],
                      "plain_sides": L["plain_sides"],
                      "buckets_lds_radix": L["buckets_lds_radix"],
+                     "buckets_listed": L["buckets_listed"],
                      "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
                      "kernels_ms": dict(zip(LEARNED_KERNELS, [round(x, 4) for x in L["pass_ms"]]))}
         model.close()

@@ -145,6 +145,8 @@ typedef struct {
     double t_bucket_ms;
     int32_t buckets_lds_radix; /* buckets sorted by the in-LDS radix sort */
     int32_t buckets_overflow_first; /* of buckets_overflow, those of the FirstSort */
+    int32_t buckets_listed;    /* buckets sorted by the 1024-thread in-LDS form from a list */
+    int32_t reserved4;
 } mpsort_gpu_stats;
 
 int mpsort_gpu_last_stats(mpsort_gpu_stats *stats);

@@ -714,7 +714,8 @@ struct LearnedModel {
     float* eff = nullptr;     // kMlpParams
     float* leaves = nullptr;  // 4 * L
     float* xstart = nullptr;  // L + 1
-    uint64_t* heavy = nullptr;
+    uint64_t* heavy = nullptr;  // duplicate-heavy keys, ascending
+    float2* heavy_q = nullptr;  // their quantile ranges in the training sample
     int nheavy = 0;
     int L = 0;
     int linear_root = 0;
@@ -808,7 +809,7 @@ static void lay_lside(A& a, int P, LearnedSide* ls) {
 }
 
 struct LearnedCounters {
-    int hard_buckets = 0;
+    const uint32_t* lists[2] = {nullptr, nullptr};  // device list counters of the sides that ran
     int overflow_buckets = 0;
     int global_fallback = 0;
     int unbucketed = 0;  // sides sorted without the bucket histogram (no bucket bases)
@@ -821,14 +822,22 @@ struct LearnedCounters {
 static int sort_listed_segments(DevCtx& c, const SegSortPlan& plain, int rw, int64_t* d_list, uint32_t nov,
                                 const void* src, void* dst, void* tmp, long long* cs_out, hipStream_t st) {
     int rc = 0, passes = 0;
-    if ((rc = launch_one_value_filter(rw, src, dst, make_lkey(plain.ks), d_list, nov, cs_out, st))) return rc;
     std::vector<int64_t> ov(2 * (size_t)nov);
     MPSG_CHECK(hipMemcpyAsync(ov.data(), d_list, sizeof(int64_t) * 2 * nov, hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipStreamSynchronize(st));
-    // the segments left to sort
+    int64_t maxc = 0;
+    for (uint32_t i = 0; i < nov; i++) maxc = std::max(maxc, ov[2 * i + 1]);
+    if ((rc = launch_one_value_filter(rw, src, dst, make_lkey(plain.ks), d_list, nov, maxc, cs_out, st))) return rc;
+    MPSG_CHECK(hipMemcpyAsync(ov.data(), d_list, sizeof(int64_t) * 2 * nov, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    // the segments left to sort: those flagged (top bit) as holding several
+    // key values
     std::vector<int> ord;
     for (int i = 0; i < (int)nov; i++)
-        if (ov[2 * i + 1] > 0) ord.push_back(i);
+        if (ov[2 * i + 1] < 0) {
+            ov[2 * i + 1] &= INT64_MAX;
+            ord.push_back(i);
+        }
     const int ns = (int)ord.size();
     if (ns == 0) return 0;
     std::vector<int64_t> ob(ns), oc(ns), ot(ns + 1), ob2(ns);
@@ -900,6 +909,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     if ((rc = launch_fill_u32(L.tickets, 0u, 4, st))) return rc;
     if ((rc = launch_fill_u32(L.err, 0u, 4, st))) return rc;
     if ((rc = launch_fill_u32(L.noverflow, 0u, 4, st))) return rc;
+    lc->lists[lc->lists[0] ? 1 : 0] = L.noverflow;
     // tiles of the learned scatter (the look-back buffer was sized by the
     // caller: it must not move while plans hold its address)
     std::vector<int64_t> cnt(P), beg(P), lt(P + 1);
@@ -967,7 +977,6 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         return rc;
     if ((rc = mark(4))) return rc;
     const uint32_t nov = novh[0];
-    lc->hard_buckets += (int)novh[1];
     lc->overflow_buckets += (int)nov;
     if (nov == 0) return mark(5);
     const size_t recbytes = (size_t)rw * 4;
@@ -1132,7 +1141,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                                    LA.count, P, LA.q0, LA.q1, LA.nedge, st)))
             return rc;
     }
-    if ((rc = launch_model_tables(md, M.kmax, LA.q0, LA.q1, P, kLBounds, LA.tables, M.heavy, M.nheavy, st,
+    if ((rc = launch_model_tables(md, M.kmax, LA.q0, LA.q1, P, kLBounds, LA.tables, M.heavy, M.heavy_q, M.nheavy, st,
                                   LA.nedge)))
         return rc;
     if (P == 1) {
@@ -1161,7 +1170,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             const size_t rng_bytes = ((size_t)P * (kLBins + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
             if ((rc = c.mbnd.ensure(bnd_bytes + rng_bytes + (size_t)P * kLBins))) return rc;
             bnd = (int32_t*)c.mbnd.p;
-            if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, st, nullptr,
+            if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.heavy_q, M.nheavy, st, nullptr,
                                           true)))
                 return rc;
             eo.keys = LB.tables;
@@ -1201,7 +1210,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
         if (!merged &&
-            (rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.nheavy, st)))
+            (rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.heavy_q, M.nheavy, st)))
             return rc;
         if (merged) {
             // no exchange: every receiver bucket is gathered from the senders'
@@ -1216,6 +1225,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = launch_merge_count(bnd, P, h.myC, LB.lhist, st))) return rc;
             if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st, kMergeBins))) return rc;
             if ((rc = launch_fill_u32(LB.noverflow, 0u, 8, st))) return rc;
+            lc.lists[1] = LB.noverflow;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
             if ((rc = markB(1)) || (rc = markB(2))) return rc;
             // groups of neighbouring buckets, one workgroup each; buckets
@@ -1267,7 +1277,6 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = markB(4))) return rc;
             const uint32_t nov = novh[0];
             lc.overflow_buckets += (int)nov;
-            lc.hard_buckets += (int)novh[1];
             if (nov > (uint32_t)kMaxFallbackSegs) {
                 // more oversized buckets than the bucket fallback takes (the
                 // list may also have stopped at its capacity): every one of
@@ -1306,6 +1315,11 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     uint32_t err0 = 0;
     long long sums[2] = {0, 0};
     std::vector<int> rounds(P > 1 ? P - 1 : 1, 0), ex(P > 1 ? P - 1 : 1, 0);
+    // list counters of each side: [1] LDS radix sort, [2] 1024-thread form
+    uint32_t lists[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    for (int side = 0; side < 2; side++)
+        if (lc.lists[side])
+            MPSG_CHECK(hipMemcpyAsync(lists[side], lc.lists[side], sizeof lists[side], hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(&err0, h.err, sizeof err0, hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(sums, h.checksum, sizeof sums, hipMemcpyDeviceToHost, st));
     if (P > 1) {
@@ -1339,7 +1353,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     stats->t_second_sort_ms = ev_ms(c.ev[kEvExchange], c.ev[kEvSecond]);
     stats->t_total_ms = ev_ms(c.ev[kEvStart], c.ev[kEvEnd]);
     stats->plain_sides = lc.global_fallback;
-    stats->buckets_lds_radix = lc.hard_buckets;
+    stats->buckets_lds_radix = (int32_t)(lists[0][1] + lists[1][1]);
+    stats->buckets_listed = (int32_t)(lists[0][2] + lists[1][2]);
     // pass_ms: bucket histogram, scatter, in-LDS bucket sort, in-LDS radix
     // sort, overflow fallback for the FirstSort and then the SecondSort
     int np = 0;
@@ -2235,24 +2250,40 @@ int mpsort_gpu_model_train(mpsort_gpu_model** out, const void* sorted, int64_t n
                     return r2;
             }
         }
-        // heavy keys: runs of at least max(4, S/4096) equal sampled keys
-        const int64_t hthr = std::max<int64_t>(4, S / 4096);
-        std::vector<std::pair<int64_t, uint64_t>> runs;
+        // heavy keys: runs of at least max(4, S / 2^18) equal sampled keys,
+        // the kMaxHeavy longest, each with its quantile range [i / S, j / S]
+        // (samples i .. j - 1 of the stratified sample)
+        const int64_t hthr = std::max<int64_t>(4, S >> 18);
+        struct Run {
+            int64_t i, j;
+            uint64_t v;
+        };
+        std::vector<Run> runs;
         for (int64_t i = 0; i < S;) {
             int64_t j = i;
             while (j < S && hk[j] == hk[i]) j++;
-            if (j - i >= hthr) runs.push_back({j - i, hk[i]});
+            if (j - i >= hthr) runs.push_back({i, j, hk[i]});
             i = j;
         }
-        std::sort(runs.begin(), runs.end(), [](const std::pair<int64_t, uint64_t>& a, const std::pair<int64_t, uint64_t>& b) { return a.first > b.first; });
-        if (runs.size() > 256) runs.resize(256);
+        if (runs.size() > (size_t)kMaxHeavy) {
+            std::sort(runs.begin(), runs.end(), [](const Run& a, const Run& b) { return a.j - a.i > b.j - b.i; });
+            runs.resize(kMaxHeavy);
+            std::sort(runs.begin(), runs.end(), [](const Run& a, const Run& b) { return a.v < b.v; });
+        }
         std::vector<uint64_t> hv;
-        for (auto& r : runs) hv.push_back(r.second);
-        std::sort(hv.begin(), hv.end());
+        std::vector<float2> hq;
+        for (auto& r : runs) {
+            hv.push_back(r.v);
+            hq.push_back(make_float2((float)((double)r.i / (double)S), (float)((double)r.j / (double)S)));
+        }
         M.nheavy = (int)hv.size();
         M.heavy = (uint64_t*)dev_alloc(sizeof(uint64_t) * (hv.size() + 1));
-        if (!M.heavy) return fail(MPSORT_ERR_NOMEM, "device allocation failed (heavy keys)");
-        if (!hv.empty()) MPSG_CHECK(hipMemcpy(M.heavy, hv.data(), sizeof(uint64_t) * hv.size(), hipMemcpyHostToDevice));
+        M.heavy_q = (float2*)dev_alloc(sizeof(float2) * (hv.size() + 1));
+        if (!M.heavy || !M.heavy_q) return fail(MPSORT_ERR_NOMEM, "device allocation failed (heavy keys)");
+        if (!hv.empty()) {
+            MPSG_CHECK(hipMemcpy(M.heavy, hv.data(), sizeof(uint64_t) * hv.size(), hipMemcpyHostToDevice));
+            MPSG_CHECK(hipMemcpy(M.heavy_q, hq.data(), sizeof(float2) * hq.size(), hipMemcpyHostToDevice));
+        }
         // the root network
         float init[kMlpParams];
         {
@@ -2334,6 +2365,7 @@ int mpsort_gpu_model_free(mpsort_gpu_model* mo) {
     dev_free(mo->m.leaves);
     dev_free(mo->m.xstart);
     dev_free(mo->m.heavy);
+    dev_free(mo->m.heavy_q);
     delete mo;
     return 0;
 }

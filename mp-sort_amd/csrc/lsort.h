@@ -151,9 +151,10 @@ int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const ui
                     const int64_t* bucket_base, const uint32_t* hard, const uint32_t* nhard, int64_t max_buckets,
                     long long* checksum, hipStream_t st);
 // the listed {begin, count} segments that hold one key value: copied src ->
-// dst when those differ (output checksum added) and their count set to 0
+// dst when those differ (output checksum added); the others get the top bit
+// of their count set.  max_count: the largest count in the list
 int launch_one_value_filter(int rw, const void* src, void* dst, const LKey& lk, int64_t* list, uint32_t nlist,
-                            long long* checksum, hipStream_t st);
+                            int64_t max_count, long long* checksum, hipStream_t st);
 constexpr int kMergeMaxPH = 64;  // largest P of the merged SecondSort
 
 }  // namespace mpsg

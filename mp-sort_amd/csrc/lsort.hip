@@ -601,11 +601,18 @@ __device__ __forceinline__ void lds_count_pass(const uint16_t* s_in, uint16_t* s
 // Records are grouped by bin with LDS atomics (in any order inside a bin),
 // and a record's final slot is its bin start + #{records of the bin with a
 // smaller key, or an equal key and a smaller position}: the stable order.
-// Buckets with a bin above kRunMax records go to the list for
-// lsd_sort_kernel.  Record i of the bucket is item i / BLOCK of thread
-// i % BLOCK.
+// In the 9-record forms, a bin of more than kLongMin records that all hold
+// one key value (a duplicate-heavy key) is placed by counting instead:
+// slot = bin start + #{its records before this one in the bucket}, item by
+// item; the 512 x 6 form, whose registers are all in use, hands a bucket with
+// such bins to the 1024-thread form (result 2) when the sample holds a
+// repeated key.
+// Buckets with a bin above kRunMax records of several key values go to the
+// list for lsd_sort_kernel (result 1).  Record i of the bucket is item
+// i / BLOCK of thread i % BLOCK.
 constexpr int kSamp = 64;
 constexpr int kInterpMax = 32;
+constexpr int kLongMin = 32;
 // Workgroup shapes of the in-LDS sort: 1024 threads x 9 records (one
 // workgroup per CU) or 512 threads x 9 records (two share a CU), in 4095
 // bins (65 sample intervals x 63), or 512 threads x 6 records (three share a
@@ -615,6 +622,7 @@ constexpr int kInterpMax = 32;
 template <int BLOCK, int ITEMS = kCItems>
 struct BSort {
     static constexpr int B = BLOCK, W = BLOCK / 64, I = ITEMS, CAP = BLOCK * ITEMS;
+    static constexpr int NL = CAP / (kLongMin + 1) + 1;  // bins of more than kLongMin records, at most
     static constexpr int SUB = CAP > 3072 ? 63 : 31;             // bins per sample interval
     static constexpr int FINE = (kSamp + 1) * SUB;               // bins
     static constexpr int NB = (FINE + 1 + BLOCK - 1) / BLOCK * BLOCK;  // bin counters
@@ -674,17 +682,22 @@ __device__ __forceinline__ void load_items(const GatherSrc<RW>& src, const int (
 }
 
 // Sorts one bucket of count (1 .. BSort<BLOCK>::CAP) records, given by src,
-// into dst[base .. base + count) with a BLOCK-thread workgroup.  Returns false (and writes nothing) when a bin
-// is crowded beyond kRunMax.  s_bins: BSort<BLOCK, ITEMS>::BW words of LDS owned by
-// the caller, who may keep data there that src reads while loading.  When
+// into dst[base .. base + count) with a BLOCK-thread workgroup.  Returns 0
+// when sorted; otherwise writes nothing and returns 1 when a bin of several
+// key values is crowded beyond kRunMax, 2 when the bucket is for the
+// 1024-thread form (one-value long bins in the 512 x 6 form, see above).
+// s_bins: BSort<BLOCK, ITEMS>::BW words of LDS owned by the caller, who may
+// keep data there that src reads while loading.  When
 // bounded, every key lies in [lo, hi] (the bucket's boundaries) and the
 // bucket's minimum and maximum are not computed.
 template <int RW, int BLOCK, class Src, int ITEMS = kCItems>
-__device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
+__device__ __forceinline__ int sort_bucket_lds(const Src& src, uint32_t* __restrict__ dst, int64_t base, int count,
                                                 const LKey& lk, uint32_t* s_bins, long long* __restrict__ checksum_out,
                                                 bool bounded = false, uint64_t lo = 0, uint64_t hi = ~0ull,
                                                 const EntryOut eo = EntryOut{}, int ej = 0, int eb = 0) {
     using C = BSort<BLOCK, ITEMS>;
+    // one-value long bins placed here (the 512 x 6 form hands them on)
+    constexpr bool DUPS = BLOCK == kCBlock || ITEMS >= 9;
     __shared__ uint64_t s_gkey[C::CAP + 8];  // key of each bin-order slot (+8: unrolled reads)
     __shared__ uint16_t s_sbin[C::CAP];      // bin of each bin-order slot, then the final
                                              // slot of each bucket position (s_pos)
@@ -700,9 +713,19 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     __shared__ uint32_t s_max[C::W];
     __shared__ long long s_sum[C::W];
     __shared__ int s_long;
+    __shared__ int s_dup;
+    __shared__ int s_defer;
+    __shared__ uint32_t s_nlong;
     uint16_t* s_start = reinterpret_cast<uint16_t*>(s_bins);  // NB u16
     uint32_t* s_fill = s_bins + C::NB / 2;                    // NB u16 counters, two per word
     uint16_t* s_pos = s_sbin;
+    // one-value long bins; each array reuses LDS that is free by then
+    static_assert(C::NB <= 32 * 2 * kSamp && (C::W + 1) * C::NL <= C::CAP && C::CAP < 0x8000 && C::NB < 0x8000,
+                  "aliased arrays of the one-value long bins");
+    uint32_t* s_mix = reinterpret_cast<uint32_t*>(s_samp);   // after the sample sort: long bins of several values
+    uint16_t* s_lidx = reinterpret_cast<uint16_t*>(s_fill);  // after the ranking: long bin numbers
+    uint16_t* s_wc = s_ord;                 // after the ranking: their records per wave, one item
+    uint16_t* s_run = s_ord + C::W * C::NL;  // and per long bin, the items before
     // one more record in bin d: the count before it
     auto bump = [&](uint32_t d) -> uint32_t {
         const uint32_t sh = (d & 1u) << 4;
@@ -738,6 +761,9 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     if (tid == 0) {
         s_crowd = 0;
         s_long = 0;
+        s_dup = 0;
+        s_defer = 0;
+        s_nlong = 0;
     }
     if (bounded) {
         mn = lo;
@@ -779,7 +805,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             const uint64_t v = mn;
             write_entries<BLOCK>(eo, ej, eb, base, [&](uint64_t t) { return t >= v ? (uint32_t)count : 0u; });
         }
-        return true;
+        return 0;
     }
     for (int i = tid; i < C::NB / 2; i += C::B) s_fill[i] = 0;
     // plain bins: offsets from the minimum shifted to 32 bits, then scaled
@@ -824,7 +850,10 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         // two neighbouring samples in one plain bin put about count / 64
         // records there: the sampled bins are used
         if (j + 1 < kSamp && plain_bin(s_sorted[j]) == plain_bin(s_sorted[j + 1])) s_crowd = 1;
+        // a key sampled twice: duplicate-heavy keys are likely
+        if (j + 1 < kSamp && s_sorted[j] == s_sorted[j + 1]) s_dup = 1;
     }
+    for (int i = tid; i < (C::NB + 31) / 32; i += C::B) s_mix[i] = 0;  // the sample is sorted
     __syncthreads();
     int mode = count >= kInterpMax * kSamp && s_crowd ? 1 : 0;  // 0: plain bins, 1: sampled bins
     // sampled: j = #{samples <= key}, then the interval's offset shifted and
@@ -842,6 +871,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         return (uint32_t)j * C::SUB + sub;
     };
     uint32_t dbin[C::I];
+    uint32_t bmax = 0;  // the fullest bin
     for (;;) {
 #pragma unroll
         for (int k = 0; k < C::I; k++) {
@@ -880,13 +910,16 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         }
         if (lane == 0) s_max[wave] = big;
         __syncthreads();
-        uint32_t bmax = 0;
+        bmax = 0;
 #pragma unroll
         for (int w = 0; w < C::W; w++) bmax = s_max[w] > bmax ? s_max[w] : bmax;
         // clustered keys the sample did not reveal: sampled bins instead
         if (mode == 1 || bmax <= (uint32_t)kInterpMax) break;
         mode = 1;
     }
+    // one-value long bins are looked for when the sample showed a repeated key
+    const bool any_long = s_dup && bmax > (uint32_t)kLongMin;
+    if (!DUPS && any_long) return 2;
     // group by bin: key, bin and bucket position of every bin-order slot
 #pragma unroll
     for (int k = 0; k < C::I; k++) {
@@ -905,6 +938,19 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
     // later bins (larger) or the padding (all ones), which add no smaller
     // key.  Equal keys keep their bucket order.
     if (tid < 8) s_gkey[count + tid] = ~0ull;
+    if (DUPS && any_long) {
+        // a long bin with a slot whose key is not its first slot's holds
+        // several key values
+#pragma unroll
+        for (int k = 0; k < C::I; k++) {
+            const int o = k * C::B + tid;
+            if (o < count) {
+                const uint32_t d = s_sbin[o], a = s_start[d];
+                if (s_start[d + 1] - a > (uint32_t)kLongMin && s_gkey[o] != s_gkey[a])
+                    atomicOr(&s_mix[d >> 5], 1u << (d & 31));
+            }
+        }
+    }
     __syncthreads();
     // the slots' bins, positions, keys and bin bounds are read for all items
     // before the first is ranked, so the reads are in flight together
@@ -930,6 +976,10 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         const uint32_t s0 = rs0[k], e = re[k];
         if (e - s0 == 1) {
             s_pos[i] = (uint16_t)s0;
+        } else if (DUPS && any_long && e - s0 > (uint32_t)kLongMin && !((s_mix[rb[k] >> 5] >> (rb[k] & 31)) & 1u)) {
+            // one key value: placed by counting below; s_pos holds 0x8000 | bin
+            s_pos[i] = (uint16_t)(0x8000u | rb[k]);
+            s_defer = 1;
         } else if (e - s0 > (uint32_t)kRunMax) {
             s_long = 1;
         } else {
@@ -956,7 +1006,75 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
         }
     }
     __syncthreads();
-    if (s_long) return false;
+    if (s_long) return 1;
+    if (DUPS && s_defer) {
+        // the records of the one-value long bins, item by item (records
+        // k B .. k B + B - 1, wave by wave): slot = bin start + the bin's
+        // records in earlier items + in earlier waves of this item + in
+        // earlier lanes of this wave.  Long bins are numbered first (in any
+        // order).
+        {
+            constexpr int PER = C::NB / C::B;
+#pragma unroll
+            for (int j = 0; j < PER; j++) {
+                const int d = tid * PER + j;
+                const bool lg = d < C::FINE && s_start[d + 1] - s_start[d] > (uint32_t)kLongMin;
+                s_lidx[d] = lg ? (uint16_t)atomicAdd(&s_nlong, 1u) : (uint16_t)0xffffu;
+            }
+            for (int q = tid; q < C::W * C::NL; q += C::B) s_wc[q] = 0;
+            for (int q = tid; q < C::NL; q += C::B) s_run[q] = 0;
+        }
+        __syncthreads();
+        for (int k = 0; k < C::I; k++) {
+            const int i = k * C::B + tid;
+            // this wave's records of each long bin in item k (the lanes of
+            // a bin found by ballots, one bin at a time)
+            {
+                const uint32_t sp = i < count ? s_pos[i] : 0u;
+                const bool def = (sp & 0x8000u) != 0;
+                const uint32_t L = def ? s_lidx[sp & 0x7fffu] : 0xffffu;
+                uint64_t act = __ballot(def);
+                while (act) {
+                    const int ld = __builtin_ctzll(act);
+                    const uint32_t Ll = (uint32_t)__builtin_amdgcn_readlane((int)L, ld);
+                    const uint64_t m = __ballot(L == Ll);
+                    if (lane == ld) s_wc[wave * C::NL + Ll] = (uint16_t)__popcll(m);
+                    act &= ~m;
+                }
+            }
+            __syncthreads();
+            {
+                const uint32_t sp = i < count ? s_pos[i] : 0u;
+                const bool def = (sp & 0x8000u) != 0;
+                const uint32_t d = sp & 0x7fffu;
+                const uint32_t L = def ? s_lidx[d] : 0xffffu;
+                uint32_t before = 0;
+                if (def) {
+                    before = s_start[d] + s_run[L];
+                    for (int w = 0; w < wave; w++) before += s_wc[w * C::NL + L];
+                }
+                uint64_t act = __ballot(def);
+                while (act) {
+                    const int ld = __builtin_ctzll(act);
+                    const uint32_t Ll = (uint32_t)__builtin_amdgcn_readlane((int)L, ld);
+                    const uint64_t m = __ballot(L == Ll);
+                    if (L == Ll) s_pos[i] = (uint16_t)(before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)));
+                    act &= ~m;
+                }
+            }
+            __syncthreads();
+            for (int q = tid; q < C::NL; q += C::B) {
+                uint32_t acc = 0;
+#pragma unroll
+                for (int w = 0; w < C::W; w++) {
+                    acc += s_wc[w * C::NL + q];
+                    s_wc[w * C::NL + q] = 0;
+                }
+                s_run[q] = (uint16_t)(s_run[q] + acc);
+            }
+            __syncthreads();
+        }
+    }
 #pragma unroll
     for (int k = 0; k < C::I; k++) {
         const int i = k * C::B + tid;
@@ -983,7 +1101,7 @@ __device__ __forceinline__ bool sort_bucket_lds(const Src& src, uint32_t* __rest
             return c;
         });
     }
-    return true;
+    return 0;
 }
 
 // One workgroup per bucket; the bucket's count, base and table bounds are
@@ -1059,9 +1177,11 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
             const RunSrc<RW> rs{src, base};
             // an interior bucket's keys lie in (t[k-1], t[k]]
             const bool bounded = tables && kb > 0 && kb < kLBins - 1 && cur.tlo < cur.thi;
-            if (!sort_bucket_lds<RW, BLOCK, RunSrc<RW>, ITEMS>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
-                                     bounded ? cur.tlo + 1 : 0, bounded ? cur.thi : ~0ull, eo, ej, kb))
-                if (tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
+            const int rc = sort_bucket_lds<RW, BLOCK, RunSrc<RW>, ITEMS>(
+                rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded, bounded ? cur.tlo + 1 : 0,
+                bounded ? cur.thi : ~0ull, eo, ej, kb);
+            if (rc == 1 && tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
+            if (rc == 2 && tid == 0) medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread form
         }
     }
 }
@@ -1088,8 +1208,8 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
     // every record is in registers before the first store: src == dst is safe
     const RunSrc<RW> rs{src, base};
     const bool bounded = k > 0 && k < nbins - 1 && tlo < thi;
-    if (!sort_bucket_lds<RW, kCBlock>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
-                                      bounded ? tlo + 1 : 0, bounded ? thi : ~0ull, eo, (int)(b / (uint32_t)nbins), k))
+    if (sort_bucket_lds<RW, kCBlock>(rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded,
+                                     bounded ? tlo + 1 : 0, bounded ? thi : ~0ull, eo, (int)(b / (uint32_t)nbins), k))
         if (threadIdx.x == 0) hard[atomicAdd(nhard, 1u)] = b;
 }
 
@@ -1374,11 +1494,13 @@ __global__ __launch_bounds__(kMergeBlock, ITEMS == kMergeItems ? 6 : 4) void mer
     const bool single = len == 1 && (ka == 0 ? thi == 0 : (ka < kMergeBounds && tlo + 1 == thi));
     // the keys of buckets [ka, kb) lie in (t[ka-1], t[kb-1]]
     const bool bounded = ka > 0 && kb < kMergeBins && tlo < thi;
-    if (!single && count <= C::CAP &&
-        sort_bucket_lds<RW, kMergeBlock, GatherSrc<RW>, ITEMS>(gs, dst, base, count, lk, s_bins, checksum_out,
+    int rc = 1;
+    if (!single && count <= C::CAP) {
+        rc = sort_bucket_lds<RW, kMergeBlock, GatherSrc<RW>, ITEMS>(gs, dst, base, count, lk, s_bins, checksum_out,
                                                                      bounded, bounded ? tlo + 1 : 0,
-                                                                     bounded ? thi : ~0ull))
-        return;
+                                                                     bounded ? thi : ~0ull);
+        if (rc == 0) return;
+    }
     // copy in sender order, bucket by bucket: final for a single-value
     // bucket, otherwise the input of the radix path
     long long csum = 0;
@@ -1402,7 +1524,7 @@ __global__ __launch_bounds__(kMergeBlock, ITEMS == kMergeItems ? 6 : 4) void mer
             }
         }
         if (!single && tid == 0) {
-            if (cq <= C::CAP) {
+            if (cq <= C::CAP && rc == 1) {
                 hard[atomicAdd(nhard, 1u)] = (uint32_t)bq;  // a crowded bin: the LDS radix sort
             } else if (cq <= kChunkCap) {
                 medium[atomicAdd(nmedium, 1u)] = (uint32_t)bq;  // the 1024-thread in-LDS sort
@@ -1482,40 +1604,47 @@ __global__ __launch_bounds__(kCBlock) void lsd_sort_kernel(
     if (checksum_out) block_checksum_add(csum, s_sum, checksum_out);
 }
 
-// The listed segments {begin, count} (oversized buckets) that hold one key
-// value are in stable order already -- a bucket's records keep their input
-// (or sender) order through the scatter and the gather -- so they need no
-// sort: copied src -> dst when those differ (with the output checksum) and
-// their count set to 0 in the list.  One workgroup per entry.
+// The listed segments ({begin, count} pairs) that hold one key value are in
+// stable order already.  one_value_check_kernel: gridDim.x workgroups per
+// segment (blockIdx.y) compare their share of its keys with its first key;
+// a segment with another key gets the top bit of its count set (negative).
+// one_value_copy_kernel: the segments left non-negative are copied to dst
+// (when it is another buffer) and added to the checksum.
 template <int RW>
-__global__ __launch_bounds__(kBlock) void one_value_filter_kernel(const uint32_t* __restrict__ src,
-                                                                  uint32_t* __restrict__ dst, LKey lk,
-                                                                  int64_t* __restrict__ list,
-                                                                  long long* __restrict__ checksum_out) {
-    __shared__ uint64_t s_mm[2 * kWaves];
-    __shared__ long long s_sum[kWaves];
-    const int64_t base = list[2 * blockIdx.x], cnt = list[2 * blockIdx.x + 1];
-    if (cnt <= 0) return;
-    uint64_t kmn = ~0ull, kmx = 0;
-    for (int64_t i = base + threadIdx.x; i < base + cnt; i += kBlock) {
+__global__ __launch_bounds__(kBlock) void one_value_check_kernel(const uint32_t* __restrict__ src, LKey lk,
+                                                                 int64_t* __restrict__ list) {
+    const int64_t base = list[2 * (int64_t)blockIdx.y], cnt = list[2 * (int64_t)blockIdx.y + 1];
+    if (cnt <= 0) return;  // empty, or another workgroup found a second key value
+    uint32_t r0[RW];
+    RecIO<RW>::load(src, base, r0);
+    const uint64_t k0 = lkey<RW>(r0, lk);
+    bool diff = false;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * kBlock) {
         uint32_t rr[RW];
-        RecIO<RW>::load(src, i, rr);
-        const uint64_t kk = lkey<RW>(rr, lk);
-        kmn = kk < kmn ? kk : kmn;
-        kmx = kk > kmx ? kk : kmx;
+        RecIO<RW>::load(src, base + i, rr);
+        diff = diff || lkey<RW>(rr, lk) != k0;
     }
-    if (!block_one_value<kWaves>(kmn, kmx, s_mm)) return;
+    if (__ballot(diff) != 0 && (threadIdx.x & 63) == 0)
+        atomicOr(reinterpret_cast<unsigned int*>(&list[2 * (int64_t)blockIdx.y + 1]) + 1, 0x80000000u);
+}
+
+template <int RW>
+__global__ __launch_bounds__(kBlock) void one_value_copy_kernel(const uint32_t* __restrict__ src,
+                                                                uint32_t* __restrict__ dst,
+                                                                const int64_t* __restrict__ list,
+                                                                long long* __restrict__ checksum_out) {
+    __shared__ long long s_sum[kWaves];
+    const int64_t base = list[2 * (int64_t)blockIdx.y], cnt = list[2 * (int64_t)blockIdx.y + 1];
+    if (cnt <= 0) return;
     long long csum = 0;
-    for (int64_t i = base + threadIdx.x; i < base + cnt; i += kBlock) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * kBlock) {
         uint32_t rr[RW];
-        RecIO<RW>::load(src, i, rr);
-        if (src != dst) RecIO<RW>::store(dst, i, rr);
+        RecIO<RW>::load(src, base + i, rr);
+        if (src != dst) RecIO<RW>::store(dst, base + i, rr);
 #pragma unroll
         for (int w = 0; w < RW; w++) csum += sbyte_sum(rr[w]);
     }
     if (checksum_out) block_checksum_add<kWaves>(csum, s_sum, checksum_out);
-    __syncthreads();
-    if (threadIdx.x == 0) list[2 * blockIdx.x + 1] = 0;
 }
 
 // range[j][b] = first entry above segment j's FirstSort boundary b - 1 (the
@@ -1797,13 +1926,25 @@ int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const ui
 }
 
 int launch_one_value_filter(int rw, const void* src, void* dst, const LKey& lk, int64_t* list, uint32_t nlist,
-                            long long* checksum, hipStream_t st) {
+                            int64_t max_count, long long* checksum, hipStream_t st) {
     if (nlist == 0) return 0;
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
+    // about 16 records per thread of the largest segment, at most 1024
+    // workgroups per segment
+    const int64_t per = (int64_t)kBlock * 16;
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (max_count + per - 1) / per));
+    const dim3 grid(gx, nlist);
+    const bool copy = src != dst || checksum != nullptr;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(one_value_filter_kernel<2>, dim3(nlist), dim3(kBlock), 0, st, s, d, lk, list, checksum); break;
-        case 4: hipLaunchKernelGGL(one_value_filter_kernel<4>, dim3(nlist), dim3(kBlock), 0, st, s, d, lk, list, checksum); break;
+        case 2:
+            hipLaunchKernelGGL(one_value_check_kernel<2>, grid, dim3(kBlock), 0, st, s, lk, list);
+            if (copy) hipLaunchKernelGGL(one_value_copy_kernel<2>, grid, dim3(kBlock), 0, st, s, d, list, checksum);
+            break;
+        case 4:
+            hipLaunchKernelGGL(one_value_check_kernel<4>, grid, dim3(kBlock), 0, st, s, lk, list);
+            if (copy) hipLaunchKernelGGL(one_value_copy_kernel<4>, grid, dim3(kBlock), 0, st, s, d, list, checksum);
+            break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());

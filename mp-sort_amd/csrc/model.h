@@ -42,11 +42,15 @@ int launch_leaves(const float* x, const float* y, int64_t S, const float* eff, i
 int launch_leaf_xstart(const float* eff, int L, int linear_root, float* xstart, hipStream_t st);
 int launch_model_eval(const ModelDev& m, const uint64_t* keys, int64_t n, float* out, hipStream_t st);
 // nedge (optional): per table, boundaries kept on each side of [q0, q1];
-// in_span: heavy keys only inside each table's own span (tables of
-// consecutive quantile ranges then stay in order one after the other)
+// heavy keys (ascending, at most kMaxHeavy) with their sampled quantile
+// ranges {qa, qb}: a boundary whose quantile lies in such a range is that
+// key, and a table isolates the heavy keys inside its span that weigh at
+// least one of its buckets; ordered: the tables are of consecutive quantile
+// ranges and are made to read as one sorted sequence
+constexpr int kMaxHeavy = 16384;
 int launch_model_tables(const ModelDev& m, uint64_t kmax, const float* q0, const float* q1, int ntables,
-                        int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st,
-                        const int* nedge = nullptr, bool in_span = false);
+                        int nb, uint64_t* tables, const uint64_t* heavy, const float2* heavy_q, int nheavy,
+                        hipStream_t st, const int* nedge = nullptr, bool ordered = false);
 // per segment of unsorted records: the quantile range of its keys and its
 // edge boundaries (see model_tables_kernel)
 int launch_seg_range(const ModelDev& m, const void* recs, int64_t elsize, int64_t key_off, int key_bytes,

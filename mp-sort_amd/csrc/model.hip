@@ -270,7 +270,8 @@ __global__ void leaf_xstart_kernel(ModelDev m, float* __restrict__ xstart) {
 // per boundary.
 __global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, const float* __restrict__ q1,
                                     const int* __restrict__ nedge, int ntables, int nb,
-                                    uint64_t* __restrict__ tables) {
+                                    uint64_t* __restrict__ tables, const uint64_t* __restrict__ heavy,
+                                    const float2* __restrict__ heavy_q, int nheavy) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (int64_t)ntables * nb) return;
     const int t = (int)(g / nb), i = (int)(g % nb);
@@ -284,7 +285,25 @@ __global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, co
     } else {
         q = a + (b - a) * (float)(i - E + 1) / (float)(nb - 2 * E + 1);
     }
-    tables[g] = model_inverse_leaves(m, q);
+    uint64_t k = model_inverse_leaves(m, q);
+    if (nheavy > 0) {
+        // exact at the heavy keys, whatever the model's error at their
+        // jumps: q inside heavy key h's sampled range [qa, qb) gives h's
+        // key; otherwise the key lies strictly between the heavy keys whose
+        // ranges end before q and start after it
+        int lo = 0, hi = nheavy;  // lo = #{heavy keys with qb <= q}
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (heavy_q[mid].y <= q) lo = mid + 1; else hi = mid;
+        }
+        if (lo < nheavy && heavy_q[lo].x <= q) {
+            k = heavy[lo];
+        } else {
+            if (lo < nheavy && heavy[lo] > 0 && k >= heavy[lo]) k = heavy[lo] - 1;
+            if (lo > 0 && k <= heavy[lo - 1] && heavy[lo - 1] < ~0ull) k = heavy[lo - 1] + 1;
+        }
+    }
+    tables[g] = k;
 }
 
 // Quantile range of each segment's keys under the model, from 256 keys read
@@ -359,16 +378,29 @@ __global__ __launch_bounds__(64) void seg_range_kernel(ModelDev m, const uint8_t
 
 // per table (one block each, nb <= 4096): running max so the table is sorted
 // even if FP32 rounding made F locally non-monotone; then every heavy key v
-// (ascending) gets the boundary pair {v - 1, v}, i.e. a bucket holding v
-// alone.  Overwriting the two boundaries around v's position keeps the table
-// sorted; a pair that would overwrite the previous heavy key's pair moves
-// just after it, or is dropped when the table has no room left.
+// (ascending) that weighs at least one of the table's buckets (sampled
+// weight qb - qa >= (q1 - q0) / (nb - 2 nedge)) and lies inside its span
+// [t[0], t[nb - 1]] or within (q1 - q0) / 4 of its quantile range (the data
+// sorted may hold a little more of v than the training sample, spilling
+// into this table's first or last bucket) gets the boundary pair {v - 1, v},
+// i.e. a bucket holding v alone.
+// Overwriting the two boundaries around v's position keeps the table sorted;
+// a pair that would overwrite the previous heavy key's pair moves just after
+// it, or is dropped when the table has no room left.
+constexpr int kFixHeavy = 16;             // heavy keys per thread (kMaxHeavy / 1024)
+constexpr int kFixList = 2048;            // heavy keys one table takes, at most
 __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__ tables, int ntables, int nb,
-                                                          const uint64_t* __restrict__ heavy, int nheavy, int in_span) {
+                                                          const uint64_t* __restrict__ heavy,
+                                                          const float2* __restrict__ heavy_q, int nheavy,
+                                                          const float* __restrict__ q0, const float* __restrict__ q1,
+                                                          const int* __restrict__ nedge) {
     (void)ntables;  // one block per table
     constexpr int E = 4;  // entries per thread
     __shared__ uint64_t s_t[1024 * E];
     __shared__ uint64_t s_w[16];
+    __shared__ uint64_t s_h[kFixList];  // this table's heavy keys, ascending
+    __shared__ uint32_t s_c[16];
+    __shared__ uint32_t s_nh;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint64_t* tb = tables + (int64_t)blockIdx.x * nb;
     // entries E tid .. E tid + E - 1: their max, then a running max over
@@ -398,15 +430,52 @@ __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__
         if (E * tid + q < nb) s_t[E * tid + q] = before;
     }
     __syncthreads();
+    {
+        // this table's heavy keys, in key order: kFixHeavy consecutive ones
+        // per thread, kept by a block prefix count
+        const int t = blockIdx.x;
+        const float qa = q0 ? q0[t] : 0.f, qb = q1 ? q1[t] : 1.f;
+        const int ne = nedge ? nedge[t] : 0;
+        const float wmin = (qb - qa) / (float)(nb - 2 * ne > 1 ? nb - 2 * ne : 1);
+        const uint64_t tfirst = s_t[0], tlast = s_t[nb - 1];
+        const float qtol = (qb - qa) * 0.25f;
+        uint32_t keep = 0, cnt = 0;
+#pragma unroll
+        for (int q = 0; q < kFixHeavy; q++) {
+            const int h = tid * kFixHeavy + q;
+            if (h < nheavy) {
+                const float2 hq = heavy_q[h];
+                const uint64_t hv = heavy[h];
+                const bool ok = hq.y - hq.x >= wmin &&
+                                ((hv >= tfirst && hv <= tlast) || (hq.y >= qa - qtol && hq.x <= qb + qtol));
+                keep |= ok ? 1u << q : 0u;
+                cnt += ok ? 1u : 0u;
+            }
+        }
+        uint32_t x = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_c[wave] = x;
+        __syncthreads();
+        uint32_t pre = x - cnt;
+        for (int w = 0; w < wave; w++) pre += s_c[w];
+        if (tid == 1023) s_nh = pre + cnt;
+#pragma unroll
+        for (int q = 0; q < kFixHeavy; q++)
+            if ((keep >> q) & 1u) {
+                if (pre < (uint32_t)kFixList) s_h[pre] = heavy[tid * kFixHeavy + q];
+                pre++;
+            }
+    }
+    __syncthreads();
     if (tid == 0 && nb >= 2) {
         int last = -1;  // slot of the previous heavy key
-        // in_span: heavy keys outside the table's span are left to the other
-        // tables, so that tables made for consecutive quantile ranges stay in
-        // order one after the other
-        const uint64_t tfirst = s_t[0], tlast = s_t[nb - 1];
-        for (int h = 0; h < nheavy; h++) {
-            const uint64_t hv = heavy[h];
-            if (in_span && (hv < tfirst || hv > tlast)) continue;
+        const int nh = s_nh < (uint32_t)kFixList ? (int)s_nh : kFixList;
+        for (int h = 0; h < nh; h++) {
+            const uint64_t hv = s_h[h];
             if (hv == 0) {
                 s_t[0] = 0;  // bucket 0 = keys <= 0
                 last = 0;
@@ -436,6 +505,30 @@ __global__ __launch_bounds__(1024) void tables_fix_kernel(uint64_t* __restrict__
 #pragma unroll
     for (int q = 0; q < E; q++)
         if (E * tid + q < nb) tb[E * tid + q] = s_t[E * tid + q];
+}
+
+// tables of consecutive quantile ranges, one after the other (one block per
+// table after the first): entries below the largest entry of the earlier
+// tables are raised to it, so that the tables read as one sorted sequence
+// (two tables that both isolate a heavy key on their common edge would
+// otherwise overlap by its pair)
+__global__ __launch_bounds__(256) void tables_order_kernel(uint64_t* __restrict__ tables, int ntables, int nb) {
+    (void)ntables;
+    __shared__ uint64_t s_m;
+    const int t = blockIdx.x + 1;
+    if (threadIdx.x == 0) {
+        uint64_t m = 0;
+        for (int u = 0; u < t; u++) {
+            const uint64_t e = tables[(int64_t)u * nb + nb - 1];  // each table is sorted
+            m = e > m ? e : m;
+        }
+        s_m = m;
+    }
+    __syncthreads();
+    const uint64_t m = s_m;
+    uint64_t* tb = tables + (int64_t)t * nb;
+    for (int i = threadIdx.x; i < nb; i += 256)
+        if (tb[i] < m) tb[i] = m;
 }
 
 // training samples from a sorted record array: S stratified positions with
@@ -523,17 +616,21 @@ int launch_model_eval(const ModelDev& m, const uint64_t* keys, int64_t n, float*
 }
 
 int launch_model_tables(const ModelDev& m, uint64_t kmax, const float* q0, const float* q1, int ntables,
-                        int nb, uint64_t* tables, const uint64_t* heavy, int nheavy, hipStream_t st,
-                        const int* nedge, bool in_span) {
+                        int nb, uint64_t* tables, const uint64_t* heavy, const float2* heavy_q, int nheavy,
+                        hipStream_t st, const int* nedge, bool ordered) {
     const int64_t tot = (int64_t)ntables * nb;
     (void)kmax;
     hipLaunchKernelGGL(model_tables_kernel, dim3(nblk(tot, 128)), dim3(128), 0, st, m, q0, q1, nedge, ntables, nb,
-                       tables);
+                       tables, heavy, heavy_q, nheavy);
     MPSG_CHECK(hipGetLastError());
-    if (nb > 4096) return -1;
-    hipLaunchKernelGGL(tables_fix_kernel, dim3(ntables), dim3(1024), 0, st, tables, ntables, nb, heavy, nheavy,
-                       in_span ? 1 : 0);
+    if (nb > 4096 || nheavy > kMaxHeavy) return -1;
+    hipLaunchKernelGGL(tables_fix_kernel, dim3(ntables), dim3(1024), 0, st, tables, ntables, nb, heavy, heavy_q,
+                       nheavy, q0, q1, nedge);
     MPSG_CHECK(hipGetLastError());
+    if (ordered && ntables > 1) {
+        hipLaunchKernelGGL(tables_order_kernel, dim3(ntables - 1), dim3(256), 0, st, tables, ntables, nb);
+        MPSG_CHECK(hipGetLastError());
+    }
     return 0;
 }
 

@@ -83,6 +83,7 @@ class Stats(ctypes.Structure):
         ("repair_rounds", ctypes.c_int32), ("plain_sides", ctypes.c_int32),
         ("t_model_ms", ctypes.c_double), ("t_bucket_ms", ctypes.c_double),
         ("buckets_lds_radix", ctypes.c_int32), ("buckets_overflow_first", ctypes.c_int32),
+        ("buckets_listed", ctypes.c_int32), ("reserved4", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -72,19 +72,28 @@ def test_learned_equals_plain_and_counts(dist, mode):
 
 @pytest.mark.parametrize("mode", ["merged", "exchange"])
 @pytest.mark.parametrize("nvalues", [40, 400])
-def test_one_value_buckets_of_a_uniform_model(nvalues, mode):
+@pytest.mark.parametrize("shape", ["one", "pairs"])
+def test_few_value_buckets_of_a_uniform_model(shape, nvalues, mode):
     """A model of uniform keys and data of few values spread over the key
     range: a bucket then holds one value hundreds or thousands of times
-    without being a single-value bucket of the tables, so the LDS sort finds a
-    crowded bin and lists it for the LDS radix sort, whose keys are then all
-    equal (40 values: FirstSort buckets; 400: SecondSort buckets)."""
+    without being a single-value bucket of the tables (40 values: FirstSort
+    buckets; 400: SecondSort buckets).  "one": each value alone, so every
+    crowded bin of the in-LDS sort holds one key value and is placed in
+    record order there, with no LDS radix sort.  "pairs": next to each value
+    v, rare keys v + 1 and v + 2^40 (0.3% each), so v + 1 usually escapes the
+    bucket's 64-key sample and shares v's bin: crowded bins of two values,
+    which go to the LDS radix sort."""
     n, P = 1 << 20, 16
-    note("one-value buckets nvalues=%d mode=%s" % (nvalues, mode))
+    note("few-value buckets shape=%s nvalues=%d mode=%s" % (shape, nvalues, mode))
     model = trained("uniform", n)
     g = numpy.random.default_rng(nvalues)
     values = (numpy.arange(nvalues, dtype=numpy.uint64) * numpy.uint64((1 << 64) // nvalues - 12345)
               + numpy.uint64(777))
-    a = workloads.records16(values[g.integers(0, nvalues, n)])
+    keys = values[g.integers(0, nvalues, n)]
+    if shape == "pairs":
+        u = g.random(n)
+        keys = keys + numpy.where(u < 0.003, 1, numpy.where(u < 0.006, 1 << 40, 0)).astype(numpy.uint64)
+    a = workloads.records16(keys)
     ref, _ = gpu_sort(a, even(n, P), even(n, P), 'key')
     flags = _lib.LEARNED_EXCHANGE if mode == "exchange" else 0
     out = learned.sort_learned(a, model, 'key', nin=even(n, P), flags=flags)
@@ -93,7 +102,10 @@ def test_one_value_buckets_of_a_uniform_model(nvalues, mode):
     note("  overflow=%d lds_radix=%d" % (s['buckets_overflow'], s['buckets_lds_radix']))
     model.close()
     assert numpy.array_equal(out, ref), "learned output differs from the plain path"
-    assert s['buckets_lds_radix'] > 0
+    if shape == "one":
+        assert s['buckets_lds_radix'] == 0
+    else:
+        assert s['buckets_lds_radix'] > 0
 
 
 @pytest.mark.parametrize("P", [1, 3, 8, 64, 100])
