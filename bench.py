@@ -237,10 +237,12 @@ def bench_batched(args):
 
 def bench_snapshot(args):
     """The snapshot-matching workload: two arrays of 2^27 particles
-    ({u64 id, f32 pos[3], f32 vel[3]}, 32 B), A with the ids a seeded
-    permutation, B a copy of A with 0.1% of the records displaced; both
-    sorted by id on 64 virtual ranks, learned path with one model (trained
-    on A's plain sort) for both.  One step = the sorts of A and B."""
+    ({u64 id, f32 pos[3], f32 vel[3]}, 32 B): A with the ids a seeded
+    permutation, B = A sorted by id with 0.1% of the records displaced (its
+    exchange is sparse).  Both sorted by id on 64 virtual ranks, learned path
+    with one model (trained on A's plain sort) for both; one step = the sorts
+    of A and B.  Beside it, the plain path's exchange with the dense and the
+    sparse run table, for A and for B, on 64 and on 4096 virtual ranks."""
     import numpy
     from mpsort import _lib, learned, workloads
     lib = _lib.lib
@@ -249,32 +251,52 @@ def bench_snapshot(args):
     a, b = workloads.snapshot_pair(n, 27)
     log("generated 2 x %d particles in %.2f s" % (n, time.perf_counter() - t0))
     key = learned._key_desc(a.dtype, 'id')
-    nin = (ctypes.c_int64 * P)(*[n * (t + 1) // P - n * t // P for t in range(P)])
+
+    def counts(p):
+        return (ctypes.c_int64 * p)(*[n * (t + 1) // p - n * t // p for t in range(p)])
+
+    nin = counts(P)
     da, db, dout = (lib.mpsort_gpu_dmalloc(a.nbytes) for _ in range(3))
     if not (da and db and dout):
         print("bench.py: device allocation failed", file=sys.stderr)
         return 4
     _lib.check(lib.mpsort_gpu_memcpy_h2d(da, a.ctypes.data, a.nbytes))
     _lib.check(lib.mpsort_gpu_memcpy_h2d(db, b.ctypes.data, b.nbytes))
+    del b
     flags = _lib.DEVICE_PTRS | _lib.TIMING
+
+    def timed(step, label, steps=None):
+        steps = steps or args.steps
+        for i in range(args.warmup):
+            step()
+        _lib.check(lib.mpsort_gpu_synchronize())
+        t0 = time.perf_counter()
+        for i in range(steps):
+            step()
+        _lib.check(lib.mpsort_gpu_synchronize())
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        log("%s: %.3f ms per step" % (label, ms))
+        return ms
+
+    # the plain path's exchange: dense and sparse run tables, A and B
+    exchange = {}
+    for p in (P, 4096):
+        cp = counts(p)
+        for name, src in (("A", da), ("B", db)):
+            for pol, f in (("dense", _lib.EXCHANGE_DENSE), ("sparse", _lib.EXCHANGE_SPARSE)):
+                def one():
+                    _lib.check(lib.mpsort_gpu_newarray(src, cp, p, dout, cp, 32, ctypes.byref(key), flags | f))
+                ms = timed(one, "plain %s P=%d %s exchange" % (name, p, pol), steps=max(2, args.steps // 2))
+                st = _lib.last_stats()
+                exchange.setdefault("P%d" % p, {}).setdefault(name, {})[pol] = {
+                    "sort_ms": round(ms, 3), "exchange_ms": round(st.t_exchange_ms, 4),
+                    "runs_walked": st.exchange_runs, "sparse": st.exchange_sparse}
 
     def plain_step():
         for src in (da, db):
             _lib.check(lib.mpsort_gpu_newarray(src, nin, P, dout, nin, 32, ctypes.byref(key), flags))
 
-    def timed(step, label):
-        for i in range(args.warmup):
-            step()
-        _lib.check(lib.mpsort_gpu_synchronize())
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            step()
-        _lib.check(lib.mpsort_gpu_synchronize())
-        ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        log("%s: %.3f ms per step (2 sorts), %.1f Mrecords/s" % (label, ms, 2 * n / (ms * 1e-3) / 1e6))
-        return ms
-
-    ms_plain = timed(plain_step, "plain")
+    ms_plain = timed(plain_step, "plain (A then B)")
     ref = numpy.empty_like(a)
     _lib.check(lib.mpsort_gpu_newarray(da, nin, P, dout, nin, 32, ctypes.byref(key), flags))
     _lib.check(lib.mpsort_gpu_memcpy_d2h(ref.ctypes.data, dout, ref.nbytes))
@@ -290,13 +312,14 @@ def bench_snapshot(args):
             _lib.check(lib.mpsort_gpu_newarray_learned(src, nin, P, dout, nin, 32, ctypes.byref(key), model._h,
                                                        flags))
 
-    ms = timed(learned_step, "learned")
+    ms = timed(learned_step, "learned (A then B)")
     st = _lib.last_stats().as_dict()
     ok = True
     out = numpy.empty_like(a)
     for src in (da, db):
         _lib.check(lib.mpsort_gpu_newarray_learned(src, nin, P, dout, nin, 32, ctypes.byref(key), model._h, flags))
         _lib.check(lib.mpsort_gpu_memcpy_d2h(out.ctypes.data, dout, out.nbytes))
+        # B holds A's records (0.1% of them moved): both sort to the same array
         ok = ok and bool(numpy.array_equal(out.view(numpy.uint8), ref.view(numpy.uint8)))
     model.close()
     for p in (da, db, dout):
@@ -312,13 +335,14 @@ def bench_snapshot(args):
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": "snapshot matching: A = seeded permutation of ids with uniform pos/vel, "
-                               "B = A with 0.1%% of records displaced, %d virtual ranks" % P,
+                               "B = A sorted by id with 0.1%% of records displaced, %d virtual ranks" % P,
                    "records_per_array": n, "record_bytes": 32},
         "ids_equal_and_sorted": ok, "train_s": round(t_train, 3),
         "plain": {"value": round(2 * n / (ms_plain * 1e-3) / 1e6, 2), "ms_per_step": round(ms_plain, 3)},
         "learned_vs_plain": round(ms_plain / ms, 3),
         "last_sort_phases_ms": {k: round(st[k], 4) for k in PHASES},
         "sort_equivalent_GBps": round(2 * 2.0 * n * 32 / (ms * 1e-3) / 1e9, 1),
+        "plain_exchange": exchange,
     }
     print(json.dumps(result), flush=True)
     return 0

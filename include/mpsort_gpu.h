@@ -70,6 +70,15 @@ typedef struct {
 #define MPSORT_GPU_LEARNED_EXCHANGE (1u << 5) /* learned path: explicit exchange, then the
                                                  SecondSort (default: each receiver bucket is
                                                  gathered from the senders' sorted runs) */
+/* The exchange's run table (MPIU_Alltoallv mp-mpiu.c:125-136): dense walks
+ * every (sender, receiver) slot, sparse only the non-empty runs.  Neither
+ * flag: the reference's AUTO vote (dense when some sender has more than 128
+ * non-empty receivers), unless mpsort_mpi_set_options holds
+ * MPSORT_DISABLE_SPARSE_ALLTOALLV or MPSORT_REQUIRE_SPARSE_ALLTOALLV (which
+ * wins, as in mpsort-mpi.c:576-583).  Sparse falls back to dense when the
+ * non-empty runs exceed 64 per rank on average.  The result is the same. */
+#define MPSORT_GPU_EXCHANGE_DENSE (1u << 6)
+#define MPSORT_GPU_EXCHANGE_SPARSE (1u << 7)
 
 /*
  * Sort the P virtual ranks of `base` (rank t holds nin[t] records, in rank
@@ -146,7 +155,9 @@ typedef struct {
     int32_t buckets_lds_radix; /* buckets sorted by the in-LDS radix sort */
     int32_t buckets_overflow_first; /* of buckets_overflow, those of the FirstSort */
     int32_t buckets_listed;    /* buckets sorted by the 1024-thread in-LDS form from a list */
-    int32_t reserved4;
+    int32_t exchange_sparse;   /* 1 when the exchange walked the non-empty runs only */
+    int32_t exchange_runs;     /* runs in the table it walked (P * P when dense) */
+    int32_t reserved5;
 } mpsort_gpu_stats;
 
 int mpsort_gpu_last_stats(mpsort_gpu_stats *stats);

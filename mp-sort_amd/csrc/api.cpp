@@ -19,6 +19,7 @@
 #include "model.h"
 #include "../../include/mpsort_gpu.h"
 #include "../../include/mp-mpiu.h"
+#include "../../include/mpsort.h"
 #include <thread>
 
 namespace mpsg {
@@ -31,12 +32,10 @@ int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sb
                   int64_t* mycle, uint64_t* split, uint64_t* pminmax, int* rounds, uint32_t* err,
                   hipStream_t st, const ModelDev* model = nullptr, int* exact = nullptr);
 int launch_layout(int P, const int64_t* C, const int64_t* scnt, const int64_t* sbeg, int64_t* myclt,
-                  int64_t* mycle, int64_t* myC, int64_t* send, int64_t* run_dst, int64_t* run_src,
-                  uint32_t* err, hipStream_t st);
-int launch_exchange(const void* src, void* dst, int64_t ntot, int rw, int nruns,
-                    const int64_t* run_dst, const int64_t* run_src, uint32_t* err, hipStream_t st);
-int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, int nruns,
-                         const int64_t* run_dst, const int64_t* run_src, uint32_t* err,
+                  int64_t* mycle, int64_t* myC, int64_t* send, const ExchRuns& ex, uint32_t* err, hipStream_t st);
+int launch_exchange(const void* src, void* dst, int64_t ntot, int rw, const RunTable& rt, uint32_t* err,
+                    hipStream_t st);
+int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, const RunTable& rt, uint32_t* err,
                          const int64_t* C, int P, const KeySpec& ks, uint32_t* hist, hipStream_t st);
 int launch_pack(const void* in, int64_t n, int64_t elsize, int64_t key_off, int key_bytes,
                 void* out, int rw, int iwords, long long* checksum, hipStream_t st);
@@ -273,8 +272,14 @@ static void lay_plan(A& a, int nseg, int npass, PlanArrays* pa) {
     }
 }
 
+// the compact run table of the sparse exchange: at most this many runs per
+// receiver on average (more: the dense table is used)
+constexpr int kSparseRunsPerRank = 64;
+static size_t sparse_cap(int P) { return std::min<size_t>((size_t)P * P, (size_t)P * kSparseRunsPerRank); }
+
 struct HistArrays {
-    int64_t *C, *win, *myclt, *mycle, *myC, *send, *run_dst, *run_src;
+    int64_t *C, *win, *myclt, *mycle, *myC, *send, *run_dst, *run_src, *run_dst_c, *run_src_c;
+    int *rnz, *snz, *roff, *sel;
     uint64_t *split, *pminmax;
     int* rounds;
     uint32_t* err;
@@ -293,6 +298,12 @@ static void lay_hist(A& a, int P, HistArrays* h) {
         h->send = a.template take<int64_t>(PP);
         h->run_dst = a.template take<int64_t>(PP);
         h->run_src = a.template take<int64_t>(PP);
+        h->run_dst_c = a.template take<int64_t>(sparse_cap(P));
+        h->run_src_c = a.template take<int64_t>(sparse_cap(P));
+        h->rnz = a.template take<int>(P);
+        h->snz = a.template take<int>(P);
+        h->roff = a.template take<int>(P1);
+        h->sel = a.template take<int>(4);
         h->split = a.template take<uint64_t>(Pm * 4);
         h->pminmax = a.template take<uint64_t>(8);
         h->rounds = a.template take<int>(Pm);
@@ -307,6 +318,12 @@ static void lay_hist(A& a, int P, HistArrays* h) {
         a.template take<int64_t>(PP);
         a.template take<int64_t>(PP);
         a.template take<int64_t>(PP);
+        a.template take<int64_t>(sparse_cap(P));
+        a.template take<int64_t>(sparse_cap(P));
+        a.template take<int>(P);
+        a.template take<int>(P);
+        a.template take<int>(P1);
+        a.template take<int>(4);
         a.template take<uint64_t>(Pm * 4);
         a.template take<uint64_t>(8);
         a.template take<int>(Pm);
@@ -350,7 +367,26 @@ struct CallShape {
     bool checksum;
     bool timing;
     bool exchange;  // learned path: keep the explicit exchange before the SecondSort
+    int exch_policy;  // ExchPolicy of the exchange's run table
 };
+
+static ExchRuns exch_runs(const HistArrays& h, int P, int policy) {
+    ExchRuns e;
+    e.dst = h.run_dst;
+    e.src = h.run_src;
+    e.dst_c = h.run_dst_c;
+    e.src_c = h.run_src_c;
+    e.rnz = h.rnz;
+    e.snz = h.snz;
+    e.roff = h.roff;
+    e.sel = h.sel;
+    e.cap = (int)sparse_cap(P);
+    e.policy = policy;
+    return e;
+}
+static RunTable run_table(const HistArrays& h, int P) {
+    return RunTable{h.run_dst, h.run_src, h.run_dst_c, h.run_src_c, h.sel, P * P};
+}
 
 static double ev_ms(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
@@ -525,14 +561,14 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
                                 h.split, h.pminmax, h.rounds, h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFindP], st));
-        if ((rc = launch_layout(P, h.C, pa.count, pa.begin, h.myclt, h.mycle, h.myC, h.send, h.run_dst,
-                                h.run_src, h.err, st)))
+        if ((rc = launch_layout(P, h.C, pa.count, pa.begin, h.myclt, h.mycle, h.myC, h.send,
+                                exch_runs(h, P, cs.exch_policy), h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
         // the exchange also builds the SecondSort's digit histograms
         if ((rc = seg_clear(planB, st))) return rc;
-        if ((rc = launch_exchange_hist(w.s0, w.s1, n, rw, P * P, h.run_dst, h.run_src, h.err, h.C, P,
-                                       cs.ks, planB.d_hist, st)))
+        if ((rc = launch_exchange_hist(w.s0, w.s1, n, rw, run_table(h, P), h.err, h.C, P, cs.ks, planB.d_hist,
+                                       st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
         SortBuffers b2{w.s1, final_dst, w.s0};
@@ -556,13 +592,18 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     MPSG_CHECK(hipMemcpyAsync(&errs[2], pb.pass_flags + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(sums, h.checksum, sizeof sums, hipMemcpyDeviceToHost, st));
     static int rounds[4096];
-    if (P > 1)
+    int sel[2] = {0, 0};
+    if (P > 1) {
         MPSG_CHECK(hipMemcpyAsync(rounds, h.rounds, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipMemcpyAsync(sel, h.sel, sizeof sel, hipMemcpyDeviceToHost, st));
+    }
     MPSG_CHECK(hipStreamSynchronize(st));
     for (int i = 0; i + 1 < P; i++) rounds_host = rounds[i] > rounds_host ? rounds[i] : rounds_host;
 
     // stats
     stats->nvranks = P;
+    stats->exchange_sparse = sel[0];
+    stats->exchange_runs = P > 1 ? (sel[0] ? sel[1] : P * P) : 0;
     stats->record_bytes = (int32_t)recbytes;
     stats->first_passes = first_passes;
     stats->second_passes = second_passes;
@@ -667,6 +708,13 @@ static int shape_for(const mpsort_key_desc* key, size_t elsize, unsigned flags, 
     cs->checksum = !(flags & MPSORT_GPU_NO_CHECKSUM);
     cs->timing = (flags & MPSORT_GPU_TIMING) != 0;
     cs->exchange = (flags & MPSORT_GPU_LEARNED_EXCHANGE) != 0;
+    // the call's flags, else the recorded options (REQUIRE over DISABLE, as
+    // mpsort-mpi.c:576-583), else the vote
+    if (flags & MPSORT_GPU_EXCHANGE_SPARSE) cs->exch_policy = kExchSparse;
+    else if (flags & MPSORT_GPU_EXCHANGE_DENSE) cs->exch_policy = kExchDense;
+    else if (g_options & MPSORT_REQUIRE_SPARSE_ALLTOALLV) cs->exch_policy = kExchSparse;
+    else if (g_options & MPSORT_DISABLE_SPARSE_ALLTOALLV) cs->exch_policy = kExchDense;
+    else cs->exch_policy = kExchAuto;
     return 0;
 }
 
@@ -1205,8 +1253,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                                 h.pminmax, h.rounds, h.err, st, &md, exact)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFindP], st));
-        if ((rc = launch_layout(P, h.C, pa.count, pa.begin, h.myclt, h.mycle, h.myC, h.send, h.run_dst,
-                                h.run_src, h.err, st)))
+        if ((rc = launch_layout(P, h.C, pa.count, pa.begin, h.myclt, h.mycle, h.myC, h.send,
+                                exch_runs(h, P, cs.exch_policy), h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
         if (!merged &&
@@ -1297,7 +1345,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             }
             if ((rc = markB(5))) return rc;
         } else {
-            if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, P * P, h.run_dst, h.run_src, h.err, st))) return rc;
+            if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, run_table(h, P), h.err, st))) return rc;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
             if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LB, planB, c.s1.p, final_dst, c.s0.p, nullptr,
                                          cs_direct ? h.checksum + 1 : nullptr, &lc, st, evB)))
@@ -1322,9 +1370,11 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             MPSG_CHECK(hipMemcpyAsync(lists[side], lc.lists[side], sizeof lists[side], hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(&err0, h.err, sizeof err0, hipMemcpyDeviceToHost, st));
     MPSG_CHECK(hipMemcpyAsync(sums, h.checksum, sizeof sums, hipMemcpyDeviceToHost, st));
+    int sel[2] = {0, 0};
     if (P > 1) {
         MPSG_CHECK(hipMemcpyAsync(rounds.data(), h.rounds, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
         MPSG_CHECK(hipMemcpyAsync(ex.data(), exact, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipMemcpyAsync(sel, h.sel, sizeof sel, hipMemcpyDeviceToHost, st));
     }
     MPSG_CHECK(hipStreamSynchronize(st));
     int rmax = 0, nexact = 0;
@@ -1355,6 +1405,9 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     stats->plain_sides = lc.global_fallback;
     stats->buckets_lds_radix = (int32_t)(lists[0][1] + lists[1][1]);
     stats->buckets_listed = (int32_t)(lists[0][2] + lists[1][2]);
+    // the merged SecondSort gathers from the senders' runs: no exchange
+    stats->exchange_sparse = P > 1 && !merged_side ? sel[0] : 0;
+    stats->exchange_runs = P > 1 && !merged_side ? (sel[0] ? sel[1] : P * P) : 0;
     // pass_ms: bucket histogram, scatter, in-LDS bucket sort, in-LDS radix
     // sort, overflow fallback for the FirstSort and then the SecondSort
     int np = 0;

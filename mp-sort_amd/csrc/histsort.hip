@@ -11,10 +11,16 @@
 //   layout_kernel   _solve_for_layout_mpi (mpsort-mpi.c:663-727), one thread
 //                   per receiving rank.
 //   runs_kernel     SendCount / RecvDispl (mpsort-mpi.c:483-510) as a table of
-//                   (sender, receiver) runs in output order.
+//                   (sender, receiver) runs in output order, and the count of
+//                   non-empty runs per receiver and per sender.
+//   runs_vote_kernel, runs_compact_kernel  the dense / sparse choice of
+//                   MPIU_Alltoallv (mp-mpiu.c:125-136) and the table of the
+//                   non-empty runs alone (the sparse form, mp-mpiu.c:158-236,
+//                   which skips the empty pairs)
 //   exchange_kernel MPIU_Alltoallv (mp-mpiu.c:69-156): every output record is
-//                   copied from its sender run; receiver buffers hold the
-//                   sender runs in sender order.
+//                   copied from its sender run, found by a search in the
+//                   chosen run table; receiver buffers hold the sender runs
+//                   in sender order.
 #include "device.h"
 #include "model_dev.h"
 
@@ -480,10 +486,11 @@ __device__ __forceinline__ long long wave_incl_scan_ll(long long v) {
 // of the count matrices.
 __global__ void layout_kernel(int P, const int64_t* __restrict__ C, const int64_t* __restrict__ scnt,
                               int64_t* __restrict__ myclt, int64_t* __restrict__ mycle,
-                              int64_t* __restrict__ myC, uint32_t* __restrict__ error_flag) {
+                              int64_t* __restrict__ myC, int* __restrict__ snz, uint32_t* __restrict__ error_flag) {
     const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (r >= P) return;  // uniform per wave
+    if (lane == 0) snz[r] = 0;  // counted by runs_kernel
     const int P1 = P + 1;
     const bool last = (r == P - 1);
     long long sure = 0;
@@ -527,12 +534,13 @@ __global__ void layout_kernel(int P, const int64_t* __restrict__ C, const int64_
 __global__ void runs_kernel(int P, const int64_t* __restrict__ C, const int64_t* __restrict__ sbeg,
                             const int64_t* __restrict__ myC, int64_t* __restrict__ send,
                             int64_t* __restrict__ run_dst, int64_t* __restrict__ run_src,
-                            uint32_t* __restrict__ error_flag) {
+                            int* __restrict__ rnz, int* __restrict__ snz, uint32_t* __restrict__ error_flag) {
     const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (r >= P) return;  // uniform per wave
     const int P1 = P + 1;
     long long displ = 0;  // counts of the senders before this chunk
+    int nz = 0;           // non-empty runs of this receiver
     for (int j0 = 0; j0 < P; j0 += 64) {
         const int j = j0 + lane;
         int64_t a = 0, b = 0;
@@ -548,10 +556,78 @@ __global__ void runs_kernel(int P, const int64_t* __restrict__ C, const int64_t*
             send[(int64_t)j * P + r] = cnt;
             run_dst[(int64_t)r * P + j] = C[r] + displ + (incl - cp);
             run_src[(int64_t)r * P + j] = sbeg[j] + a;
+            if (cnt > 0) atomicAdd(&snz[j], 1);
         }
+        nz += __popcll(__ballot(j < P && cnt > 0));
         displ += __shfl(incl, 63, 64);
     }
+    if (lane == 0) rnz[r] = nz;
     if (lane == 0 && displ != C[r + 1] - C[r]) atomicOr(error_flag, 16u);
+}
+
+// The exchange's choice (MPIU_Alltoallv mp-mpiu.c:125-136): dense when some
+// sender has more than 128 non-empty peers (AUTO), or when asked; sparse
+// otherwise, if the non-empty runs fit the compact table.  One block: the
+// prefix of the receivers' non-empty runs (roff, P + 1 entries) and sel.
+constexpr int kSparseVotePeers = 128;
+__global__ __launch_bounds__(1024) void runs_vote_kernel(int P, const int* __restrict__ rnz,
+                                                         const int* __restrict__ snz, int* __restrict__ roff,
+                                                         int* __restrict__ sel, int policy, int cap) {
+    __shared__ int s_w[16];
+    __shared__ int s_dense;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_dense = 0;
+    __syncthreads();
+    int carry = 0;  // runs of the receivers before this chunk
+    for (int r0 = 0; r0 < P; r0 += 1024) {
+        const int r = r0 + tid;
+        const int v = r < P ? rnz[r] : 0;
+        if (r < P && snz[r] > kSparseVotePeers) s_dense = 1;  // r as a sender
+        int x = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_w[wave] = x;
+        __syncthreads();
+        int pre = carry + x - v;
+        for (int w = 0; w < wave; w++) pre += s_w[w];
+        if (r < P) roff[r] = pre;
+        int tot = 0;
+        for (int w = 0; w < 16; w++) tot += s_w[w];
+        __syncthreads();
+        carry += tot;
+    }
+    if (tid == 0) {
+        roff[P] = carry;
+        const bool dense = policy == kExchDense || (policy == kExchAuto && s_dense) || carry > cap;
+        sel[0] = dense ? 0 : 1;
+        sel[1] = carry;
+    }
+}
+
+// the compact table: a wave per receiver copies its non-empty runs (when
+// the sparse table was chosen)
+__global__ void runs_compact_kernel(int P, const int64_t* __restrict__ send, const int64_t* __restrict__ run_dst,
+                                    const int64_t* __restrict__ run_src, const int* __restrict__ roff,
+                                    const int* __restrict__ sel, int64_t* __restrict__ dst_c,
+                                    int64_t* __restrict__ src_c) {
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= P || sel[0] == 0) return;  // uniform per wave
+    int o = roff[r];
+    for (int j0 = 0; j0 < P; j0 += 64) {
+        const int j = j0 + lane;
+        const bool nz = j < P && send[(int64_t)j * P + r] > 0;
+        const uint64_t m = __ballot(nz);
+        if (nz) {
+            const int q = o + (int)__popcll(m & ((1ull << lane) - 1ull));
+            dst_c[q] = run_dst[(int64_t)r * P + j];
+            src_c[q] = run_src[(int64_t)r * P + j];
+        }
+        o += (int)__popcll(m);
+    }
 }
 
 constexpr int kExchChunk = 4096;
@@ -559,13 +635,15 @@ constexpr int kExchLdsRuns = 512;
 
 template <int RW>
 __global__ __launch_bounds__(kBlock) void exchange_kernel(const uint32_t* __restrict__ src,
-                                                          uint32_t* __restrict__ dst, int64_t ntot,
-                                                          int nruns, const int64_t* __restrict__ run_dst,
-                                                          const int64_t* __restrict__ run_src,
+                                                          uint32_t* __restrict__ dst, int64_t ntot, RunTable rt,
                                                           uint32_t* __restrict__ error_flag) {
     __shared__ int64_t s_rd[kExchLdsRuns], s_rs[kExchLdsRuns];
     __shared__ int s_a, s_n;
     const int tid = threadIdx.x;
+    const bool sparse = rt.sel && rt.sel[0] != 0;
+    const int64_t* __restrict__ run_dst = sparse ? rt.dst_c : rt.dst;
+    const int64_t* __restrict__ run_src = sparse ? rt.src_c : rt.src;
+    const int nruns = sparse ? rt.sel[1] : rt.ndense;
     const int64_t o0 = (int64_t)blockIdx.x * kExchChunk;
     const int64_t o1 = min(ntot, o0 + (int64_t)kExchChunk);
     if (tid == 0) {
@@ -623,14 +701,17 @@ __global__ __launch_bounds__(kBlock) void exchange_kernel(const uint32_t* __rest
 // move).  hist is [P][npass][256], cleared beforehand.
 template <int RW>
 __global__ __launch_bounds__(kBlock) void exchange_hist_kernel(
-    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t ntot, int nruns,
-    const int64_t* __restrict__ run_dst, const int64_t* __restrict__ run_src,
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t ntot, RunTable rt,
     uint32_t* __restrict__ error_flag, const int64_t* __restrict__ C, int P, int npass,
     PassTable pt, uint32_t* __restrict__ hist) {
     __shared__ int64_t s_rd[kExchLdsRuns], s_rs[kExchLdsRuns];
     __shared__ uint32_t s_hist[kMaxPasses * kBins];
     __shared__ int s_a, s_n, s_r0, s_r1;
     const int tid = threadIdx.x;
+    const bool sparse = rt.sel && rt.sel[0] != 0;
+    const int64_t* __restrict__ run_dst = sparse ? rt.dst_c : rt.dst;
+    const int64_t* __restrict__ run_src = sparse ? rt.src_c : rt.src;
+    const int nruns = sparse ? rt.sel[1] : rt.ndense;
     const int nb = npass * kBins;
     const int64_t o0 = (int64_t)blockIdx.x * kExchChunk;
     const int64_t o1 = min(ntot, o0 + (int64_t)kExchChunk);
@@ -838,38 +919,42 @@ int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sb
 }
 
 int launch_layout(int P, const int64_t* C, const int64_t* scnt, const int64_t* sbeg, int64_t* myclt,
-                  int64_t* mycle, int64_t* myC, int64_t* send, int64_t* run_dst, int64_t* run_src,
-                  uint32_t* err, hipStream_t st) {
+                  int64_t* mycle, int64_t* myC, int64_t* send, const ExchRuns& ex, uint32_t* err, hipStream_t st) {
     const int blocks = (P + kWaves - 1) / kWaves;  // a wave per receiving rank
     hipLaunchKernelGGL(layout_kernel, dim3(blocks), dim3(kBlock), 0, st, P, C, scnt, myclt, mycle,
-                       myC, err);
+                       myC, ex.snz, err);
     MPSG_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(runs_kernel, dim3(blocks), dim3(kBlock), 0, st, P, C, sbeg, myC, send, run_dst,
-                       run_src, err);
+    hipLaunchKernelGGL(runs_kernel, dim3(blocks), dim3(kBlock), 0, st, P, C, sbeg, myC, send, ex.dst, ex.src,
+                       ex.rnz, ex.snz, err);
+    MPSG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(runs_vote_kernel, dim3(1), dim3(1024), 0, st, P, ex.rnz, ex.snz, ex.roff, ex.sel, ex.policy,
+                       ex.cap);
+    MPSG_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(runs_compact_kernel, dim3(blocks), dim3(kBlock), 0, st, P, send, ex.dst, ex.src, ex.roff,
+                       ex.sel, ex.dst_c, ex.src_c);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_exchange(const void* src, void* dst, int64_t ntot, int rw, int nruns,
-                    const int64_t* run_dst, const int64_t* run_src, uint32_t* err, hipStream_t st) {
+int launch_exchange(const void* src, void* dst, int64_t ntot, int rw, const RunTable& rt, uint32_t* err,
+                    hipStream_t st) {
     if (ntot <= 0) return 0;
     const int64_t blocks = (ntot + kExchChunk - 1) / kExchChunk;
     dim3 g((unsigned)blocks), b(kBlock);
     const uint32_t* s = (const uint32_t*)src;
     uint32_t* d = (uint32_t*)dst;
     switch (rw) {
-        case 1: hipLaunchKernelGGL(exchange_kernel<1>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
-        case 2: hipLaunchKernelGGL(exchange_kernel<2>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
-        case 4: hipLaunchKernelGGL(exchange_kernel<4>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
-        case 8: hipLaunchKernelGGL(exchange_kernel<8>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err); break;
+        case 1: hipLaunchKernelGGL(exchange_kernel<1>, g, b, 0, st, s, d, ntot, rt, err); break;
+        case 2: hipLaunchKernelGGL(exchange_kernel<2>, g, b, 0, st, s, d, ntot, rt, err); break;
+        case 4: hipLaunchKernelGGL(exchange_kernel<4>, g, b, 0, st, s, d, ntot, rt, err); break;
+        case 8: hipLaunchKernelGGL(exchange_kernel<8>, g, b, 0, st, s, d, ntot, rt, err); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
 
-int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, int nruns,
-                         const int64_t* run_dst, const int64_t* run_src, uint32_t* err,
+int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, const RunTable& rt, uint32_t* err,
                          const int64_t* C, int P, const KeySpec& ks, uint32_t* hist, hipStream_t st) {
     if (ntot <= 0) return 0;
     const int64_t blocks = (ntot + kExchChunk - 1) / kExchChunk;
@@ -879,10 +964,10 @@ int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, int n
     const PassTable pt = make_pass_table(ks);
     const int np = ks.key_bytes;
     switch (rw) {
-        case 1: hipLaunchKernelGGL(exchange_hist_kernel<1>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err, C, P, np, pt, hist); break;
-        case 2: hipLaunchKernelGGL(exchange_hist_kernel<2>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err, C, P, np, pt, hist); break;
-        case 4: hipLaunchKernelGGL(exchange_hist_kernel<4>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err, C, P, np, pt, hist); break;
-        case 8: hipLaunchKernelGGL(exchange_hist_kernel<8>, g, b, 0, st, s, d, ntot, nruns, run_dst, run_src, err, C, P, np, pt, hist); break;
+        case 1: hipLaunchKernelGGL(exchange_hist_kernel<1>, g, b, 0, st, s, d, ntot, rt, err, C, P, np, pt, hist); break;
+        case 2: hipLaunchKernelGGL(exchange_hist_kernel<2>, g, b, 0, st, s, d, ntot, rt, err, C, P, np, pt, hist); break;
+        case 4: hipLaunchKernelGGL(exchange_hist_kernel<4>, g, b, 0, st, s, d, ntot, rt, err, C, P, np, pt, hist); break;
+        case 8: hipLaunchKernelGGL(exchange_hist_kernel<8>, g, b, 0, st, s, d, ntot, rt, err, C, P, np, pt, hist); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());

@@ -89,6 +89,25 @@ int set_hip_error(hipError_t e, const char* what, const char* file, int line);
     } while (0)
 
 // ---- launchers (onesweep.hip) ----
+// The exchange's runs (receiver-major, sender order): the dense table of all
+// P * P (receiver, sender) slots, empty ones included, and the table of the
+// non-empty runs only, with the choice between them made on the device
+// (sel[0] = 1: the compact table, sel[1] = its length).  rnz / snz: per
+// receiver / sender, its non-empty runs; roff: prefix of rnz.
+enum ExchPolicy { kExchAuto = 0, kExchDense = 1, kExchSparse = 2 };
+struct ExchRuns {
+    int64_t *dst = nullptr, *src = nullptr;      // P * P
+    int64_t *dst_c = nullptr, *src_c = nullptr;  // cap
+    int *rnz = nullptr, *snz = nullptr, *roff = nullptr, *sel = nullptr;
+    int cap = 0;
+    int policy = kExchAuto;
+};
+struct RunTable {
+    const int64_t *dst, *src, *dst_c, *src_c;
+    const int* sel;
+    int ndense;
+};
+
 struct SortBuffers {
     void* src;        // input sort records (read only unless == dst)
     void* dst;        // output

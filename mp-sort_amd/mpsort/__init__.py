@@ -97,8 +97,10 @@ def sort(source, orderby=None, out=None, comm=None, tuning=[]):
     are u8, i8, u4 or i4, scalars or 1-d vectors whose later elements are
     more significant.  out: same total length and item size as source.
     tuning: option names ('DISABLE_SPARSE_ALLTOALLV', 'DISABLE_GATHER_SORT',
-    'REQUIRE_GATHER_SORT', 'REQUIRE_SPARSE_ALLTOALLV'); recorded, they do not
-    change the result.  source, orderby and out may be flat iterators.
+    'REQUIRE_GATHER_SORT', 'REQUIRE_SPARSE_ALLTOALLV'); the SPARSE ones pick
+    the exchange's run table (every rank pair, or the non-empty ones only),
+    the GATHER ones are recorded; none changes the result.  source, orderby
+    and out may be flat iterators.
     Returns out (None for a field-name key, as the reference).
     """
     if isinstance(orderby, (str, bytes)):

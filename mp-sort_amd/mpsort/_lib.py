@@ -29,6 +29,8 @@ TIMING = 1 << 2
 LEARNED = 1 << 3
 FORCE_GENERAL = 1 << 4
 LEARNED_EXCHANGE = 1 << 5
+EXCHANGE_DENSE = 1 << 6
+EXCHANGE_SPARSE = 1 << 7
 
 # mpsort.h:17-20
 MPSORT_DISABLE_SPARSE_ALLTOALLV = 1 << 1
@@ -83,7 +85,8 @@ class Stats(ctypes.Structure):
         ("repair_rounds", ctypes.c_int32), ("plain_sides", ctypes.c_int32),
         ("t_model_ms", ctypes.c_double), ("t_bucket_ms", ctypes.c_double),
         ("buckets_lds_radix", ctypes.c_int32), ("buckets_overflow_first", ctypes.c_int32),
-        ("buckets_listed", ctypes.c_int32), ("reserved4", ctypes.c_int32),
+        ("buckets_listed", ctypes.c_int32), ("exchange_sparse", ctypes.c_int32),
+        ("exchange_runs", ctypes.c_int32), ("reserved5", ctypes.c_int32),
     ]
 
     def as_dict(self):

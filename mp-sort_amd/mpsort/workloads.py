@@ -140,16 +140,21 @@ def mix64(z):
 def snapshot_pair(n, seed, displaced=0.001):
     """Two snapshots of n particles ({u64 id, f32 pos[3], f32 vel[3]}, 32 B):
     in A the ids are a seeded permutation of 0..n-1 and pos/vel uniform in
-    [0, 1); B is A with round(displaced * n) records moved to other chosen
-    positions (a cyclic shift among those positions)."""
+    [0, 1); B is A sorted by id with round(displaced * n) records moved to
+    other chosen positions (a cyclic shift among those positions): the next
+    snapshot of a code that keeps its particles in id order, whose exchange
+    is sparse (each rank keeps most of its records)."""
     a = numpy.empty(n, PARTICLE32)
     a['id'] = permutation(n, seed)
     g = numpy.random.default_rng(seed)
     a['pos'] = g.random((n, 3), dtype=numpy.float32)
     a['vel'] = g.random((n, 3), dtype=numpy.float32)
+    inv = numpy.empty(n, numpy.int64)
+    inv[a['id'].astype(numpy.int64)] = numpy.arange(n, dtype=numpy.int64)
+    b = a[inv]  # sorted by id: the ids are a permutation of 0..n-1
+    del inv
     m = int(round(displaced * n))
-    b = a.copy()
     if m > 1:
         pos = numpy.sort(g.choice(n, m, replace=False))
-        b[pos] = a[numpy.roll(pos, 1)]
+        b[pos] = b[numpy.roll(pos, 1)]
     return a, b
