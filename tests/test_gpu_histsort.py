@@ -319,3 +319,25 @@ def test_radix_callback_three_word_radix():
     note("callback 3-word radix n=%d" % n)
     out, st = _callback_sort(a, nin, nout, 0, 3)
     assert numpy.array_equal(out, a[numpy.lexsort(a['k'].T)])
+
+
+@pytest.mark.parametrize("entry", ["radix_sort", "mpsort_omp"])
+def test_radix_sort_and_mpsort_omp_exports(entry):
+    """The reference's single-process entry points (mpsort.h:1-11) with a C
+    radix callback: 16-byte records sorted in place by a u64 radix, equal to
+    numpy's stable sort (ties keep their input order)."""
+    import ctypes
+    from mpsort import _lib, workloads as w
+    from mpsort._lib import lib
+    n = 300000
+    a = rec16(n, 71, 1 << 12)
+    want = a[numpy.argsort(a['key'], kind='stable')]
+    fn = ctypes.cast(w._gen().mpsort_radix_u64_words, _lib.RADIX_FN)
+    arg = (ctypes.c_int64 * 2)(0, 1)
+    f = getattr(lib, entry)
+    f.restype = None
+    f.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, _lib.RADIX_FN, ctypes.c_size_t,
+                  ctypes.c_void_p]
+    note("%s n=%d" % (entry, n))
+    f(a.ctypes.data, n, a.dtype.itemsize, fn, 8, ctypes.cast(arg, ctypes.c_void_p))
+    assert numpy.array_equal(a, want)
