@@ -162,6 +162,29 @@ typedef struct {
 
 int mpsort_gpu_last_stats(mpsort_gpu_stats *stats);
 
+/* Fault hooks, for tests of the paths that the checks of the reference
+ * guard (it aborts instead, mpsort-mpi.c:324-330,496-510,707-716); no
+ * counterpart in the reference.  They stay set for the following calls of
+ * this process until cleared with faults = 0.
+ *   MPSORT_GPU_FAULT_SPLITTERS   learned path: the model's first proposal for
+ *       every splitter is moved by 0.37 in quantile, so the repair rounds
+ *       find every splitter
+ *   MPSORT_GPU_FAULT_BUCKET_CAP  learned path: buckets above bucket_cap
+ *       records (>= 1) take the overflow fallback, as if above the in-LDS
+ *       capacity
+ *   MPSORT_GPU_FAULT_PLAIN       learned path: every local sort takes the
+ *       plain radix sort of its whole side
+ *   MPSORT_GPU_FAULT_CORRUPT     mpsort_gpu_newarray / _learned: a byte past
+ *       the key of one record is flipped between FirstSort and SecondSort
+ *       (plain path, more than one rank, key not filling the record), else
+ *       the recorded input sum is changed; the call then fails with
+ *       MPSORT_ERR_CHECKSUM (unless MPSORT_GPU_NO_CHECKSUM) */
+#define MPSORT_GPU_FAULT_SPLITTERS (1u << 0)
+#define MPSORT_GPU_FAULT_BUCKET_CAP (1u << 1)
+#define MPSORT_GPU_FAULT_PLAIN (1u << 2)
+#define MPSORT_GPU_FAULT_CORRUPT (1u << 3)
+int mpsort_gpu_set_faults(unsigned faults, int64_t bucket_cap);
+
 /* Prints the phase times of the last call in the reference's format
  * (mpsort-mpi.c:110-119). */
 void mpsort_gpu_report_last_run(void);

@@ -30,11 +30,12 @@ namespace mpsg {
 int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sbeg,
                   const int64_t* scnt, int P, const int64_t* C, int64_t* win, int64_t* myclt,
                   int64_t* mycle, uint64_t* split, uint64_t* pminmax, int* rounds, uint32_t* err,
-                  hipStream_t st, const ModelDev* model = nullptr, int* exact = nullptr);
+                  hipStream_t st, const ModelDev* model = nullptr, int* exact = nullptr, bool perturb = false);
 int launch_layout(int P, const int64_t* C, const int64_t* scnt, const int64_t* sbeg, int64_t* myclt,
                   int64_t* mycle, int64_t* myC, int64_t* send, const ExchRuns& ex, uint32_t* err, hipStream_t st);
 int launch_exchange(const void* src, void* dst, int64_t ntot, int rw, const RunTable& rt, uint32_t* err,
                     hipStream_t st);
+int launch_fault_corrupt(void* byte, long long* checksum_in, hipStream_t st);
 int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, const RunTable& rt, uint32_t* err,
                          const int64_t* C, int P, const KeySpec& ks, uint32_t* hist, hipStream_t st);
 int launch_pack(const void* in, int64_t n, int64_t elsize, int64_t key_off, int key_bytes,
@@ -51,6 +52,12 @@ static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 static char g_err[1024];
 static int g_options = 0;
+// fault hooks (mpsort_gpu_set_faults): the bits and the bucket capacity
+static unsigned g_faults = 0;
+static int64_t g_fault_cap = 0;
+static uint32_t fault_cap() {
+    return (g_faults & MPSORT_GPU_FAULT_BUCKET_CAP) ? (uint32_t)g_fault_cap : (uint32_t)kChunkCap;
+}
 static int g_vranks = 0;  // 0 = not set yet (environment decides)
 static mpsort_gpu_malloc_fn g_malloc = nullptr;
 static mpsort_gpu_free_fn g_free = nullptr;
@@ -542,6 +549,10 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         final_dst = w.rbuf;
     }
     const bool cs_direct = cs.checksum && !cs.general;
+    // the corruption fault hook changes a byte past the key of the first
+    // FirstSort output record (more than one rank, direct records), else
+    // the recorded input sum
+    const bool corrupt_record = P > 1 && !cs.general && cs.ks.key_off + cs.ks.key_bytes < rw * 4;
 
     int first_passes = 0, second_passes = 0;
     hipEvent_t* pevA = cs.timing ? c.pass_ev[0] : nullptr;
@@ -556,6 +567,9 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     } else {
         SortBuffers b1{recs_in, w.s0, w.s1};
         if ((rc = segmented_sort(planA, b1, st, cs_direct, false, &first_passes, pevA))) return rc;
+        if ((g_faults & MPSORT_GPU_FAULT_CORRUPT) && corrupt_record &&
+            (rc = launch_fault_corrupt((char*)w.s0 + rw * 4 - 1, nullptr, st)))
+            return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         if ((rc = launch_bisect(w.s0, rw, cs.ks, pa.begin, pa.count, P, h.C, h.win, h.myclt, h.mycle,
                                 h.split, h.pminmax, h.rounds, h.err, st)))
@@ -581,6 +595,9 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
                                 cs.checksum ? h.checksum + 1 : nullptr, h.err, st)))
             return rc;
     }
+    if ((g_faults & MPSORT_GPU_FAULT_CORRUPT) && !corrupt_record &&
+        (rc = launch_fault_corrupt(nullptr, h.checksum, st)))
+        return rc;
     MPSG_CHECK(hipEventRecord(c.ev[kEvEnd], st));
 
     // ---- results
@@ -967,7 +984,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         // segments whose average bucket would exceed half the in-LDS capacity
         // would mostly overflow: such a side takes the plain radix sort
         const int64_t big = *std::max_element(cnt.begin(), cnt.end());
-        if (big > (int64_t)kLBins * kChunkCap / 2) {
+        if (big > (int64_t)kLBins * kChunkCap / 2 || (g_faults & MPSORT_GPU_FAULT_PLAIN)) {
             lc->global_fallback++;
             lc->unbucketed++;
             if ((rc = mark(0))) return rc;
@@ -1009,7 +1026,7 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     // crowded ones by the LDS radix sort
     if ((rc = launch_chunk_sort_small(rw, tmp, dst, lk, L.lhist, L.bbase, L.tables, L.tstride, (int64_t)P * kLBins,
                                       L.overflow, L.noverflow, kMaxOverflowList, L.hard, L.noverflow + 1, L.medium,
-                                      L.noverflow + 2, cs_out, eq_direct, st, eo)))
+                                      L.noverflow + 2, cs_out, eq_direct, st, eo, fault_cap())))
         return rc;
     if ((rc = mark(3))) return rc;
     uint32_t novh[3] = {0, 0, 0}, errs = 0;
@@ -1250,7 +1267,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         // splitters: values from the model, each checked with exact counts
         if ((rc = launch_bisect(c.s0.p, rw, cs.ks, pa.begin, pa.count, P, h.C, h.win, h.myclt, h.mycle, h.split,
-                                h.pminmax, h.rounds, h.err, st, &md, exact)))
+                                h.pminmax, h.rounds, h.err, st, &md, exact,
+                                (g_faults & MPSORT_GPU_FAULT_SPLITTERS) != 0)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFindP], st));
         if ((rc = launch_layout(P, h.C, pa.count, pa.begin, h.myclt, h.mycle, h.myC, h.send,
@@ -1306,7 +1324,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                                               LB.bbase, desc + o, sub_beg + (size_t)o * 64,
                                               sub_pref + (size_t)o * 64, big ? nbig : nsmall, LB.overflow,
                                               LB.noverflow, kMaxOverflowList, LB.hard, LB.noverflow + 1, LB.medium,
-                                              LB.noverflow + 2, cs_direct ? h.checksum + 1 : nullptr, st)))
+                                              LB.noverflow + 2, cs_direct ? h.checksum + 1 : nullptr, st,
+                                              fault_cap())))
                     return rc;
             }
             if ((rc = markB(7))) return rc;
@@ -1325,7 +1344,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = markB(4))) return rc;
             const uint32_t nov = novh[0];
             lc.overflow_buckets += (int)nov;
-            if (nov > (uint32_t)kMaxFallbackSegs) {
+            if (nov > (uint32_t)kMaxFallbackSegs || (g_faults & MPSORT_GPU_FAULT_PLAIN)) {
                 // more oversized buckets than the bucket fallback takes (the
                 // list may also have stopped at its capacity): every one of
                 // them was copied to its slots in sender order, so a radix
@@ -1358,6 +1377,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
                                 cs.checksum ? h.checksum + 1 : nullptr, h.err, st)))
             return rc;
     }
+    if ((g_faults & MPSORT_GPU_FAULT_CORRUPT) && (rc = launch_fault_corrupt(nullptr, h.checksum, st))) return rc;
     MPSG_CHECK(hipEventRecord(c.ev[kEvEnd], st));
 
     uint32_t err0 = 0;
@@ -2123,6 +2143,17 @@ static void parse_env_options(void) {
     if (getenv("MPSORT_REQUIRE_GATHER_SORT")) g_options |= (1 << 4);
     if (getenv("MPSORT_REQUIRE_SPARSE_ALLTOALLV")) g_options |= (1 << 6);
 }
+int mpsort_gpu_set_faults(unsigned faults, int64_t bucket_cap) {
+    const unsigned known = MPSORT_GPU_FAULT_SPLITTERS | MPSORT_GPU_FAULT_BUCKET_CAP | MPSORT_GPU_FAULT_PLAIN |
+                           MPSORT_GPU_FAULT_CORRUPT;
+    if (faults & ~known) return fail(MPSORT_ERR_ARG, "unknown fault bits 0x%x", faults & ~known);
+    if ((faults & MPSORT_GPU_FAULT_BUCKET_CAP) && (bucket_cap < 1 || bucket_cap > kChunkCap))
+        return fail(MPSORT_ERR_ARG, "bucket_cap %lld outside [1, %d]", (long long)bucket_cap, kChunkCap);
+    g_faults = faults;
+    g_fault_cap = (faults & MPSORT_GPU_FAULT_BUCKET_CAP) ? bucket_cap : 0;
+    return MPSORT_OK;
+}
+
 void mpsort_mpi_set_options(int options) {
     parse_env_options();
     g_options |= options;

@@ -276,8 +276,9 @@ __global__ __launch_bounds__(kBlock) void bisect_kernel(
                 if (select_direct(recs, rw, ks, sbeg, P, wi, wsum, mine, (long long)Ci - 1 - below, nw, s_cand[wave], &c))
                     v[nv++] = c;
             } else if (rounds == 1) {
-                v[nv++] = Key{{model_inverse_leaves(md, (float)(((double)Ci - 0.5) / (double)scnt_total(scnt, P))), 0,
-                               0, 0}};
+                double q = ((double)Ci - 0.5) / (double)scnt_total(scnt, P);
+                if (use_model == 2) q = q + 0.37 < 1.0 ? q + 0.37 : q - 0.63;  // the fault hook: a wrong proposal
+                v[nv++] = Key{{model_inverse_leaves(md, (float)q), 0, 0, 0}};
             } else if (wsum > 0) {
                 double t = (double)Ci - 0.5;
                 if (repeats >= 2) t += (moved < 0 ? 1.0 : -1.0) * (double)wsum * 0.125;
@@ -906,14 +907,14 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(const uint8_t* __restric
 int launch_bisect(const void* recs, int rw, const KeySpec& ks, const int64_t* sbeg,
                   const int64_t* scnt, int P, const int64_t* C, int64_t* win, int64_t* myclt,
                   int64_t* mycle, uint64_t* split, uint64_t* pminmax, int* rounds, uint32_t* err,
-                  hipStream_t st, const ModelDev* model, int* exact) {
+                  hipStream_t st, const ModelDev* model, int* exact, bool perturb) {
     if (P < 2) return 0;
     const int blocks = (P - 1 + kWaves - 1) / kWaves;
     ModelDev md{};
     if (model) md = *model;
     hipLaunchKernelGGL(bisect_kernel, dim3(blocks), dim3(kBlock), 0, st, (const uint32_t*)recs, rw,
                        ks, sbeg, scnt, P, C, win, myclt, mycle, split, pminmax, rounds, err, md,
-                       model ? 1 : 0, exact);
+                       model ? (perturb ? 2 : 1) : 0, exact);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
@@ -994,6 +995,20 @@ int launch_gather(const void* in, int64_t n, int64_t elsize, const void* srec, i
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(gather_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, (const uint8_t*)in,
                        n, elsize, (const uint32_t*)srec, rw, iwords, (uint8_t*)out, checksum, err);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+// The corruption fault hook: one byte of a sort record flipped (bit 0), or,
+// without a record to change, 1 added to the recorded input sum; either way
+// the output sum no longer matches.
+__global__ void fault_corrupt_kernel(uint8_t* __restrict__ byte, long long* __restrict__ checksum_in) {
+    if (byte) *byte ^= 1u;
+    else if (checksum_in) *checksum_in += 1;
+}
+
+int launch_fault_corrupt(void* byte, long long* checksum_in, hipStream_t st) {
+    hipLaunchKernelGGL(fault_corrupt_kernel, dim3(1), dim3(1), 0, st, (uint8_t*)byte, checksum_in);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }

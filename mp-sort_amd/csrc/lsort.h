@@ -83,11 +83,14 @@ int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegD
 // wrote the single-value buckets to dst.  Buckets above kChunkCap go to the
 // overflow list (*noverflow entries of {base, count}); hard (one uint32 per
 // bucket) and *nhard (zeroed by the caller) list the buckets handed from the
-// interpolation sort to the LSD sort.
+// interpolation sort to the LSD sort.  ovf_cap (here and in the other bucket
+// launchers): buckets above it take the overflow list as if they were above
+// the in-LDS capacity (the capacity fault hook; kChunkCap changes nothing).
 int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                       const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
                       int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
-                      long long* checksum, bool eq_done, hipStream_t st, hipEvent_t mid = nullptr);
+                      long long* checksum, bool eq_done, hipStream_t st, hipEvent_t mid = nullptr,
+                      uint32_t ovf_cap = (uint32_t)kChunkCap);
 
 // SecondSort straight from the senders' sorted segments (no exchange), for
 // P <= 64 receivers with kMergeBounds-boundary tables each:
@@ -130,7 +133,7 @@ int launch_merge_bucket(int rw, bool big, const void* recs, void* dst, const LKe
                         const MergeTask* desc, const int64_t* sub_beg,
                         const int32_t* sub_pref, uint32_t ntasks, int64_t* overflow, uint32_t* noverflow,
                         int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium,
-                        long long* checksum, hipStream_t st);
+                        long long* checksum, hipStream_t st, uint32_t ovf_cap = (uint32_t)kChunkCap);
 // in-LDS sort (1024-thread form) of the nlist listed buckets, src -> dst at
 // the same slots (src may equal dst); nbins buckets per segment, segment s
 // uses tables + s * tstride; crowded ones are appended to hard
@@ -145,7 +148,7 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
                             const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
                             int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
                             uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st,
-                            const EntryOut* eo = nullptr);
+                            const EntryOut* eo = nullptr, uint32_t ovf_cap = (uint32_t)kChunkCap);
 // LDS radix sort of the listed buckets (src may equal dst)
 int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                     const int64_t* bucket_base, const uint32_t* hard, const uint32_t* nhard, int64_t max_buckets,

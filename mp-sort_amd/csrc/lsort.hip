@@ -1115,7 +1115,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
     const uint64_t* __restrict__ tables, int64_t tstride, int64_t nbuckets, int64_t* __restrict__ overflow,
     uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard, uint32_t* __restrict__ nhard,
     uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium, long long* __restrict__ checksum_out,
-    int eq_done, EntryOut eo) {
+    int eq_done, EntryOut eo, uint32_t ovf_cap) {
     using C = BSort<BLOCK, ITEMS>;
     __shared__ long long s_sum[C::W];
     __shared__ uint32_t s_bins[C::BW];
@@ -1163,9 +1163,9 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
                 }
                 block_checksum_add<C::W>(csum, s_sum, checksum_out);
             }
-        } else if (cnt32 > (uint32_t)C::CAP && cnt32 <= (uint32_t)kChunkCap) {
+        } else if (cnt32 > (uint32_t)C::CAP && cnt32 <= (uint32_t)kChunkCap && cnt32 <= ovf_cap) {
             if (tid == 0) medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread form
-        } else if (cnt32 > (uint32_t)kChunkCap) {
+        } else if (cnt32 > (uint32_t)kChunkCap || cnt32 > ovf_cap) {
             if (tid == 0) {
                 const uint32_t slot = atomicAdd(noverflow, 1u);
                 if ((int)slot < max_overflow) {
@@ -1469,7 +1469,7 @@ __global__ __launch_bounds__(kMergeBlock, ITEMS == kMergeItems ? 6 : 4) void mer
     const MergeTask* __restrict__ desc, const int64_t* __restrict__ sub_beg, const int32_t* __restrict__ sub_pref,
     int64_t* __restrict__ overflow, uint32_t* __restrict__ noverflow, int max_overflow, uint32_t* __restrict__ hard,
     uint32_t* __restrict__ nhard, uint32_t* __restrict__ medium, uint32_t* __restrict__ nmedium,
-    long long* __restrict__ checksum_out) {
+    long long* __restrict__ checksum_out, uint32_t ovf_cap) {
     using C = BSort<kMergeBlock, ITEMS>;
     __shared__ int64_t s_beg[kMergeMaxP];
     __shared__ int s_pref[kMergeMaxP + 1];
@@ -1495,7 +1495,7 @@ __global__ __launch_bounds__(kMergeBlock, ITEMS == kMergeItems ? 6 : 4) void mer
     // the keys of buckets [ka, kb) lie in (t[ka-1], t[kb-1]]
     const bool bounded = ka > 0 && kb < kMergeBins && tlo < thi;
     int rc = 1;
-    if (!single && count <= C::CAP) {
+    if (!single && count <= C::CAP && (uint32_t)count <= ovf_cap) {
         rc = sort_bucket_lds<RW, kMergeBlock, GatherSrc<RW>, ITEMS>(gs, dst, base, count, lk, s_bins, checksum_out,
                                                                      bounded, bounded ? tlo + 1 : 0,
                                                                      bounded ? thi : ~0ull);
@@ -1524,7 +1524,13 @@ __global__ __launch_bounds__(kMergeBlock, ITEMS == kMergeItems ? 6 : 4) void mer
             }
         }
         if (!single && tid == 0) {
-            if (cq <= C::CAP && rc == 1) {
+            if ((uint32_t)cq > ovf_cap) {
+                const uint32_t slot = atomicAdd(noverflow, 1u);  // above the capacity given
+                if ((int)slot < max_overflow) {
+                    overflow[2 * slot] = baseq;
+                    overflow[2 * slot + 1] = (int64_t)cq;
+                }
+            } else if (cq <= C::CAP && rc == 1) {
                 hard[atomicAdd(nhard, 1u)] = (uint32_t)bq;  // a crowded bin: the LDS radix sort
             } else if (cq <= kChunkCap) {
                 medium[atomicAdd(nmedium, 1u)] = (uint32_t)bq;  // the 1024-thread in-LDS sort
@@ -1766,7 +1772,7 @@ int launch_lscatter(int rw, const void* src, void* dst, void* eq_dst, const SegD
 int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                       const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
                       int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
-                      long long* checksum, bool eq_done, hipStream_t st, hipEvent_t mid) {
+                      long long* checksum, bool eq_done, hipStream_t st, hipEvent_t mid, uint32_t ovf_cap) {
     if (nbuckets <= 0) {
         if (mid) MPSG_CHECK(hipEventRecord(mid, st));
         return 0;
@@ -1780,7 +1786,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
         case 2:
             hipLaunchKernelGGL((interp_sort_kernel<2, kCBlock>), dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, nullptr, nullptr,
-                               checksum, ed, EntryOut{});
+                               checksum, ed, EntryOut{}, ovf_cap);
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<2>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
@@ -1789,7 +1795,7 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
         case 4:
             hipLaunchKernelGGL((interp_sort_kernel<4, kCBlock>), dim3(grid), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
                                tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard, nullptr, nullptr,
-                               checksum, ed, EntryOut{});
+                               checksum, ed, EntryOut{}, ovf_cap);
             MPSG_CHECK(hipGetLastError());
             if (mid) MPSG_CHECK(hipEventRecord(mid, st));
             hipLaunchKernelGGL(lsd_sort_kernel<4>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, hist, bucket_base,
@@ -1805,7 +1811,7 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
                             const int64_t* bucket_base, const uint64_t* tables, int64_t tstride, int64_t nbuckets,
                             int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
                             uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st,
-                            const EntryOut* eop) {
+                            const EntryOut* eop, uint32_t ovf_cap) {
     if (nbuckets <= 0) return 0;
     const EntryOut eo = eop ? *eop : EntryOut{};
     const uint32_t* s = (const uint32_t*)src;
@@ -1816,12 +1822,12 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
         case 2:
             hipLaunchKernelGGL((interp_sort_kernel<2, kMergeBlock, kSmallItems>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
                                bucket_base, tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard,
-                               medium, nmedium, checksum, ed, eo);
+                               medium, nmedium, checksum, ed, eo, ovf_cap);
             break;
         case 4:
             hipLaunchKernelGGL((interp_sort_kernel<4, kMergeBlock, kSmallItems>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, hist,
                                bucket_base, tables, tstride, nbuckets, overflow, noverflow, max_overflow, hard, nhard,
-                               medium, nmedium, checksum, ed, eo);
+                               medium, nmedium, checksum, ed, eo, ovf_cap);
             break;
         default: return -1;
     }
@@ -1874,12 +1880,12 @@ int launch_merge_bucket(int rw, bool big, const void* recs, void* dst, const LKe
                         const MergeTask* desc, const int64_t* sub_beg,
                         const int32_t* sub_pref, uint32_t ntasks, int64_t* overflow, uint32_t* noverflow,
                         int max_overflow, uint32_t* hard, uint32_t* nhard, uint32_t* medium, uint32_t* nmedium,
-                        long long* checksum, hipStream_t st) {
+                        long long* checksum, hipStream_t st, uint32_t ovf_cap) {
     if (P > kMergeMaxP) return -1;
     if (ntasks == 0) return 0;
     const uint32_t* r = (const uint32_t*)recs;
     uint32_t* d = (uint32_t*)dst;
-#define MPSG_MERGE(R, I) hipLaunchKernelGGL((merge_bucket_kernel<R, I>), dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, desc, sub_beg, sub_pref, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum)
+#define MPSG_MERGE(R, I) hipLaunchKernelGGL((merge_bucket_kernel<R, I>), dim3(ntasks), dim3(kMergeBlock), 0, st, r, d, lk, seg_begin, bnd, P, hist, bucket_base, desc, sub_beg, sub_pref, overflow, noverflow, max_overflow, hard, nhard, medium, nmedium, checksum, ovf_cap)
     switch (rw * 2 + (big ? 1 : 0)) {
         case 4: MPSG_MERGE(2, kMergeItems); break;
         case 5: MPSG_MERGE(2, kSmallItems); break;

@@ -31,6 +31,10 @@ FORCE_GENERAL = 1 << 4
 LEARNED_EXCHANGE = 1 << 5
 EXCHANGE_DENSE = 1 << 6
 EXCHANGE_SPARSE = 1 << 7
+FAULT_SPLITTERS = 1 << 0
+FAULT_BUCKET_CAP = 1 << 1
+FAULT_PLAIN = 1 << 2
+FAULT_CORRUPT = 1 << 3
 
 # mpsort.h:17-20
 MPSORT_DISABLE_SPARSE_ALLTOALLV = 1 << 1
@@ -52,7 +56,7 @@ EXPORTS = [
     "mpsort_gpu_newarray_learned", "mpsort_gpu_segplan_create", "mpsort_gpu_segplan_add",
     "mpsort_gpu_segplan_run", "mpsort_gpu_segplan_info", "mpsort_gpu_segplan_free",
     "mpsort_gpu_sort_segments",
-    "mpiu_malloc", "mpiu_free", "mpiu_set_malloc", "mpsort_gpu_newarray_noalloc",
+    "mpiu_malloc", "mpiu_free", "mpiu_set_malloc", "mpsort_gpu_newarray_noalloc", "mpsort_gpu_set_faults",
 ]
 
 SEGPLAN_GRAPH = 1 << 0
@@ -130,6 +134,7 @@ def _load():
                                               ctypes.c_void_p, ctypes.c_uint]
     lib.mpsort_gpu_newarray_radix.restype = ctypes.c_int
     lib.mpsort_gpu_last_stats.argtypes = [P(Stats)]
+    lib.mpsort_gpu_set_faults.argtypes = [ctypes.c_uint, ctypes.c_int64]
     lib.mpsort_gpu_last_stats.restype = ctypes.c_int
     lib.mpsort_gpu_last_counts.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            i64p, i64p, i64p, i64p, i64p, i64p]

@@ -214,3 +214,12 @@ def test_globalrange_and_globalindices_on_one_process():
 def test_version():
     """test_mpsort.py:309-311"""
     assert hasattr(mpsort, "__version__")
+
+
+def test_fault_hook_arguments():
+    """mpsort_gpu_set_faults checks its bits and the capacity; 0 clears"""
+    assert lib.mpsort_gpu_set_faults(1 << 9, 0) == _lib.MPSORT_ERR_ARG and "unknown fault" in err()
+    assert lib.mpsort_gpu_set_faults(_lib.FAULT_BUCKET_CAP, 0) == _lib.MPSORT_ERR_ARG and "bucket_cap" in err()
+    assert lib.mpsort_gpu_set_faults(_lib.FAULT_BUCKET_CAP, 9217) == _lib.MPSORT_ERR_ARG
+    assert lib.mpsort_gpu_set_faults(_lib.FAULT_BUCKET_CAP | _lib.FAULT_PLAIN, 100) == 0
+    assert lib.mpsort_gpu_set_faults(0, 0) == 0
