@@ -29,8 +29,17 @@
 
 namespace mpsg {
 
-template <int RW>
+// FAST: the common layout (the key in the record's first words, no bit
+// shift) is taken without the word selection when the key has it (the
+// condition is the same for the whole kernel; used where the registers
+// allow the second code path)
+template <int RW, bool FAST = false>
 __device__ __forceinline__ uint64_t lkey(const uint32_t (&r)[RW], const LKey& k) {
+    if (FAST && k.w0 == 0 && k.sh == 0) {
+        uint64_t v = r[0];
+        if constexpr (RW > 1) v |= (uint64_t)r[1] << 32;
+        return (v & k.mask) ^ k.xmask;
+    }
     const uint32_t a = pick_word<RW>(r, k.w0), b = pick_word<RW>(r, k.w0 + 1),
                    c = pick_word<RW>(r, k.w0 + 2);
     const uint64_t lo = (uint64_t)a | ((uint64_t)b << 32);
@@ -752,12 +761,12 @@ __device__ __forceinline__ int sort_bucket_lds(const Src& src, uint32_t* __restr
 #pragma unroll
     for (int k = 0; k < C::I; k++) {
         const bool valid = k * C::B + tid < count;
-        const uint64_t key = lkey<RW>(r[k], lk);
+        const uint64_t key = lkey<RW, (ITEMS >= 9)>(r[k], lk);
         kk[k] = valid ? key : 0;
         mn = valid && key < mn ? key : mn;
         mx = valid && key > mx ? key : mx;
     }
-    if (tid < kSamp) s_samp[tid] = lkey<RW>(r[C::I], lk);
+    if (tid < kSamp) s_samp[tid] = lkey<RW, (ITEMS >= 9)>(r[C::I], lk);
     if (tid == 0) {
         s_crowd = 0;
         s_long = 0;
@@ -1146,6 +1155,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
         // bucket k holds one key value (see single_bucket)
         const bool single = tables && (kb == 0 ? cur.thi == 0 : (kb < kLBounds && cur.tlo + 1 == cur.thi));
         const int ej = (int)(b / kLBins);
+        const uint32_t cap = ovf_cap < (uint32_t)kChunkCap ? ovf_cap : (uint32_t)kChunkCap;
         if (cnt32 == 0) {
             if (eo.keys) write_entries<BLOCK>(eo, ej, kb, base, [](uint64_t) { return 0u; });
         } else if (eq_done && single) {
@@ -1163,9 +1173,9 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
                 }
                 block_checksum_add<C::W>(csum, s_sum, checksum_out);
             }
-        } else if (cnt32 > (uint32_t)C::CAP && cnt32 <= (uint32_t)kChunkCap && cnt32 <= ovf_cap) {
+        } else if (cnt32 > (uint32_t)C::CAP && cnt32 <= cap) {
             if (tid == 0) medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread form
-        } else if (cnt32 > (uint32_t)kChunkCap || cnt32 > ovf_cap) {
+        } else if (cnt32 > cap) {
             if (tid == 0) {
                 const uint32_t slot = atomicAdd(noverflow, 1u);
                 if ((int)slot < max_overflow) {
@@ -1174,6 +1184,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
                 }
             }
         } else {
+            __builtin_assume(cnt32 <= (uint32_t)C::CAP && cnt32 > 0u);
             const RunSrc<RW> rs{src, base};
             // an interior bucket's keys lie in (t[k-1], t[k]]
             const bool bounded = tables && kb > 0 && kb < kLBins - 1 && cur.tlo < cur.thi;
