@@ -113,9 +113,12 @@ int mpsort_gpu_newarray_noalloc(const void *base, const int64_t *nin, int P,
                                 const mpsort_key_desc *key, unsigned flags);
 
 /* The same with the reference's opaque radix callback (mpsort.h:1-4): the
- * radix of every record is materialised once on the host (rsize <= 28 bytes,
- * compared as a little-endian unsigned integer, radixsort.c:146-195), the
- * {radix, index} pairs are sorted on the GPU and the records gathered.  Host
+ * radix of every record is materialised once on the host (compared as a
+ * little-endian unsigned integer, radixsort.c:146-195), the {radix, index}
+ * pairs are sorted on the GPU and the records gathered.  Any rsize: up to 28
+ * bytes (24 past 2^32 records) the pair is one sort record; a longer radix
+ * is sorted by stable passes over its 24-byte chunks, least significant
+ * first, and the counts recorded are those of the top chunk's pass.  Host
  * pointers only. */
 int mpsort_gpu_newarray_radix(const void *base, const int64_t *nin, int P,
                               void *out, const int64_t *nout, size_t elsize,

@@ -1907,6 +1907,105 @@ int mpsort_gpu_newarray_noalloc(const void* base, const int64_t* nin, int P, voi
     return rc;
 }
 
+// A radix longer than the packed sort record takes (rsize above 28 bytes,
+// 24 with a 64-bit index): every radix is materialised once by host
+// threads, then stable passes over its 24-byte chunks, least significant
+// first (an LSD sort whose digits are whole chunks: radixsort.c:146-195
+// compares the radix as one little-endian integer), each sorting 32-byte
+// records {chunk, original index} in place through mpsort_gpu_newarray --
+// one rank for the lower chunks, the caller's ranks for the top one, whose
+// counts and splitters are the ones recorded.  The caller's records are
+// gathered by index at the end; both checksums are the host's.
+static int sort_long_radix(const void* base, const int64_t* nin, int P, void* out, const int64_t* nout,
+                           size_t elsize, void (*radix)(const void* ptr, void* radix, void* arg), size_t rsize,
+                           void* arg, unsigned flags, int64_t ntot) {
+    constexpr size_t kChunk = 24;
+    struct Rec {
+        uint64_t k[3];  // the chunk, little-endian words (the last most significant)
+        uint64_t idx;   // the record's input position
+    };
+    static_assert(sizeof(Rec) == 32, "chunk record");
+    const int nch = (int)((rsize + kChunk - 1) / kChunk);
+    if (ntot == 0) {
+        memset(&g_last, 0, sizeof g_last);
+        return 0;
+    }
+    unsigned char* rad = (unsigned char*)mpiu_malloc("mpsort_gpu:radix", (size_t)ntot * rsize, __FILE__, __LINE__);
+    Rec* recs = (Rec*)mpiu_malloc("mpsort_gpu:chunks", (size_t)ntot * sizeof(Rec), __FILE__, __LINE__);
+    unsigned char* src = nullptr;  // the input, copied when the output overlaps it
+    const size_t bytes = (size_t)ntot * elsize;
+    const bool overlap = (const unsigned char*)out < (const unsigned char*)base + bytes &&
+                         (const unsigned char*)base < (const unsigned char*)out + bytes;
+    if (overlap) src = (unsigned char*)mpiu_malloc("mpsort_gpu:input", bytes, __FILE__, __LINE__);
+    auto release = [&]() {
+        if (rad) mpiu_free(rad, __FILE__, __LINE__);
+        if (recs) mpiu_free(recs, __FILE__, __LINE__);
+        if (src) mpiu_free(src, __FILE__, __LINE__);
+    };
+    if (!rad || !recs || (overlap && !src)) {
+        release();
+        return fail(MPSORT_ERR_NOMEM, "host allocation for a %zu-byte radix failed", rsize);
+    }
+    const int T = host_threads(ntot);
+    std::vector<long long> part(T, 0);
+    parallel_ranges(ntot, T, [&](int t, int64_t lo, int64_t hi) {
+        long long sum = 0;
+        for (int64_t i = lo; i < hi; i++) {
+            const unsigned char* rec = (const unsigned char*)base + (size_t)i * elsize;
+            memset(rad + (size_t)i * rsize, 0, rsize);
+            radix(rec, rad + (size_t)i * rsize, arg);
+            recs[i].idx = (uint64_t)i;
+            for (size_t b = 0; b < elsize; b++) sum += (signed char)rec[b];
+        }
+        part[t] = sum;
+    });
+    long long sum_in = 0;
+    for (long long v : part) sum_in += v;
+    if (src) memcpy(src, base, bytes);
+    const unsigned char* from = src ? src : (const unsigned char*)base;
+    mpsort_key_desc key;
+    memset(&key, 0, sizeof key);
+    key.offset = 0;
+    key.width = 8;
+    key.nwords = 3;
+    const unsigned pass_flags = flags & (MPSORT_GPU_NO_CHECKSUM | MPSORT_GPU_TIMING);
+    int rc = 0;
+    for (int ch = 0; ch < nch && !rc; ch++) {
+        const size_t off = (size_t)ch * kChunk, len = std::min(kChunk, rsize - off);
+        parallel_ranges(ntot, T, [&](int, int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; i++) {
+                Rec& r = recs[i];
+                r.k[0] = r.k[1] = r.k[2] = 0;
+                memcpy(r.k, rad + (size_t)r.idx * rsize + off, len);
+            }
+        });
+        const bool top = ch == nch - 1;
+        const int64_t one = ntot;
+        rc = mpsort_gpu_newarray(recs, top ? nin : &one, top ? P : 1, recs, top ? nout : &one, sizeof(Rec), &key,
+                                 pass_flags);
+    }
+    if (!rc) {
+        parallel_ranges(ntot, T, [&](int t, int64_t lo, int64_t hi) {
+            long long sum = 0;
+            for (int64_t i = lo; i < hi; i++) {
+                unsigned char* dst = (unsigned char*)out + (size_t)i * elsize;
+                memcpy(dst, from + (size_t)recs[i].idx * elsize, elsize);
+                for (size_t b = 0; b < elsize; b++) sum += (signed char)dst[b];
+            }
+            part[t] = sum;
+        });
+        long long sum_out = 0;
+        for (long long v : part) sum_out += v;
+        g_last.checksum_in = sum_in;
+        g_last.checksum_out = sum_out;
+        if (!(flags & MPSORT_GPU_NO_CHECKSUM) && sum_in != sum_out)
+            rc = fail(MPSORT_ERR_CHECKSUM, "MPSort: Data changed after sorting; checksum mismatch (%lld != %lld)",
+                      sum_in, sum_out);
+    }
+    release();
+    return rc;
+}
+
 int mpsort_gpu_newarray_radix(const void* base, const int64_t* nin, int P, void* out, const int64_t* nout,
                               size_t elsize, void (*radix)(const void* ptr, void* radix, void* arg), size_t rsize,
                               void* arg, unsigned flags) {
@@ -1918,11 +2017,10 @@ int mpsort_gpu_newarray_radix(const void* base, const int64_t* nin, int P, void*
     if (elsize == 0) return fail(MPSORT_ERR_ARG, "elsize is 0");
     if (flags & MPSORT_GPU_DEVICE_PTRS) return fail(MPSORT_ERR_ARG, "the radix-callback form takes host pointers");
     const int iwords = ntot >= ((int64_t)1 << 32) ? 2 : 1;
-    const int rw = rsize > 0 && rsize <= 32 ? packed_words((int64_t)rsize, iwords) : 0;
-    if (!rw)
-        return fail(MPSORT_ERR_UNSUPPORTED, "rsize %zu and a %d-byte index exceed the 32-byte packed record", rsize,
-                    4 * iwords);
+    if (rsize == 0) return fail(MPSORT_ERR_UNSUPPORTED, "rsize is 0");
+    const int rw = rsize <= 32 ? packed_words((int64_t)rsize, iwords) : 0;
     if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");
+    if (!rw) return sort_long_radix(base, nin, P, out, nout, elsize, radix, rsize, arg, flags, ntot);
     DevCtx* c = nullptr;
     if ((rc = get_ctx(&c))) return rc;
     mpsort_gpu_stats st;

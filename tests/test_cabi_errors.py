@@ -83,7 +83,6 @@ RADIX = _lib.RADIX_FN(lambda p, r, a: None)
 @pytest.mark.parametrize("rsize, flags, fn, code", [
     (8, 0, None, _lib.MPSORT_ERR_ARG),
     (0, 0, RADIX, _lib.MPSORT_ERR_UNSUPPORTED),
-    (29, 0, RADIX, _lib.MPSORT_ERR_UNSUPPORTED),
     (8, _lib.DEVICE_PTRS, RADIX, _lib.MPSORT_ERR_ARG),
 ])
 def test_radix_callback_argument_errors(rsize, flags, fn, code):
@@ -92,6 +91,17 @@ def test_radix_callback_argument_errors(rsize, flags, fn, code):
     rc = lib.mpsort_gpu_newarray_radix(buf.ctypes.data, counts(4), 1, buf.ctypes.data, counts(4), 16, f, rsize,
                                        None, flags)
     assert rc == code, err()
+
+
+def test_long_radix_is_accepted():
+    """a radix longer than the packed record (radixsort.c takes any rsize):
+    sorted in 24-byte chunks, so not rejected (no GPU here: NODEVICE)"""
+    buf = numpy.zeros(64, numpy.uint8)
+    rc = lib.mpsort_gpu_newarray_radix(buf.ctypes.data, counts(4), 1, buf.ctypes.data, counts(4), 16, RADIX, 40,
+                                       None, 0)
+    assert rc != _lib.MPSORT_ERR_UNSUPPORTED
+    if _lib.device_count() == 0:
+        assert rc == _lib.MPSORT_ERR_NODEVICE
 
 
 def test_learned_entry_rejects_a_null_model():

@@ -341,3 +341,24 @@ def test_radix_sort_and_mpsort_omp_exports(entry):
     note("%s n=%d" % (entry, n))
     f(a.ctypes.data, n, a.dtype.itemsize, fn, 8, ctypes.cast(arg, ctypes.c_void_p))
     assert numpy.array_equal(a, want)
+
+
+@pytest.mark.parametrize("nwords", [5, 8])
+def test_radix_callback_long_radix(nwords):
+    """radixes of 40 and 64 bytes (5 and 8 u64 words, the last most
+    significant), longer than one packed sort record: sorted in 24-byte
+    chunks, equal to numpy's lexsort (ties in input order) on uneven
+    partitions"""
+    g = numpy.random.default_rng(65 + nwords)
+    n = 150000
+    a = numpy.zeros(n, dtype=[('k', ('u8', nwords)), ('v', 'u8')])
+    for w in range(nwords):
+        a['k'][:, w] = g.integers(0, 3 if w >= nwords - 3 else 2 ** 64, n, dtype=numpy.uint64)
+    a['v'] = numpy.arange(n)
+    nin = [n // 4, 0, n - n // 4]
+    nout = [n // 3, n - n // 3 - 11, 11]
+    note("callback %d-byte radix n=%d" % (8 * nwords, n))
+    out, st = _callback_sort(a, nin, nout, 0, nwords)
+    order = numpy.lexsort([a['v']] + [a['k'][:, w] for w in range(nwords)])
+    assert numpy.array_equal(out, a[order])
+    assert st.checksum_in == st.checksum_out
