@@ -165,7 +165,8 @@ def bench_batched(args):
         plan.add(g["src"], g["dst"], g["sizes"], g["dtype"], 'key')
     t0 = time.perf_counter()
     plan.run()  # tunes, captures
-    log("plan tuned and captured in %.2f s: %s" % (time.perf_counter() - t0, [plan.info(i) for i in range(2)]))
+    plan_info = [plan.info(i) for i in range(2)]
+    log("plan tuned and captured in %.2f s: %s" % (time.perf_counter() - t0, plan_info))
     for i in range(args.warmup):
         plan.run()
     _lib.check(lib.mpsort_gpu_synchronize())
@@ -230,6 +231,7 @@ def bench_batched(args):
         "per_segment_loop": {"value": round(loop_value, 2), "s": round(loop_s, 3)},
         "speedup_vs_loop": round(value / loop_value, 2),
         "sort_equivalent_GBps": round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1),
+        "tuned_plan": {"i32_f32": plan_info[0], "i64_i64": plan_info[1]},
     }
     print(json.dumps(result), flush=True)
     return 0

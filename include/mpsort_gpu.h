@@ -261,12 +261,19 @@ int mpsort_gpu_newarray_learned(const void *base, const int64_t *nin, int P, voi
  * MPSORT_GPU_SEGPLAN_GRAPH captures the whole plan once in a hipGraph and
  * replays it on every run (the segment sizes and pointers are fixed by the
  * plan); MPSORT_GPU_SEGPLAN_TUNE times tile shapes and the in-LDS size
- * threshold on the first run and caches the choice per (record size, key
+ * threshold, then the workgroup shape of the in-LDS segments (the waves that
+ * share a CU), on the first run and caches the choice per (record size, key
  * width, size class) in memory and in $MPSORT_GPU_TUNE_CACHE (default
  * ~/.cache/mpsort_gpu_segsort.tune). */
 typedef struct mpsort_gpu_segplan_s mpsort_gpu_segplan;
 #define MPSORT_GPU_SEGPLAN_GRAPH (1u << 0)
 #define MPSORT_GPU_SEGPLAN_TUNE (1u << 1)
+/* the workgroup shape of the segments sorted in LDS when not tuned (default:
+ * 1024 threads x 9 records, one workgroup per CU): 512 x 9 (two per CU,
+ * segments up to 4608 records) or 512 x 6 (three per CU, up to 3072); larger
+ * ones below the threshold take the 1024-thread form */
+#define MPSORT_GPU_SEGPLAN_SMALL_512x9 (1u << 2)
+#define MPSORT_GPU_SEGPLAN_SMALL_512x6 (1u << 3)
 int mpsort_gpu_segplan_create(mpsort_gpu_segplan **plan, unsigned flags);
 int mpsort_gpu_segplan_add(mpsort_gpu_segplan *plan, const void *d_in, void *d_out, const int64_t *seg_count,
                            int nseg, size_t elsize, const mpsort_key_desc *key);
@@ -275,6 +282,9 @@ int mpsort_gpu_segplan_run(mpsort_gpu_segplan *plan, mpsort_gpu_stats *stats);
 /* the tuning a group runs with: segments sorted in LDS / by onesweep, tile */
 int mpsort_gpu_segplan_info(const mpsort_gpu_segplan *plan, int group, int *nsmall, int *nlarge, int *block,
                             int *items, int *threshold);
+/* the workgroup shape of a group's in-LDS segments (0: 1024 x 9, 1: 512 x 9,
+ * 2: 512 x 6) and the waves that then share a CU */
+int mpsort_gpu_segplan_shape(const mpsort_gpu_segplan *plan, int group, int *form, int *waves_per_cu);
 int mpsort_gpu_segplan_free(mpsort_gpu_segplan *plan);
 /* one group, no graph: create + add + run + free */
 int mpsort_gpu_sort_segments(const void *d_in, void *d_out, const int64_t *seg_count, int nseg, size_t elsize,

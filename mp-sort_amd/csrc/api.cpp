@@ -1469,10 +1469,17 @@ using namespace mpsg;
 // the segment as the bucket); the larger ones by the segmented onesweep over
 // just those segments.  Every launch of a group depends only on device data,
 // so the whole batch can be captured once in a hipGraph and replayed.
+// tile: the onesweep tile of the large segments; threshold: the largest
+// segment sorted in LDS; form: the workgroup shape of those, which sets how
+// many share a CU (0: 1024 x 9, one per CU, 16 waves; 1: 512 x 9, two per
+// CU, 16 waves; 2: 512 x 6, three per CU, 24 waves), the segments above its
+// capacity going to the 1024-thread form
 struct SegTune {
     TileCfg tile;
     int threshold;
+    int form;
 };
+static int form_waves_per_cu(int form) { return form == 2 ? 24 : 16; }
 
 struct SegGroup {
     const void* in = nullptr;
@@ -1484,7 +1491,7 @@ struct SegGroup {
     int nseg = 0;
     int64_t nrec = 0;
     std::vector<int64_t> count, begin;
-    SegTune tune{{1024, 8}, kChunkCap};
+    SegTune tune{{1024, 8}, kChunkCap, 0};
     bool tuned = false;
     // device side
     Buf small;     // small-segment counts, bases, hard list, counters; epoch
@@ -1492,10 +1499,11 @@ struct SegGroup {
     Buf tmp;       // record buffer for the digit passes
     Buf lookback;
     int nsmall = 0, nlarge = 0;
+    int nsmall_form = 0;  // of the small segments, those sorted by the form chosen (the others: 1024 threads)
     uint32_t* d_cnt = nullptr;
     int64_t* d_base = nullptr;
     uint32_t* d_hard = nullptr;
-    uint32_t* d_counters = nullptr;  // [0] oversized, [1] hard, [2] epoch
+    uint32_t* d_counters = nullptr;  // [0] oversized, [1] hard, [2] epoch, [4] hard of the 1024-thread list
     int64_t* d_ovf = nullptr;
     SegSortPlan plan;
     uint64_t epoch_used = 0;  // epoch steps consumed since the last reset
@@ -1535,13 +1543,15 @@ static void seg_tune_load() {
     if (!fp) return;
     char line[256];
     while (fgets(line, sizeof line, fp)) {
-        int rw, kb, cls, blk, items, thr;
-        if (sscanf(line, "rw=%d keybytes=%d class=%d block=%d items=%d threshold=%d", &rw, &kb, &cls, &blk, &items,
-                   &thr) == 6) {
+        int rw, kb, cls, blk, items, thr, form = 0;
+        const int got = sscanf(line, "rw=%d keybytes=%d class=%d block=%d items=%d threshold=%d form=%d", &rw, &kb,
+                               &cls, &blk, &items, &thr, &form);
+        if (got >= 6) {
             char k[96];
             snprintf(k, sizeof k, "rw=%d keybytes=%d class=%d", rw, kb, cls);
-            SegTune t{{blk, items}, thr};
-            if (tile_supported(rw, t.tile) && thr >= 0 && thr <= kChunkCap) seg_tune_cache()[k] = t;
+            SegTune t{{blk, items}, thr, form};
+            if (tile_supported(rw, t.tile) && thr >= 0 && thr <= kChunkCap && form >= 0 && form <= 2)
+                seg_tune_cache()[k] = t;
         }
     }
     fclose(fp);
@@ -1553,8 +1563,8 @@ static void seg_tune_store() {
     FILE* fp = fopen(f.c_str(), "w");
     if (!fp) return;
     for (auto& kv : seg_tune_cache())
-        fprintf(fp, "%s block=%d items=%d threshold=%d\n", kv.first.c_str(), kv.second.tile.block,
-                kv.second.tile.items, kv.second.threshold);
+        fprintf(fp, "%s block=%d items=%d threshold=%d form=%d\n", kv.first.c_str(), kv.second.tile.block,
+                kv.second.tile.items, kv.second.threshold, kv.second.form);
     fclose(fp);
 }
 
@@ -1564,38 +1574,46 @@ static int seg_group_prepare(SegGroup& G, hipStream_t st) {
     const int npass = G.ks.key_bytes;
     const bool lds_ok = (G.rw == 2 || G.rw == 4) && G.ks.key_bytes <= 8;
     const int thr = lds_ok ? G.tune.threshold : 0;
-    std::vector<uint32_t> scnt;
-    std::vector<int64_t> sbase, lcnt, lbeg;
+    const int fcap = segments_lds_cap(G.tune.form);
+    std::vector<uint32_t> scnt, mcnt;
+    std::vector<int64_t> sbase, mbase, lcnt, lbeg;
     for (int i = 0; i < G.nseg; i++) {
         if (G.count[i] == 0) continue;
-        if (G.count[i] <= thr) {
+        if (G.count[i] <= thr && G.count[i] <= fcap) {
             scnt.push_back((uint32_t)G.count[i]);
             sbase.push_back(G.begin[i]);
+        } else if (G.count[i] <= thr) {
+            mcnt.push_back((uint32_t)G.count[i]);
+            mbase.push_back(G.begin[i]);
         } else {
             lcnt.push_back(G.count[i]);
             lbeg.push_back(G.begin[i]);
         }
     }
+    // the form's segments, then those above its capacity
+    G.nsmall_form = (int)scnt.size();
+    scnt.insert(scnt.end(), mcnt.begin(), mcnt.end());
+    sbase.insert(sbase.end(), mbase.begin(), mbase.end());
     G.nsmall = (int)scnt.size();
     G.nlarge = (int)lcnt.size();
     Sizer sz;
     sz.take<uint32_t>((size_t)G.nsmall + 1);
     sz.take<int64_t>((size_t)G.nsmall + 1);
     sz.take<uint32_t>((size_t)G.nsmall + 1);
-    sz.take<uint32_t>(4);
+    sz.take<uint32_t>(8);
     sz.take<int64_t>(2);
     if ((rc = G.small.ensure(sz.off + 4096))) return rc;
     Arena a{(char*)G.small.p};
     G.d_cnt = a.take<uint32_t>((size_t)G.nsmall + 1);
     G.d_base = a.take<int64_t>((size_t)G.nsmall + 1);
     G.d_hard = a.take<uint32_t>((size_t)G.nsmall + 1);
-    G.d_counters = a.take<uint32_t>(4);
+    G.d_counters = a.take<uint32_t>(8);
     G.d_ovf = a.take<int64_t>(2);
     if (G.nsmall) {
         MPSG_CHECK(hipMemcpyAsync(G.d_cnt, scnt.data(), sizeof(uint32_t) * G.nsmall, hipMemcpyHostToDevice, st));
         MPSG_CHECK(hipMemcpyAsync(G.d_base, sbase.data(), sizeof(int64_t) * G.nsmall, hipMemcpyHostToDevice, st));
     }
-    if ((rc = launch_fill_u32(G.d_counters, 0u, 4, st))) return rc;
+    if ((rc = launch_fill_u32(G.d_counters, 0u, 8, st))) return rc;
     G.epoch_used = 0;
 
     SegSortPlan& p = G.plan;
@@ -1653,9 +1671,11 @@ static int seg_group_prepare(SegGroup& G, hipStream_t st) {
 static int seg_group_enqueue(SegGroup& G, hipStream_t st) {
     int rc = 0;
     if ((rc = launch_fill_u32(G.d_counters, 0u, 2, st))) return rc;
+    if ((rc = launch_fill_u32(G.d_counters + 4, 0u, 1, st))) return rc;
     if (G.nsmall) {
-        if ((rc = launch_chunk_sort(G.rw, G.in, G.out, G.lk, G.d_cnt, G.d_base, nullptr, 0, G.nsmall, G.d_ovf,
-                                    G.d_counters, 1, G.d_hard, G.d_counters + 1, nullptr, false, st)))
+        if ((rc = launch_segments_lds(G.rw, G.tune.form, G.in, G.out, G.lk, G.d_cnt, G.d_base, G.nsmall_form,
+                                      G.nsmall - G.nsmall_form, G.d_ovf, G.d_counters, G.d_hard, G.d_counters + 1,
+                                      G.d_counters + 4, st)))
             return rc;
     }
     if (G.nlarge) {
@@ -1719,8 +1739,31 @@ static int seg_group_tune(SegGroup& G, DevCtx& c) {
     for (const TileCfg& t : tiles) {
         if (!tile_supported(G.rw, t)) continue;
         for (int thr : thresholds) {
-            G.tune = SegTune{t, thr};
+            G.tune = SegTune{t, thr, 0};
             if ((rc = seg_group_prepare(G, c.st))) return rc;
+            float best_here = 1e30f;
+            for (int rep = 0; rep < 2; rep++) {
+                if ((rc = seg_group_epoch_guard(G, c.st))) return rc;
+                MPSG_CHECK(hipEventRecord(c.ev[kEvStart], c.st));
+                if ((rc = seg_group_enqueue(G, c.st))) return rc;
+                MPSG_CHECK(hipEventRecord(c.ev[kEvEnd], c.st));
+                if ((rc = seg_group_check(G, c.st))) return rc;
+                best_here = std::min(best_here, (float)ev_ms(c.ev[kEvStart], c.ev[kEvEnd]));
+            }
+            if (best_here < best) {
+                best = best_here;
+                pick = G.tune;
+            }
+        }
+    }
+    // then the workgroup shape of the segments sorted in LDS (how many
+    // workgroups and waves share a CU), with that tile and threshold
+    const bool lds_ok = (G.rw == 2 || G.rw == 4) && G.ks.key_bytes <= 8;
+    if (lds_ok && pick.threshold > 0) {
+        for (int form = 1; form <= 2; form++) {
+            G.tune = SegTune{pick.tile, pick.threshold, form};
+            if ((rc = seg_group_prepare(G, c.st))) return rc;
+            if (!G.nsmall) break;
             float best_here = 1e30f;
             for (int rep = 0; rep < 2; rep++) {
                 if ((rc = seg_group_epoch_guard(G, c.st))) return rc;
@@ -2716,6 +2759,7 @@ int mpsort_gpu_segplan_add(mpsort_gpu_segplan* p, const void* d_in, void* d_out,
     fill_keyspec(&G->ks, G->rw, (int)key->offset, (int)kb, key->width, key->is_signed);
     G->lk = make_lkey(G->ks);
     G->tune.tile = default_tile(G->rw);
+    G->tune.form = (p->flags & MPSORT_GPU_SEGPLAN_SMALL_512x6) ? 2 : ((p->flags & MPSORT_GPU_SEGPLAN_SMALL_512x9) ? 1 : 0);
     G->nseg = nseg;
     G->count.assign(seg_count, seg_count + nseg);
     G->begin.resize(nseg);
@@ -2792,6 +2836,14 @@ int mpsort_gpu_segplan_info(const mpsort_gpu_segplan* p, int group, int* nsmall,
     if (block) *block = G->tune.tile.block;
     if (items) *items = G->tune.tile.items;
     if (threshold) *threshold = G->tune.threshold;
+    return 0;
+}
+
+int mpsort_gpu_segplan_shape(const mpsort_gpu_segplan* p, int group, int* form, int* waves_per_cu) {
+    if (!p || group < 0 || group >= (int)p->groups.size()) return fail(MPSORT_ERR_ARG, "no such group");
+    const SegGroup* G = p->groups[group];
+    if (form) *form = G->tune.form;
+    if (waves_per_cu) *waves_per_cu = form_waves_per_cu(G->tune.form);
     return 0;
 }
 

@@ -149,6 +149,17 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
                             int64_t* overflow, uint32_t* noverflow, int max_overflow, uint32_t* hard, uint32_t* nhard,
                             uint32_t* medium, uint32_t* nmedium, long long* checksum, bool eq_done, hipStream_t st,
                             const EntryOut* eo = nullptr, uint32_t ovf_cap = (uint32_t)kChunkCap);
+// independent segments (the batched sort) in LDS, one workgroup each:
+// segments [0, na) of the lists cnt / base by the form chosen -- 0: 1024
+// threads x 9 records (one workgroup per CU), 1: 512 x 9 (two per CU), 2:
+// 512 x 6 (three per CU) -- and [na, na + nb), those above the form's
+// capacity (segments_lds_cap), by the 1024-thread form; crowded ones by the
+// LDS radix sort (hard: na + nb entries; counts *nhard_a, *nhard_b), which
+// also takes the segments the 512 x 6 form hands on (one-value crowded bins)
+int launch_segments_lds(int rw, int form, const void* src, void* dst, const LKey& lk, const uint32_t* cnt,
+                        const int64_t* base, int64_t na, int64_t nb, int64_t* overflow, uint32_t* noverflow,
+                        uint32_t* hard, uint32_t* nhard_a, uint32_t* nhard_b, hipStream_t st);
+int segments_lds_cap(int form);
 // LDS radix sort of the listed buckets (src may equal dst)
 int launch_lsd_sort(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,
                     const int64_t* bucket_base, const uint32_t* hard, const uint32_t* nhard, int64_t max_buckets,

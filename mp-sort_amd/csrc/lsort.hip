@@ -1118,7 +1118,7 @@ __device__ __forceinline__ int sort_bucket_lds(const Src& src, uint32_t* __restr
 // BLOCK = 512: buckets above BSort<512>::CAP (up to kChunkCap) go to the
 // medium list for the 1024-thread form (medium must then be given).
 template <int RW, int BLOCK, int ITEMS = kCItems>
-__global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
+__global__ __launch_bounds__(BLOCK, ITEMS <= 6 ? 6 : 2048 / BLOCK) void interp_sort_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, LKey lk,
     const uint32_t* __restrict__ hist, const int64_t* __restrict__ bucket_base,
     const uint64_t* __restrict__ tables, int64_t tstride, int64_t nbuckets, int64_t* __restrict__ overflow,
@@ -1174,7 +1174,11 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
                 block_checksum_add<C::W>(csum, s_sum, checksum_out);
             }
         } else if (cnt32 > (uint32_t)C::CAP && cnt32 <= cap) {
-            if (tid == 0) medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread form
+            // the 1024-thread form, or without a medium list the LDS radix sort
+            if (tid == 0) {
+                if (medium) medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;
+                else hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
+            }
         } else if (cnt32 > cap) {
             if (tid == 0) {
                 const uint32_t slot = atomicAdd(noverflow, 1u);
@@ -1192,7 +1196,10 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void interp_sort_kernel(
                 rs, dst, base, (int)cnt32, lk, s_bins, checksum_out, bounded, bounded ? cur.tlo + 1 : 0,
                 bounded ? cur.thi : ~0ull, eo, ej, kb);
             if (rc == 1 && tid == 0) hard[atomicAdd(nhard, 1u)] = (uint32_t)b;
-            if (rc == 2 && tid == 0) medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread form
+            if (rc == 2 && tid == 0) {
+                if (medium) medium[atomicAdd(nmedium, 1u)] = (uint32_t)b;  // the 1024-thread form
+                else hard[atomicAdd(nhard, 1u)] = (uint32_t)b;             // no medium list
+            }
         }
     }
 }
@@ -1816,6 +1823,43 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
     }
     MPSG_CHECK(hipGetLastError());
     return 0;
+}
+
+int launch_segments_lds(int rw, int form, const void* src, void* dst, const LKey& lk, const uint32_t* cnt,
+                        const int64_t* base, int64_t na, int64_t nb, int64_t* overflow, uint32_t* noverflow,
+                        uint32_t* hard, uint32_t* nhard_a, uint32_t* nhard_b, hipStream_t st) {
+    if (form == 0 || na == 0)
+        return launch_chunk_sort(rw, src, dst, lk, cnt, base, nullptr, 0, na + nb, overflow, noverflow, 1, hard,
+                                 nhard_a, nullptr, false, st);
+    const uint32_t* s = (const uint32_t*)src;
+    uint32_t* d = (uint32_t*)dst;
+    const unsigned grid = (unsigned)na, persistent = (unsigned)std::min<int64_t>(na, 512);
+    const uint32_t cap = (uint32_t)kChunkCap;
+#define MPSG_SEGFORM(R, I) do { \
+        hipLaunchKernelGGL((interp_sort_kernel<R, kMergeBlock, I>), dim3(grid), dim3(kMergeBlock), 0, st, s, d, lk, \
+                           cnt, base, nullptr, 0, na, overflow, noverflow, 1, hard, nhard_a, nullptr, nullptr, nullptr, \
+                           0, EntryOut{}, cap); \
+        MPSG_CHECK(hipGetLastError()); \
+        hipLaunchKernelGGL(lsd_sort_kernel<R>, dim3(persistent), dim3(kCBlock), 0, st, s, d, lk, cnt, base, \
+                           (const uint32_t*)hard, (const uint32_t*)nhard_a, nullptr); \
+    } while (0)
+    switch (rw * 4 + form) {
+        case 9: MPSG_SEGFORM(2, kSmallItems); break;
+        case 10: MPSG_SEGFORM(2, kMergeItems); break;
+        case 17: MPSG_SEGFORM(4, kSmallItems); break;
+        case 18: MPSG_SEGFORM(4, kMergeItems); break;
+        default: return -1;
+    }
+#undef MPSG_SEGFORM
+    MPSG_CHECK(hipGetLastError());
+    if (nb == 0) return 0;
+    // the segments above the form's capacity: the 1024-thread form
+    return launch_chunk_sort(rw, src, dst, lk, cnt + na, base + na, nullptr, 0, nb, overflow, noverflow, 1, hard + na,
+                             nhard_b, nullptr, false, st);
+}
+
+int segments_lds_cap(int form) {
+    return form == 1 ? BSort<kMergeBlock, kSmallItems>::CAP : (form == 2 ? BSort<kMergeBlock, kMergeItems>::CAP : kChunkCap);
 }
 
 int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, const uint32_t* hist,

@@ -57,10 +57,13 @@ EXPORTS = [
     "mpsort_gpu_segplan_run", "mpsort_gpu_segplan_info", "mpsort_gpu_segplan_free",
     "mpsort_gpu_sort_segments",
     "mpiu_malloc", "mpiu_free", "mpiu_set_malloc", "mpsort_gpu_newarray_noalloc", "mpsort_gpu_set_faults",
+    "mpsort_gpu_segplan_shape",
 ]
 
 SEGPLAN_GRAPH = 1 << 0
 SEGPLAN_TUNE = 1 << 1
+SEGPLAN_SMALL_512x9 = 1 << 2
+SEGPLAN_SMALL_512x6 = 1 << 3
 
 MODEL_LINEAR_ROOT = 1 << 8
 MLP_PARAMS = 1153

@@ -16,10 +16,14 @@ class SegmentPlan(object):
     Each group is a device buffer of records of one dtype holding its
     segments back to back.  ``graph=True`` captures the whole plan in a
     hipGraph on the first run and replays it afterwards; ``tune=True`` picks
-    the tile shape and in-LDS threshold by timing them on the first run."""
+    the tile shape, the in-LDS threshold and the workgroup shape of the
+    in-LDS segments by timing them on the first run.  ``small_form`` sets that
+    shape without tuning: 0 (1024 threads x 9 records, one workgroup per CU),
+    1 (512 x 9, two per CU) or 2 (512 x 6, three per CU)."""
 
-    def __init__(self, graph=False, tune=False):
+    def __init__(self, graph=False, tune=False, small_form=0):
         flags = (_lib.SEGPLAN_GRAPH if graph else 0) | (_lib.SEGPLAN_TUNE if tune else 0)
+        flags |= {0: 0, 1: _lib.SEGPLAN_SMALL_512x9, 2: _lib.SEGPLAN_SMALL_512x6}[small_form]
         h = ctypes.c_void_p()
         check(lib.mpsort_gpu_segplan_create(ctypes.byref(h), flags))
         self._h = h
@@ -45,7 +49,11 @@ class SegmentPlan(object):
     def info(self, group=0):
         v = [ctypes.c_int() for _ in range(5)]
         check(lib.mpsort_gpu_segplan_info(self._h, group, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("small_segments", "large_segments", "block", "items", "threshold"), [x.value for x in v]))
+        d = dict(zip(("small_segments", "large_segments", "block", "items", "threshold"), [x.value for x in v]))
+        f, w = ctypes.c_int(), ctypes.c_int()
+        check(lib.mpsort_gpu_segplan_shape(self._h, group, ctypes.byref(f), ctypes.byref(w)))
+        d["small_form"], d["small_waves_per_cu"] = f.value, w.value
+        return d
 
     def close(self):
         if self._h:

@@ -144,3 +144,42 @@ def test_32_byte_records_default_and_tuned():
     for tune in (False, True):
         out = segments.sort_segments(a, sizes, orderby='key', tune=tune)
         assert numpy.array_equal(out, want), tune
+
+
+def _plan_sort(a, sizes, form, graph=False):
+    d = lib.mpsort_gpu_dmalloc(max(a.nbytes, 1))
+    try:
+        _lib.check(lib.mpsort_gpu_memcpy_h2d(d, a.ctypes.data, a.nbytes))
+        plan = segments.SegmentPlan(graph=graph, small_form=form)
+        plan.add(d, d, sizes, a.dtype, 'key')
+        plan.run()
+        info = plan.info()
+        g = numpy.empty_like(a)
+        _lib.check(lib.mpsort_gpu_memcpy_d2h(g.ctypes.data, d, a.nbytes))
+        plan.close()
+    finally:
+        lib.mpsort_gpu_dfree(d)
+    return g, info
+
+
+@pytest.mark.parametrize("key_bits", [None, 6], ids=["spread", "dup6"])
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("dtype", [workloads.REC8, workloads.REC16I], ids=["8B", "16B"])
+def test_segments_small_forms(dtype, form, key_bits):
+    """the in-LDS segments by 512 x 9 (two workgroups per CU) or 512 x 6
+    (three per CU) workgroups, those above the form's capacity by the
+    1024-thread form; duplicate-heavy keys take the LDS radix sort in the
+    512 x 6 form"""
+    sizes = sizes_mixed(200, 13 + form, 1 << 20)
+    a = make(dtype, sizes, 41 + form, key_bits=key_bits)
+    note("segments form %d keybits %s n=%d" % (form, key_bits, int(sizes.sum())))
+    g, info = _plan_sort(a, sizes, form)
+    assert info["small_form"] == form and info["small_waves_per_cu"] == (24 if form == 2 else 16)
+    assert numpy.array_equal(g, expected(a, sizes))
+
+
+def test_segments_small_form_graph_replays():
+    sizes = sizes_mixed(150, 23, 1 << 20)
+    a = make(workloads.REC8, sizes, 51)
+    g, _ = _plan_sort(a, sizes, 2, graph=True)
+    assert numpy.array_equal(g, expected(a, sizes))
