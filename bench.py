@@ -582,7 +582,7 @@ def bench_histsort(args):
                             "source": ROCPROF_SOURCE}
         return r
 
-    plain_roof = roof([x for st in st_plain for x in st["pass_ms"]],
+    plain_roof = roof([x for st in st_plain for x in st["pass_ms"][:st["first_passes"] + st["second_passes"]]],
                       "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP,
                       ROCPROF_MS.get("onesweep"))
     if result_learned is not None:
@@ -605,6 +605,19 @@ def bench_histsort(args):
                               last["t_exchange_ms"], last["t_second_sort_ms"], last["t_total_ms"],
                               last["first_passes"], last["second_passes"], last["bisect_rounds"]))
     log("plain pass ms: " + " ".join("%.3f" % x for x in last["pass_ms"]))
+    plain_second = None
+    if last["second_merged"]:
+        # the SecondSort without an exchange: its stages after the digit passes
+        k0 = last["first_passes"]
+        plain_second = {
+            "form": "merged (receiver buckets gathered from the senders' sorted segments, sampled tables)",
+            "stages_ms": {name: round(sum(st["pass_ms"][k0 + i] for st in st_plain) / len(st_plain), 4)
+                          for i, name in enumerate(("tables_bounds_groups", "gather_sort", "listed", "fallback"))},
+            "buckets_overflow": last["buckets_overflow"], "buckets_listed": last["buckets_listed"],
+            "buckets_lds_radix": last["buckets_lds_radix"], "plain_sides": last["plain_sides"]}
+        log("plain merged SecondSort: %s" % json.dumps(plain_second))
+    else:
+        plain_second = {"form": "exchange, then %d digit passes" % last["second_passes"]}
 
     cpu = None
     if not args.no_cpu:
@@ -623,7 +636,7 @@ def bench_histsort(args):
                                "clustered field, %d virtual ranks" % (args.log2n, P),
                    "records": n, "record_bytes": 16, "vranks": P, "path": path},
         "plain": {"value": round(value_plain, 2), "ms_per_step": round(ms_plain, 3),
-                  "phases_ms": {k2: round(last[k2], 4) for k2 in PHASES}},
+                  "phases_ms": {k2: round(last[k2], 4) for k2 in PHASES}, "second_sort": plain_second},
         "learned": result_learned,
         "host_pointer_call": host_leg,
         "learned_vs_plain": round(value / value_plain, 3) if result_learned else None,

@@ -67,16 +67,20 @@ typedef struct {
 #define MPSORT_GPU_TIMING (1u << 2)      /* record per-pass kernel times */
 #define MPSORT_GPU_LEARNED (1u << 3)     /* learned-CDF splitters + bucket sort */
 #define MPSORT_GPU_FORCE_GENERAL (1u << 4) /* always sort packed {key, index} records */
-#define MPSORT_GPU_LEARNED_EXCHANGE (1u << 5) /* learned path: explicit exchange, then the
-                                                 SecondSort (default: each receiver bucket is
-                                                 gathered from the senders' sorted runs) */
+#define MPSORT_GPU_LEARNED_EXCHANGE (1u << 5) /* explicit exchange, then the SecondSort
+                                                 (default: with at most 64 ranks and keys of at
+                                                 most 8 bytes, each receiver bucket is gathered
+                                                 from the senders' sorted runs); either path */
+#define MPSORT_GPU_EXCHANGE MPSORT_GPU_LEARNED_EXCHANGE
 /* The exchange's run table (MPIU_Alltoallv mp-mpiu.c:125-136): dense walks
  * every (sender, receiver) slot, sparse only the non-empty runs.  Neither
  * flag: the reference's AUTO vote (dense when some sender has more than 128
  * non-empty receivers), unless mpsort_mpi_set_options holds
  * MPSORT_DISABLE_SPARSE_ALLTOALLV or MPSORT_REQUIRE_SPARSE_ALLTOALLV (which
  * wins, as in mpsort-mpi.c:576-583).  Sparse falls back to dense when the
- * non-empty runs exceed 64 per rank on average.  The result is the same. */
+ * non-empty runs exceed 64 per rank on average.  A table chosen by either
+ * flag or by a recorded option implies MPSORT_GPU_EXCHANGE.  The result is
+ * the same. */
 #define MPSORT_GPU_EXCHANGE_DENSE (1u << 6)
 #define MPSORT_GPU_EXCHANGE_SPARSE (1u << 7)
 
@@ -160,7 +164,8 @@ typedef struct {
     int32_t buckets_listed;    /* buckets sorted by the 1024-thread in-LDS form from a list */
     int32_t exchange_sparse;   /* 1 when the exchange walked the non-empty runs only */
     int32_t exchange_runs;     /* runs in the table it walked (P * P when dense) */
-    int32_t reserved5;
+    int32_t second_merged;     /* 1 when the SecondSort gathered every receiver bucket from the
+                                  senders' sorted segments (no exchange, no digit passes) */
 } mpsort_gpu_stats;
 
 int mpsort_gpu_last_stats(mpsort_gpu_stats *stats);

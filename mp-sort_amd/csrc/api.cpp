@@ -373,7 +373,7 @@ struct CallShape {
     int iwords;            // general path: words of the source index (1: u32, 2: u64)
     bool checksum;
     bool timing;
-    bool exchange;  // learned path: keep the explicit exchange before the SecondSort
+    bool exchange;  // keep the explicit exchange before the SecondSort
     int exch_policy;  // ExchPolicy of the exchange's run table
 };
 
@@ -401,6 +401,9 @@ static double ev_ms(hipEvent_t a, hipEvent_t b) {
     return (double)ms;
 }
 
+constexpr int kMaxOverflowList = 1 << 16;  // bucket overflow list capacity
+constexpr int kMaxFallbackSegs = 4096;     // larger lists re-sort the whole side plainly
+
 // Device workspaces of one plain histogram sort and their sizes.  The
 // sort-record buffers hold n sort records each; s0 is needed with more than
 // one rank or packed records, rbuf with packed records only.
@@ -413,6 +416,7 @@ struct HistWork {
     uint64_t* lookback = nullptr;
     int64_t* host_tab = nullptr;   // 7 P + 4 int64 (tables staged to the device)
     bool clear_lookback = false;   // the flag words may hold other data: clear them
+    bool merge = false;            // the merged SecondSort may allocate its tables
 };
 
 static int64_t tiles_of(const int64_t* counts, int P, int64_t T) {
@@ -421,7 +425,8 @@ static int64_t tiles_of(const int64_t* counts, int P, int64_t T) {
     return t;
 }
 
-static HistNeed histsort_need(const CallShape& cs, const int64_t* nin, const int64_t* nout) {
+// merge: the flag words also serve the merged SecondSort's bucket fallback
+static HistNeed histsort_need(const CallShape& cs, const int64_t* nin, const int64_t* nout, bool merge = false) {
     HistNeed w;
     Sizer sz;
     lay_plan(sz, cs.P, cs.ks.key_bytes, nullptr);
@@ -435,7 +440,7 @@ static HistNeed histsort_need(const CallShape& cs, const int64_t* nin, const int
     const TileCfg tile = default_tile(cs.rw);
     const int64_t T = (int64_t)tile.block * tile.items;
     const int64_t nt = std::max<int64_t>(1, std::max(tiles_of(nin, cs.P, T), tiles_of(nout, cs.P, T)));
-    w.lookback = (size_t)nt * kBins * sizeof(uint64_t);
+    w.lookback = (size_t)(nt + (merge ? kMaxFallbackSegs + 1 : 0)) * kBins * sizeof(uint64_t);
     return w;
 }
 
@@ -455,8 +460,27 @@ static int histwork_cached(DevCtx& c, const CallShape& cs, const HistNeed& need,
     w->lookback = (uint64_t*)c.lookback.p;
     w->host_tab = (int64_t*)c.pinned;
     w->clear_lookback = false;
+    w->merge = true;
     return 0;
 }
+
+// The SecondSort gathers every receiver bucket from the senders' sorted
+// segments, with no exchange, when no exchange is asked for: at most
+// kMergeMaxPH ranks, records of 8 or 16 bytes with keys of at most 8 bytes,
+// positions inside a segment below 2^31.
+static bool merge_eligible(const CallShape& cs) {
+    return !cs.exchange && cs.P > 1 && cs.P <= kMergeMaxPH && (cs.rw == 2 || cs.rw == 4) && cs.ks.key_bytes <= 8 &&
+           cs.ntot < ((int64_t)1 << 31);
+}
+// what the plain path's merged SecondSort reports
+struct MergedReport {
+    uint32_t lists[3] = {0, 0, 0};  // oversized, LDS radix, 1024-thread form (read at the call's end)
+    int overflow = 0;
+    int global_fallback = 0;
+};
+static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan& planB, const HistArrays& h,
+                               const PlanArrays& pa, const void* src, void* dst, void* tmp, long long* cs_out,
+                               hipEvent_t* ev, MergedReport* rep, hipStream_t st);
 
 // The histogram sort on device-resident data.  d_in / d_out are the caller's
 // records (device); for the general path they are packed into sort records.
@@ -557,6 +581,8 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     int first_passes = 0, second_passes = 0;
     hipEvent_t* pevA = cs.timing ? c.pass_ev[0] : nullptr;
     hipEvent_t* pevB = cs.timing ? c.pass_ev[1] : nullptr;
+    const bool merged = w.merge && merge_eligible(cs);
+    MergedReport mrep;
     if (P == 1) {
         // one rank: FirstSort is the whole sort (the exchange is the identity
         // and SecondSort re-sorts a sorted run)
@@ -579,15 +605,23 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
                                 exch_runs(h, P, cs.exch_policy), h.err, st)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvLayout], st));
-        // the exchange also builds the SecondSort's digit histograms
-        if ((rc = seg_clear(planB, st))) return rc;
-        if ((rc = launch_exchange_hist(w.s0, w.s1, n, rw, run_table(h, P), h.err, h.C, P, cs.ks, planB.d_hist,
-                                       st)))
-            return rc;
-        MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
-        SortBuffers b2{w.s1, final_dst, w.s0};
-        if ((rc = seg_passes(planB, b2, st, cs_direct ? h.checksum + 1 : nullptr, &second_passes, pevB)))
-            return rc;
+        if (merged) {
+            // every receiver bucket gathered from the senders' sorted
+            // segments and sorted in LDS (receiver tables from a sample)
+            if ((rc = plain_merged_second(c, cs, planB, h, pa, w.s0, final_dst, w.s1,
+                                          cs_direct ? h.checksum + 1 : nullptr, pevB, &mrep, st)))
+                return rc;
+        } else {
+            // the exchange also builds the SecondSort's digit histograms
+            if ((rc = seg_clear(planB, st))) return rc;
+            if ((rc = launch_exchange_hist(w.s0, w.s1, n, rw, run_table(h, P), h.err, h.C, P, cs.ks,
+                                           planB.d_hist, st)))
+                return rc;
+            MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
+            SortBuffers b2{w.s1, final_dst, w.s0};
+            if ((rc = seg_passes(planB, b2, st, cs_direct ? h.checksum + 1 : nullptr, &second_passes, pevB)))
+                return rc;
+        }
         MPSG_CHECK(hipEventRecord(c.ev[kEvSecond], st));
     }
     if (cs.general) {
@@ -619,8 +653,16 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
 
     // stats
     stats->nvranks = P;
-    stats->exchange_sparse = sel[0];
-    stats->exchange_runs = P > 1 ? (sel[0] ? sel[1] : P * P) : 0;
+    stats->exchange_sparse = merged ? 0 : sel[0];
+    stats->exchange_runs = P > 1 && !merged ? (sel[0] ? sel[1] : P * P) : 0;
+    stats->second_merged = P > 1 && merged ? 1 : 0;
+    if (P > 1 && merged) {
+        stats->buckets = P * kMergeBins;
+        stats->buckets_overflow = mrep.overflow;
+        stats->buckets_lds_radix = (int32_t)mrep.lists[1];
+        stats->buckets_listed = (int32_t)mrep.lists[2];
+        stats->plain_sides = mrep.global_fallback;
+    }
     stats->record_bytes = (int32_t)recbytes;
     stats->first_passes = first_passes;
     stats->second_passes = second_passes;
@@ -641,6 +683,15 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
             stats->pass_ms[np++] = ev_ms(c.pass_ev[0][2 * i], c.pass_ev[0][2 * i + 1]);
         for (int i = 0; i < second_passes && np < 64; i++)
             stats->pass_ms[np++] = ev_ms(c.pass_ev[1][2 * i], c.pass_ev[1][2 * i + 1]);
+        if (P > 1 && merged) {
+            // tables, bounds, counts and groups; the gather-and-sort launches;
+            // the listed buckets; the overflow fallback
+            hipEvent_t* e = c.pass_ev[1];
+            stats->pass_ms[np++] = ev_ms(e[8], e[6]);
+            stats->pass_ms[np++] = ev_ms(e[6], e[7]);
+            stats->pass_ms[np++] = ev_ms(e[3], e[4]);
+            stats->pass_ms[np++] = ev_ms(e[4], e[5]);
+        }
     }
     stats->npass_ms = np;
 
@@ -732,6 +783,8 @@ static int shape_for(const mpsort_key_desc* key, size_t elsize, unsigned flags, 
     else if (g_options & MPSORT_REQUIRE_SPARSE_ALLTOALLV) cs->exch_policy = kExchSparse;
     else if (g_options & MPSORT_DISABLE_SPARSE_ALLTOALLV) cs->exch_policy = kExchDense;
     else cs->exch_policy = kExchAuto;
+    // a run table chosen by the caller means an exchange
+    if (cs->exch_policy != kExchAuto) cs->exchange = true;
     return 0;
 }
 
@@ -810,8 +863,6 @@ struct LearnedModel {
     }
 };
 
-constexpr int kMaxOverflowList = 1 << 16;  // bucket overflow list capacity
-constexpr int kMaxFallbackSegs = 4096;     // larger lists re-sort the whole side plainly
 
 struct LearnedSide {
     int64_t *begin, *count, *ltiles;  // segment tables (learned tile size)
@@ -1067,6 +1118,144 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
 
 // The histogram sort with the learned local sorts and learned splitter
 // proposals (same stages and outputs as run_histsort).
+// The SecondSort without an exchange (at most kMergeMaxPH senders,
+// positions in a segment below 2^31): the bounds bnd of every receiver
+// bucket in the senders' segments (written before, see launch_merge_bounds)
+// are clamped to the receivers' windows, the buckets grouped, gathered in
+// sender order from the senders' sorted segments src and sorted in LDS
+// into dst; buckets above the in-LDS capacity take the 1024-thread form, the
+// LDS radix sort or the radix fallback (tmp: its third buffer).  LB holds
+// the receivers' tables and lists; ev (optional): 8 timing events.
+static int merged_second_sort(DevCtx& c, int rw, const LKey& lk, int P, int64_t n, const LearnedSide& LB,
+                              const SegSortPlan& planB, const HistArrays& h, const int64_t* seg_begin, int32_t* bnd,
+                              const void* src, void* dst, void* tmp, long long* cs_out, LearnedCounters& lc,
+                              hipEvent_t* ev, hipStream_t st) {
+    int rc = 0;
+    // no exchange: every receiver bucket is gathered from the senders'
+    // sorted segments and sorted in LDS
+    auto markB = [&](int i) -> int {
+        if (ev) MPSG_CHECK(hipEventRecord(ev[i], st));
+        return 0;
+    };
+    if ((rc = markB(0))) return rc;
+    // the bounds, clamped to each receiver's window in each sender
+    if ((rc = launch_merge_count(bnd, P, h.myC, LB.lhist, st))) return rc;
+    if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st, kMergeBins))) return rc;
+    if ((rc = launch_fill_u32(LB.noverflow, 0u, 8, st))) return rc;
+    lc.lists[1] = LB.noverflow;
+    MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
+    if ((rc = markB(1)) || (rc = markB(2))) return rc;
+    // groups of neighbouring buckets, one workgroup each; buckets
+    // above the group capacity alone, in the larger form
+    uint32_t novh[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if ((rc = launch_merge_group(LB.lhist, P, LB.tasks, LB.noverflow + 3, st))) return rc;
+    MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    // descriptors (groups, then the larger buckets): group, then per
+    // sender sub-run start and position
+    const uint32_t nsmall = novh[3], nbig = novh[4], ntasks = nsmall + nbig;
+    const size_t d_bytes = ((size_t)(ntasks ? ntasks : 1) * sizeof(MergeTask) + 255) & ~(size_t)255;
+    const size_t b_bytes = (size_t)(ntasks ? ntasks : 1) * 64 * sizeof(int64_t);
+    if ((rc = c.mdesc.ensure(d_bytes + b_bytes + (size_t)(ntasks ? ntasks : 1) * 64 * sizeof(int32_t))))
+        return rc;
+    MergeTask* desc = (MergeTask*)c.mdesc.p;
+    int64_t* sub_beg = (int64_t*)((char*)c.mdesc.p + d_bytes);
+    int32_t* sub_pref = (int32_t*)((char*)c.mdesc.p + d_bytes + b_bytes);
+    if ((rc = launch_merge_desc(LB.tasks, nsmall, bnd, seg_begin, P, LB.bbase, LB.tables, desc, sub_beg,
+                                sub_pref, st)))
+        return rc;
+    if ((rc = launch_merge_desc(LB.tasks + (size_t)P * kMergeBins, nbig, bnd, seg_begin, P, LB.bbase,
+                                LB.tables, desc + nsmall, sub_beg + (size_t)nsmall * 64,
+                                sub_pref + (size_t)nsmall * 64, st)))
+        return rc;
+    if ((rc = markB(6))) return rc;
+    for (int big = 0; big < 2; big++) {
+        const uint32_t o = big ? nsmall : 0;
+        if ((rc = launch_merge_bucket(rw, big != 0, src, dst, lk, seg_begin, bnd, P, LB.lhist,
+                                      LB.bbase, desc + o, sub_beg + (size_t)o * 64,
+                                      sub_pref + (size_t)o * 64, big ? nbig : nsmall, LB.overflow,
+                                      LB.noverflow, kMaxOverflowList, LB.hard, LB.noverflow + 1, LB.medium,
+                                      LB.noverflow + 2, cs_out, st,
+                                      fault_cap())))
+            return rc;
+    }
+    if ((rc = markB(7))) return rc;
+    MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if ((rc = markB(3))) return rc;
+    // buckets above the workgroup's capacity and crowded ones were
+    // copied in place in sender order: the 1024-thread in-LDS sort,
+    // then the LDS radix sort
+    if ((rc = launch_listed_sort(rw, dst, dst, lk, LB.medium, novh[2], LB.lhist, LB.bbase,
+                                 LB.tables, kMergeBins, kMergeBounds, LB.hard, LB.noverflow + 1, nullptr, st)))
+        return rc;
+    if ((rc = launch_lsd_sort(rw, dst, dst, lk, LB.lhist, LB.bbase, LB.hard, LB.noverflow + 1,
+                              (int64_t)P * kMergeBins, nullptr, st)))
+        return rc;
+    if ((rc = markB(4))) return rc;
+    const uint32_t nov = novh[0];
+    lc.overflow_buckets += (int)nov;
+    if (nov > (uint32_t)kMaxFallbackSegs || (g_faults & MPSORT_GPU_FAULT_PLAIN)) {
+        // more oversized buckets than the bucket fallback takes (the
+        // list may also have stopped at its capacity): every one of
+        // them was copied to its slots in sender order, so a radix
+        // sort of each receiver's whole segment finishes the sort
+        lc.global_fallback++;
+        if (cs_out && (rc = launch_fill_u32((uint32_t*)cs_out, 0u, 2, st))) return rc;
+        int passes = 0;
+        SortBuffers b{dst, dst, tmp};
+        if ((rc = segmented_sort(planB, b, st, false, cs_out != nullptr, &passes, nullptr))) return rc;
+        // no digit pass ran (one key value): the output checksum is
+        // taken by a copy
+        if (passes == 0 && cs_out && (rc = launch_copy(dst, tmp, n, rw, cs_out, st)))
+            return rc;
+    } else if (nov && (rc = sort_listed_segments(c, planB, rw, LB.overflow, nov, dst, dst, tmp,
+                                                 nullptr, st))) {
+        return rc;
+    }
+    if ((rc = markB(5))) return rc;
+    return 0;
+}
+
+// The plain path's merged SecondSort: receiver tables from an even sample
+// of each receiver's records, their positions searched in the receivers'
+// windows of the senders' sorted segments src (no model, no exchange).
+// ev: the SecondSort's timing events (ev[8] marks the tables' start).
+static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan& planB, const HistArrays& h,
+                               const PlanArrays& pa, const void* src, void* dst, void* tmp, long long* cs_out,
+                               hipEvent_t* ev, MergedReport* rep, hipStream_t st) {
+    const int P = cs.P, rw = cs.rw;
+    int rc = 0;
+    Sizer lz;
+    lay_lside(lz, P, nullptr);
+    if ((rc = c.lsmall.ensure(lz.off + 4096))) return rc;
+    Arena la{(char*)c.lsmall.p};
+    LearnedSide LB;
+    lay_lside(la, P, &LB);
+    LB.tstride = kMergeBounds;
+    // bounds, then the receivers' samples and their window prefixes
+    const size_t bnd_bytes = ((size_t)P * P * (kMergeBins + 1) * sizeof(int32_t) + 255) & ~(size_t)255;
+    const size_t samp_bytes = (size_t)P * kMergeSamples * sizeof(uint64_t);
+    if ((rc = c.mbnd.ensure(bnd_bytes + samp_bytes + (size_t)P * (kMergeMaxPH + 1) * sizeof(int64_t)))) return rc;
+    int32_t* bnd = (int32_t*)c.mbnd.p;
+    uint64_t* samp = (uint64_t*)((char*)c.mbnd.p + bnd_bytes);
+    int64_t* spref = (int64_t*)((char*)c.mbnd.p + bnd_bytes + samp_bytes);
+    const LKey lk = make_lkey(cs.ks);
+    if (ev) MPSG_CHECK(hipEventRecord(ev[8], st));
+    if ((rc = launch_merge_sample_tables(rw, src, lk, pa.begin, P, h.myC, LB.tables, samp, spref, st))) return rc;
+    if ((rc = launch_merge_bounds(rw, src, lk, pa.begin, pa.count, P, LB.tables, nullptr, 0, nullptr, bnd, st,
+                                  h.myC, samp, spref)))
+        return rc;
+    LearnedCounters lc;
+    if ((rc = merged_second_sort(c, rw, lk, P, cs.ntot, LB, planB, h, pa.begin, bnd, src, dst, tmp, cs_out, lc, ev,
+                                 st)))
+        return rc;
+    MPSG_CHECK(hipMemcpyAsync(rep->lists, LB.noverflow, sizeof rep->lists, hipMemcpyDeviceToHost, st));
+    rep->overflow = lc.overflow_buckets;
+    rep->global_fallback = lc.global_fallback;
+    return 0;
+}
+
 static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in, void* d_out,
                                 const int64_t* nin, const int64_t* nout, const LearnedModel& M,
                                 mpsort_gpu_stats* stats) {
@@ -1220,7 +1409,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         // SecondSort: receiver r holds the quantile range [C[r], C[r+1]) / n;
         // the merged form uses kMergeBins buckets per receiver (positions
         // inside a sender's segment are kept as int32)
-        const bool merged = !cs.exchange && P <= kMergeMaxPH && (rw == 2 || rw == 4) && n < ((int64_t)1 << 31);
+        const bool merged = merge_eligible(cs);
         merged_side = merged;
         const int nbB = merged ? kMergeBounds : kLBounds;
         LB.tstride = nbB;
@@ -1279,90 +1468,9 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             (rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.heavy_q, M.nheavy, st)))
             return rc;
         if (merged) {
-            // no exchange: every receiver bucket is gathered from the senders'
-            // sorted segments and sorted in LDS
-            auto markB = [&](int i) -> int {
-                if (evB) MPSG_CHECK(hipEventRecord(evB[i], st));
-                return 0;
-            };
-            const LKey lk = lkB;
-            if ((rc = markB(0))) return rc;
-            // the bounds, clamped to each receiver's window in each sender
-            if ((rc = launch_merge_count(bnd, P, h.myC, LB.lhist, st))) return rc;
-            if ((rc = launch_lscan(P, LB.lhist, h.C, LB.bbase, st, kMergeBins))) return rc;
-            if ((rc = launch_fill_u32(LB.noverflow, 0u, 8, st))) return rc;
-            lc.lists[1] = LB.noverflow;
-            MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
-            if ((rc = markB(1)) || (rc = markB(2))) return rc;
-            // groups of neighbouring buckets, one workgroup each; buckets
-            // above the group capacity alone, in the larger form
-            uint32_t novh[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            if ((rc = launch_merge_group(LB.lhist, P, LB.tasks, LB.noverflow + 3, st))) return rc;
-            MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
-            MPSG_CHECK(hipStreamSynchronize(st));
-            // descriptors (groups, then the larger buckets): group, then per
-            // sender sub-run start and position
-            const uint32_t nsmall = novh[3], nbig = novh[4], ntasks = nsmall + nbig;
-            const size_t d_bytes = ((size_t)(ntasks ? ntasks : 1) * sizeof(MergeTask) + 255) & ~(size_t)255;
-            const size_t b_bytes = (size_t)(ntasks ? ntasks : 1) * 64 * sizeof(int64_t);
-            if ((rc = c.mdesc.ensure(d_bytes + b_bytes + (size_t)(ntasks ? ntasks : 1) * 64 * sizeof(int32_t))))
+            if ((rc = merged_second_sort(c, rw, lkB, P, n, LB, planB, h, pa.begin, bnd, c.s0.p, final_dst, c.s1.p,
+                                         cs_direct ? h.checksum + 1 : nullptr, lc, evB, st)))
                 return rc;
-            MergeTask* desc = (MergeTask*)c.mdesc.p;
-            int64_t* sub_beg = (int64_t*)((char*)c.mdesc.p + d_bytes);
-            int32_t* sub_pref = (int32_t*)((char*)c.mdesc.p + d_bytes + b_bytes);
-            if ((rc = launch_merge_desc(LB.tasks, nsmall, bnd, pa.begin, P, LB.bbase, LB.tables, desc, sub_beg,
-                                        sub_pref, st)))
-                return rc;
-            if ((rc = launch_merge_desc(LB.tasks + (size_t)P * kMergeBins, nbig, bnd, pa.begin, P, LB.bbase,
-                                        LB.tables, desc + nsmall, sub_beg + (size_t)nsmall * 64,
-                                        sub_pref + (size_t)nsmall * 64, st)))
-                return rc;
-            if ((rc = markB(6))) return rc;
-            for (int big = 0; big < 2; big++) {
-                const uint32_t o = big ? nsmall : 0;
-                if ((rc = launch_merge_bucket(rw, big != 0, c.s0.p, final_dst, lk, pa.begin, bnd, P, LB.lhist,
-                                              LB.bbase, desc + o, sub_beg + (size_t)o * 64,
-                                              sub_pref + (size_t)o * 64, big ? nbig : nsmall, LB.overflow,
-                                              LB.noverflow, kMaxOverflowList, LB.hard, LB.noverflow + 1, LB.medium,
-                                              LB.noverflow + 2, cs_direct ? h.checksum + 1 : nullptr, st,
-                                              fault_cap())))
-                    return rc;
-            }
-            if ((rc = markB(7))) return rc;
-            MPSG_CHECK(hipMemcpyAsync(novh, LB.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
-            MPSG_CHECK(hipStreamSynchronize(st));
-            if ((rc = markB(3))) return rc;
-            // buckets above the workgroup's capacity and crowded ones were
-            // copied in place in sender order: the 1024-thread in-LDS sort,
-            // then the LDS radix sort
-            if ((rc = launch_listed_sort(rw, final_dst, final_dst, lk, LB.medium, novh[2], LB.lhist, LB.bbase,
-                                         LB.tables, kMergeBins, kMergeBounds, LB.hard, LB.noverflow + 1, nullptr, st)))
-                return rc;
-            if ((rc = launch_lsd_sort(rw, final_dst, final_dst, lk, LB.lhist, LB.bbase, LB.hard, LB.noverflow + 1,
-                                      (int64_t)P * kMergeBins, nullptr, st)))
-                return rc;
-            if ((rc = markB(4))) return rc;
-            const uint32_t nov = novh[0];
-            lc.overflow_buckets += (int)nov;
-            if (nov > (uint32_t)kMaxFallbackSegs || (g_faults & MPSORT_GPU_FAULT_PLAIN)) {
-                // more oversized buckets than the bucket fallback takes (the
-                // list may also have stopped at its capacity): every one of
-                // them was copied to its slots in sender order, so a radix
-                // sort of each receiver's whole segment finishes the sort
-                lc.global_fallback++;
-                if (cs_direct && (rc = launch_fill_u32((uint32_t*)(h.checksum + 1), 0u, 2, st))) return rc;
-                int passes = 0;
-                SortBuffers b{final_dst, final_dst, c.s1.p};
-                if ((rc = segmented_sort(planB, b, st, false, cs_direct, &passes, nullptr))) return rc;
-                // no digit pass ran (one key value): the output checksum is
-                // taken by a copy
-                if (passes == 0 && cs_direct && (rc = launch_copy(final_dst, c.s1.p, n, rw, h.checksum + 1, st)))
-                    return rc;
-            } else if (nov && (rc = sort_listed_segments(c, planB, rw, LB.overflow, nov, final_dst, final_dst,
-                                                         c.s1.p, nullptr, st))) {
-                return rc;
-            }
-            if ((rc = markB(5))) return rc;
         } else {
             if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, run_table(h, P), h.err, st))) return rc;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
@@ -1877,7 +1985,7 @@ int mpsort_gpu_newarray(const void* base, const int64_t* nin, int P, void* out, 
     remember_key(*c, key, 0, 0);
     rc = staged_call(c, cs, base, out, devptr, P, &st, [&](const void* d_in, void* d_out) {
         HistWork w;
-        int r2 = histwork_cached(*c, cs, histsort_need(cs, nin, nout), &w);
+        int r2 = histwork_cached(*c, cs, histsort_need(cs, nin, nout, true), &w);
         return r2 ? r2 : run_histsort(*c, cs, d_in, d_out, nin, nout, nullptr, &st, w);
     });
     g_last = st;
@@ -2125,7 +2233,7 @@ int mpsort_gpu_newarray_radix(const void* base, const int64_t* nin, int P, void*
     // host's (the device sees only the packed records)
     cs.checksum = !(flags & MPSORT_GPU_NO_CHECKSUM);
     HistWork w;
-    rc = histwork_cached(*c, cs, histsort_need(cs, nin, nout), &w);
+    rc = histwork_cached(*c, cs, histsort_need(cs, nin, nout, true), &w);
     if (!rc) rc = run_histsort(*c, cs, c->din.p, c->dout.p, nin, nout, packed, &st, w);
     mpiu_free(packed, __FILE__, __LINE__);
     if (!rc) {

@@ -37,12 +37,19 @@ inline LKey make_lkey(const KeySpec& ks) {
     return k;
 }
 
+// The merged SecondSort's bounds: bucket k of receiver r starts at
+// bnd[merge_bnd_at(r, j, k, P)] in sender j's segment (k = 0 .. kMergeBins;
+// one sender's bounds of one receiver side by side).
+__host__ __device__ inline int64_t merge_bnd_at(int r, int j, int k, int P) {
+    return ((int64_t)r * P + j) * (kMergeBins + 1) + k;
+}
+
 // Positions of the merged SecondSort's receiver-table entries in the
 // senders' sorted segments, written by the FirstSort's bucket sorts while a
 // sorted bucket is in LDS (keys == nullptr: not wanted).  Entry e is entry
 // k - 1 of receiver r's table (e = r * kMergeBounds + k - 1); its position in
 // segment j (#{keys <= entry}, relative to the segment's start) goes to
-// bnd[(r * (kMergeBins + 1) + k) * P + j].
+// bnd[merge_bnd_at(r, j, k, P)].
 struct EntryOut {
     const uint64_t* keys;      // P * kMergeBounds entries, non-decreasing
     const uint32_t* range;     // per segment kLBins + 1: bucket b holds entries [range[b], range[b + 1])
@@ -94,15 +101,29 @@ int launch_chunk_sort(int rw, const void* src, void* dst, const LKey& lk, const 
 
 // SecondSort straight from the senders' sorted segments (no exchange), for
 // P <= 64 receivers with kMergeBounds-boundary tables each:
-//   bnd: P * P * (kMergeBins + 1) int32; hist / bucket_base: P * kMergeBins
+//   bnd: P * P * (kMergeBins + 1) int32 (merge_bnd_at); hist / bucket_base:
+//   P * kMergeBins
 // positions of the table entries in the senders' whole segments (needs no
 // splitters), then clamped to the receivers' windows myC with the sizes.
 // With the FirstSort's tables ftab (kLBounds entries, sender j's at
 // ftab + j * ftstride) and its bucket bases fbase (absolute, kLBins per
 // segment), each search covers one bucket.
+// win (optional, the receivers' windows myC): each search covers receiver
+// r's window in the sender only; samp / spref (optional, with win): the
+// receivers' samples from launch_merge_sample_tables narrow it further.
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
                         int P, const uint64_t* tables, const uint64_t* ftab, int64_t ftstride, const int64_t* fbase,
-                        int32_t* bnd, hipStream_t st);
+                        int32_t* bnd, hipStream_t st, const int64_t* win = nullptr, const uint64_t* samp = nullptr,
+                        const int64_t* spref = nullptr);
+// receiver tables (P * kMergeBounds, each non-decreasing) from an even
+// sample of kMergeSamples keys of every receiver's records in the senders'
+// sorted segments, for callers without a model; myC: the receivers' windows
+// in the senders.  samp (P * kMergeSamples) and spref (P * 65) keep the
+// sample and the window prefixes for launch_merge_bounds.
+constexpr int kMergeSamples = 4 * kMergeBins;
+int launch_merge_sample_tables(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, int P,
+                               const int64_t* myC, uint64_t* tables, uint64_t* samp, int64_t* spref,
+                               hipStream_t st);
 int launch_merge_count(int32_t* bnd, int P, const int64_t* myC, uint32_t* hist, hipStream_t st);
 // groups of neighbouring receiver buckets (together at most kMergeCap
 // records) in tasks[0 .. ntasks[0]), and the larger single buckets in

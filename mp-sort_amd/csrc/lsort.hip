@@ -496,7 +496,7 @@ __device__ __forceinline__ void write_entries(const EntryOut& eo, int ej, int eb
     const int64_t rel = base - eo.seg_begin[ej];
     for (uint32_t e = e0 + threadIdx.x; e < e1; e += BLOCK) {
         const uint32_t r = e / kMergeBounds, k = e % kMergeBounds + 1;
-        eo.bnd[((int64_t)r * (kMergeBins + 1) + k) * eo.P + ej] = (int32_t)(rel + count_le(eo.keys[e]));
+        eo.bnd[merge_bnd_at(r, ej, k, eo.P)] = (int32_t)(rel + count_le(eo.keys[e]));
     }
     if (threadIdx.x == 0) eo.done[(int64_t)ej * kLBins + eb] = 1;
 }
@@ -962,7 +962,8 @@ __device__ __forceinline__ int sort_bucket_lds(const Src& src, uint32_t* __restr
     }
     __syncthreads();
     // the slots' bins, positions, keys and bin bounds are read for all items
-    // before the first is ranked, so the reads are in flight together
+    // before the first is ranked, so the reads are in flight together (bin
+    // and position, start and end, are below 2^16: two to a register)
     uint32_t rb[C::I], ri[C::I], rs0[C::I], re[C::I];
     uint64_t rk[C::I];
 #pragma unroll
@@ -981,13 +982,13 @@ __device__ __forceinline__ int sort_bucket_lds(const Src& src, uint32_t* __restr
 #pragma unroll
     for (int k = 0; k < C::I; k++) {
         if (k * C::B + tid >= count) continue;
-        const uint32_t i = ri[k];
+        const uint32_t i = ri[k], b = rb[k];
         const uint32_t s0 = rs0[k], e = re[k];
         if (e - s0 == 1) {
             s_pos[i] = (uint16_t)s0;
-        } else if (DUPS && any_long && e - s0 > (uint32_t)kLongMin && !((s_mix[rb[k] >> 5] >> (rb[k] & 31)) & 1u)) {
+        } else if (DUPS && any_long && e - s0 > (uint32_t)kLongMin && !((s_mix[b >> 5] >> (b & 31)) & 1u)) {
             // one key value: placed by counting below; s_pos holds 0x8000 | bin
-            s_pos[i] = (uint16_t)(0x8000u | rb[k]);
+            s_pos[i] = (uint16_t)(0x8000u | b);
             s_defer = 1;
         } else if (e - s0 > (uint32_t)kRunMax) {
             s_long = 1;
@@ -1084,18 +1085,6 @@ __device__ __forceinline__ int sort_bucket_lds(const Src& src, uint32_t* __restr
             __syncthreads();
         }
     }
-#pragma unroll
-    for (int k = 0; k < C::I; k++) {
-        const int i = k * C::B + tid;
-        if (i < count) {
-            RecIO<RW>::store(dst, base + s_pos[i], r[k]);
-            if (checksum_out) {
-#pragma unroll
-                for (int w = 0; w < RW; w++) csum += sbyte_sum(r[k][w]);
-            }
-        }
-    }
-    if (checksum_out) block_checksum_add<C::W>(csum, s_sum, checksum_out);
     if (eo.keys) {
         // #{keys <= t}: every key of the bins before t's bin, and those of
         // its bin that are <= t (bins are non-decreasing in the key; the keys
@@ -1110,6 +1099,18 @@ __device__ __forceinline__ int sort_bucket_lds(const Src& src, uint32_t* __restr
             return c;
         });
     }
+#pragma unroll
+    for (int k = 0; k < C::I; k++) {
+        const int i = k * C::B + tid;
+        if (i < count) {
+            RecIO<RW>::store(dst, base + s_pos[i], r[k]);
+            if (checksum_out) {
+#pragma unroll
+                for (int w = 0; w < RW; w++) csum += sbyte_sum(r[k][w]);
+            }
+        }
+    }
+    if (checksum_out) block_checksum_add<C::W>(csum, s_sum, checksum_out);
     return 0;
 }
 
@@ -1243,9 +1244,9 @@ __global__ __launch_bounds__(kCBlock) void listed_sort_kernel(
 // sender) and sorted in one pass by a kMergeBlock-thread workgroup of
 // kMergeItems records per thread (three share a CU); a bucket above
 // kMergeCap is sorted alone by kSmallItems records per thread (two per CU).
-//   merge_bounds_kernel  bnd[r][k][j] = start of bucket k's sub-run in sender
-//                        j (relative to j's segment), k = 0..kMergeBins (the
-//                        P senders of one bucket side by side)
+//   merge_bounds_kernel  bnd[r][j][k] = start of bucket k's sub-run in sender
+//                        j (relative to j's segment), k = 0..kMergeBins (one
+//                        sender's bounds of one receiver side by side)
 //   merge_count_kernel   bucket sizes (sum over senders)
 //   merge_group_kernel   the groups, one workgroup per receiver: greedy over
 //                        chunks of kGroupChunk buckets, one thread per chunk
@@ -1268,14 +1269,21 @@ static_assert(BSort<kMergeBlock, kSmallItems>::CAP == kMergeBigCap, "merged Seco
 template <int RW>
 // Positions in the whole sorted segment of every sender (the receivers'
 // windows are applied by merge_count_kernel, so this runs before the
-// splitters are known): bnd[r][k][j] = #{keys of sender j <= t_r[k-1]} for
+// splitters are known): bnd[r][j][k] = #{keys of sender j <= t_r[k-1]} for
 // k = 1 .. kMergeBins - 1, 0 for k = 0, the segment's size for kMergeBins.
+// win (the windows myC, when known): each search covers receiver r's
+// window in sender j only (the same value once clamped to it); with the
+// receivers' even samples (samp, spref: merge_sample_tables_kernel) only
+// the records between sender j's two samples that bracket the entry.
 __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __restrict__ recs, LKey lk,
                                                            const int64_t* __restrict__ seg_begin,
                                                            const int64_t* __restrict__ seg_count, int P,
                                                            const uint64_t* __restrict__ tables,
                                                            const uint64_t* __restrict__ ftab, int64_t ftstride,
                                                            const int64_t* __restrict__ fbase,
+                                                           const int64_t* __restrict__ win,
+                                                           const uint64_t* __restrict__ samp,
+                                                           const int64_t* __restrict__ spref,
                                                            int32_t* __restrict__ bnd) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, j, k), k = 0..kMergeBins
     if (g >= (int64_t)P * P * (kMergeBins + 1)) return;
@@ -1305,6 +1313,36 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
             const int64_t* fb = fbase + (int64_t)j * kLBins;
             a = fb[m] - sb;
             e = m + 1 < kLBins ? fb[m + 1] - sb : cnt;
+        } else if (win) {
+            const int64_t* wj = win + (int64_t)j * (P + 1);
+            a = wj[r];
+            e = wj[r + 1];
+            if (samp && a < e) {
+                // sample u lies at position pos(u) of the receiver's records;
+                // sender j's samples are u0 .. u1 - 1, at window offsets
+                // pos(u) - pj, keys non-decreasing
+                const int64_t* pr = spref + (int64_t)r * (kMergeMaxP + 1);
+                const int64_t nr = pr[P], pj = pr[j], pj1 = pr[j + 1];
+                auto pos = [&](int64_t u) -> int64_t { return ((2 * u + 1) * nr) / (2 * kMergeSamples); };
+                auto first_at = [&](int64_t p) -> int64_t {  // least u with pos(u) >= p
+                    int64_t x = 0, y = kMergeSamples;
+                    while (x < y) {
+                        const int64_t m = (x + y) >> 1;
+                        if (pos(m) < p) x = m + 1; else y = m;
+                    }
+                    return x;
+                };
+                const int64_t u0 = first_at(pj), u1 = first_at(pj1);
+                const uint64_t* sr = samp + (int64_t)r * kMergeSamples;
+                int64_t qa = u0, qe = u1;  // first of sender j's samples above t
+                while (qa < qe) {
+                    const int64_t m = (qa + qe) >> 1;
+                    if (sr[m] <= t) qa = m + 1; else qe = m;
+                }
+                const int64_t w0 = a;
+                if (qa > u0) a = w0 + (pos(qa - 1) - pj) + 1;
+                if (qa < u1) e = w0 + (pos(qa) - pj);
+            }
         }
         while (a < e) {
             const int64_t m = a + ((e - a) >> 1);
@@ -1314,34 +1352,121 @@ __global__ __launch_bounds__(256) void merge_bounds_kernel(const uint32_t* __res
         }
         res = a;
     }
-    bnd[((int64_t)r * (kMergeBins + 1) + k) * P + j] = (int32_t)res;
+    bnd[g] = (int32_t)res;  // merge_bnd_at(r, j, k, P)
+}
+
+// Receiver tables of the merged SecondSort taken from the records (for
+// callers without a model): kMergeSamples keys of receiver r at evenly spaced
+// positions of its records (the windows [myC[j][r], myC[j][r+1]) of the
+// senders' sorted segments, in sender order), sorted in LDS (bitonic over
+// kTabSort slots, the rest padded with the largest key); entry e is sample
+// 4e + 3, so that a bucket holds about four samples' share of the records.
+// Where entries repeat a key v, the first of them becomes v - 1: the bucket
+// (v - 1, v] then holds that key alone and is copied, not sorted.  One
+// 1024-thread workgroup per receiver.
+constexpr int kTabSort = 16384;
+static_assert(kMergeSamples <= kTabSort && (kTabSort & (kTabSort - 1)) == 0, "bitonic sort of the samples");
+static_assert(4 * (kMergeBounds - 1) + 3 < kMergeSamples, "entries are samples");
+// The unsorted sample (samp[r][u]: the key at position pos(u) of receiver
+// r's records) and the prefixes over the senders' windows (spref[r][j], j =
+// 0 .. P) are kept for merge_bounds_kernel.
+template <int RW>
+__global__ __launch_bounds__(1024) void merge_sample_tables_kernel(const uint32_t* __restrict__ recs, LKey lk,
+                                                                   const int64_t* __restrict__ seg_begin, int P,
+                                                                   const int64_t* __restrict__ myC,
+                                                                   uint64_t* __restrict__ tables,
+                                                                   uint64_t* __restrict__ samp,
+                                                                   int64_t* __restrict__ spref) {
+    __shared__ uint64_t s_k[kTabSort];
+    __shared__ int64_t s_pref[kMergeMaxP + 1];  // receiver r's records in the windows before sender j's
+    __shared__ int64_t s_lo[kMergeMaxP];        // first record of sender j's window (absolute)
+    const int r = blockIdx.x, tid = threadIdx.x;
+    if (tid < 64) {
+        int64_t len = 0, lo = 0;
+        if (tid < P) {
+            const int64_t* cj = myC + (int64_t)tid * (P + 1);
+            lo = seg_begin[tid] + cj[r];
+            len = cj[r + 1] - cj[r];
+        }
+        int64_t x = len;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int64_t y = __shfl_up(x, off, 64);
+            if (tid >= off) x += y;
+        }
+        if (tid < P) {
+            s_pref[tid] = x - len;
+            s_lo[tid] = lo;
+            spref[(int64_t)r * (kMergeMaxP + 1) + tid] = x - len;
+        }
+        if (tid == P - 1) {
+            s_pref[P] = x;
+            spref[(int64_t)r * (kMergeMaxP + 1) + P] = x;
+        }
+    }
+    __syncthreads();
+    const int64_t nr = s_pref[P];
+    for (int u = tid; u < kTabSort; u += 1024) {
+        uint64_t key = ~0ull;
+        if (u < kMergeSamples && nr > 0) {
+            const int64_t pos = ((2 * (int64_t)u + 1) * nr) / (2 * kMergeSamples);
+            // the sender whose window holds pos: the last j with s_pref[j] <= pos
+            int j = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1)
+                if (j + step < P && s_pref[j + step] <= pos) j += step;
+            uint32_t rr[RW];
+            RecIO<RW>::load(recs, s_lo[j] + (pos - s_pref[j]), rr);
+            key = lkey<RW>(rr, lk);
+        }
+        s_k[u] = key;
+        if (u < kMergeSamples) samp[(int64_t)r * kMergeSamples + u] = key;
+    }
+    for (int size = 2; size <= kTabSort; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = tid; t < kTabSort / 2; t += 1024) {
+                const int i = 2 * t - (t & (stride - 1));  // lower slot of the pair
+                const uint64_t a = s_k[i], b = s_k[i + stride];
+                if ((a > b) == ((i & size) == 0)) {
+                    s_k[i] = b;
+                    s_k[i + stride] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < kMergeBounds; e += 1024) {
+        const uint64_t v = s_k[4 * e + 3];
+        const bool first = e == 0 || s_k[4 * e - 1] < v;
+        const bool repeated = e + 1 < kMergeBounds && s_k[4 * e + 7] == v;
+        tables[(int64_t)r * kMergeBounds + e] = first && repeated && v > 0 ? v - 1 : v;
+    }
 }
 
 // The bounds clamped to receiver r's window [myC[j][r], myC[j][r+1]) in
-// every sender j (written back: row k by the wave of (r, k), the last row
-// also by the wave of (r, kMergeBins - 1); a row read while its owner writes
-// it clamps to the same value), and the bucket sizes.  One wave per (r, k),
-// lane j for sender j.
+// every sender j (written back: entry k by the thread of (r, k), the last
+// entry also by the thread of (r, kMergeBins - 1); an entry read while its
+// owner writes it clamps to the same value), and the bucket sizes.  One
+// thread per (r, k), walking the senders; the lanes of a wave hold
+// neighbouring k, so every read and write is one contiguous run.
 __global__ __launch_bounds__(256) void merge_count_kernel(int32_t* __restrict__ bnd, int P,
                                                           const int64_t* __restrict__ myC,
                                                           uint32_t* __restrict__ hist) {
-    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // (r, k)
-    const int j = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (r, k)
     if (g >= (int64_t)P * kMergeBins) return;
     const int k = (int)(g % kMergeBins), r = (int)(g / kMergeBins);
-    int32_t* bk = bnd + ((int64_t)r * (kMergeBins + 1) + k) * P;
-    int c = 0;
-    if (j < P) {
+    uint32_t c = 0;
+    for (int j = 0; j < P; j++) {
         const int64_t* cj = myC + (int64_t)j * (P + 1);
         const int32_t lo0 = (int32_t)cj[r], hi0 = (int32_t)cj[r + 1];
-        const int32_t a = min(max(bk[j], lo0), hi0), b = min(max(bk[P + j], lo0), hi0);
-        c = b - a;
-        bk[j] = a;
-        if (k == kMergeBins - 1) bk[P + j] = b;
+        int32_t* bk = bnd + merge_bnd_at(r, j, k, P);
+        const int32_t a = min(max(bk[0], lo0), hi0), b = min(max(bk[1], lo0), hi0);
+        c += (uint32_t)(b - a);
+        bk[0] = a;
+        if (k == kMergeBins - 1) bk[1] = b;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-    if (j == 0) hist[g] = (uint32_t)c;
+    hist[g] = c;
 }
 
 // groups of the merged SecondSort: tasks[] (receiver by receiver, in bucket
@@ -1413,10 +1538,9 @@ __device__ __forceinline__ void merge_subruns(const int32_t* __restrict__ bnd, c
     int len = 0;
     int64_t beg = 0;
     if (lane < P) {
-        const int32_t* ba = bnd + ((int64_t)r * (kMergeBins + 1) + ka) * P;
-        const int32_t* bb = bnd + ((int64_t)r * (kMergeBins + 1) + kb) * P;
-        beg = seg_begin[lane] + ba[lane];
-        len = bb[lane] - ba[lane];
+        const int32_t a = bnd[merge_bnd_at(r, lane, ka, P)];
+        beg = seg_begin[lane] + a;
+        len = bnd[merge_bnd_at(r, lane, kb, P)] - a;
     }
     int x = len;
 #pragma unroll
@@ -1450,10 +1574,9 @@ __global__ __launch_bounds__(256) void merge_desc_kernel(const uint32_t* __restr
     int len = 0;
     int64_t beg = 0;
     if (lane < P) {
-        const int32_t* ba = bnd + ((int64_t)r * (kMergeBins + 1) + ka) * P;
-        const int32_t* bb = bnd + ((int64_t)r * (kMergeBins + 1) + kb) * P;
-        beg = seg_begin[lane] + ba[lane];
-        len = bb[lane] - ba[lane];
+        const int32_t a = bnd[merge_bnd_at(r, lane, ka, P)];
+        beg = seg_begin[lane] + a;
+        len = bnd[merge_bnd_at(r, lane, kb, P)] - a;
     }
     int x = len;
 #pragma unroll
@@ -1712,8 +1835,8 @@ __global__ __launch_bounds__(kBlock) void entries_fixup_kernel(const uint32_t* _
         const int64_t q = bb - nb;  // (r, j)
         const int r = (int)(q / eo.P), j = (int)(q % eo.P);
         if (threadIdx.x == 0) {
-            eo.bnd[((int64_t)r * (kMergeBins + 1)) * eo.P + j] = 0;
-            eo.bnd[((int64_t)r * (kMergeBins + 1) + kMergeBins) * eo.P + j] = (int32_t)seg_count[j];
+            eo.bnd[merge_bnd_at(r, j, 0, eo.P)] = 0;
+            eo.bnd[merge_bnd_at(r, j, kMergeBins, eo.P)] = (int32_t)seg_count[j];
         }
         return;
     }
@@ -1892,13 +2015,28 @@ int launch_chunk_sort_small(int rw, const void* src, void* dst, const LKey& lk, 
 
 int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
                         int P, const uint64_t* tables, const uint64_t* ftab, int64_t ftstride, const int64_t* fbase,
-                        int32_t* bnd, hipStream_t st) {
+                        int32_t* bnd, hipStream_t st, const int64_t* win, const uint64_t* samp,
+                        const int64_t* spref) {
     const int64_t n = (int64_t)P * P * (kMergeBins + 1);
     const unsigned g = (unsigned)((n + 255) / 256);
     const uint32_t* r = (const uint32_t*)recs;
     switch (rw) {
-        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, ftstride, fbase, bnd); break;
-        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, ftstride, fbase, bnd); break;
+        case 2: hipLaunchKernelGGL(merge_bounds_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, ftstride, fbase, win, samp, spref, bnd); break;
+        case 4: hipLaunchKernelGGL(merge_bounds_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, tables, ftab, ftstride, fbase, win, samp, spref, bnd); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_merge_sample_tables(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, int P,
+                               const int64_t* myC, uint64_t* tables, uint64_t* samp, int64_t* spref,
+                               hipStream_t st) {
+    if (P < 1 || P > kMergeMaxP) return -1;
+    const uint32_t* r = (const uint32_t*)recs;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(merge_sample_tables_kernel<2>, dim3(P), dim3(1024), 0, st, r, lk, seg_begin, P, myC, tables, samp, spref); break;
+        case 4: hipLaunchKernelGGL(merge_sample_tables_kernel<4>, dim3(P), dim3(1024), 0, st, r, lk, seg_begin, P, myC, tables, samp, spref); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());
@@ -1906,8 +2044,8 @@ int launch_merge_bounds(int rw, const void* recs, const LKey& lk, const int64_t*
 }
 
 int launch_merge_count(int32_t* bnd, int P, const int64_t* myC, uint32_t* hist, hipStream_t st) {
-    const int64_t n = (int64_t)P * kMergeBins;  // one wave each
-    hipLaunchKernelGGL(merge_count_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, bnd, P, myC, hist);
+    const int64_t n = (int64_t)P * kMergeBins;  // one thread each
+    hipLaunchKernelGGL(merge_count_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, bnd, P, myC, hist);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }

@@ -29,6 +29,7 @@ TIMING = 1 << 2
 LEARNED = 1 << 3
 FORCE_GENERAL = 1 << 4
 LEARNED_EXCHANGE = 1 << 5
+EXCHANGE = LEARNED_EXCHANGE  # explicit exchange before the SecondSort (either path)
 EXCHANGE_DENSE = 1 << 6
 EXCHANGE_SPARSE = 1 << 7
 FAULT_SPLITTERS = 1 << 0
@@ -93,7 +94,7 @@ class Stats(ctypes.Structure):
         ("t_model_ms", ctypes.c_double), ("t_bucket_ms", ctypes.c_double),
         ("buckets_lds_radix", ctypes.c_int32), ("buckets_overflow_first", ctypes.c_int32),
         ("buckets_listed", ctypes.c_int32), ("exchange_sparse", ctypes.c_int32),
-        ("exchange_runs", ctypes.c_int32), ("reserved5", ctypes.c_int32),
+        ("exchange_runs", ctypes.c_int32), ("second_merged", ctypes.c_int32),
     ]
 
     def as_dict(self):

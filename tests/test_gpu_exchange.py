@@ -4,7 +4,9 @@ non-empty runs.  Both give the oracle's bytes and counts; the choice follows
 the call's flags, then the recorded options (REQUIRE over DISABLE,
 mpsort-mpi.c:576-583), then the reference's AUTO vote (dense when some sender
 has more than 128 non-empty receivers, mp-mpiu.c:125-129); sparse falls back
-to dense when the non-empty runs exceed 64 per rank on average."""
+to dense when the non-empty runs exceed 64 per rank on average.  With no
+table chosen and at most 64 ranks there is no exchange at all: the SecondSort
+gathers every receiver bucket from the senders' sorted segments."""
 import sys
 
 import numpy
@@ -40,17 +42,19 @@ def nearly_sorted16(n, seed, frac=0.001):
 
 
 def expected(policy, P, send):
-    """(sparse?, runs walked) for the oracle's send matrix"""
+    """(sparse?, runs walked, merged?) for the oracle's send matrix"""
     nnz = int(numpy.count_nonzero(send))
     cap = min(P * P, 64 * P)
     peers = int((numpy.asarray(send) > 0).sum(axis=1).max())
+    if policy == "auto" and P <= 64:
+        return False, 0, True
     if policy == "dense":
         sparse = False
     elif policy == "sparse":
         sparse = nnz <= cap
     else:
         sparse = peers <= 128 and nnz <= cap
-    return sparse, (nnz if sparse else P * P)
+    return sparse, (nnz if sparse else P * P), False
 
 
 @pytest.mark.parametrize("shape", ["random", "nearly_sorted"])
@@ -62,13 +66,13 @@ def test_dense_and_sparse_tables_match_the_oracle(P, shape):
     key = oracle.key_for(a.dtype, 'key')
     ref, info = oracle.histogram_sort(a, even(n, P), even(n, P), key)
     outs = []
-    for policy, flags in (("dense", DENSE), ("sparse", SPARSE), ("auto", 0)):
+    for policy, flags in (("dense", DENSE), ("sparse", SPARSE), ("auto", 0), ("vote", _lib.EXCHANGE)):
         note("exchange P=%d shape=%s policy=%s" % (P, shape, policy))
         out, st = check_parity(a, even(n, P), even(n, P), 'key', flags=flags, device=True)
-        sp, runs = expected(policy, P, info['send'])
-        assert (st.exchange_sparse, st.exchange_runs) == (int(sp), runs), policy
+        sp, runs, merged = expected(policy, P, info['send'])
+        assert (st.exchange_sparse, st.exchange_runs, st.second_merged) == (int(sp), runs, int(merged)), policy
         outs.append(out)
-    assert numpy.array_equal(outs[0], outs[1]) and numpy.array_equal(outs[0], outs[2])
+    assert all(numpy.array_equal(outs[0], o) for o in outs[1:])
 
 
 def test_nearly_sorted_input_walks_few_runs():
@@ -77,8 +81,8 @@ def test_nearly_sorted_input_walks_few_runs():
     4096 slots"""
     n, P = 1 << 16, 64
     a = nearly_sorted16(n, 5)
-    out, st = check_parity(a, even(n, P), even(n, P), 'key', device=True)
-    assert st.exchange_sparse == 1
+    out, st = check_parity(a, even(n, P), even(n, P), 'key', flags=_lib.EXCHANGE, device=True)
+    assert st.exchange_sparse == 1 and st.second_merged == 0
     assert P <= st.exchange_runs < P * P // 8
 
 
@@ -99,14 +103,16 @@ def test_recorded_options_choose_the_table(opts, sparse):
     assert st.exchange_sparse == sparse
 
 
-@pytest.mark.parametrize("tuning, sparse", [([], 1), (['DISABLE_SPARSE_ALLTOALLV'], 0),
-                                            (['REQUIRE_SPARSE_ALLTOALLV'], 1)])
-def test_python_tuning_names(tuning, sparse):
+@pytest.mark.parametrize("tuning, sparse, merged", [([], 0, 1), (['DISABLE_SPARSE_ALLTOALLV'], 0, 0),
+                                                    (['REQUIRE_SPARSE_ALLTOALLV'], 1, 0)])
+def test_python_tuning_names(tuning, sparse, merged):
     """test_mpsort.py:85-107: the tunings sort the same; here they also pick
-    the run table"""
+    the run table (none named: no exchange, the SecondSort gathers from the
+    senders' sorted segments)"""
     g = numpy.random.default_rng(9)
     a = g.integers(-2 ** 62, 2 ** 62, 30000).astype('i8')
     want = numpy.sort(a, kind='stable')
     mpsort.sort(a, orderby=None, comm=mpsort.VirtualComm(vranks=7), tuning=tuning)
     assert numpy.array_equal(a, want)
-    assert mpsort.last_stats()['exchange_sparse'] == sparse
+    st = mpsort.last_stats()
+    assert (st['exchange_sparse'], st['second_merged']) == (sparse, merged)
