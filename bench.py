@@ -582,9 +582,23 @@ def bench_histsort(args):
                             "source": ROCPROF_SOURCE}
         return r
 
-    plain_roof = roof([x for st in st_plain for x in st["pass_ms"][:st["first_passes"] + st["second_passes"]]],
-                      "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP,
-                      ROCPROF_MS.get("onesweep"))
+    def plain_first_form(st):
+        """(FirstSort form, entries of pass_ms before the SecondSort's)"""
+        nsec = 4 if st["second_merged"] else st["second_passes"]
+        if st["first_passes"] == 0 and len(st["pass_ms"]) - nsec == 6:
+            return "buckets", 6
+        return "radix", st["first_passes"]
+
+    first_form, k0 = plain_first_form(st_plain[-1])
+    if first_form == "radix":
+        plain_roof = roof([x for st in st_plain for x in st["pass_ms"][:st["first_passes"] + st["second_passes"]]],
+                          "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP,
+                          ROCPROF_MS.get("onesweep"))
+    else:
+        plain_roof = roof([st["pass_ms"][3] for st in st_plain],
+                          "interp_sort_kernel<4,512,9> (the plain FirstSort's in-LDS bucket sort, sampled tables, "
+                          "one launch per step over all n records: 2*n*16 B per launch)", TRAFFIC_BUCKET_SORT,
+                          ROCPROF_MS.get("plain_bucket_sort"))
     if result_learned is not None:
         ib = LEARNED_KERNELS.index
         # the step's largest kernel: the FirstSort's in-LDS bucket sort, one
@@ -605,16 +619,27 @@ def bench_histsort(args):
                               last["t_exchange_ms"], last["t_second_sort_ms"], last["t_total_ms"],
                               last["first_passes"], last["second_passes"], last["bisect_rounds"]))
     log("plain pass ms: " + " ".join("%.3f" % x for x in last["pass_ms"]))
+    if first_form == "buckets":
+        plain_first = {
+            "form": "bucket sort (tables from a sorted sample of 65536 keys per rank)",
+            "stages_ms": {name: round(sum(st["pass_ms"][i] for st in st_plain) / len(st_plain), 4)
+                          for i, name in enumerate(("sample_tables", "hist", "scatter", "bucket_sort", "lds_radix",
+                                                    "fallback"))},
+            "buckets_overflow": last["buckets_overflow_first"]}
+        log("plain bucket FirstSort: %s" % json.dumps(plain_first))
+    else:
+        plain_first = {"form": "%d digit passes" % last["first_passes"]}
     plain_second = None
     if last["second_merged"]:
-        # the SecondSort without an exchange: its stages after the digit passes
-        k0 = last["first_passes"]
+        # the SecondSort without an exchange: its stages after the FirstSort's
         plain_second = {
             "form": "merged (receiver buckets gathered from the senders' sorted segments, sampled tables)",
             "stages_ms": {name: round(sum(st["pass_ms"][k0 + i] for st in st_plain) / len(st_plain), 4)
                           for i, name in enumerate(("tables_bounds_groups", "gather_sort", "listed", "fallback"))},
-            "buckets_overflow": last["buckets_overflow"], "buckets_listed": last["buckets_listed"],
-            "buckets_lds_radix": last["buckets_lds_radix"], "plain_sides": last["plain_sides"]}
+            "buckets_overflow": last["buckets_overflow"] - (last["buckets_overflow_first"]
+                                                            if first_form == "buckets" else 0),
+            "buckets_listed_both_sorts": last["buckets_listed"],
+            "buckets_lds_radix_both_sorts": last["buckets_lds_radix"], "plain_sides": last["plain_sides"]}
         log("plain merged SecondSort: %s" % json.dumps(plain_second))
     else:
         plain_second = {"form": "exchange, then %d digit passes" % last["second_passes"]}
@@ -636,7 +661,8 @@ def bench_histsort(args):
                                "clustered field, %d virtual ranks" % (args.log2n, P),
                    "records": n, "record_bytes": 16, "vranks": P, "path": path},
         "plain": {"value": round(value_plain, 2), "ms_per_step": round(ms_plain, 3),
-                  "phases_ms": {k2: round(last[k2], 4) for k2 in PHASES}, "second_sort": plain_second},
+                  "phases_ms": {k2: round(last[k2], 4) for k2 in PHASES}, "first_sort": plain_first,
+                  "second_sort": plain_second},
         "learned": result_learned,
         "host_pointer_call": host_leg,
         "learned_vs_plain": round(value / value_plain, 3) if result_learned else None,

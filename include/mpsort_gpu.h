@@ -83,6 +83,16 @@ typedef struct {
  * the same. */
 #define MPSORT_GPU_EXCHANGE_DENSE (1u << 6)
 #define MPSORT_GPU_EXCHANGE_SPARSE (1u << 7)
+/* The plain path's FirstSort (mpsort-mpi.c:369, a stable local sort of
+ * every rank): by default, ranks of at least 2^18 records (and at most
+ * 4.7 million: 1024 buckets of the in-LDS capacity's half) with keys of at
+ * most 8 bytes in 8- or 16-byte sort records, at most 256 ranks, are
+ * bucket-sorted -- 1024 buckets per rank from a sorted sample of its keys,
+ * a scatter, an in-LDS sort of every bucket; otherwise LSD radix passes.
+ * FIRST_RADIX keeps the radix passes; FIRST_BUCKETS takes the bucket sort
+ * below the size threshold too.  The result is the same. */
+#define MPSORT_GPU_FIRST_RADIX (1u << 8)
+#define MPSORT_GPU_FIRST_BUCKETS (1u << 9)
 
 /*
  * Sort the P virtual ranks of `base` (rank t holds nin[t] records, in rank

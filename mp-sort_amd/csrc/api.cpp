@@ -167,6 +167,7 @@ struct DevCtx {
     Buf ltile;          // learned scatter: per-tile bucket counts and offsets
     Buf mbnd;           // merged SecondSort: sub-run bounds per (receiver, sender, bucket)
     Buf mdesc;          // merged SecondSort: group descriptors and their sub-run tables
+    Buf fsamp;          // plain bucket FirstSort: the segments' sample keys (three buffers)
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     void* scratch = nullptr;      // this device's copy of the scratch array
@@ -260,6 +261,8 @@ struct PlanArrays {
 
 template <class A>
 static void lay_plan(A& a, int nseg, int npass, PlanArrays* pa) {
+    PlanArrays unused;  // an Arena given no target still takes the space
+    if (!pa) pa = &unused;
     if constexpr (std::is_same<A, Arena>::value) {
         pa->begin = a.template take<int64_t>(nseg);
         pa->count = a.template take<int64_t>(nseg);
@@ -295,6 +298,8 @@ struct HistArrays {
 
 template <class A>
 static void lay_hist(A& a, int P, HistArrays* h) {
+    HistArrays unused;  // an Arena given no target still takes the space
+    if (!h) h = &unused;
     const size_t P1 = (size_t)P + 1, PP = (size_t)P * P, Pm = P > 1 ? (size_t)P - 1 : 1;
     if constexpr (std::is_same<A, Arena>::value) {
         h->C = a.template take<int64_t>(P1);
@@ -375,6 +380,7 @@ struct CallShape {
     bool timing;
     bool exchange;  // keep the explicit exchange before the SecondSort
     int exch_policy;  // ExchPolicy of the exchange's run table
+    int first_mode;   // plain FirstSort: 0 by size, 1 radix passes, 2 buckets
 };
 
 static ExchRuns exch_runs(const HistArrays& h, int P, int policy) {
@@ -472,7 +478,23 @@ static bool merge_eligible(const CallShape& cs) {
     return !cs.exchange && cs.P > 1 && cs.P <= kMergeMaxPH && (cs.rw == 2 || cs.rw == 4) && cs.ks.key_bytes <= 8 &&
            cs.ntot < ((int64_t)1 << 31);
 }
-// what the plain path's merged SecondSort reports
+// The plain FirstSort is a bucket sort (tables from a sample of every
+// segment) for segments of keys up to 8 bytes in 8- or 16-byte sort records,
+// each at most half of 1024 in-LDS buckets, at most kFirstMaxP of them; by
+// default only from kFirstMinAvg records per segment on.
+constexpr int kFirstMaxP = 256;
+constexpr int64_t kFirstMinAvg = (int64_t)1 << 18;
+constexpr int kFirstSamples = 65536;  // sample keys per segment (64 per bucket)
+static bool first_buckets_eligible(const CallShape& cs, const int64_t* nin) {
+    if (cs.first_mode == 1 || cs.P > kFirstMaxP || !(cs.rw == 2 || cs.rw == 4) || cs.ks.key_bytes > 8 ||
+        cs.ntot >= ((int64_t)1 << 31))
+        return false;
+    int64_t big = 0;
+    for (int t = 0; t < cs.P; t++) big = std::max(big, nin[t]);
+    if (big > (int64_t)kLBins * kChunkCap / 2) return false;
+    return cs.first_mode == 2 || cs.ntot >= kFirstMinAvg * cs.P;
+}
+// what the plain path's merged SecondSort (and bucket FirstSort) reports
 struct MergedReport {
     uint32_t lists[3] = {0, 0, 0};  // oversized, LDS radix, 1024-thread form (read at the call's end)
     int overflow = 0;
@@ -481,6 +503,10 @@ struct MergedReport {
 static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan& planB, const HistArrays& h,
                                const PlanArrays& pa, const void* src, void* dst, void* tmp, long long* cs_out,
                                hipEvent_t* ev, MergedReport* rep, hipStream_t st);
+static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan& planA, const HistArrays& h,
+                              const int64_t* hbegin, const int64_t* hcount, const void* src, void* dst, void* tmp,
+                              long long* cs_in, long long* cs_out, hipEvent_t* ev, MergedReport* rep,
+                              hipStream_t st);
 
 // The histogram sort on device-resident data.  d_in / d_out are the caller's
 // records (device); for the general path they are packed into sort records.
@@ -582,17 +608,31 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     hipEvent_t* pevA = cs.timing ? c.pass_ev[0] : nullptr;
     hipEvent_t* pevB = cs.timing ? c.pass_ev[1] : nullptr;
     const bool merged = w.merge && merge_eligible(cs);
-    MergedReport mrep;
+    const bool first_buckets = w.merge && first_buckets_eligible(cs, hcA);
+    MergedReport mrep, frep;
     if (P == 1) {
         // one rank: FirstSort is the whole sort (the exchange is the identity
         // and SecondSort re-sorts a sorted run)
-        SortBuffers b{recs_in, final_dst, w.s1};
-        if ((rc = segmented_sort(planA, b, st, cs_direct, cs_direct, &first_passes, pevA))) return rc;
+        if (first_buckets) {
+            if ((rc = plain_bucket_first(c, cs, planA, h, hbA, hcA, recs_in, final_dst, w.s1,
+                                         cs_direct ? h.checksum : nullptr, cs_direct ? h.checksum + 1 : nullptr, pevA,
+                                         &frep, st)))
+                return rc;
+        } else {
+            SortBuffers b{recs_in, final_dst, w.s1};
+            if ((rc = segmented_sort(planA, b, st, cs_direct, cs_direct, &first_passes, pevA))) return rc;
+        }
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         for (int e = kEvFindP; e <= kEvSecond; e++) MPSG_CHECK(hipEventRecord(c.ev[e], st));
     } else {
-        SortBuffers b1{recs_in, w.s0, w.s1};
-        if ((rc = segmented_sort(planA, b1, st, cs_direct, false, &first_passes, pevA))) return rc;
+        if (first_buckets) {
+            if ((rc = plain_bucket_first(c, cs, planA, h, hbA, hcA, recs_in, w.s0, w.s1,
+                                         cs_direct ? h.checksum : nullptr, nullptr, pevA, &frep, st)))
+                return rc;
+        } else {
+            SortBuffers b1{recs_in, w.s0, w.s1};
+            if ((rc = segmented_sort(planA, b1, st, cs_direct, false, &first_passes, pevA))) return rc;
+        }
         if ((g_faults & MPSORT_GPU_FAULT_CORRUPT) && corrupt_record &&
             (rc = launch_fault_corrupt((char*)w.s0 + rw * 4 - 1, nullptr, st)))
             return rc;
@@ -663,6 +703,14 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         stats->buckets_listed = (int32_t)mrep.lists[2];
         stats->plain_sides = mrep.global_fallback;
     }
+    if (first_buckets) {
+        stats->buckets += P * kLBins;
+        stats->buckets_overflow += frep.overflow;
+        stats->buckets_overflow_first = frep.overflow;
+        stats->buckets_lds_radix += (int32_t)frep.lists[1];
+        stats->buckets_listed += (int32_t)frep.lists[2];
+        stats->plain_sides += frep.global_fallback;
+    }
     stats->record_bytes = (int32_t)recbytes;
     stats->first_passes = first_passes;
     stats->second_passes = second_passes;
@@ -681,6 +729,13 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     if (cs.timing) {
         for (int i = 0; i < first_passes && np < 64; i++)
             stats->pass_ms[np++] = ev_ms(c.pass_ev[0][2 * i], c.pass_ev[0][2 * i + 1]);
+        if (first_buckets) {
+            // sample and tables; bucket histogram, scatter, in-LDS bucket
+            // sort, LDS radix sort of crowded buckets, overflow fallback
+            hipEvent_t* e = c.pass_ev[0];
+            stats->pass_ms[np++] = ev_ms(e[8], e[0]);
+            for (int k = 0; k < 5; k++) stats->pass_ms[np++] = ev_ms(e[k], e[k + 1]);
+        }
         for (int i = 0; i < second_passes && np < 64; i++)
             stats->pass_ms[np++] = ev_ms(c.pass_ev[1][2 * i], c.pass_ev[1][2 * i + 1]);
         if (P > 1 && merged) {
@@ -785,6 +840,7 @@ static int shape_for(const mpsort_key_desc* key, size_t elsize, unsigned flags, 
     else cs->exch_policy = kExchAuto;
     // a run table chosen by the caller means an exchange
     if (cs->exch_policy != kExchAuto) cs->exchange = true;
+    cs->first_mode = (flags & MPSORT_GPU_FIRST_RADIX) ? 1 : ((flags & MPSORT_GPU_FIRST_BUCKETS) ? 2 : 0);
     return 0;
 }
 
@@ -887,6 +943,8 @@ struct LearnedSide {
 // sized for the larger of the two bucket counts (kLBins, kMergeBins per segment)
 template <class A>
 static void lay_lside(A& a, int P, LearnedSide* ls) {
+    LearnedSide unused;  // an Arena given no target still takes the space
+    if (!ls) ls = &unused;
     if constexpr (std::is_same<A, Arena>::value) {
         ls->begin = a.template take<int64_t>(P);
         ls->count = a.template take<int64_t>(P);
@@ -1226,7 +1284,10 @@ static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan
                                hipEvent_t* ev, MergedReport* rep, hipStream_t st) {
     const int P = cs.P, rw = cs.rw;
     int rc = 0;
+    // the first of the two learned-side layouts (the bucket FirstSort uses
+    // the second)
     Sizer lz;
+    lay_lside(lz, P, nullptr);
     lay_lside(lz, P, nullptr);
     if ((rc = c.lsmall.ensure(lz.off + 4096))) return rc;
     Arena la{(char*)c.lsmall.p};
@@ -1251,6 +1312,89 @@ static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan
                                  st)))
         return rc;
     MPSG_CHECK(hipMemcpyAsync(rep->lists, LB.noverflow, sizeof rep->lists, hipMemcpyDeviceToHost, st));
+    rep->overflow = lc.overflow_buckets;
+    rep->global_fallback = lc.global_fallback;
+    return 0;
+}
+
+// The plain path's bucket FirstSort: kFirstSamples keys of every segment at
+// evenly spaced positions, sorted (segmented radix sort of the sample), give
+// each segment a table of kLBounds boundaries; then the learned path's local
+// sort (bucket histogram, scatter, in-LDS bucket sorts, fallbacks) from src
+// into dst.  hbegin / hcount: the segments on the host.
+static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan& planA, const HistArrays& h,
+                              const int64_t* hbegin, const int64_t* hcount, const void* src, void* dst, void* tmp,
+                              long long* cs_in, long long* cs_out, hipEvent_t* ev, MergedReport* rep,
+                              hipStream_t st) {
+    const int P = cs.P, rw = cs.rw, S = kFirstSamples;
+    int rc = 0;
+    Sizer lz;
+    lay_lside(lz, P, nullptr);
+    lay_lside(lz, P, nullptr);
+    if ((rc = c.lsmall.ensure(lz.off + 4096))) return rc;
+    Arena la{(char*)c.lsmall.p};
+    LearnedSide LM, LA;
+    lay_lside(la, P, &LM);  // the merged SecondSort's
+    lay_lside(la, P, &LA);
+    LA.tstride = kLBounds;
+    MPSG_CHECK(hipMemcpyAsync(LA.begin, hbegin, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipMemcpyAsync(LA.count, hcount, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+    // the sample (three buffers of the segmented radix sort) and its plan
+    const size_t sbytes = ((size_t)P * S * sizeof(uint64_t) + 255) & ~(size_t)255;
+    if ((rc = c.fsamp.ensure(3 * sbytes))) return rc;
+    uint64_t* s0 = (uint64_t*)c.fsamp.p;
+    uint64_t* s1 = (uint64_t*)((char*)c.fsamp.p + sbytes);
+    uint64_t* s2 = (uint64_t*)((char*)c.fsamp.p + 2 * sbytes);
+    KeySpec sk;
+    fill_keyspec(&sk, 2, 0, 8, 8, 0);
+    std::vector<int64_t> sc(P, S), sb(P), st_(P + 1);
+    const TileCfg stile = default_tile(2);
+    const int64_t snt = make_tiles(sc.data(), P, (int64_t)stile.block * stile.items, sb.data(), st_.data());
+    if ((size_t)snt * kBins * sizeof(uint64_t) > c.lookback.bytes)
+        return fail(MPSORT_ERR_ARG, "look-back buffer too small for the FirstSort sample");
+    Sizer pz;
+    lay_plan(pz, P, 8, nullptr);
+    if ((rc = c.ovf.ensure(pz.off + 4096))) return rc;
+    Arena pr{(char*)c.ovf.p};
+    PlanArrays spa;
+    lay_plan(pr, P, 8, &spa);
+    MPSG_CHECK(hipMemcpyAsync(spa.begin, sb.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipMemcpyAsync(spa.count, sc.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipMemcpyAsync(spa.tile_start, st_.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+    if ((rc = launch_fill_u32(spa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
+    SegSortPlan sp = planA;
+    sp.rw = 2;
+    sp.tile = stile;
+    sp.ks = sk;
+    sp.nrec = (int64_t)P * S;
+    sp.nseg = P;
+    sp.ntiles = snt;
+    sp.d_begin = spa.begin;
+    sp.d_count = spa.count;
+    sp.d_tile_start = spa.tile_start;
+    sp.d_hist = spa.hist;
+    sp.d_digit_base = spa.digit_base;
+    sp.d_pass_flags = spa.pass_flags;
+    sp.d_tickets = spa.tickets;
+    sp.sparse = false;
+    sp.all_passes = false;
+    sp.d_epoch_base = nullptr;
+    const LKey lk = make_lkey(cs.ks);
+    if (ev) MPSG_CHECK(hipEventRecord(ev[8], st));
+    if ((rc = launch_seg_sample(rw, src, lk, LA.begin, LA.count, P, S, s0, st))) return rc;
+    int passes = 0;
+    SortBuffers sbuf{s0, s1, s2};
+    if ((rc = segmented_sort(sp, sbuf, st, false, false, &passes, nullptr))) return rc;
+    uint32_t flag = 0;
+    MPSG_CHECK(hipMemcpyAsync(&flag, spa.pass_flags + kMaxPasses, sizeof flag, hipMemcpyDeviceToHost, st));
+    // no pass ran: every sample key is equal and the sample stayed in s0
+    if ((rc = launch_sample_tables(passes ? s1 : s0, P, S, LA.tables, st))) return rc;
+    MPSG_CHECK(hipStreamSynchronize(st));
+    if (flag) return fail(MPSORT_ERR_TIMEOUT, "FirstSort sample sort: device flag 0x%x", flag);
+    LearnedCounters lc;
+    if ((rc = learned_local_sort(c, rw, cs.ks, P, cs.ntot, LA, planA, src, dst, tmp, cs_in, cs_out, &lc, st, ev)))
+        return rc;
+    MPSG_CHECK(hipMemcpyAsync(rep->lists, LA.noverflow, sizeof rep->lists, hipMemcpyDeviceToHost, st));
     rep->overflow = lc.overflow_buckets;
     rep->global_fallback = lc.global_fallback;
     return 0;

@@ -192,4 +192,12 @@ int launch_one_value_filter(int rw, const void* src, void* dst, const LKey& lk, 
                             int64_t max_count, long long* checksum, hipStream_t st);
 constexpr int kMergeMaxPH = 64;  // largest P of the merged SecondSort
 
+// FirstSort tables without a model: S (a multiple of kLBins) keys of every
+// segment at evenly spaced positions into out (P * S; the caller sorts each
+// segment's S keys), then kLBounds entries per segment taken from the
+// sorted sample (sample-sort tables, same form as the model's)
+int launch_seg_sample(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
+                      int P, int S, uint64_t* out, hipStream_t st);
+int launch_sample_tables(const uint64_t* sorted, int P, int S, uint64_t* tables, hipStream_t st);
+
 }  // namespace mpsg

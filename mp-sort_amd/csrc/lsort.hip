@@ -1444,6 +1444,43 @@ __global__ __launch_bounds__(1024) void merge_sample_tables_kernel(const uint32_
     }
 }
 
+// FirstSort tables without a model: S keys of every segment at evenly
+// spaced positions (out[j * S + u] = the key of record floor((2u + 1) cnt_j
+// / 2S) of segment j; all ones for an empty segment), sorted by the caller;
+// then entry e of segment j's table is sorted sample (e + 1) S / kLBins - 1,
+// and the first of a run of equal entries v becomes v - 1, as in the
+// receiver tables (bucket (v - 1, v] holds that key alone).
+template <int RW>
+__global__ __launch_bounds__(256) void seg_sample_kernel(const uint32_t* __restrict__ recs, LKey lk,
+                                                         const int64_t* __restrict__ seg_begin,
+                                                         const int64_t* __restrict__ seg_count, int P, int S,
+                                                         uint64_t* __restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)P * S) return;
+    const int j = (int)(g / S);
+    const int64_t u = g % S, cnt = seg_count[j];
+    uint64_t key = ~0ull;
+    if (cnt > 0) {
+        uint32_t rr[RW];
+        RecIO<RW>::load(recs, seg_begin[j] + ((2 * u + 1) * cnt) / (2 * (int64_t)S), rr);
+        key = lkey<RW>(rr, lk);
+    }
+    out[g] = key;
+}
+
+__global__ __launch_bounds__(256) void sample_tables_kernel(const uint64_t* __restrict__ sorted, int P, int S,
+                                                            uint64_t* __restrict__ tables) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (j, e)
+    if (g >= (int64_t)P * kLBounds) return;
+    const int j = (int)(g / kLBounds), e = (int)(g % kLBounds);
+    const int step = S / kLBins;
+    const uint64_t* sj = sorted + (int64_t)j * S;
+    const uint64_t v = sj[(e + 1) * step - 1];
+    const bool first = e == 0 || sj[e * step - 1] < v;
+    const bool repeated = e + 1 < kLBounds && sj[(e + 2) * step - 1] == v;
+    tables[g] = first && repeated && v > 0 ? v - 1 : v;
+}
+
 // The bounds clamped to receiver r's window [myC[j][r], myC[j][r+1]) in
 // every sender j (written back: entry k by the thread of (r, k), the last
 // entry also by the thread of (r, kMergeBins - 1); an entry read while its
@@ -2039,6 +2076,28 @@ int launch_merge_sample_tables(int rw, const void* recs, const LKey& lk, const i
         case 4: hipLaunchKernelGGL(merge_sample_tables_kernel<4>, dim3(P), dim3(1024), 0, st, r, lk, seg_begin, P, myC, tables, samp, spref); break;
         default: return -1;
     }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_seg_sample(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, const int64_t* seg_count,
+                      int P, int S, uint64_t* out, hipStream_t st) {
+    const int64_t n = (int64_t)P * S;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    const uint32_t* r = (const uint32_t*)recs;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(seg_sample_kernel<2>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, S, out); break;
+        case 4: hipLaunchKernelGGL(seg_sample_kernel<4>, dim3(g), dim3(256), 0, st, r, lk, seg_begin, seg_count, P, S, out); break;
+        default: return -1;
+    }
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_sample_tables(const uint64_t* sorted, int P, int S, uint64_t* tables, hipStream_t st) {
+    if (S < kLBins || S % kLBins) return -1;
+    const int64_t n = (int64_t)P * kLBounds;
+    hipLaunchKernelGGL(sample_tables_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sorted, P, S, tables);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }

@@ -32,6 +32,8 @@ LEARNED_EXCHANGE = 1 << 5
 EXCHANGE = LEARNED_EXCHANGE  # explicit exchange before the SecondSort (either path)
 EXCHANGE_DENSE = 1 << 6
 EXCHANGE_SPARSE = 1 << 7
+FIRST_RADIX = 1 << 8
+FIRST_BUCKETS = 1 << 9
 FAULT_SPLITTERS = 1 << 0
 FAULT_BUCKET_CAP = 1 << 1
 FAULT_PLAIN = 1 << 2

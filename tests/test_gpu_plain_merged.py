@@ -159,3 +159,84 @@ def test_forced_plain_fallback():
     _lib.check(lib.mpsort_gpu_set_faults(_lib.FAULT_PLAIN, 0))
     out, st = check_parity(a, even(n, P), even(n, P), 'key', device=True)
     assert st.second_merged == 1 and st.plain_sides == 1
+
+
+# The plain FirstSort as a bucket sort: 1024 buckets per rank from a sorted
+# sample of 65536 of its keys, then the in-LDS bucket sorts (mpsort-mpi.c:369
+# asks only for a stable local sort).  FIRST_BUCKETS takes it below its size
+# threshold, FIRST_RADIX keeps the digit passes.
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("P", [1, 4, 64])
+def test_bucket_first_sort_matches_the_oracle(shape, P):
+    n = 90000 + 77 * P
+    a = workloads.records16(keys(shape, n, 31 + P))
+    note("plain bucket FirstSort shape=%s P=%d n=%d" % (shape, P, n))
+    out, st = check_parity(a, even(n, P), even(n, P), 'key', flags=_lib.FIRST_BUCKETS, device=True)
+    assert st.first_passes == 0
+    out2, st2 = check_parity(a, even(n, P), even(n, P), 'key', flags=_lib.FIRST_RADIX, device=True)
+    assert st2.first_passes > 0 or len(numpy.unique(a['key'])) == 1
+    assert numpy.array_equal(out, out2)
+
+
+@pytest.mark.parametrize("kind", ["u8", "i8", "u4_pairs"])
+def test_bucket_first_sort_eight_byte_records(kind):
+    n, P = 70000, 4
+    g = numpy.random.default_rng(13)
+    if kind == "u8":
+        a = g.integers(0, 2 ** 64, n, dtype=numpy.uint64)
+        field = None
+    elif kind == "i8":
+        a = g.integers(-2 ** 63, 2 ** 63, n, dtype=numpy.int64)
+        field = None
+    else:
+        a = numpy.zeros(n, [('key', 'u4'), ('v', 'u4')])
+        a['key'] = g.integers(0, 1000, n)
+        a['v'] = numpy.arange(n)
+        field = 'key'
+    note("plain bucket FirstSort 8-byte records kind=%s" % kind)
+    key = oracle.key_for(a.dtype, field)
+    ref, info = oracle.histogram_sort(a, even(n, P), even(n, P), key)
+    out, st = gpu_sort(a, even(n, P), even(n, P), field, flags=_lib.FIRST_BUCKETS)
+    assert numpy.array_equal(out.view(numpy.uint8), ref.view(numpy.uint8))
+    assert st.first_passes == 0
+    assert st.checksum_in == info['checksum_in'] == st.checksum_out
+
+
+def test_bucket_first_sort_general_path_inplace():
+    """packed records sorted in place (one rank: the FirstSort is the sort)"""
+    n = 80000
+    g = numpy.random.default_rng(6)
+    a = numpy.zeros(n, [('x', 'f8'), ('key', 'i8'), ('w', 'u8')])
+    a['key'] = g.integers(-500, 500, n)
+    a['w'] = numpy.arange(n)
+    for P in (1, 3):
+        out, st = check_parity(a, even(n, P), even(n, P), 'key', flags=_lib.FIRST_BUCKETS)
+        assert st.general_path == 1 and st.first_passes == 0
+
+
+def test_bucket_first_sort_by_size():
+    """2^21 clustered records on 4 ranks (2^19 each): the bucket FirstSort
+    without being asked, the merged SecondSort after it"""
+    n, P = 1 << 21, 4
+    a = workloads.clustered_records(n, seed=6)
+    out, st = check_parity(a, even(n, P), even(n, P), 'key', device=True)
+    assert st.first_passes == 0 and st.second_merged == 1 and st.plain_sides == 0
+
+
+@pytest.mark.parametrize("faults, cap", [(_lib.FAULT_BUCKET_CAP, 48), (_lib.FAULT_BUCKET_CAP, 1),
+                                         (_lib.FAULT_PLAIN, 0)])
+def test_bucket_first_sort_fallbacks(faults, cap):
+    """FirstSort buckets above a forced capacity (88 records on average):
+    the bucket fallback, or past its limit the radix sort of the whole side;
+    and the forced radix sort of both sides"""
+    n, P = 1 << 18, 3
+    a = workloads.records16(keys("random", n, 17))
+    _lib.check(lib.mpsort_gpu_set_faults(faults, cap))
+    note("plain bucket FirstSort faults=%d cap=%d" % (faults, cap))
+    out, st = check_parity(a, even(n, P), even(n, P), 'key', flags=_lib.FIRST_BUCKETS, device=True)
+    if faults == _lib.FAULT_PLAIN:
+        assert st.plain_sides == 2
+    else:
+        assert st.buckets_overflow_first > 0
+        assert st.plain_sides >= (1 if cap == 1 else 0)
