@@ -494,6 +494,17 @@ def bench_histsort(args):
     # ---- plain path ------------------------------------------------------
     ms_plain, st_plain, _ = _timed(plain_step, "plain", args.steps, args.warmup)
     log("plain: %.3f ms per step, %.1f Mrecords/s" % (ms_plain, n / (ms_plain * 1e-3) / 1e6))
+
+    # the reference's shape on the GPU: radix FirstSort, exchange, radix
+    # SecondSort (fewer steps: it is a reported figure, not `value`)
+    rflags = flags | _lib.FIRST_RADIX | _lib.EXCHANGE
+
+    def radix_step():
+        _lib.check(lib.mpsort_gpu_newarray(din, nin, P, dout, nin, 16, ctypes.byref(key), rflags))
+
+    rsteps = max(3, args.steps // 4)
+    ms_radix, st_radix, _ = _timed(radix_step, "plain radix", rsteps, min(args.warmup, 2))
+    log("plain, reference shape (radix passes + exchange): %.3f ms per step" % ms_radix)
     if args.copy_probe:
         _lib.check(lib.mpsort_gpu_dcopy(dout, din, nb))
         log("copy probe: copy_kernel<4> moved %d bytes in and %d bytes out" % (nb, nb))
@@ -589,6 +600,9 @@ def bench_histsort(args):
             return "buckets", 6
         return "radix", st["first_passes"]
 
+    radix_roof = roof([x for st in st_radix for x in st["pass_ms"][:st["first_passes"] + st["second_passes"]]],
+                      "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP,
+                      ROCPROF_MS.get("onesweep"))
     first_form, k0 = plain_first_form(st_plain[-1])
     if first_form == "radix":
         plain_roof = roof([x for st in st_plain for x in st["pass_ms"][:st["first_passes"] + st["second_passes"]]],
@@ -669,6 +683,12 @@ def bench_histsort(args):
         "sort_equivalent_GBps": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
         "roofline": roofline,
         "plain_roofline": plain_roof,
+        "plain_radix": {"ms_per_step": round(ms_radix, 3), "steps": rsteps,
+                        "value": round(n / (ms_radix * 1e-3) / 1e6, 2),
+                        "form": "radix FirstSort (%d digit passes), exchange, radix SecondSort (%d passes)" % (
+                            st_radix[-1]["first_passes"], st_radix[-1]["second_passes"]),
+                        "phases_ms": {k2: round(st_radix[-1][k2], 4) for k2 in PHASES},
+                        "roofline": radix_roof},
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

@@ -2119,6 +2119,15 @@ int mpsort_gpu_newarray(const void* base, const int64_t* nin, int P, void* out, 
     const bool devptr = (flags & MPSORT_GPU_DEVICE_PTRS) != 0;
     rc = shape_for(key, elsize, flags, devptr ? base : nullptr, devptr ? out : nullptr, ntot, &cs);
     if (rc) return rc;
+    if (!cs.general && cs.rw == 8 && cs.ks.key_bytes <= 8 && !cs.exchange && cs.first_mode != 1 &&
+        ntot >= kFirstMinAvg * P) {
+        // 32-byte records of large ranks: sorted as packed {key, index}
+        // records, so that the bucket FirstSort and the merged SecondSort
+        // take them (a quarter of the radix passes' traffic, gather included)
+        if ((rc = shape_for(key, elsize, flags | MPSORT_GPU_FORCE_GENERAL, devptr ? base : nullptr,
+                            devptr ? out : nullptr, ntot, &cs)))
+            return rc;
+    }
     cs.P = P;
     cs.ntot = ntot;
     if (ntot > 0 && (!base || !out)) return fail(MPSORT_ERR_ARG, "NULL data pointer");

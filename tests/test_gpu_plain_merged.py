@@ -240,3 +240,19 @@ def test_bucket_first_sort_fallbacks(faults, cap):
     else:
         assert st.buckets_overflow_first > 0
         assert st.plain_sides >= (1 if cap == 1 else 0)
+
+
+def test_large_32_byte_records_are_packed():
+    """32-byte particles in ranks of 2^18 records: sorted as packed {key,
+    index} records (bucket FirstSort, merged SecondSort), then gathered"""
+    n, P = 1 << 20, 4
+    g = numpy.random.default_rng(21)
+    a = numpy.zeros(n, [('id', 'u8'), ('pos', 'f4', 3), ('vel', 'f4', 3)])
+    a['id'] = g.permutation(n).astype(numpy.uint64)
+    a['id'][g.random(n) < 0.01] = 7  # a heavy id as well
+    a['pos'] = g.random((n, 3))
+    a['vel'] = g.random((n, 3))
+    out, st = check_parity(a, even(n, P), even(n, P), 'id', device=True)
+    assert st.general_path == 1 and st.first_passes == 0 and st.second_merged == 1
+    out2, st2 = check_parity(a, even(n, P), even(n, P), 'id', flags=_lib.FIRST_RADIX, device=True)
+    assert st2.general_path == 0 and numpy.array_equal(out, out2)
