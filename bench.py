@@ -32,13 +32,14 @@ PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_
 # HBM bytes per launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # each scaled by the counter's reading for a plain copy kernel of known size
 # (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
-TRAFFIC_ONESWEEP = 9117953756
-TRAFFIC_BUCKET_SORT = 9445766080
+TRAFFIC_ONESWEEP = 9120342587
+TRAFFIC_BUCKET_SORT = 11557167104
+TRAFFIC_PLAIN_BUCKET_SORT = 7983348736
 # average launch time of the same kernels over the same launches (the timed
 # steps of `python bench.py --steps 20 --warmup 5`) from its rocprofv3 kernel
-# trace; filled from the committed trace summary
+# trace (profiles/histsort_clustered_2e28x16B_kernel_trace.md)
 ROCPROF_SOURCE = "profiles/histsort_clustered_2e28x16B_kernel_trace.md"
-ROCPROF_MS = {}
+ROCPROF_MS = {"bucket_sort_A": 2.9333, "plain_bucket_sort": 2.4582, "onesweep": 2.3693}
 # per-side stages of the learned local sort, as timed by the library; with
 # the merged SecondSort also its two gather-and-sort launches alone
 # (merge_B: groups of buckets, then the larger single buckets), which
@@ -588,8 +589,12 @@ def bench_histsort(args):
              "timing": "HIP events around each launch on the library's stream, the timed steps only "
                        "(warmup launches, the cold first one included, are left out)"}
         if rocprof_ms:
+            # the committed rocprofv3 trace of the driver's command, averaged
+            # over the same launches (timed steps only, warmups left out)
             r["rocprof"] = {"avg_launch_ms": rocprof_ms, "frac": round(launch_bytes / (rocprof_ms * 1e-3) / 1e9 /
                                                                         HBM_PEAK_GBS, 4),
+                            "launches": "the timed steps' launches of `rocprofv3 --kernel-trace -- python3 "
+                                        "bench.py --gpus 1 --steps 20 --warmup 5 --no-cpu`",
                             "source": ROCPROF_SOURCE}
         return r
 
@@ -611,7 +616,7 @@ def bench_histsort(args):
     else:
         plain_roof = roof([st["pass_ms"][3] for st in st_plain],
                           "interp_sort_kernel<4,512,9> (the plain FirstSort's in-LDS bucket sort, sampled tables, "
-                          "one launch per step over all n records: 2*n*16 B per launch)", TRAFFIC_BUCKET_SORT,
+                          "one launch per step over all n records: 2*n*16 B per launch)", TRAFFIC_PLAIN_BUCKET_SORT,
                           ROCPROF_MS.get("plain_bucket_sort"))
     if result_learned is not None:
         ib = LEARNED_KERNELS.index
