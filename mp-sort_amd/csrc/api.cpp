@@ -1571,6 +1571,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.heavy_q, M.nheavy, st, nullptr,
                                           true)))
                 return rc;
+            // the edge entries are placed once the windows are known
+            if ((rc = launch_tables_reserve_edges(LB.tables, P, st))) return rc;
             eo.keys = LB.tables;
             eo.range = (uint32_t*)((char*)c.mbnd.p + bnd_bytes);
             eo.seg_begin = pa.begin;
@@ -1612,6 +1614,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             (rc = launch_model_tables(md, M.kmax, LB.q0, LB.q1, P, nbB, LB.tables, M.heavy, M.heavy_q, M.nheavy, st)))
             return rc;
         if (merged) {
+            if ((rc = launch_merge_edges(rw, c.s0.p, lkB, pa.begin, P, h.myC, LB.tables, bnd, st))) return rc;
             if ((rc = merged_second_sort(c, rw, lkB, P, n, LB, planB, h, pa.begin, bnd, c.s0.p, final_dst, c.s1.p,
                                          cs_direct ? h.checksum + 1 : nullptr, lc, evB, st)))
                 return rc;

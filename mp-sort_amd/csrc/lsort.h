@@ -192,6 +192,18 @@ int launch_one_value_filter(int rw, const void* src, void* dst, const LKey& lk, 
                             int64_t max_count, long long* checksum, hipStream_t st);
 constexpr int kMergeMaxPH = 64;  // largest P of the merged SecondSort
 
+// Receiver tables made before the splitters (the model's): kEdgeReserve
+// entries at each end repeat their neighbours until, with the receivers'
+// windows known, launch_merge_edges spreads them evenly by value over the
+// receiver's keys at or below its first and above its last other entry and
+// writes their bounds
+// (the model's receiver edges miss the splitters by its error; without
+// them those keys would all land in the first or the last bucket)
+constexpr int kEdgeReserve = 16;
+int launch_tables_reserve_edges(uint64_t* tables, int P, hipStream_t st);
+int launch_merge_edges(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, int P, const int64_t* myC,
+                       uint64_t* tables, int32_t* bnd, hipStream_t st);
+
 // FirstSort tables without a model: S (a multiple of kLBins) keys of every
 // segment at evenly spaced positions into out (P * S; the caller sorts each
 // segment's S keys), then kLBounds entries per segment taken from the

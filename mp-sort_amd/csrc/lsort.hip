@@ -1481,6 +1481,115 @@ __global__ __launch_bounds__(256) void sample_tables_kernel(const uint64_t* __re
     tables[g] = first && repeated && v > 0 ? v - 1 : v;
 }
 
+// Receiver tables made before the splitters are known (from the model)
+// keep kEdgeReserve entries at each end free: they repeat the first and last
+// of the others (empty buckets) until merge_edges_kernel gives them keys.
+__global__ __launch_bounds__(256) void tables_reserve_edges_kernel(uint64_t* __restrict__ tables, int P) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;  // (r, i), i < 2 kEdgeReserve
+    if (g >= P * 2 * kEdgeReserve) return;
+    const int r = g / (2 * kEdgeReserve), i = g % (2 * kEdgeReserve);
+    uint64_t* t = tables + (int64_t)r * kMergeBounds;
+    if (i < kEdgeReserve) t[i] = t[kEdgeReserve];
+    else t[kMergeBounds - 2 * kEdgeReserve + i] = t[kMergeBounds - kEdgeReserve - 1];
+}
+
+// Once the receivers' windows are known: the keys of receiver r at or
+// below its first table entry t[E] (E = kEdgeReserve) or above its last
+// t[nb - E - 1] (the model placed the receiver's edge off its splitter)
+// would all fall in bucket 0 or the last bucket.  The reserved entries
+// split those key ranges evenly by value: entry i < E = lo + (t[E] - lo)
+// (i + 1) / (E + 1), lo the receiver's smallest key (the least first key of
+// its windows), and likewise from t[nb - E - 1] up to its largest key (the
+// ranges are thin slices of the distribution, where the keys lie about
+// evenly; a sample of them, one or two keys per sender, is not); then each
+// one's position in every sender's window by binary search (bnd index i +
+// 1).  One 1024-thread workgroup per receiver: thread (entry, sender).
+template <int RW>
+__global__ __launch_bounds__(1024) void merge_edges_kernel(const uint32_t* __restrict__ recs, LKey lk,
+                                                           const int64_t* __restrict__ seg_begin, int P,
+                                                           const int64_t* __restrict__ myC,
+                                                           uint64_t* __restrict__ tables,
+                                                           int32_t* __restrict__ bnd) {
+    constexpr int E = kEdgeReserve;
+    __shared__ uint64_t s_lo, s_hi;
+    __shared__ uint64_t s_e[2 * E];
+    const int r = blockIdx.x, tid = threadIdx.x;
+    uint64_t* t = tables + (int64_t)r * kMergeBounds;
+    if (tid < 64) {
+        uint64_t lo = ~0ull, hi = 0;
+        if (tid < P) {
+            const int64_t* cj = myC + (int64_t)tid * (P + 1);
+            if (cj[r + 1] > cj[r]) {
+                uint32_t rr[RW];
+                RecIO<RW>::load(recs, seg_begin[tid] + cj[r], rr);
+                lo = lkey<RW>(rr, lk);
+                RecIO<RW>::load(recs, seg_begin[tid] + cj[r + 1] - 1, rr);
+                hi = lkey<RW>(rr, lk);
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t x = __shfl_xor(lo, off, 64), y = __shfl_xor(hi, off, 64);
+            lo = x < lo ? x : lo;
+            hi = y > hi ? y : hi;
+        }
+        if (tid == 0) {
+            s_lo = lo;
+            s_hi = hi;
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * E) {
+        const uint64_t lo = s_lo, hi = s_hi;
+        const uint64_t a = t[E], b = t[kMergeBounds - E - 1];
+        const int i = tid < E ? tid : tid - E;
+        const double f = (double)(i + 1) / (double)(E + 1);
+        uint64_t e;
+        if (tid < E) {
+            // [lo, a] split (a itself when no key lies below it)
+            e = lo < a ? lo + (uint64_t)((double)(a - lo) * f) : a;
+            e = e > a ? a : e;
+        } else {
+            e = hi > b ? b + (uint64_t)((double)(hi - b) * f) : b;
+            e = e < b ? b : e;
+        }
+        s_e[tid] = e;
+    }
+    __syncthreads();
+    // non-decreasing (the rounding through double may not keep the order)
+    uint64_t ev = 0;
+    if (tid < 2 * E) {
+        ev = s_e[tid];
+        if (tid < E) {
+            for (int q = tid + 1; q < E; q++) ev = s_e[q] < ev ? s_e[q] : ev;
+        } else {
+            for (int q = E; q < tid; q++) ev = s_e[q] > ev ? s_e[q] : ev;
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * E) {
+        t[tid < E ? tid : kMergeBounds - 2 * E + tid] = ev;
+        s_e[tid] = ev;
+    }
+    __syncthreads();
+    // positions: thread (entry q of 2E, sender j)
+    for (int w = tid; w < 2 * E * P; w += 1024) {
+        const int q = w / P, j = w % P;
+        const int k = (q < E ? q : kMergeBounds - 2 * E + q) + 1;
+        const uint64_t key = s_e[q];
+        const int64_t* cj = myC + (int64_t)j * (P + 1);
+        int64_t lo = cj[r], hi = cj[r + 1];
+        const int64_t sb = seg_begin[j];
+        while (lo < hi) {
+            const int64_t m = lo + ((hi - lo) >> 1);
+            uint32_t rr[RW];
+            RecIO<RW>::load(recs, sb + m, rr);
+            if (lkey<RW>(rr, lk) <= key) lo = m + 1; else hi = m;
+        }
+        bnd[merge_bnd_at(r, j, k, P)] = (int32_t)lo;
+    }
+}
+
 // The bounds clamped to receiver r's window [myC[j][r], myC[j][r+1]) in
 // every sender j (written back: entry k by the thread of (r, k), the last
 // entry also by the thread of (r, kMergeBins - 1); an entry read while its
@@ -2098,6 +2207,26 @@ int launch_sample_tables(const uint64_t* sorted, int P, int S, uint64_t* tables,
     if (S < kLBins || S % kLBins) return -1;
     const int64_t n = (int64_t)P * kLBounds;
     hipLaunchKernelGGL(sample_tables_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sorted, P, S, tables);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_tables_reserve_edges(uint64_t* tables, int P, hipStream_t st) {
+    const int n = P * 2 * kEdgeReserve;
+    hipLaunchKernelGGL(tables_reserve_edges_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tables, P);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_merge_edges(int rw, const void* recs, const LKey& lk, const int64_t* seg_begin, int P, const int64_t* myC,
+                       uint64_t* tables, int32_t* bnd, hipStream_t st) {
+    if (P < 1 || P > kMergeMaxP) return -1;
+    const uint32_t* r = (const uint32_t*)recs;
+    switch (rw) {
+        case 2: hipLaunchKernelGGL(merge_edges_kernel<2>, dim3(P), dim3(1024), 0, st, r, lk, seg_begin, P, myC, tables, bnd); break;
+        case 4: hipLaunchKernelGGL(merge_edges_kernel<4>, dim3(P), dim3(1024), 0, st, r, lk, seg_begin, P, myC, tables, bnd); break;
+        default: return -1;
+    }
     MPSG_CHECK(hipGetLastError());
     return 0;
 }

@@ -112,7 +112,12 @@ def test_python_tuning_names(tuning, sparse, merged):
     g = numpy.random.default_rng(9)
     a = g.integers(-2 ** 62, 2 ** 62, 30000).astype('i8')
     want = numpy.sort(a, kind='stable')
-    mpsort.sort(a, orderby=None, comm=mpsort.VirtualComm(vranks=7), tuning=tuning)
+    try:
+        mpsort.sort(a, orderby=None, comm=mpsort.VirtualComm(vranks=7), tuning=tuning)
+        st = mpsort.last_stats()
+    finally:
+        # the tuning names stay recorded after the call, as in the reference
+        # binding (binding.pyx:195-204): clear them for the later tests
+        lib.mpsort_mpi_unset_options(-1)
     assert numpy.array_equal(a, want)
-    st = mpsort.last_stats()
     assert (st['exchange_sparse'], st['second_merged']) == (sparse, merged)

@@ -27,7 +27,9 @@ def note(*a):
 
 
 @pytest.fixture(autouse=True)
-def no_faults_after():
+def no_faults_or_options():
+    # recorded exchange options would ask for an exchange
+    lib.mpsort_mpi_unset_options(-1)
     yield
     _lib.check(lib.mpsort_gpu_set_faults(0, 0))
 
@@ -256,3 +258,31 @@ def test_large_32_byte_records_are_packed():
     assert st.general_path == 1 and st.first_passes == 0 and st.second_merged == 1
     out2, st2 = check_parity(a, even(n, P), even(n, P), 'id', flags=_lib.FIRST_RADIX, device=True)
     assert st2.general_path == 0 and numpy.array_equal(out, out2)
+
+
+@pytest.mark.parametrize("P", [1, 4])
+@pytest.mark.parametrize("rec", ["rec16", "particle32"])
+def test_bucket_first_sort_device_in_place(P, rec):
+    """base == out in device memory: the bucket FirstSort reads the input
+    before anything writes it (16-byte records), and the packed 32-byte path
+    copies the input aside before gathering"""
+    from test_gpu_histsort import _device_inplace
+    n = 1 << 20 if rec == "particle32" else 120000
+    g = numpy.random.default_rng(40 + P)
+    if rec == "rec16":
+        a = workloads.records16(keys("heavy", n, 40 + P))
+        field = 'key'
+        flags = _lib.FIRST_BUCKETS
+    else:
+        a = numpy.zeros(n, workloads.PARTICLE32)
+        a['id'] = g.permutation(n).astype(numpy.uint64)
+        a['pos'] = g.random((n, 3), dtype=numpy.float32)
+        field = 'id'
+        flags = 0  # ranks of 2^18 records or more: packed by size
+    note("plain bucket FirstSort in place rec=%s P=%d n=%d" % (rec, P, n))
+    key = oracle.key_for(a.dtype, field)
+    ref, info = oracle.histogram_sort(a, even(n, P), even(n, P), key)
+    out, st = _device_inplace(a, even(n, P), field, flags=flags)
+    assert numpy.array_equal(out.view(numpy.uint8), ref.view(numpy.uint8))
+    assert st.first_passes == 0
+    assert st.checksum_in == info['checksum_in'] == st.checksum_out
