@@ -289,6 +289,11 @@ typedef struct mpsort_gpu_segplan_s mpsort_gpu_segplan;
  * ones below the threshold take the 1024-thread form */
 #define MPSORT_GPU_SEGPLAN_SMALL_512x9 (1u << 2)
 #define MPSORT_GPU_SEGPLAN_SMALL_512x6 (1u << 3)
+/* every segment sorted by launches of its own, one segment after the other
+ * on the stream with no host wait between them (the loop of per-segment
+ * sorts the batched form replaces, with the same kernels and workspaces;
+ * for comparison) */
+#define MPSORT_GPU_SEGPLAN_EACH (1u << 4)
 int mpsort_gpu_segplan_create(mpsort_gpu_segplan **plan, unsigned flags);
 int mpsort_gpu_segplan_add(mpsort_gpu_segplan *plan, const void *d_in, void *d_out, const int64_t *seg_count,
                            int nseg, size_t elsize, const mpsort_key_desc *key);

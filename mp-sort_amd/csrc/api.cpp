@@ -1762,6 +1762,11 @@ struct SegGroup {
     int64_t* d_ovf = nullptr;
     SegSortPlan plan;
     uint64_t epoch_used = 0;  // epoch steps consumed since the last reset
+    // one segment at a time (MPSORT_GPU_SEGPLAN_EACH): the large segments'
+    // counts and tile counts, and a {0, tiles} table per large segment
+    bool each = false;
+    std::vector<int64_t> lcount, ltiles;
+    int64_t* d_tsv = nullptr;
 };
 
 static std::map<std::string, SegTune>& seg_tune_cache() {
@@ -1892,10 +1897,17 @@ static int seg_group_prepare(SegGroup& G, hipStream_t st) {
         p.ntiles = t;
         Sizer pz;
         lay_plan(pz, G.nlarge, npass, nullptr);
+        pz.take<int64_t>(2 * (size_t)G.nlarge);
         if ((rc = G.plan_buf.ensure(pz.off + 4096))) return rc;
         Arena pa_a{(char*)G.plan_buf.p};
         PlanArrays pa;
         lay_plan(pa_a, G.nlarge, npass, &pa);
+        G.d_tsv = pa_a.take<int64_t>(2 * (size_t)G.nlarge);
+        G.lcount = lcnt;
+        G.ltiles.assign(G.nlarge, 0);
+        std::vector<int64_t> tsv(2 * (size_t)G.nlarge, 0);
+        for (int i = 0; i < G.nlarge; i++) G.ltiles[i] = tsv[2 * i + 1] = ts[i + 1] - ts[i];
+        MPSG_CHECK(hipMemcpyAsync(G.d_tsv, tsv.data(), sizeof(int64_t) * 2 * G.nlarge, hipMemcpyHostToDevice, st));
         MPSG_CHECK(hipMemcpyAsync(pa.begin, lbeg.data(), sizeof(int64_t) * G.nlarge, hipMemcpyHostToDevice, st));
         MPSG_CHECK(hipMemcpyAsync(pa.count, lcnt.data(), sizeof(int64_t) * G.nlarge, hipMemcpyHostToDevice, st));
         MPSG_CHECK(hipMemcpyAsync(pa.tile_start, ts.data(), sizeof(int64_t) * (G.nlarge + 1), hipMemcpyHostToDevice,
@@ -1923,10 +1935,12 @@ static int seg_group_prepare(SegGroup& G, hipStream_t st) {
 
 // the device work of one group; records nothing but kernels, so it can be
 // captured (look-back epochs come from the counter bumped here)
+static int seg_group_enqueue_each(SegGroup& G, hipStream_t st);
 static int seg_group_enqueue(SegGroup& G, hipStream_t st) {
     int rc = 0;
     if ((rc = launch_fill_u32(G.d_counters, 0u, 2, st))) return rc;
     if ((rc = launch_fill_u32(G.d_counters + 4, 0u, 1, st))) return rc;
+    if (G.each) return seg_group_enqueue_each(G, st);
     if (G.nsmall) {
         if ((rc = launch_segments_lds(G.rw, G.tune.form, G.in, G.out, G.lk, G.d_cnt, G.d_base, G.nsmall_form,
                                       G.nsmall - G.nsmall_form, G.d_ovf, G.d_counters, G.d_hard, G.d_counters + 1,
@@ -1947,18 +1961,56 @@ static int seg_group_enqueue(SegGroup& G, hipStream_t st) {
     return 0;
 }
 
+// The same work one segment at a time: each small segment by its own
+// in-LDS launch (its crowded-segment count cleared before), each large one
+// by a one-segment view of the plan (its counts, histograms and digit bases;
+// a {0, tiles} tile table; look-back epochs advanced per segment), all
+// enqueued back to back.
+static int seg_group_enqueue_each(SegGroup& G, hipStream_t st) {
+    int rc = 0;
+    for (int i = 0; i < G.nsmall; i++) {
+        if ((rc = launch_fill_u32(G.d_counters + 1, 0u, 1, st))) return rc;
+        if ((rc = launch_fill_u32(G.d_counters + 4, 0u, 1, st))) return rc;
+        const bool in_form = i < G.nsmall_form;
+        if ((rc = launch_segments_lds(G.rw, G.tune.form, G.in, G.out, G.lk, G.d_cnt + i, G.d_base + i, in_form ? 1 : 0,
+                                      in_form ? 0 : 1, G.d_ovf, G.d_counters, G.d_hard, G.d_counters + 1,
+                                      G.d_counters + 4, st)))
+            return rc;
+    }
+    const int npass = G.ks.key_bytes;
+    for (int i = 0; i < G.nlarge; i++) {
+        if ((rc = launch_add_u32(G.d_counters + 2, (uint32_t)kMaxPasses, st))) return rc;
+        SegSortPlan v = G.plan;
+        v.nseg = 1;
+        v.nrec = G.lcount[i];
+        v.ntiles = G.ltiles[i];
+        v.d_begin = G.plan.d_begin + i;
+        v.d_count = G.plan.d_count + i;
+        v.d_tile_start = G.d_tsv + 2 * i;
+        v.d_hist = G.plan.d_hist + (int64_t)i * npass * kBins;
+        v.d_digit_base = G.plan.d_digit_base + (int64_t)i * npass * kBins;
+        if ((rc = seg_count(v, G.in, nullptr, st))) return rc < 0 && !g_err[0] ? fail(MPSORT_ERR_UNSUPPORTED, "segmented sort: tile not supported") : rc;
+        SortBuffers b{const_cast<void*>(G.in), G.out, G.tmp.p};
+        int passes = 0;
+        if ((rc = seg_passes(v, b, st, nullptr, &passes, nullptr)))
+            return rc < 0 && !g_err[0] ? fail(MPSORT_ERR_UNSUPPORTED, "segmented sort: tile not supported") : rc;
+    }
+    return 0;
+}
+
 // look-back epochs must stay below 2^23: reset the flags and the counter
 // (outside any capture) before a run that could pass the limit
 static int seg_group_epoch_guard(SegGroup& G, hipStream_t st) {
     if (!G.nlarge) return 0;
-    if (G.epoch_used + 2 * kMaxPasses < (1u << 23)) {
-        G.epoch_used += kMaxPasses;
+    const uint64_t step = (uint64_t)kMaxPasses * (G.each ? (uint64_t)G.nlarge + 1 : 1);
+    if (G.epoch_used + 2 * step < (1u << 23)) {
+        G.epoch_used += step;
         return 0;
     }
     int rc = launch_fill_u32((uint32_t*)G.lookback.p, 0u, (int64_t)(G.lookback.bytes / 4), st);
     if (rc) return rc;
     if ((rc = launch_fill_u32(G.d_counters + 2, 0u, 1, st))) return rc;
-    G.epoch_used = kMaxPasses;
+    G.epoch_used = step;
     return 0;
 }
 
@@ -3024,6 +3076,7 @@ int mpsort_gpu_segplan_add(mpsort_gpu_segplan* p, const void* d_in, void* d_out,
     G->lk = make_lkey(G->ks);
     G->tune.tile = default_tile(G->rw);
     G->tune.form = (p->flags & MPSORT_GPU_SEGPLAN_SMALL_512x6) ? 2 : ((p->flags & MPSORT_GPU_SEGPLAN_SMALL_512x9) ? 1 : 0);
+    G->each = (p->flags & MPSORT_GPU_SEGPLAN_EACH) != 0;
     G->nseg = nseg;
     G->count.assign(seg_count, seg_count + nseg);
     G->begin.resize(nseg);

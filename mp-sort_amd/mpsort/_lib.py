@@ -67,6 +67,7 @@ SEGPLAN_GRAPH = 1 << 0
 SEGPLAN_TUNE = 1 << 1
 SEGPLAN_SMALL_512x9 = 1 << 2
 SEGPLAN_SMALL_512x6 = 1 << 3
+SEGPLAN_EACH = 1 << 4
 
 MODEL_LINEAR_ROOT = 1 << 8
 MLP_PARAMS = 1153

@@ -21,9 +21,12 @@ class SegmentPlan(object):
     shape without tuning: 0 (1024 threads x 9 records, one workgroup per CU),
     1 (512 x 9, two per CU) or 2 (512 x 6, three per CU)."""
 
-    def __init__(self, graph=False, tune=False, small_form=0):
+    def __init__(self, graph=False, tune=False, small_form=0, each=False):
         flags = (_lib.SEGPLAN_GRAPH if graph else 0) | (_lib.SEGPLAN_TUNE if tune else 0)
         flags |= {0: 0, 1: _lib.SEGPLAN_SMALL_512x9, 2: _lib.SEGPLAN_SMALL_512x6}[small_form]
+        # each: every segment by launches of its own, one after the other (the
+        # per-segment loop the batched form replaces)
+        flags |= _lib.SEGPLAN_EACH if each else 0
         h = ctypes.c_void_p()
         check(lib.mpsort_gpu_segplan_create(ctypes.byref(h), flags))
         self._h = h

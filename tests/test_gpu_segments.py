@@ -146,11 +146,11 @@ def test_32_byte_records_default_and_tuned():
         assert numpy.array_equal(out, want), tune
 
 
-def _plan_sort(a, sizes, form, graph=False):
+def _plan_sort(a, sizes, form, graph=False, each=False):
     d = lib.mpsort_gpu_dmalloc(max(a.nbytes, 1))
     try:
         _lib.check(lib.mpsort_gpu_memcpy_h2d(d, a.ctypes.data, a.nbytes))
-        plan = segments.SegmentPlan(graph=graph, small_form=form)
+        plan = segments.SegmentPlan(graph=graph, small_form=form, each=each)
         plan.add(d, d, sizes, a.dtype, 'key')
         plan.run()
         info = plan.info()
@@ -183,3 +183,17 @@ def test_segments_small_form_graph_replays():
     a = make(workloads.REC8, sizes, 51)
     g, _ = _plan_sort(a, sizes, 2, graph=True)
     assert numpy.array_equal(g, expected(a, sizes))
+
+
+@pytest.mark.parametrize("form", [0, 2])
+@pytest.mark.parametrize("dtype", [workloads.REC8, workloads.REC16I], ids=["8B", "16B"])
+def test_segments_one_at_a_time(dtype, form):
+    """MPSORT_GPU_SEGPLAN_EACH: every segment by launches of its own (the
+    per-segment loop the batched plan replaces), small ones in LDS and large
+    ones by one-segment views of the radix plan, duplicate-heavy keys too"""
+    sizes = sizes_mixed(120, 61 + form, 1 << 20)
+    for key_bits in (None, 6):
+        a = make(dtype, sizes, 71 + form, key_bits=key_bits)
+        note("segments one at a time form %d keybits %s n=%d" % (form, key_bits, int(sizes.sum())))
+        g, _ = _plan_sort(a, sizes, form, each=True)
+        assert numpy.array_equal(g, expected(a, sizes))
