@@ -430,6 +430,34 @@ def _device_copy(recs):
     return d
 
 
+def plain_first_form(st):
+    """(FirstSort form, entries of pass_ms before the SecondSort's) of a
+    plain-path call's statistics"""
+    nsec = 4 if st["second_merged"] else st["second_passes"]
+    if st["first_passes"] == 0 and len(st["pass_ms"]) - nsec == 6:
+        return "buckets", 6
+    return "radix", st["first_passes"]
+
+
+def plain_stages(st):
+    """the plain path's stage times (ms) from one call's pass_ms"""
+    form, k0 = plain_first_form(st)
+    ms = st["pass_ms"]
+    out = {}
+    if form == "buckets":
+        for i, name in enumerate(("sample_tables", "hist", "scatter", "bucket_sort", "lds_radix", "fallback")):
+            out["first_" + name] = round(ms[i], 4)
+    else:
+        out["first_radix_passes"] = round(sum(ms[:k0]), 4)
+    rest = ms[k0:]
+    if st["second_merged"]:
+        for i, name in enumerate(("tables_bounds_groups", "gather_sort", "listed", "fallback")):
+            out["second_" + name] = round(rest[i], 4)
+    elif rest:
+        out["second_radix_passes"] = round(sum(rest), 4)
+    return out
+
+
 def _timed(step, label, steps, warmup):
     """warmup + timed calls of step; (ms per step, per-call stats of the timed
     calls, per-call stats of the warmup calls)"""
@@ -622,13 +650,6 @@ def bench_histsort(args):
                             "source": ROCPROF_SOURCE}
         return r
 
-    def plain_first_form(st):
-        """(FirstSort form, entries of pass_ms before the SecondSort's)"""
-        nsec = 4 if st["second_merged"] else st["second_passes"]
-        if st["first_passes"] == 0 and len(st["pass_ms"]) - nsec == 6:
-            return "buckets", 6
-        return "radix", st["first_passes"]
-
     radix_roof = roof([x for st in st_radix for x in st["pass_ms"][:st["first_passes"] + st["second_passes"]]],
                       "onesweep_kernel<4,1024,8> (one 8-bit digit pass, 2*n*16 B per launch)", TRAFFIC_ONESWEEP,
                       ROCPROF_MS.get("onesweep"))
@@ -780,6 +801,7 @@ def bench_dists(args):
                      "learned_Mrecords_s": round(n / (ms_l * 1e-3) / 1e6, 1),
                      "plain_stable_sorted": ok, "learned_bit_identical_to_plain": same,
                      "train_s": round(t_train, 3), "model_err_max": model.info()["err_max"],
+                     "model_nheavy": model.info()["nheavy"],
                      "splitters_exact": L["splitters_exact"], "splitters_repaired": L["splitters_repaired"],
                      "repair_rounds": L["repair_rounds"], "plain_bisect_rounds": st_p[-1]["bisect_rounds"],
                      "t_findp_ms": {"learned": round(L["t_findp_ms"], 4), "plain": round(st_p[-1]["t_findp_ms"], 4)},
@@ -789,7 +811,10 @@ def bench_dists(args):
                      "buckets_lds_radix": L["buckets_lds_radix"],
                      "buckets_listed": L["buckets_listed"],
                      "phases_ms": {k2: round(L[k2], 4) for k2 in PHASES},
-                     "kernels_ms": dict(zip(LEARNED_KERNELS, [round(x, 4) for x in L["pass_ms"]]))}
+                     "kernels_ms": dict(zip(LEARNED_KERNELS, [round(x, 4) for x in L["pass_ms"]])),
+                     "plain_stages_ms": plain_stages(st_p[-1]),
+                     "plain_buckets_overflow": st_p[-1]["buckets_overflow"],
+                     "plain_buckets_lds_radix": st_p[-1]["buckets_lds_radix"]}
         model.close()
         log("%s: %s" % (dist, json.dumps(out[dist])))
         if not (ok and same):

@@ -1317,29 +1317,15 @@ static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan
     return 0;
 }
 
-// The plain path's bucket FirstSort: kFirstSamples keys of every segment at
-// evenly spaced positions, sorted (segmented radix sort of the sample), give
-// each segment a table of kLBounds boundaries; then the learned path's local
-// sort (bucket histogram, scatter, in-LDS bucket sorts, fallbacks) from src
-// into dst.  hbegin / hcount: the segments on the host.
-static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan& planA, const HistArrays& h,
-                              const int64_t* hbegin, const int64_t* hcount, const void* src, void* dst, void* tmp,
-                              long long* cs_in, long long* cs_out, hipEvent_t* ev, MergedReport* rep,
-                              hipStream_t st) {
+// FirstSort tables from the records: kFirstSamples keys of every segment
+// (L.begin / L.count on the device) at evenly spaced positions, sorted by a
+// segmented radix sort of the sample, give each segment a table of
+// kLBounds boundaries in L.tables (stride kLBounds).  plan supplies the
+// look-back flags and the checksum words.
+static int sample_first_tables(DevCtx& c, const CallShape& cs, const SegSortPlan& plan, const LearnedSide& L,
+                               const void* src, hipStream_t st) {
     const int P = cs.P, rw = cs.rw, S = kFirstSamples;
     int rc = 0;
-    Sizer lz;
-    lay_lside(lz, P, nullptr);
-    lay_lside(lz, P, nullptr);
-    if ((rc = c.lsmall.ensure(lz.off + 4096))) return rc;
-    Arena la{(char*)c.lsmall.p};
-    LearnedSide LM, LA;
-    lay_lside(la, P, &LM);  // the merged SecondSort's
-    lay_lside(la, P, &LA);
-    LA.tstride = kLBounds;
-    MPSG_CHECK(hipMemcpyAsync(LA.begin, hbegin, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(LA.count, hcount, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    // the sample (three buffers of the segmented radix sort) and its plan
     const size_t sbytes = ((size_t)P * S * sizeof(uint64_t) + 255) & ~(size_t)255;
     if ((rc = c.fsamp.ensure(3 * sbytes))) return rc;
     uint64_t* s0 = (uint64_t*)c.fsamp.p;
@@ -1362,7 +1348,7 @@ static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan&
     MPSG_CHECK(hipMemcpyAsync(spa.count, sc.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
     MPSG_CHECK(hipMemcpyAsync(spa.tile_start, st_.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
     if ((rc = launch_fill_u32(spa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
-    SegSortPlan sp = planA;
+    SegSortPlan sp = plan;
     sp.rw = 2;
     sp.tile = stile;
     sp.ks = sk;
@@ -1380,19 +1366,45 @@ static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan&
     sp.all_passes = false;
     sp.d_epoch_base = nullptr;
     const LKey lk = make_lkey(cs.ks);
-    if (ev) MPSG_CHECK(hipEventRecord(ev[8], st));
-    if ((rc = launch_seg_sample(rw, src, lk, LA.begin, LA.count, P, S, s0, st))) return rc;
+    if ((rc = launch_seg_sample(rw, src, lk, L.begin, L.count, P, S, s0, st))) return rc;
     int passes = 0;
     SortBuffers sbuf{s0, s1, s2};
     if ((rc = segmented_sort(sp, sbuf, st, false, false, &passes, nullptr))) return rc;
     uint32_t flag = 0;
     MPSG_CHECK(hipMemcpyAsync(&flag, spa.pass_flags + kMaxPasses, sizeof flag, hipMemcpyDeviceToHost, st));
     // no pass ran: every sample key is equal and the sample stayed in s0
-    if ((rc = launch_sample_tables(passes ? s1 : s0, P, S, LA.tables, st))) return rc;
+    if ((rc = launch_sample_tables(passes ? s1 : s0, P, S, L.tables, st))) return rc;
     MPSG_CHECK(hipStreamSynchronize(st));
     if (flag) return fail(MPSORT_ERR_TIMEOUT, "FirstSort sample sort: device flag 0x%x", flag);
+    return 0;
+}
+
+// The plain path's bucket FirstSort: tables from a sample of every segment
+// (sample_first_tables), then the learned path's local sort (bucket
+// histogram, scatter, in-LDS bucket sorts, fallbacks) from src into dst.
+// hbegin / hcount: the segments on the host.
+static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan& planA, const HistArrays& h,
+                              const int64_t* hbegin, const int64_t* hcount, const void* src, void* dst, void* tmp,
+                              long long* cs_in, long long* cs_out, hipEvent_t* ev, MergedReport* rep,
+                              hipStream_t st) {
+    const int P = cs.P;
+    int rc = 0;
+    (void)h;
+    Sizer lz;
+    lay_lside(lz, P, nullptr);
+    lay_lside(lz, P, nullptr);
+    if ((rc = c.lsmall.ensure(lz.off + 4096))) return rc;
+    Arena la{(char*)c.lsmall.p};
+    LearnedSide LM, LA;
+    lay_lside(la, P, &LM);  // the merged SecondSort's
+    lay_lside(la, P, &LA);
+    LA.tstride = kLBounds;
+    MPSG_CHECK(hipMemcpyAsync(LA.begin, hbegin, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+    MPSG_CHECK(hipMemcpyAsync(LA.count, hcount, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+    if (ev) MPSG_CHECK(hipEventRecord(ev[8], st));
+    if ((rc = sample_first_tables(c, cs, planA, LA, src, st))) return rc;
     LearnedCounters lc;
-    if ((rc = learned_local_sort(c, rw, cs.ks, P, cs.ntot, LA, planA, src, dst, tmp, cs_in, cs_out, &lc, st, ev)))
+    if ((rc = learned_local_sort(c, cs.rw, cs.ks, P, cs.ntot, LA, planA, src, dst, tmp, cs_in, cs_out, &lc, st, ev)))
         return rc;
     MPSG_CHECK(hipMemcpyAsync(rep->lists, LA.noverflow, sizeof rep->lists, hipMemcpyDeviceToHost, st));
     rep->overflow = lc.overflow_buckets;

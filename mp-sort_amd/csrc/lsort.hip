@@ -1094,6 +1094,10 @@ __device__ __forceinline__ int sort_bucket_lds(const Src& src, uint32_t* __restr
             if (t >= mx) return (uint32_t)count;
             const uint32_t d = bin_of(t);
             const uint32_t a = s_start[d], e = s_start[d + 1];
+            // a long bin of one key value (duplicate-heavy keys, where many
+            // entries fall): all of it or none, without reading it
+            if (DUPS && any_long && e - a > (uint32_t)kLongMin && !((s_mix[d >> 5] >> (d & 31)) & 1u))
+                return t >= s_gkey[a] ? e : a;
             uint32_t c = a;
             for (uint32_t q = a; q < e; q++) c += s_gkey[q] <= t ? 1u : 0u;
             return c;
