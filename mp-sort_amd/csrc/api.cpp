@@ -503,10 +503,9 @@ struct MergedReport {
 static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan& planB, const HistArrays& h,
                                const PlanArrays& pa, const void* src, void* dst, void* tmp, long long* cs_out,
                                hipEvent_t* ev, MergedReport* rep, hipStream_t st);
-static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan& planA, const HistArrays& h,
-                              const int64_t* hbegin, const int64_t* hcount, const void* src, void* dst, void* tmp,
-                              long long* cs_in, long long* cs_out, hipEvent_t* ev, MergedReport* rep,
-                              hipStream_t st);
+static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan& planA, const int64_t* hbegin,
+                              const int64_t* hcount, const void* src, void* dst, void* tmp, long long* cs_in,
+                              long long* cs_out, hipEvent_t* ev, MergedReport* rep, hipStream_t st);
 
 // The histogram sort on device-resident data.  d_in / d_out are the caller's
 // records (device); for the general path they are packed into sort records.
@@ -614,7 +613,7 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         // one rank: FirstSort is the whole sort (the exchange is the identity
         // and SecondSort re-sorts a sorted run)
         if (first_buckets) {
-            if ((rc = plain_bucket_first(c, cs, planA, h, hbA, hcA, recs_in, final_dst, w.s1,
+            if ((rc = plain_bucket_first(c, cs, planA, hbA, hcA, recs_in, final_dst, w.s1,
                                          cs_direct ? h.checksum : nullptr, cs_direct ? h.checksum + 1 : nullptr, pevA,
                                          &frep, st)))
                 return rc;
@@ -626,7 +625,7 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         for (int e = kEvFindP; e <= kEvSecond; e++) MPSG_CHECK(hipEventRecord(c.ev[e], st));
     } else {
         if (first_buckets) {
-            if ((rc = plain_bucket_first(c, cs, planA, h, hbA, hcA, recs_in, w.s0, w.s1,
+            if ((rc = plain_bucket_first(c, cs, planA, hbA, hcA, recs_in, w.s0, w.s1,
                                          cs_direct ? h.checksum : nullptr, nullptr, pevA, &frep, st)))
                 return rc;
         } else {
@@ -1383,13 +1382,11 @@ static int sample_first_tables(DevCtx& c, const CallShape& cs, const SegSortPlan
 // (sample_first_tables), then the learned path's local sort (bucket
 // histogram, scatter, in-LDS bucket sorts, fallbacks) from src into dst.
 // hbegin / hcount: the segments on the host.
-static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan& planA, const HistArrays& h,
-                              const int64_t* hbegin, const int64_t* hcount, const void* src, void* dst, void* tmp,
-                              long long* cs_in, long long* cs_out, hipEvent_t* ev, MergedReport* rep,
-                              hipStream_t st) {
+static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan& planA, const int64_t* hbegin,
+                              const int64_t* hcount, const void* src, void* dst, void* tmp, long long* cs_in,
+                              long long* cs_out, hipEvent_t* ev, MergedReport* rep, hipStream_t st) {
     const int P = cs.P;
     int rc = 0;
-    (void)h;
     Sizer lz;
     lay_lside(lz, P, nullptr);
     lay_lside(lz, P, nullptr);
