@@ -241,6 +241,17 @@ def bench_batched(args):
     value = nrec / (ms * 1e-3) / 1e6
     loop_value = nrec / loop_s / 1e6
     each_value = nrec / each_s / 1e6
+    # bytes the plan's passes move: a segment up to the in-LDS threshold is
+    # read and written once; a larger one is read by the digit histogram and
+    # read and written by one onesweep pass per key byte (uniform keys leave
+    # no digit place trivial)
+    pass_bytes = 0
+    for g, info in zip(groups, plan_info):
+        es = g["dtype"].itemsize
+        kb = g["dtype"]['key'].itemsize
+        small = int(g["sizes"][g["sizes"] <= info["threshold"]].sum())
+        pass_bytes += small * 2 * es + (g["n"] - small) * es * (1 + 2 * kb)
+    pass_gbs = pass_bytes / (ms * 1e-3) / 1e9
     result = {
         "metric": "Mrecords/s, batched segmented sort (10,000 segments, 2^30 records)", "value": round(value, 2),
         "unit": "Mrecords/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -256,6 +267,10 @@ def bench_batched(args):
         "per_call_loop": {"value": round(loop_value, 2), "s": round(loop_s, 3),
                           "form": "one mpsort_gpu_newarray call per segment, each waiting for its result"},
         "sort_equivalent_GBps": round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1),
+        "passes": {"bytes": pass_bytes, "GBps": round(pass_gbs, 1), "frac_of_hbm_peak": round(pass_gbs / HBM_PEAK_GBS, 3),
+                   "form": "segments up to the in-LDS threshold: one read and one write; larger ones: the "
+                           "histogram read plus one read and one write per key byte (4 passes for i32 keys, "
+                           "8 for i64)"},
         "tuned_plan": {"i32_f32": plan_info[0], "i64_i64": plan_info[1]},
     }
     print(json.dumps(result), flush=True)

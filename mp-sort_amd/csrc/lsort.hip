@@ -345,6 +345,7 @@ __global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void lscatter_kernel(
             s_single[d / 32 + 1] = (uint32_t)(m >> 32);
         }
     }
+    __syncthreads();  // every wave's counters are cleared by threads of all waves
 
     uint32_t dl[ITEMS];  // bucket | wave rank << 16
     volatile uint16_t* wh = s_whist[wave];

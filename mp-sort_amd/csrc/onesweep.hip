@@ -39,7 +39,8 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict
                                                       int npass, PassTable pt,
                                                       uint32_t* __restrict__ hist,
                                                       long long* __restrict__ checksum) {
-    __shared__ uint32_t s_hist[kMaxPasses * kBins];
+    extern __shared__ uint32_t s_hist[];  // npass * kBins counters (dynamic: short keys leave room
+                                          // for more resident workgroups)
     __shared__ long long s_sum[kWaves];
     const int tid = threadIdx.x;
     const int nb = npass * kBins;
@@ -50,8 +51,41 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict
     const int64_t t1 = min(ntiles, t0 + tpb);
     long long csum = 0;
     int cur = -1;
+    // U records per thread at a time; the next U records' loads are issued
+    // before this group is counted, so they are in flight meanwhile (the
+    // next group may be the next tile's first).  Every thread runs the same
+    // number of groups per tile (positions past the tile are ignored).
+    constexpr int U = 4, STEP = U * kBlock;
+    auto tile_range = [&](int64_t tl, int& s, int64_t& b, int64_t& e) {
+        s = upper_index<int64_t>(seg.tile_start, seg.nseg, tl);
+        const int64_t sb = seg.begin[s];
+        b = sb + (tl - seg.tile_start[s]) * tile_recs;
+        e = min(sb + seg.count[s], b + (int64_t)tile_recs);
+    };
+    auto load_group = [&](uint32_t (&r)[U][RW], int64_t i0, int64_t end) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (i0 + (int64_t)u * kBlock < end) {
+                RecIO<RW>::load(src, i0 + (int64_t)u * kBlock, r[u]);
+            } else {
+#pragma unroll
+                for (int w = 0; w < RW; w++) r[u][w] = 0;
+            }
+        }
+    };
+    int s_nx = 0;
+    int64_t b_nx = 0, e_nx = 0;
+    uint32_t r[U][RW];
+    if (t0 < t1) tile_range(t0, s_nx, b_nx, e_nx);
+    load_group(r, b_nx + tid, e_nx);
     for (int64_t tile = t0; tile < t1; tile++) {
-        const int s = upper_index<int64_t>(seg.tile_start, seg.nseg, tile);
+        const int s = s_nx;
+        const int64_t begin = b_nx, end = e_nx;
+        if (tile + 1 < t1) {
+            tile_range(tile + 1, s_nx, b_nx, e_nx);
+        } else {
+            b_nx = e_nx = 0;
+        }
         if (s != cur) {
             if (cur >= 0) {
                 __syncthreads();
@@ -65,18 +99,15 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict
             }
             cur = s;
         }
-        const int64_t t = tile - seg.tile_start[s];
-        const int64_t sbeg = seg.begin[s];
-        const int64_t begin = sbeg + t * tile_recs;
-        const int64_t end = min(sbeg + seg.count[s], begin + (int64_t)tile_recs);
-        // U records per thread at a time, all U loads in flight before the
-        // first is used
-        constexpr int U = 4;
-        for (int64_t i0 = begin + tid; i0 < end; i0 += U * kBlock) {
-            uint32_t r[U][RW];
-#pragma unroll
-            for (int u = 0; u < U; u++)
-                if (i0 + (int64_t)u * kBlock < end) RecIO<RW>::load(src, i0 + (int64_t)u * kBlock, r[u]);
+        const int ngroups = (int)((end - begin + STEP - 1) / STEP);
+        for (int q = 0; q < ngroups; q++) {
+            const int64_t i0 = begin + tid + (int64_t)q * STEP;
+            uint32_t rn[U][RW];
+            if (q + 1 < ngroups) {
+                load_group(rn, i0 + STEP, end);
+            } else {
+                load_group(rn, b_nx + tid, e_nx);
+            }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 if (i0 + (int64_t)u * kBlock < end) {
@@ -90,6 +121,10 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict
                     }
                 }
             }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int w = 0; w < RW; w++) r[u][w] = rn[u][w];
         }
     }
     __syncthreads();
@@ -500,7 +535,7 @@ static void launch_hist(const SegSortPlan& pl, const void* src, int npass, const
     // about 2048 blocks (8 per CU), each over a contiguous run of tiles
     const int64_t tpb = (pl.ntiles + 2047) / 2048;
     const int64_t blocks = (pl.ntiles + tpb - 1) / tpb;
-    hipLaunchKernelGGL(hist_kernel<RW>, dim3((unsigned)blocks), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL(hist_kernel<RW>, dim3((unsigned)blocks), dim3(kBlock), (size_t)npass * kBins * sizeof(uint32_t), st,
                        (const uint32_t*)src, sd, pl.ntiles, tpb, pl.tile.block * pl.tile.items, npass,
                        pt, pl.d_hist, cs);
 }
