@@ -58,6 +58,30 @@ __device__ __forceinline__ uint32_t digit_of(const uint32_t (&r)[RW], const Pass
     return ((pick_word<RW>(r, pd.widx) >> pd.shift) & 0xffu) ^ pd.xmask;
 }
 
+// Counts the digit of every pass of record r in h[p * kBins + digit] (LDS
+// or global counters).  b0 >= 0: the passes are the record's bytes b0,
+// b0 + 1, ... (at most 8 of them, consecutive_key_bytes), read as two words
+// once; otherwise each pass picks its own word.
+template <int RW>
+__device__ __forceinline__ void count_digits(const uint32_t (&r)[RW], int npass, const PassTable& pt, int b0,
+                                             uint32_t* h) {
+    if (b0 >= 0) {
+        const int w = b0 >> 2, sh = (b0 & 3) * 8;
+        const uint32_t a = pick_word<RW>(r, w), b = pick_word<RW>(r, w + 1), c = pick_word<RW>(r, w + 2);
+        const uint32_t lo = sh ? ((a >> sh) | (b << (32 - sh))) : a;  // bytes b0 .. b0 + 3
+        const uint32_t hi = sh ? ((b >> sh) | (c << (32 - sh))) : b;  // bytes b0 + 4 .. b0 + 7
+#pragma unroll
+        for (int p = 0; p < 8; p++) {
+            if (p < npass) {
+                const uint32_t d = (((p < 4 ? lo : hi) >> (8 * (p & 3))) & 0xffu) ^ pt.p[p].xmask;
+                atomicAdd(&h[p * kBins + d], 1u);
+            }
+        }
+    } else {
+        for (int p = 0; p < npass; p++) atomicAdd(&h[p * kBins + digit_of<RW>(r, pt.p[p])], 1u);
+    }
+}
+
 template <int RW> struct RecIO;
 template <> struct RecIO<1> {
     __device__ static void load(const uint32_t* p, int64_t i, uint32_t (&r)[1]) { r[0] = p[i]; }

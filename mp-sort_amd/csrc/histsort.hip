@@ -704,9 +704,9 @@ template <int RW>
 __global__ __launch_bounds__(kBlock) void exchange_hist_kernel(
     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int64_t ntot, RunTable rt,
     uint32_t* __restrict__ error_flag, const int64_t* __restrict__ C, int P, int npass,
-    PassTable pt, uint32_t* __restrict__ hist) {
+    PassTable pt, int b0, uint32_t* __restrict__ hist) {
     __shared__ int64_t s_rd[kExchLdsRuns], s_rs[kExchLdsRuns];
-    __shared__ uint32_t s_hist[kMaxPasses * kBins];
+    extern __shared__ uint32_t s_hist[];  // npass * kBins counters
     __shared__ int s_a, s_n, s_r0, s_r1;
     const int tid = threadIdx.x;
     const bool sparse = rt.sel && rt.sel[0] != 0;
@@ -769,11 +769,10 @@ __global__ __launch_bounds__(kBlock) void exchange_hist_kernel(
             const int64_t o = oo + (int64_t)u * kBlock;
             RecIO<RW>::store(dst, o, r[u]);
             if (one_receiver) {
-                for (int p = 0; p < npass; p++) atomicAdd(&s_hist[p * kBins + digit_of<RW>(r[u], pt.p[p])], 1u);
+                count_digits<RW>(r[u], npass, pt, b0, s_hist);
             } else {
                 const int rr = upper_index<int64_t>(C, P, o);
-                uint32_t* h = hist + (int64_t)rr * nb;
-                for (int p = 0; p < npass; p++) atomicAdd(&h[p * kBins + digit_of<RW>(r[u], pt.p[p])], 1u);
+                count_digits<RW>(r[u], npass, pt, b0, hist + (int64_t)rr * nb);
             }
         }
     }
@@ -964,11 +963,13 @@ int launch_exchange_hist(const void* src, void* dst, int64_t ntot, int rw, const
     uint32_t* d = (uint32_t*)dst;
     const PassTable pt = make_pass_table(ks);
     const int np = ks.key_bytes;
+    const int b0 = consecutive_key_bytes(pt, np);
+    const size_t lds = (size_t)np * kBins * sizeof(uint32_t);
     switch (rw) {
-        case 1: hipLaunchKernelGGL(exchange_hist_kernel<1>, g, b, 0, st, s, d, ntot, rt, err, C, P, np, pt, hist); break;
-        case 2: hipLaunchKernelGGL(exchange_hist_kernel<2>, g, b, 0, st, s, d, ntot, rt, err, C, P, np, pt, hist); break;
-        case 4: hipLaunchKernelGGL(exchange_hist_kernel<4>, g, b, 0, st, s, d, ntot, rt, err, C, P, np, pt, hist); break;
-        case 8: hipLaunchKernelGGL(exchange_hist_kernel<8>, g, b, 0, st, s, d, ntot, rt, err, C, P, np, pt, hist); break;
+        case 1: hipLaunchKernelGGL(exchange_hist_kernel<1>, g, b, lds, st, s, d, ntot, rt, err, C, P, np, pt, b0, hist); break;
+        case 2: hipLaunchKernelGGL(exchange_hist_kernel<2>, g, b, lds, st, s, d, ntot, rt, err, C, P, np, pt, b0, hist); break;
+        case 4: hipLaunchKernelGGL(exchange_hist_kernel<4>, g, b, lds, st, s, d, ntot, rt, err, C, P, np, pt, b0, hist); break;
+        case 8: hipLaunchKernelGGL(exchange_hist_kernel<8>, g, b, lds, st, s, d, ntot, rt, err, C, P, np, pt, b0, hist); break;
         default: return -1;
     }
     MPSG_CHECK(hipGetLastError());

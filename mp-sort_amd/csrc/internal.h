@@ -45,6 +45,16 @@ struct KeySpec {
     uint8_t xmask[kMaxKeyBytes];  // per key byte XOR mask
 };
 
+// first record byte of a pass table whose npass (1 .. 8) passes are
+// consecutive record bytes in order; -1 otherwise
+inline int consecutive_key_bytes(const PassTable& pt, int npass) {
+    if (npass < 1 || npass > 8) return -1;
+    const int b0 = pt.p[0].widx * 4 + pt.p[0].shift / 8;
+    for (int p = 0; p < npass; p++)
+        if (pt.p[p].shift % 8 != 0 || pt.p[p].widx * 4 + pt.p[p].shift / 8 != b0 + p) return -1;
+    return b0;
+}
+
 inline PassDesc make_pass(const KeySpec& ks, int key_byte) {
     PassDesc p;
     int b = ks.key_off + key_byte;

@@ -36,7 +36,7 @@ namespace mpsg {
 template <int RW>
 __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict__ src, SegDev seg,
                                                       int64_t ntiles, int64_t tpb, int tile_recs,
-                                                      int npass, PassTable pt,
+                                                      int npass, PassTable pt, int b0,
                                                       uint32_t* __restrict__ hist,
                                                       long long* __restrict__ checksum) {
     extern __shared__ uint32_t s_hist[];  // npass * kBins counters (dynamic: short keys leave room
@@ -115,10 +115,7 @@ __global__ __launch_bounds__(kBlock) void hist_kernel(const uint32_t* __restrict
 #pragma unroll
                         for (int w = 0; w < RW; w++) csum += sbyte_sum(r[u][w]);
                     }
-                    for (int p = 0; p < npass; p++) {
-                        const uint32_t d = digit_of<RW>(r[u], pt.p[p]);
-                        atomicAdd(&s_hist[p * kBins + d], 1u);
-                    }
+                    count_digits<RW>(r[u], npass, pt, b0, s_hist);
                 }
             }
 #pragma unroll
@@ -537,7 +534,7 @@ static void launch_hist(const SegSortPlan& pl, const void* src, int npass, const
     const int64_t blocks = (pl.ntiles + tpb - 1) / tpb;
     hipLaunchKernelGGL(hist_kernel<RW>, dim3((unsigned)blocks), dim3(kBlock), (size_t)npass * kBins * sizeof(uint32_t), st,
                        (const uint32_t*)src, sd, pl.ntiles, tpb, pl.tile.block * pl.tile.items, npass,
-                       pt, pl.d_hist, cs);
+                       pt, consecutive_key_bytes(pt, npass), pl.d_hist, cs);
 }
 
 template <int RW, int BLOCK, int ITEMS>
