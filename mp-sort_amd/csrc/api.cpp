@@ -170,6 +170,9 @@ struct DevCtx {
     Buf fsamp;          // plain bucket FirstSort: the segments' sample keys (three buffers)
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    Buf hmirror;        // device copy of staged host tables (Upload)
+    void* rpin = nullptr;  // pinned: per-call staging and small reads (pin_take), reset per call
+    size_t rpin_bytes = 0, rpin_off = 0;
     void* scratch = nullptr;      // this device's copy of the scratch array
     hipError_t scratch_err = hipSuccess;
     hipEvent_t ev[kEvPhase];
@@ -231,6 +234,111 @@ static int ensure_pinned(DevCtx& c, size_t bytes) {
     c.pinned_bytes = want;
     return 0;
 }
+
+// the per-call pinned area: staging for host tables and small reads, taken
+// in order and reset at the start of every call (the previous call's copies
+// have all completed by then)
+static int ensure_rpin(DevCtx& c, size_t bytes) {
+    c.rpin_off = 0;
+    if (bytes <= c.rpin_bytes) return 0;
+    if (c.rpin) (void)hipHostFree(c.rpin);
+    c.rpin = nullptr;
+    c.rpin_bytes = 0;
+    const size_t want = bytes + 4096;
+    MPSG_CHECK(hipHostMalloc(&c.rpin, want, hipHostMallocDefault));
+    c.rpin_bytes = want;
+    return 0;
+}
+static void* pin_take(DevCtx& c, size_t bytes) {
+    const size_t at = (c.rpin_off + 63) & ~(size_t)63;
+    if (!c.rpin || at + bytes > c.rpin_bytes) return nullptr;
+    c.rpin_off = at + bytes;
+    return (char*)c.rpin + at;
+}
+
+// Host tables bound for device arrays and small counters to clear: the
+// tables lie in one pinned buffer (base); flush() copies their span to the
+// device mirror once and one fan-out launch writes every array and fill,
+// instead of a copy or a launch per array.  The host tables must stay as
+// they are until the stream has run the copy.
+struct Upload {
+    DevCtx& c;
+    hipStream_t st;
+    const char* base;
+    size_t base_bytes;
+    FanOut f;
+    size_t lo = ~(size_t)0, hi = 0;
+    Upload(DevCtx& c_, hipStream_t st_, const void* b, size_t bb)
+        : c(c_), st(st_), base((const char*)b), base_bytes(bb) { f.n = 0; }
+    int add(void* dst, const void* hsrc, size_t bytes) {
+        const size_t at = (size_t)((const char*)hsrc - base);
+        if (!base || (const char*)hsrc < base || at + bytes > base_bytes || at % 4 || bytes % 4)
+            return fail(MPSORT_ERR_ARG, "host table outside the staging buffer");
+        if (f.n == kFanMax) {
+            int rc = flush();
+            if (rc) return rc;
+        }
+        f.e[f.n++] = FanEntry{(uint32_t*)dst, (uint32_t)(at / 4), (uint32_t)(bytes / 4), 0u, 0};
+        lo = std::min(lo, at);
+        hi = std::max(hi, at + bytes);
+        return 0;
+    }
+    int fill(uint32_t* dst, uint32_t v, int64_t n) {
+        if (n <= 0) return 0;
+        if (n > (1 << 20)) return launch_fill_u32(dst, v, n, st);
+        if (f.n == kFanMax) {
+            int rc = flush();
+            if (rc) return rc;
+        }
+        f.e[f.n++] = FanEntry{dst, 0u, (uint32_t)n, v, 1};
+        return 0;
+    }
+    int flush() {
+        int rc = 0;
+        if (hi > lo) {
+            if ((rc = c.hmirror.ensure(base_bytes))) return rc;
+            MPSG_CHECK(hipMemcpyAsync((char*)c.hmirror.p + lo, base + lo, hi - lo, hipMemcpyHostToDevice, st));
+        }
+        if (f.n && (rc = launch_fanout((const uint32_t*)c.hmirror.p, f, st))) return rc;
+        f.n = 0;
+        lo = ~(size_t)0;
+        hi = 0;
+        return 0;
+    }
+};
+
+// Small device-to-host reads: with use, each copy lands in the per-call
+// pinned area and one wait covers them all (a copy into pageable memory
+// waits by itself); otherwise each goes straight to its target.
+struct Readback {
+    DevCtx& c;
+    hipStream_t st;
+    bool use;
+    struct Item {
+        void* host;
+        const void* at;
+        size_t bytes;
+    };
+    Item it[16];
+    int n = 0;
+    Readback(DevCtx& c_, hipStream_t st_, bool use_) : c(c_), st(st_), use(use_) {}
+    int add(void* host, const void* dev, size_t bytes) {
+        void* at = use && n < 16 ? pin_take(c, bytes) : nullptr;
+        if (!at) {
+            MPSG_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
+            return 0;
+        }
+        MPSG_CHECK(hipMemcpyAsync(at, dev, bytes, hipMemcpyDeviceToHost, st));
+        it[n++] = Item{host, at, bytes};
+        return 0;
+    }
+    int wait() {
+        MPSG_CHECK(hipStreamSynchronize(st));
+        for (int i = 0; i < n; i++) memcpy(it[i].host, it[i].at, it[i].bytes);
+        n = 0;
+        return 0;
+    }
+};
 
 // bump allocator over one device buffer
 struct Arena {
@@ -545,18 +653,33 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
         (rc = launch_fill_u32((uint32_t*)w.lookback, 0u, (int64_t)(ntiles > 0 ? ntiles : 1) * kBins * 2, st)))
         return rc;
 
-    MPSG_CHECK(hipMemcpyAsync(pa.begin, hbA, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(pa.count, hcA, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(pa.tile_start, htA, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(pb.begin, hbB, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(pb.count, hcB, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(pb.tile_start, htB, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(h.C, hC, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
-    if ((rc = launch_fill_u32(h.err, 0u, 4, st))) return rc;
-    if ((rc = launch_fill_u32((uint32_t*)h.checksum, 0u, 4, st))) return rc;
-    // both plans' flag words (the second plan is unused when P == 1)
-    if ((rc = launch_fill_u32(pa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
-    if ((rc = launch_fill_u32(pb.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
+    if (w.merge) {
+        // the allocating entry points: the tables are in the pinned buffer,
+        // uploaded by one copy and one fan-out launch with the clears
+        if ((rc = ensure_rpin(c, (size_t)64 * (P + 16) * sizeof(int64_t)))) return rc;
+        Upload up(c, st, c.pinned, c.pinned_bytes);
+        if ((rc = up.add(pa.begin, hbA, sizeof(int64_t) * P)) || (rc = up.add(pa.count, hcA, sizeof(int64_t) * P)) ||
+            (rc = up.add(pa.tile_start, htA, sizeof(int64_t) * (P + 1))) ||
+            (rc = up.add(pb.begin, hbB, sizeof(int64_t) * P)) || (rc = up.add(pb.count, hcB, sizeof(int64_t) * P)) ||
+            (rc = up.add(pb.tile_start, htB, sizeof(int64_t) * (P + 1))) ||
+            (rc = up.add(h.C, hC, sizeof(int64_t) * (P + 1))) || (rc = up.fill(h.err, 0u, 4)) ||
+            (rc = up.fill((uint32_t*)h.checksum, 0u, 4)) || (rc = up.fill(pa.pass_flags, 0u, kMaxPasses + 1)) ||
+            (rc = up.fill(pb.pass_flags, 0u, kMaxPasses + 1)) || (rc = up.flush()))
+            return rc;
+    } else {
+        MPSG_CHECK(hipMemcpyAsync(pa.begin, hbA, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(pa.count, hcA, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(pa.tile_start, htA, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(pb.begin, hbB, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(pb.count, hcB, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(pb.tile_start, htB, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(h.C, hC, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+        if ((rc = launch_fill_u32(h.err, 0u, 4, st))) return rc;
+        if ((rc = launch_fill_u32((uint32_t*)h.checksum, 0u, 4, st))) return rc;
+        // both plans' flag words (the second plan is unused when P == 1)
+        if ((rc = launch_fill_u32(pa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
+        if ((rc = launch_fill_u32(pb.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
+    }
 
     auto mkplan = [&](const PlanArrays& a, int64_t nt) {
         SegSortPlan p;
@@ -677,17 +800,19 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     uint32_t errs[3] = {0, 0, 0};
     long long sums[2] = {0, 0};
     int rounds_host = 0;
-    MPSG_CHECK(hipMemcpyAsync(&errs[0], h.err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipMemcpyAsync(&errs[1], pa.pass_flags + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipMemcpyAsync(&errs[2], pb.pass_flags + kMaxPasses, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipMemcpyAsync(sums, h.checksum, sizeof sums, hipMemcpyDeviceToHost, st));
     static int rounds[4096];
     int sel[2] = {0, 0};
-    if (P > 1) {
-        MPSG_CHECK(hipMemcpyAsync(rounds, h.rounds, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
-        MPSG_CHECK(hipMemcpyAsync(sel, h.sel, sizeof sel, hipMemcpyDeviceToHost, st));
+    {
+        Readback rb(c, st, w.merge);
+        if ((rc = rb.add(&errs[0], h.err, sizeof(uint32_t))) ||
+            (rc = rb.add(&errs[1], pa.pass_flags + kMaxPasses, sizeof(uint32_t))) ||
+            (rc = rb.add(&errs[2], pb.pass_flags + kMaxPasses, sizeof(uint32_t))) ||
+            (rc = rb.add(sums, h.checksum, sizeof sums)))
+            return rc;
+        if (P > 1 && ((rc = rb.add(rounds, h.rounds, sizeof(int) * (P - 1))) || (rc = rb.add(sel, h.sel, sizeof sel))))
+            return rc;
+        if ((rc = rb.wait())) return rc;
     }
-    MPSG_CHECK(hipStreamSynchronize(st));
     for (int i = 0; i + 1 < P; i++) rounds_host = rounds[i] > rounds_host ? rounds[i] : rounds_host;
 
     // stats
@@ -1067,7 +1192,7 @@ static int sort_listed_segments(DevCtx& c, const SegSortPlan& plain, int rw, int
 static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64_t n, const LearnedSide& L,
                               const SegSortPlan& plain, const void* src, void* dst, void* tmp, long long* cs_in,
                               long long* cs_out, LearnedCounters* lc, hipStream_t st, hipEvent_t* ev,
-                              const EntryOut* eo = nullptr) {
+                              const EntryOut* eo = nullptr, const int64_t* hcount = nullptr) {
     int rc = 0;
     // ev (optional): 6 events around bucket histogram, scatter, in-LDS
     // bucket sort, in-LDS radix sort of crowded buckets and the overflow
@@ -1085,9 +1210,14 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     lc->lists[lc->lists[0] ? 1 : 0] = L.noverflow;
     // tiles of the learned scatter (the look-back buffer was sized by the
     // caller: it must not move while plans hold its address)
+    // (hcount: the segment sizes on the host, as uploaded to L.count)
     std::vector<int64_t> cnt(P), beg(P), lt(P + 1);
-    MPSG_CHECK(hipMemcpyAsync(cnt.data(), L.count, sizeof(int64_t) * P, hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipStreamSynchronize(st));
+    if (hcount) {
+        std::copy(hcount, hcount + P, cnt.begin());
+    } else {
+        MPSG_CHECK(hipMemcpyAsync(cnt.data(), L.count, sizeof(int64_t) * P, hipMemcpyDeviceToHost, st));
+        MPSG_CHECK(hipStreamSynchronize(st));
+    }
     {
         // segments whose average bucket would exceed half the in-LDS capacity
         // would mostly overflow: such a side takes the plain radix sort
@@ -1105,8 +1235,15 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         }
     }
     const int64_t ntiles = make_tiles(cnt.data(), P, trecs, beg.data(), lt.data());
-    MPSG_CHECK(hipMemcpyAsync(L.ltiles, lt.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipStreamSynchronize(st));
+    // staged in the per-call pinned area when it has room (no wait for the
+    // copy); otherwise copied from lt, which must outlive the copy
+    if (int64_t* plt = (int64_t*)pin_take(c, sizeof(int64_t) * (P + 1))) {
+        std::copy(lt.begin(), lt.end(), plt);
+        MPSG_CHECK(hipMemcpyAsync(L.ltiles, plt, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+    } else {
+        MPSG_CHECK(hipMemcpyAsync(L.ltiles, lt.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipStreamSynchronize(st));
+    }
     // per-tile bucket counts (u16) and per-tile offsets (u32)
     const size_t tc_bytes = ((size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint16_t) + 255) & ~(size_t)255;
     const size_t to_bytes = ((size_t)(ntiles > 0 ? ntiles : 1) * kLBins * sizeof(uint32_t) + 255) & ~(size_t)255;
@@ -1138,9 +1275,12 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
         return rc;
     if ((rc = mark(3))) return rc;
     uint32_t novh[3] = {0, 0, 0}, errs = 0;
-    MPSG_CHECK(hipMemcpyAsync(novh, L.noverflow, sizeof novh, hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipMemcpyAsync(&errs, L.err, sizeof errs, hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipStreamSynchronize(st));
+    {
+        Readback rb(c, st, true);
+        if ((rc = rb.add(novh, L.noverflow, sizeof novh)) || (rc = rb.add(&errs, L.err, sizeof errs)) ||
+            (rc = rb.wait()))
+            return rc;
+    }
     if (errs) return fail(MPSORT_ERR_TIMEOUT, "learned bucket scatter: device flag 0x%x", errs);
     if ((rc = launch_listed_sort(rw, tmp, dst, lk, L.medium, novh[2], L.lhist, L.bbase, L.tables, kLBins, L.tstride,
                                  L.hard, L.noverflow + 1, cs_out, st, eo)))
@@ -1401,7 +1541,8 @@ static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan&
     if (ev) MPSG_CHECK(hipEventRecord(ev[8], st));
     if ((rc = sample_first_tables(c, cs, planA, LA, src, st))) return rc;
     LearnedCounters lc;
-    if ((rc = learned_local_sort(c, cs.rw, cs.ks, P, cs.ntot, LA, planA, src, dst, tmp, cs_in, cs_out, &lc, st, ev)))
+    if ((rc = learned_local_sort(c, cs.rw, cs.ks, P, cs.ntot, LA, planA, src, dst, tmp, cs_in, cs_out, &lc, st, ev,
+                                 nullptr, hcount)))
         return rc;
     MPSG_CHECK(hipMemcpyAsync(rep->lists, LA.noverflow, sizeof rep->lists, hipMemcpyDeviceToHost, st));
     rep->overflow = lc.overflow_buckets;
@@ -1473,28 +1614,22 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
         const size_t need = (size_t)(ntiles + kMaxFallbackSegs + 1) * kBins * sizeof(uint64_t);
         if ((rc = c.lookback.ensure(need))) return rc;
     }
-    auto up = [&](void* d, const void* hsrc, size_t bytes) {
-        return hipMemcpyAsync(d, hsrc, bytes, hipMemcpyHostToDevice, st);
-    };
-    MPSG_CHECK(up(pa.begin, hbA, sizeof(int64_t) * P));
-    MPSG_CHECK(up(pa.count, hcA, sizeof(int64_t) * P));
-    MPSG_CHECK(up(pa.tile_start, htA, sizeof(int64_t) * (P + 1)));
-    MPSG_CHECK(up(pb.begin, hbB, sizeof(int64_t) * P));
-    MPSG_CHECK(up(pb.count, hcB, sizeof(int64_t) * P));
-    MPSG_CHECK(up(pb.tile_start, htB, sizeof(int64_t) * (P + 1)));
-    MPSG_CHECK(up(h.C, hC, sizeof(int64_t) * (P + 1)));
-    MPSG_CHECK(up(LA.begin, hbA, sizeof(int64_t) * P));
-    MPSG_CHECK(up(LA.count, hcA, sizeof(int64_t) * P));
-    MPSG_CHECK(up(LB.begin, hbB, sizeof(int64_t) * P));
-    MPSG_CHECK(up(LB.count, hcB, sizeof(int64_t) * P));
-    MPSG_CHECK(up(LA.q0, hq, sizeof(float) * P));
-    MPSG_CHECK(up(LA.q1, hq + P, sizeof(float) * P));
-    MPSG_CHECK(up(LB.q0, hq + 2 * P, sizeof(float) * P));
-    MPSG_CHECK(up(LB.q1, hq + 3 * P, sizeof(float) * P));
-    if ((rc = launch_fill_u32(h.err, 0u, 4, st))) return rc;
-    if ((rc = launch_fill_u32((uint32_t*)h.checksum, 0u, 4, st))) return rc;
-    if ((rc = launch_fill_u32(pa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
-    if ((rc = launch_fill_u32(pb.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
+    if ((rc = ensure_rpin(c, (size_t)64 * (P + 16) * sizeof(int64_t)))) return rc;
+    {
+        Upload up(c, st, c.pinned, c.pinned_bytes);
+        if ((rc = up.add(pa.begin, hbA, sizeof(int64_t) * P)) || (rc = up.add(pa.count, hcA, sizeof(int64_t) * P)) ||
+            (rc = up.add(pa.tile_start, htA, sizeof(int64_t) * (P + 1))) ||
+            (rc = up.add(pb.begin, hbB, sizeof(int64_t) * P)) || (rc = up.add(pb.count, hcB, sizeof(int64_t) * P)) ||
+            (rc = up.add(pb.tile_start, htB, sizeof(int64_t) * (P + 1))) ||
+            (rc = up.add(h.C, hC, sizeof(int64_t) * (P + 1))) || (rc = up.add(LA.begin, hbA, sizeof(int64_t) * P)) ||
+            (rc = up.add(LA.count, hcA, sizeof(int64_t) * P)) || (rc = up.add(LB.begin, hbB, sizeof(int64_t) * P)) ||
+            (rc = up.add(LB.count, hcB, sizeof(int64_t) * P)) || (rc = up.add(LA.q0, hq, sizeof(float) * P)) ||
+            (rc = up.add(LA.q1, hq + P, sizeof(float) * P)) || (rc = up.add(LB.q0, hq + 2 * P, sizeof(float) * P)) ||
+            (rc = up.add(LB.q1, hq + 3 * P, sizeof(float) * P)) || (rc = up.fill(h.err, 0u, 4)) ||
+            (rc = up.fill((uint32_t*)h.checksum, 0u, 4)) || (rc = up.fill(pa.pass_flags, 0u, kMaxPasses + 1)) ||
+            (rc = up.fill(pb.pass_flags, 0u, kMaxPasses + 1)) || (rc = up.flush()))
+            return rc;
+    }
 
     auto mkplan = [&](const PlanArrays& a, int64_t nt) {
         SegSortPlan p;
@@ -1554,7 +1689,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     if (P == 1) {
         if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, final_dst, c.s1.p,
                                      cs_direct ? h.checksum : nullptr, cs_direct ? h.checksum + 1 : nullptr, &lc,
-                                     st, evA)))
+                                     st, evA, nullptr, hcA)))
             return rc;
         MPSG_CHECK(hipEventRecord(c.ev[kEvFirst], st));
         for (int e = kEvFindP; e <= kEvSecond; e++) MPSG_CHECK(hipEventRecord(c.ev[e], st));
@@ -1593,7 +1728,8 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = launch_fill_u32((uint32_t*)eo.done, 0u, (int64_t)P * kLBins / 4, st))) return rc;
         }
         if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LA, planA, recs_in, c.s0.p, c.s1.p,
-                                     cs_direct ? h.checksum : nullptr, nullptr, &lc, st, evA, merged ? &eo : nullptr)))
+                                     cs_direct ? h.checksum : nullptr, nullptr, &lc, st, evA, merged ? &eo : nullptr,
+                                     hcA)))
             return rc;
         overflow_first = lc.overflow_buckets;
         if (merged) {
@@ -1631,7 +1767,7 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
             if ((rc = launch_exchange(c.s0.p, c.s1.p, n, rw, run_table(h, P), h.err, st))) return rc;
             MPSG_CHECK(hipEventRecord(c.ev[kEvExchange], st));
             if ((rc = learned_local_sort(c, rw, cs.ks, P, n, LB, planB, c.s1.p, final_dst, c.s0.p, nullptr,
-                                         cs_direct ? h.checksum + 1 : nullptr, &lc, st, evB)))
+                                         cs_direct ? h.checksum + 1 : nullptr, &lc, st, evB, nullptr, hcB)))
                 return rc;
         }
         MPSG_CHECK(hipEventRecord(c.ev[kEvSecond], st));
@@ -1649,18 +1785,17 @@ static int run_histsort_learned(DevCtx& c, const CallShape& cs, const void* d_in
     std::vector<int> rounds(P > 1 ? P - 1 : 1, 0), ex(P > 1 ? P - 1 : 1, 0);
     // list counters of each side: [1] LDS radix sort, [2] 1024-thread form
     uint32_t lists[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    for (int side = 0; side < 2; side++)
-        if (lc.lists[side])
-            MPSG_CHECK(hipMemcpyAsync(lists[side], lc.lists[side], sizeof lists[side], hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipMemcpyAsync(&err0, h.err, sizeof err0, hipMemcpyDeviceToHost, st));
-    MPSG_CHECK(hipMemcpyAsync(sums, h.checksum, sizeof sums, hipMemcpyDeviceToHost, st));
     int sel[2] = {0, 0};
-    if (P > 1) {
-        MPSG_CHECK(hipMemcpyAsync(rounds.data(), h.rounds, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
-        MPSG_CHECK(hipMemcpyAsync(ex.data(), exact, sizeof(int) * (P - 1), hipMemcpyDeviceToHost, st));
-        MPSG_CHECK(hipMemcpyAsync(sel, h.sel, sizeof sel, hipMemcpyDeviceToHost, st));
+    {
+        Readback rb(c, st, true);
+        for (int side = 0; side < 2; side++)
+            if (lc.lists[side] && (rc = rb.add(lists[side], lc.lists[side], sizeof lists[side]))) return rc;
+        if ((rc = rb.add(&err0, h.err, sizeof err0)) || (rc = rb.add(sums, h.checksum, sizeof sums))) return rc;
+        if (P > 1 && ((rc = rb.add(rounds.data(), h.rounds, sizeof(int) * (P - 1))) ||
+                      (rc = rb.add(ex.data(), exact, sizeof(int) * (P - 1))) || (rc = rb.add(sel, h.sel, sizeof sel))))
+            return rc;
+        if ((rc = rb.wait())) return rc;
     }
-    MPSG_CHECK(hipStreamSynchronize(st));
     int rmax = 0, nexact = 0;
     for (int i = 0; i + 1 < P; i++) {
         rmax = std::max(rmax, rounds[i]);
@@ -2552,6 +2687,8 @@ int mpsort_gpu_release(void) {
     c.ltile.release();
     c.mbnd.release();
     c.mdesc.release();
+    c.fsamp.release();
+    c.hmirror.release();
     c.last_P = 0;
     return 0;
 }

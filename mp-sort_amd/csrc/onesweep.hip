@@ -435,6 +435,11 @@ __global__ void add_u32_kernel(uint32_t* p, uint32_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
 }
 
+__global__ __launch_bounds__(256) void fanout_kernel(const uint32_t* __restrict__ mirror, FanOut f) {
+    const FanEntry e = f.e[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < e.nwords; i += blockDim.x) e.dst[i] = e.is_fill ? e.fill : mirror[e.src + i];
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -480,6 +485,14 @@ int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
     int64_t blocks = (n + kBlock - 1) / kBlock;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, p, v, n);
+    MPSG_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_fanout(const uint32_t* mirror, const FanOut& f, hipStream_t st) {
+    if (f.n <= 0) return 0;
+    if (f.n > kFanMax) return -1;
+    hipLaunchKernelGGL(fanout_kernel, dim3((unsigned)f.n), dim3(256), 0, st, mirror, f);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
