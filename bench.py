@@ -32,14 +32,14 @@ PHASES = ("t_first_sort_ms", "t_findp_ms", "t_laysolve_ms", "t_exchange_ms", "t_
 # HBM bytes per launch at 2^28 x 16 B from rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # each scaled by the counter's reading for a plain copy kernel of known size
 # (profiles/histsort_clustered_2e28x16B_hbm_traffic.md)
-TRAFFIC_ONESWEEP = 9117204461
-TRAFFIC_BUCKET_SORT = 11860260864
-TRAFFIC_PLAIN_BUCKET_SORT = 7986922154
+TRAFFIC_ONESWEEP = 9117058048
+TRAFFIC_BUCKET_SORT = 11778213312
+TRAFFIC_PLAIN_BUCKET_SORT = 7984629536
 # average launch time of the same kernels over the same launches (the timed
 # steps of `python bench.py --steps 20 --warmup 5`) from its rocprofv3 kernel
 # trace (profiles/histsort_clustered_2e28x16B_kernel_trace.md)
 ROCPROF_SOURCE = "profiles/histsort_clustered_2e28x16B_kernel_trace.md"
-ROCPROF_MS = {"bucket_sort_A": 2.9402, "plain_bucket_sort": 2.4895, "onesweep": 2.3727}
+ROCPROF_MS = {"bucket_sort_A": 2.9063, "plain_bucket_sort": 2.4642, "onesweep": 2.3706}
 # per-side stages of the learned local sort, as timed by the library; with
 # the merged SecondSort also its two gather-and-sort launches alone
 # (merge_B: groups of buckets, then the larger single buckets), which
