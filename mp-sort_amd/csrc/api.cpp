@@ -168,9 +168,12 @@ struct DevCtx {
     Buf mbnd;           // merged SecondSort: sub-run bounds per (receiver, sender, bucket)
     Buf mdesc;          // merged SecondSort: group descriptors and their sub-run tables
     Buf fsamp;          // plain bucket FirstSort: the segments' sample keys (three buffers)
+    Buf fsplan;         // plain bucket FirstSort: the sample sort's plan arrays
+    int fsplan_P = 0;   // segments those arrays were laid out for (0: none)
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     Buf hmirror;        // device copy of staged host tables (Upload)
+    Buf gstage;         // small device results gathered for one download (Readback)
     void* rpin = nullptr;  // pinned: per-call staging and small reads (pin_take), reset per call
     size_t rpin_bytes = 0, rpin_off = 0;
     void* scratch = nullptr;      // this device's copy of the scratch array
@@ -262,35 +265,41 @@ static void* pin_take(DevCtx& c, size_t bytes) {
 // instead of a copy or a launch per array.  The host tables must stay as
 // they are until the stream has run the copy.
 struct Upload {
+    struct Item {
+        uint32_t* dst;
+        size_t at;  // byte offset in base, or ~0 for a fill
+        uint32_t nwords, fill;
+    };
     DevCtx& c;
     hipStream_t st;
     const char* base;
     size_t base_bytes;
-    FanOut f;
+    Item it[kFanMax];
+    int n = 0;
     size_t lo = ~(size_t)0, hi = 0;
     Upload(DevCtx& c_, hipStream_t st_, const void* b, size_t bb)
-        : c(c_), st(st_), base((const char*)b), base_bytes(bb) { f.n = 0; }
+        : c(c_), st(st_), base((const char*)b), base_bytes(bb) {}
     int add(void* dst, const void* hsrc, size_t bytes) {
         const size_t at = (size_t)((const char*)hsrc - base);
         if (!base || (const char*)hsrc < base || at + bytes > base_bytes || at % 4 || bytes % 4)
             return fail(MPSORT_ERR_ARG, "host table outside the staging buffer");
-        if (f.n == kFanMax) {
+        if (n == kFanMax) {
             int rc = flush();
             if (rc) return rc;
         }
-        f.e[f.n++] = FanEntry{(uint32_t*)dst, (uint32_t)(at / 4), (uint32_t)(bytes / 4), 0u, 0};
+        it[n++] = Item{(uint32_t*)dst, at, (uint32_t)(bytes / 4), 0u};
         lo = std::min(lo, at);
         hi = std::max(hi, at + bytes);
         return 0;
     }
-    int fill(uint32_t* dst, uint32_t v, int64_t n) {
-        if (n <= 0) return 0;
-        if (n > (1 << 20)) return launch_fill_u32(dst, v, n, st);
-        if (f.n == kFanMax) {
+    int fill(uint32_t* dst, uint32_t v, int64_t nw) {
+        if (nw <= 0) return 0;
+        if (nw > (1 << 20)) return launch_fill_u32(dst, v, nw, st);
+        if (n == kFanMax) {
             int rc = flush();
             if (rc) return rc;
         }
-        f.e[f.n++] = FanEntry{dst, 0u, (uint32_t)n, v, 1};
+        it[n++] = Item{dst, ~(size_t)0, (uint32_t)nw, v};
         return 0;
     }
     int flush() {
@@ -299,42 +308,72 @@ struct Upload {
             if ((rc = c.hmirror.ensure(base_bytes))) return rc;
             MPSG_CHECK(hipMemcpyAsync((char*)c.hmirror.p + lo, base + lo, hi - lo, hipMemcpyHostToDevice, st));
         }
-        if (f.n && (rc = launch_fanout((const uint32_t*)c.hmirror.p, f, st))) return rc;
-        f.n = 0;
+        FanOut f;
+        f.n = n;
+        for (int i = 0; i < n; i++)
+            f.e[i] = FanEntry{it[i].dst,
+                              it[i].at == ~(size_t)0 ? nullptr : (const uint32_t*)((const char*)c.hmirror.p + it[i].at),
+                              it[i].nwords, it[i].fill};
+        if (n && (rc = launch_fanout(f, st))) return rc;
+        n = 0;
         lo = ~(size_t)0;
         hi = 0;
         return 0;
     }
 };
 
-// Small device-to-host reads: with use, each copy lands in the per-call
-// pinned area and one wait covers them all (a copy into pageable memory
-// waits by itself); otherwise each goes straight to its target.
+// Small device-to-host reads: with use, wait() gathers them into one device
+// area with one fan-out launch, copies that area into the per-call pinned
+// area once and waits once; otherwise (or when they do not fit) each copy
+// goes straight to its target (a copy into pageable memory waits by itself).
 struct Readback {
     DevCtx& c;
     hipStream_t st;
     bool use;
     struct Item {
         void* host;
-        const void* at;
+        const void* dev;
         size_t bytes;
     };
-    Item it[16];
+    Item it[kFanMax];
     int n = 0;
     Readback(DevCtx& c_, hipStream_t st_, bool use_) : c(c_), st(st_), use(use_) {}
     int add(void* host, const void* dev, size_t bytes) {
-        void* at = use && n < 16 ? pin_take(c, bytes) : nullptr;
-        if (!at) {
+        if (!use || n == kFanMax || bytes % 4 || (uintptr_t)dev % 4) {
             MPSG_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
             return 0;
         }
-        MPSG_CHECK(hipMemcpyAsync(at, dev, bytes, hipMemcpyDeviceToHost, st));
-        it[n++] = Item{host, at, bytes};
+        it[n++] = Item{host, dev, bytes};
         return 0;
     }
     int wait() {
-        MPSG_CHECK(hipStreamSynchronize(st));
-        for (int i = 0; i < n; i++) memcpy(it[i].host, it[i].at, it[i].bytes);
+        int rc = 0;
+        size_t total = 0;
+        for (int i = 0; i < n; i++) total += it[i].bytes;
+        char* pin = n ? (char*)pin_take(c, total) : nullptr;
+        if (n && pin && !(rc = c.gstage.ensure(total))) {
+            FanOut f;
+            f.n = n;
+            size_t off = 0;
+            for (int i = 0; i < n; i++) {
+                f.e[i] = FanEntry{(uint32_t*)((char*)c.gstage.p + off), (const uint32_t*)it[i].dev,
+                                  (uint32_t)(it[i].bytes / 4), 0u};
+                off += it[i].bytes;
+            }
+            if ((rc = launch_fanout(f, st))) return rc;
+            MPSG_CHECK(hipMemcpyAsync(pin, c.gstage.p, total, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipStreamSynchronize(st));
+            off = 0;
+            for (int i = 0; i < n; i++) {
+                memcpy(it[i].host, pin + off, it[i].bytes);
+                off += it[i].bytes;
+            }
+        } else {
+            if (rc) return rc;
+            for (int i = 0; i < n; i++)
+                MPSG_CHECK(hipMemcpyAsync(it[i].host, it[i].dev, it[i].bytes, hipMemcpyDeviceToHost, st));
+            MPSG_CHECK(hipStreamSynchronize(st));
+        }
         n = 0;
         return 0;
     }
@@ -605,6 +644,8 @@ static bool first_buckets_eligible(const CallShape& cs, const int64_t* nin) {
 // what the plain path's merged SecondSort (and bucket FirstSort) reports
 struct MergedReport {
     uint32_t lists[3] = {0, 0, 0};  // oversized, LDS radix, 1024-thread form (read at the call's end)
+    const uint32_t* lists_dev = nullptr;  // where they are on the device
+    const uint32_t* flag_dev = nullptr;   // a device error flag to check at the call's end, or null
     int overflow = 0;
     int global_fallback = 0;
 };
@@ -802,6 +843,7 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
     int rounds_host = 0;
     static int rounds[4096];
     int sel[2] = {0, 0};
+    uint32_t sample_flag = 0;
     {
         Readback rb(c, st, w.merge);
         if ((rc = rb.add(&errs[0], h.err, sizeof(uint32_t))) ||
@@ -811,8 +853,12 @@ static int run_histsort(DevCtx& c, const CallShape& cs, const void* d_in, void* 
             return rc;
         if (P > 1 && ((rc = rb.add(rounds, h.rounds, sizeof(int) * (P - 1))) || (rc = rb.add(sel, h.sel, sizeof sel))))
             return rc;
+        for (MergedReport* r : {&mrep, &frep})
+            if (r->lists_dev && (rc = rb.add(r->lists, r->lists_dev, sizeof r->lists))) return rc;
+        if (frep.flag_dev && (rc = rb.add(&sample_flag, frep.flag_dev, sizeof sample_flag))) return rc;
         if ((rc = rb.wait())) return rc;
     }
+    if (sample_flag) return fail(MPSORT_ERR_TIMEOUT, "FirstSort sample sort: device flag 0x%x", sample_flag);
     for (int i = 0; i + 1 < P; i++) rounds_host = rounds[i] > rounds_host ? rounds[i] : rounds_host;
 
     // stats
@@ -1450,7 +1496,7 @@ static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan
     if ((rc = merged_second_sort(c, rw, lk, P, cs.ntot, LB, planB, h, pa.begin, bnd, src, dst, tmp, cs_out, lc, ev,
                                  st)))
         return rc;
-    MPSG_CHECK(hipMemcpyAsync(rep->lists, LB.noverflow, sizeof rep->lists, hipMemcpyDeviceToHost, st));
+    rep->lists_dev = LB.noverflow;
     rep->overflow = lc.overflow_buckets;
     rep->global_fallback = lc.global_fallback;
     return 0;
@@ -1462,7 +1508,7 @@ static int plain_merged_second(DevCtx& c, const CallShape& cs, const SegSortPlan
 // kLBounds boundaries in L.tables (stride kLBounds).  plan supplies the
 // look-back flags and the checksum words.
 static int sample_first_tables(DevCtx& c, const CallShape& cs, const SegSortPlan& plan, const LearnedSide& L,
-                               const void* src, hipStream_t st) {
+                               const void* src, hipStream_t st, const uint32_t** flag_dev) {
     const int P = cs.P, rw = cs.rw, S = kFirstSamples;
     int rc = 0;
     const size_t sbytes = ((size_t)P * S * sizeof(uint64_t) + 255) & ~(size_t)255;
@@ -1477,15 +1523,23 @@ static int sample_first_tables(DevCtx& c, const CallShape& cs, const SegSortPlan
     const int64_t snt = make_tiles(sc.data(), P, (int64_t)stile.block * stile.items, sb.data(), st_.data());
     if ((size_t)snt * kBins * sizeof(uint64_t) > c.lookback.bytes)
         return fail(MPSORT_ERR_ARG, "look-back buffer too small for the FirstSort sample");
+    // the sample's segments depend on P only: their arrays stay on the
+    // device from one call to the next
     Sizer pz;
     lay_plan(pz, P, 8, nullptr);
-    if ((rc = c.ovf.ensure(pz.off + 4096))) return rc;
-    Arena pr{(char*)c.ovf.p};
+    const void* was = c.fsplan.p;
+    if ((rc = c.fsplan.ensure(pz.off + 4096))) return rc;
+    Arena pr{(char*)c.fsplan.p};
     PlanArrays spa;
     lay_plan(pr, P, 8, &spa);
-    MPSG_CHECK(hipMemcpyAsync(spa.begin, sb.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(spa.count, sc.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(spa.tile_start, st_.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+    if (c.fsplan.p != was || c.fsplan_P != P) {
+        c.fsplan_P = 0;
+        MPSG_CHECK(hipMemcpyAsync(spa.begin, sb.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(spa.count, sc.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipMemcpyAsync(spa.tile_start, st_.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
+        MPSG_CHECK(hipStreamSynchronize(st));
+        c.fsplan_P = P;
+    }
     if ((rc = launch_fill_u32(spa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
     SegSortPlan sp = plan;
     sp.rw = 2;
@@ -1509,12 +1563,11 @@ static int sample_first_tables(DevCtx& c, const CallShape& cs, const SegSortPlan
     int passes = 0;
     SortBuffers sbuf{s0, s1, s2};
     if ((rc = segmented_sort(sp, sbuf, st, false, false, &passes, nullptr))) return rc;
-    uint32_t flag = 0;
-    MPSG_CHECK(hipMemcpyAsync(&flag, spa.pass_flags + kMaxPasses, sizeof flag, hipMemcpyDeviceToHost, st));
     // no pass ran: every sample key is equal and the sample stayed in s0
     if ((rc = launch_sample_tables(passes ? s1 : s0, P, S, L.tables, st))) return rc;
-    MPSG_CHECK(hipStreamSynchronize(st));
-    if (flag) return fail(MPSORT_ERR_TIMEOUT, "FirstSort sample sort: device flag 0x%x", flag);
+    // the sample sort's error flag is read at the call's end (whatever the
+    // tables, the bucket sort's result is the sorted order)
+    *flag_dev = spa.pass_flags + kMaxPasses;
     return 0;
 }
 
@@ -1536,15 +1589,20 @@ static int plain_bucket_first(DevCtx& c, const CallShape& cs, const SegSortPlan&
     lay_lside(la, P, &LM);  // the merged SecondSort's
     lay_lside(la, P, &LA);
     LA.tstride = kLBounds;
-    MPSG_CHECK(hipMemcpyAsync(LA.begin, hbegin, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
-    MPSG_CHECK(hipMemcpyAsync(LA.count, hcount, sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
+    {
+        // hbegin / hcount lie in the pinned table buffer
+        Upload up(c, st, c.pinned, c.pinned_bytes);
+        if ((rc = up.add(LA.begin, hbegin, sizeof(int64_t) * P)) || (rc = up.add(LA.count, hcount, sizeof(int64_t) * P)) ||
+            (rc = up.flush()))
+            return rc;
+    }
     if (ev) MPSG_CHECK(hipEventRecord(ev[8], st));
-    if ((rc = sample_first_tables(c, cs, planA, LA, src, st))) return rc;
+    if ((rc = sample_first_tables(c, cs, planA, LA, src, st, &rep->flag_dev))) return rc;
     LearnedCounters lc;
     if ((rc = learned_local_sort(c, cs.rw, cs.ks, P, cs.ntot, LA, planA, src, dst, tmp, cs_in, cs_out, &lc, st, ev,
                                  nullptr, hcount)))
         return rc;
-    MPSG_CHECK(hipMemcpyAsync(rep->lists, LA.noverflow, sizeof rep->lists, hipMemcpyDeviceToHost, st));
+    rep->lists_dev = LA.noverflow;
     rep->overflow = lc.overflow_buckets;
     rep->global_fallback = lc.global_fallback;
     return 0;
@@ -2688,7 +2746,10 @@ int mpsort_gpu_release(void) {
     c.mbnd.release();
     c.mdesc.release();
     c.fsamp.release();
+    c.fsplan.release();
+    c.fsplan_P = 0;
     c.hmirror.release();
+    c.gstage.release();
     c.last_P = 0;
     return 0;
 }

@@ -176,22 +176,22 @@ int segmented_sort(const SegSortPlan& plan, const SortBuffers& bufs, hipStream_t
 int launch_copy(const void* src, void* dst, int64_t nrec, int rw, long long* checksum,
                 hipStream_t st);
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
-// Host tables staged in one copy and small counters cleared by one launch:
-// entry i writes nwords words at dst, taken from mirror[src ..] or, with
-// is_fill, equal to fill (one workgroup per entry).
+// Many small device copies and fills in one launch (host tables staged in
+// one upload, counters cleared, small results gathered for one download):
+// entry i writes nwords words at dst, copied from src or, when src is
+// null, equal to fill (one workgroup per entry).
 constexpr int kFanMax = 24;
 struct FanEntry {
     uint32_t* dst;
-    uint32_t src;
+    const uint32_t* src;
     uint32_t nwords;
     uint32_t fill;
-    int32_t is_fill;
 };
 struct FanOut {
     int n;
     FanEntry e[kFanMax];
 };
-int launch_fanout(const uint32_t* mirror, const FanOut& f, hipStream_t st);
+int launch_fanout(const FanOut& f, hipStream_t st);
 int launch_add_u32(uint32_t* p, uint32_t v, hipStream_t st);  // *p += v on the device
 
 // epoch counter for the look-back flags (process wide, per device)

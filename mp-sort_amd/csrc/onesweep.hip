@@ -435,9 +435,9 @@ __global__ void add_u32_kernel(uint32_t* p, uint32_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
 }
 
-__global__ __launch_bounds__(256) void fanout_kernel(const uint32_t* __restrict__ mirror, FanOut f) {
+__global__ __launch_bounds__(256) void fanout_kernel(FanOut f) {
     const FanEntry e = f.e[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < e.nwords; i += blockDim.x) e.dst[i] = e.is_fill ? e.fill : mirror[e.src + i];
+    for (uint32_t i = threadIdx.x; i < e.nwords; i += blockDim.x) e.dst[i] = e.src ? e.src[i] : e.fill;
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n) {
@@ -489,10 +489,10 @@ int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
     return 0;
 }
 
-int launch_fanout(const uint32_t* mirror, const FanOut& f, hipStream_t st) {
+int launch_fanout(const FanOut& f, hipStream_t st) {
     if (f.n <= 0) return 0;
     if (f.n > kFanMax) return -1;
-    hipLaunchKernelGGL(fanout_kernel, dim3((unsigned)f.n), dim3(256), 0, st, mirror, f);
+    hipLaunchKernelGGL(fanout_kernel, dim3((unsigned)f.n), dim3(256), 0, st, f);
     MPSG_CHECK(hipGetLastError());
     return 0;
 }
