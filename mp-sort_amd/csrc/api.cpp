@@ -169,7 +169,7 @@ struct DevCtx {
     Buf mdesc;          // merged SecondSort: group descriptors and their sub-run tables
     Buf fsamp;          // plain bucket FirstSort: the segments' sample keys (three buffers)
     Buf fsplan;         // plain bucket FirstSort: the sample sort's plan arrays
-    int fsplan_P = 0;   // segments those arrays were laid out for (0: none)
+    int64_t fsplan_key = 0;  // segments and tile shape those arrays were made for (0: none)
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     Buf hmirror;        // device copy of staged host tables (Upload)
@@ -1532,13 +1532,14 @@ static int sample_first_tables(DevCtx& c, const CallShape& cs, const SegSortPlan
     Arena pr{(char*)c.fsplan.p};
     PlanArrays spa;
     lay_plan(pr, P, 8, &spa);
-    if (c.fsplan.p != was || c.fsplan_P != P) {
-        c.fsplan_P = 0;
+    const int64_t key = ((int64_t)P << 32) | ((int64_t)stile.block << 8) | stile.items;
+    if (c.fsplan.p != was || c.fsplan_key != key) {
+        c.fsplan_key = 0;
         MPSG_CHECK(hipMemcpyAsync(spa.begin, sb.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
         MPSG_CHECK(hipMemcpyAsync(spa.count, sc.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, st));
         MPSG_CHECK(hipMemcpyAsync(spa.tile_start, st_.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, st));
         MPSG_CHECK(hipStreamSynchronize(st));
-        c.fsplan_P = P;
+        c.fsplan_key = key;
     }
     if ((rc = launch_fill_u32(spa.pass_flags, 0u, kMaxPasses + 1, st))) return rc;
     SegSortPlan sp = plan;
@@ -2747,7 +2748,7 @@ int mpsort_gpu_release(void) {
     c.mdesc.release();
     c.fsamp.release();
     c.fsplan.release();
-    c.fsplan_P = 0;
+    c.fsplan_key = 0;
     c.hmirror.release();
     c.gstage.release();
     c.last_P = 0;

@@ -168,6 +168,23 @@ def test_forced_plain_fallback():
 # asks only for a stable local sort).  FIRST_BUCKETS takes it below its size
 # threshold, FIRST_RADIX keeps the digit passes.
 
+def test_bucket_first_sort_back_to_back_shapes(monkeypatch):
+    """The sample sort's segment arrays stay on the device from one call to
+    the next; a call with another rank count or tile shape lays them out
+    again."""
+    steps = [(64, None), (4, None), (64, None), (7, "512:16"), (7, None), (64, "512:16"), (64, None)]
+    for i, (P, tile) in enumerate(steps):
+        if tile:
+            monkeypatch.setenv("MPSORT_GPU_TILE", tile)
+        else:
+            monkeypatch.delenv("MPSORT_GPU_TILE", raising=False)
+        n = 70000 + 313 * i
+        a = workloads.records16(keys("random", n, 101 + i))
+        note("plain bucket FirstSort back to back: step %d P=%d tile=%s n=%d" % (i, P, tile, n))
+        out, st = check_parity(a, even(n, P), even(n, P), 'key', flags=_lib.FIRST_BUCKETS, device=True)
+        assert st.first_passes == 0
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("P", [1, 4, 64])
 def test_bucket_first_sort_matches_the_oracle(shape, P):
