@@ -317,46 +317,39 @@ __global__ void model_tables_kernel(ModelDev m, const float* __restrict__ q0, co
 // kEdgeBounds boundaries on each side for the keys outside it; the others get
 // [0, 1] and no edges.  One wave per segment.
 constexpr int kEdgeBounds = 32;
-__global__ __launch_bounds__(64) void seg_range_kernel(ModelDev m, const uint8_t* __restrict__ recs, int64_t elsize,
-                                                       int64_t key_off, int key_bytes, uint64_t key_xmask,
-                                                       const int64_t* __restrict__ begin,
-                                                       const int64_t* __restrict__ count, float* __restrict__ q0,
-                                                       float* __restrict__ q1, int* __restrict__ nedge) {
-    __shared__ float s_f[256];
+__global__ __launch_bounds__(256) void seg_range_kernel(ModelDev m, const uint8_t* __restrict__ recs, int64_t elsize,
+                                                        int64_t key_off, int key_bytes, uint64_t key_xmask,
+                                                        const int64_t* __restrict__ begin,
+                                                        const int64_t* __restrict__ count, float* __restrict__ q0,
+                                                        float* __restrict__ q1, int* __restrict__ nedge) {
+    __shared__ float s_v[256];  // the model's value at sample i
+    __shared__ float s_f[256];  // the same values, sorted
     __shared__ float s_w[64];
-    const int s = blockIdx.x, lane = threadIdx.x;
+    const int s = blockIdx.x, tid = threadIdx.x, lane = tid;
     const int64_t b = begin[s], c = count[s];
-    float f[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        f[j] = 0.f;
-        if (c > 0) {
-            const int i = lane * 4 + j;
-            int64_t pos = b + (int64_t)(((double)i + 0.5) * (double)c / 256.0);
-            pos = pos >= b + c ? b + c - 1 : pos;
-            const uint8_t* p = recs + pos * elsize + key_off;
-            uint64_t k = 0;
-            for (int q = 0; q < key_bytes; q++) k |= (uint64_t)p[q] << (8 * q);
-            f[j] = model_eval(m.eff, m, k ^ key_xmask);
-        }
+    // sample i = tid: the record at b + floor((i + 1/2) c / 256)
+    float f = 0.f;
+    if (c > 0) {
+        int64_t pos = b + (int64_t)(((double)tid + 0.5) * (double)c / 256.0);
+        pos = pos >= b + c ? b + c - 1 : pos;
+        const uint8_t* p = recs + pos * elsize + key_off;
+        uint64_t k = 0;
+        for (int q = 0; q < key_bytes; q++) k |= (uint64_t)p[q] << (8 * q);
+        f = model_eval(m.eff, m, k ^ key_xmask);
     }
+    s_v[tid] = f;
+    __syncthreads();
     // rank of each value among the 256 (ties by sample index)
-    int rank[4] = {0, 0, 0, 0};
-    for (int src = 0; src < 64; src++) {
-#pragma unroll
-        for (int jj = 0; jj < 4; jj++) {
-            const float v = __shfl(f[jj], src, 64);
-            const int vi = src * 4 + jj;
-#pragma unroll
-            for (int j = 0; j < 4; j++) rank[j] += (v < f[j] || (v == f[j] && vi < lane * 4 + j)) ? 1 : 0;
-        }
+    int rank = 0;
+    for (int i = 0; i < 256; i++) {
+        const float v = s_v[i];
+        rank += (v < f || (v == f && i < tid)) ? 1 : 0;
     }
-#pragma unroll
-    for (int j = 0; j < 4; j++) s_f[rank[j]] = f[j];
+    s_f[rank] = f;
     __syncthreads();
     // windows of 246 consecutive sorted values starting at 0 .. 10
     constexpr int kKeep = 246;
-    s_w[lane] = lane <= 256 - kKeep ? s_f[lane + kKeep - 1] - s_f[lane] : 2.f;
+    if (lane < 64) s_w[lane] = lane <= 256 - kKeep ? s_f[lane + kKeep - 1] - s_f[lane] : 2.f;
     __syncthreads();
     if (lane == 0) {
         int best = 0;
@@ -638,7 +631,7 @@ int launch_seg_range(const ModelDev& m, const void* recs, int64_t elsize, int64_
                      uint64_t key_xmask, const int64_t* begin, const int64_t* count, int nseg, float* q0, float* q1,
                      int* nedge, hipStream_t st) {
     if (nseg <= 0) return 0;
-    hipLaunchKernelGGL(seg_range_kernel, dim3(nseg), dim3(64), 0, st, m, (const uint8_t*)recs, elsize, key_off,
+    hipLaunchKernelGGL(seg_range_kernel, dim3(nseg), dim3(256), 0, st, m, (const uint8_t*)recs, elsize, key_off,
                        key_bytes, key_xmask, begin, count, q0, q1, nedge);
     MPSG_CHECK(hipGetLastError());
     return 0;
