@@ -201,6 +201,19 @@ def bench_batched(args):
             return 5
     plan.close()
 
+    if args.plan_only:
+        for g in groups:
+            lib.mpsort_gpu_dfree(g["src"])
+            lib.mpsort_gpu_dfree(g["dst"])
+        print(json.dumps({"metric": "Mrecords/s, batched segmented sort (%d segments, plan only)" % nseg,
+                          "value": round(nrec / (ms * 1e-3) / 1e6, 2), "unit": "Mrecords/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "i32/i64",
+                          "data": "synthetic", "config": {"workload": "batched, plan only", "segments": nseg,
+                                                          "records": nrec, "bytes": nbytes},
+                          "tuned_plan": {"i32_f32": plan_info[0], "i64_i64": plan_info[1]}}), flush=True)
+        return 0
+
     # the per-segment loop the plan replaces, with the same kernels, tile
     # shapes and workspaces: every segment by launches of its own, enqueued
     # back to back (no host wait between segments; prepared outside the
@@ -409,6 +422,9 @@ def main():
     ap.add_argument("--cpu-quick", action="store_true",
                     help="CPU baseline: the -O3 mpsort_omp only (no default build, no mpirun)")
     ap.add_argument("--segments", type=int, default=10000)
+    ap.add_argument("--plan-only", action="store_true",
+                    help="batched workload: time the plan only (no per-segment or per-call loop), "
+                         "e.g. to keep a kernel trace of it small")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain copy kernel of n*16 bytes (byte-counter calibration)")
     args = ap.parse_args()
