@@ -213,9 +213,13 @@ static int get_ctx(DevCtx** out) {
     if (!c.init) {
         c.device = dev;
         MPSG_CHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
-        for (int i = 0; i < kEvPhase; i++) MPSG_CHECK(hipEventCreate(&c.ev[i]));
+        // the phase and pass events only time the work (the host waits on
+        // the stream): no system-scope fence, which would write back and
+        // invalidate the caches between kernels at every event
+        for (int i = 0; i < kEvPhase; i++) MPSG_CHECK(hipEventCreateWithFlags(&c.ev[i], hipEventDisableSystemFence));
         for (int a = 0; a < 2; a++)
-            for (int i = 0; i < 2 * kMaxPasses; i++) MPSG_CHECK(hipEventCreate(&c.pass_ev[a][i]));
+            for (int i = 0; i < 2 * kMaxPasses; i++)
+                MPSG_CHECK(hipEventCreateWithFlags(&c.pass_ev[a][i], hipEventDisableSystemFence));
         c.lookback.zero_on_grow = true;
         // the device address of this GPU's scratch array (the device is
         // current); only the allocation-free entry point uses it
