@@ -39,7 +39,7 @@ TRAFFIC_PLAIN_BUCKET_SORT = 7984629536
 # steps of `python bench.py --steps 20 --warmup 5`) from its rocprofv3 kernel
 # trace (profiles/histsort_clustered_2e28x16B_kernel_trace.md)
 ROCPROF_SOURCE = "profiles/histsort_clustered_2e28x16B_kernel_trace.md"
-ROCPROF_MS = {"bucket_sort_A": 2.9063, "plain_bucket_sort": 2.4642, "onesweep": 2.3706}
+ROCPROF_MS = {"bucket_sort_A": 2.8528, "plain_bucket_sort": 2.4325, "onesweep": 2.3646}
 # per-side stages of the learned local sort, as timed by the library; with
 # the merged SecondSort also its two gather-and-sort launches alone
 # (merge_B: groups of buckets, then the larger single buckets), which
