@@ -1253,10 +1253,23 @@ static int learned_local_sort(DevCtx& c, int rw, const KeySpec& ks, int P, int64
     };
     const LKey lk = make_lkey(ks);
     const int64_t trecs = lscatter_tile(rw);
-    if ((rc = launch_fill_u32(L.lhist, 0u, (int64_t)P * kLBins, st))) return rc;
-    if ((rc = launch_fill_u32(L.tickets, 0u, 4, st))) return rc;
-    if ((rc = launch_fill_u32(L.err, 0u, 4, st))) return rc;
-    if ((rc = launch_fill_u32(L.noverflow, 0u, 4, st))) return rc;
+    {
+        // the counters cleared by one launch (the bucket counts in pieces
+        // of 8192 words, one workgroup each)
+        FanOut f;
+        f.n = 0;
+        const int64_t nh = (int64_t)P * kLBins;
+        for (int64_t o = 0; o < nh && f.n < kFanMax - 3; o += 8192)
+            f.e[f.n++] = FanEntry{L.lhist + o, nullptr, (uint32_t)std::min<int64_t>(8192, nh - o), 0u};
+        if (nh > (int64_t)8192 * (kFanMax - 3)) {
+            f.n = 0;
+            if ((rc = launch_fill_u32(L.lhist, 0u, nh, st))) return rc;
+        }
+        f.e[f.n++] = FanEntry{L.tickets, nullptr, 4u, 0u};
+        f.e[f.n++] = FanEntry{L.err, nullptr, 4u, 0u};
+        f.e[f.n++] = FanEntry{L.noverflow, nullptr, 4u, 0u};
+        if ((rc = launch_fanout(f, st))) return rc;
+    }
     lc->lists[lc->lists[0] ? 1 : 0] = L.noverflow;
     // tiles of the learned scatter (the look-back buffer was sized by the
     // caller: it must not move while plans hold its address)

@@ -199,14 +199,27 @@ uint32_t next_epoch(int device);
 
 // workgroups of kernel f (threads each) resident at once on the current
 // device; 2048 when the runtime cannot say
+// (queried once per kernel, block size and device: the query costs host
+// time at every launch otherwise)
 template <typename F>
 inline int64_t resident_blocks(F f, int threads) {
+    struct Entry {
+        const void* f;
+        int threads, dev;
+        int64_t n;
+    };
+    static Entry cache[64];
+    static int ncache = 0;
     int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+    if (hipGetDevice(&dev) != hipSuccess) return 2048;
+    for (int i = 0; i < ncache; i++)
+        if (cache[i].f == (const void*)f && cache[i].threads == threads && cache[i].dev == dev) return cache[i].n;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, threads, 0) != hipSuccess || cus <= 0 || per <= 0)
         return 2048;
-    return (int64_t)cus * per;
+    const int64_t n = (int64_t)cus * per;
+    if (ncache < 64) cache[ncache++] = Entry{(const void*)f, threads, dev, n};
+    return n;
 }
 
 }  // namespace mpsg
